@@ -1,0 +1,112 @@
+// device_math.hpp -- pair terms, minimum image, counter-based random numbers
+// and wave64 reductions shared by the engine's kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace graspa {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): a counter-based generator,
+// so every insertion owns a stream addressed by (seed, stream, index, draw).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (uint32_t)p1;
+    c[3] = (uint32_t)p0;
+    c[0] = n0;
+    c[2] = n2;
+  }
+}
+
+// Uniform double in (0,1): 53 bits from two output words of one Philox block.
+// Draws 2j and 2j+1 share the block with counter word 2 = j.
+__host__ __device__ inline double philox_uniform(uint64_t seed, uint32_t stream, uint64_t index, uint32_t draw)
+{
+  uint32_t c[4] = {(uint32_t)index, (uint32_t)(index >> 32), draw >> 1, stream};
+  philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint32_t a = (draw & 1u) ? c[2] : c[0];
+  const uint32_t b = (draw & 1u) ? c[3] : c[1];
+  const uint64_t x = (((uint64_t)a << 32) | b) >> 11;
+  return ((double)x + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// ---------------------------------------------------------------------------
+// Minimum image in FP64, rounding half away from zero (gRASPA maths.cuh:427-450).
+// ---------------------------------------------------------------------------
+__device__ inline void pbc_fp64(double& x, double& y, double& z, const double* cell, const double* inv, int cubic)
+{
+  if (cubic) {
+    x -= (double)(int)(x * inv[0] + ((x >= 0.0) ? 0.5 : -0.5)) * cell[0];
+    y -= (double)(int)(y * inv[4] + ((y >= 0.0) ? 0.5 : -0.5)) * cell[4];
+    z -= (double)(int)(z * inv[8] + ((z >= 0.0) ? 0.5 : -0.5)) * cell[8];
+  } else {
+    double s0 = inv[0] * x + inv[3] * y + inv[6] * z;
+    double s1 = inv[1] * x + inv[4] * y + inv[7] * z;
+    double s2 = inv[2] * x + inv[5] * y + inv[8] * z;
+    s0 -= (double)(int)(s0 + ((s0 >= 0.0) ? 0.5 : -0.5));
+    s1 -= (double)(int)(s1 + ((s1 >= 0.0) ? 0.5 : -0.5));
+    s2 -= (double)(int)(s2 + ((s2 >= 0.0) ? 0.5 : -0.5));
+    x = cell[0] * s0 + cell[3] * s1 + cell[6] * s2;
+    y = cell[1] * s0 + cell[4] * s1 + cell[7] * s2;
+    z = cell[2] * s0 + cell[5] * s1 + cell[8] * s2;
+  }
+}
+
+// 12-6 LJ at full coupling (gRASPA maths.cuh:452-494 with scaling = 1):
+// ff = {4 eps, sigma^2, shift}.
+__device__ inline double lj_fp64(double rr, double a4e, double s2, double shift)
+{
+  const double temp = rr / s2;
+  const double temp3 = temp * temp * temp;
+  const double rri3 = 1.0 / (temp3 + 0.0);
+  return a4e * (rri3 * (rri3 - 1.0)) - shift;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: positions are 32-bit fixed-point cell fractions, so the
+// difference of two positions wraps to the minimum image exactly in integer
+// arithmetic (fraction in [-1/2, 1/2)) before one conversion to FP32.
+// ---------------------------------------------------------------------------
+__device__ inline float r2_fixed(uint32_t ax, uint32_t ay, uint32_t az, uint32_t bx, uint32_t by, uint32_t bz,
+                                 const float* h, int cubic)
+{
+  const float fx = (float)(int32_t)(ax - bx);
+  const float fy = (float)(int32_t)(ay - by);
+  const float fz = (float)(int32_t)(az - bz);
+  if (cubic) {
+    const float dx = fx * h[0], dy = fy * h[4], dz = fz * h[8];
+    return dx * dx + dy * dy + dz * dz;
+  }
+  const float dx = h[0] * fx + h[3] * fy + h[6] * fz;
+  const float dy = h[1] * fx + h[4] * fy + h[7] * fz;
+  const float dz = h[2] * fx + h[5] * fy + h[8] * fz;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// ---------------------------------------------------------------------------
+// wave64 reductions
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+__device__ inline uint32_t wave_or(uint32_t v)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v |= __shfl_xor(v, m, 64);
+  return v;
+}
+
+}  // namespace graspa

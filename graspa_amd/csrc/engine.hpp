@@ -1,0 +1,155 @@
+// engine.hpp -- internal state of a box resident on one MI355X and the
+// launchers of the engine's kernels.  Not part of the public C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/graspa_engine.h"
+
+namespace graspa {
+
+// Sites passed to the per-move kernels by value in the kernel argument block:
+// one CBMC trial batch or one moved molecule (old + new) fits.
+constexpr int kMaxSites = 64;
+constexpr int kMaxTypes = 32;
+constexpr int kMaxComps = 8;
+constexpr int kBlock = 256;          // 4 wave64 per workgroup
+
+struct SiteBatch {
+  int n;
+  int _pad;
+  double x[kMaxSites], y[kMaxSites], z[kMaxSites], q[kMaxSites];
+  int type[kMaxSites];
+};
+
+// Geometry and force field as the kernels see them.
+struct BoxConst {
+  double cell[9], inv[9];
+  int cubic;
+  int ntypes;
+  double cut_vdw2, cut_coul2, overlap;
+  double alpha, prefactor;
+  int use_charges;
+  int kmax[3];
+  int nvec;
+  // mixed-precision helpers: fixed-point fraction -> Cartesian, per column
+  float h32[9];                      // cell * 2^-32
+  float cut_vdw2f, cut_coul2f, overlapf, alphaf, preff;
+};
+
+// Guest segment of one component in the combined guest store.
+struct CompSeg {
+  int base;      // first atom
+  int n_atoms;   // n_mol * molsize
+};
+
+struct GuestView {
+  int n_comp;
+  int total;     // atoms in use over all components (for bounds)
+  CompSeg seg[kMaxComps];   // index 1..n_comp-1 used, 0 unused (framework)
+};
+
+struct Component {
+  int molsize = 0, capacity = 0, n_mol = 0, base = 0;
+  std::vector<double> pos;      // host mirror, capacity*molsize*3
+  std::vector<double> q;
+  std::vector<int> type;
+  std::vector<int64_t> molid;
+};
+
+struct TimingState {
+  std::string family;
+  bool on = false;
+  std::vector<hipEvent_t> ev;        // pairs
+  int64_t launches = 0;
+};
+
+}  // namespace graspa
+
+struct graspa_box {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int precision = GRASPA_FP64;
+  graspa::BoxConst bc{};
+  double volume = 0.0;
+  double recip_cut = 0.0;
+  int lammps_ewald = 0;
+  std::vector<double> eps, sig, shift;   // host copies [T*T]
+
+  // force field tables: {4 eps, sigma^2, shift, 0} per type pair
+  double* d_ff64 = nullptr;
+  float* d_ff32 = nullptr;
+
+  // framework (component 0)
+  int n_fw = 0;
+  double* d_fw = nullptr;       // SoA [5][n_fw]: x, y, z, q, (type as double unused)
+  int* d_fw_type = nullptr;
+  uint4* d_fw_fix = nullptr;    // fixed-point fraction x,y,z + float q bits
+  std::vector<double> fw_pos, fw_q;
+  std::vector<int> fw_type;
+
+  // guests: one store, a segment per component
+  std::vector<graspa::Component> comps;  // [0] = framework placeholder
+  int guest_cap = 0;
+  double* d_g = nullptr;        // SoA [4][guest_cap]: x, y, z, q
+  int* d_g_type = nullptr;
+  int* d_g_molid = nullptr;
+  int* d_g_comp = nullptr;
+  uint4* d_g_fix = nullptr;
+
+  // Ewald
+  int nvec = 0, n_active = 0;
+  double* d_ktemp = nullptr;    // [nvec] prefactor * exp(-k^2/4a^2)/k^2 (x2 for kx>0), 0 if inactive
+  double2* d_sk_fw = nullptr;
+  double2* d_sk_ads = nullptr;
+  double2* d_sk_stage = nullptr;
+  int* d_kact = nullptr;        // active kxyz list
+  std::vector<double> ktemp;
+  bool sk_staged = false;
+
+  // scratch for per-move results
+  double* d_part = nullptr;     // block partials
+  size_t part_cap = 0;
+  double* h_part = nullptr;     // pinned mirror
+  unsigned* d_ticket = nullptr;
+
+  // Widom scratch
+  double* d_wsum = nullptr;     // per-workgroup partial sums
+  graspa_widom_result* d_wres = nullptr;
+  size_t wres_cap = 0;
+  int n_cu = 256;
+
+  graspa::TimingState timing;
+};
+
+namespace graspa {
+
+// error reporting
+int fail(int code, const std::string& msg);
+int hip_check(hipError_t e, const char* what);
+
+// geometry helpers shared by host code
+void fixed_from_cart(const BoxConst& bc, const double* p, uint32_t out[3]);
+void guest_view(const graspa_box* b, GuestView* gv);
+
+// kernel launchers (engine_kernels.hip)
+int launch_site_energies(graspa_box* b, const SiteBatch& sb, int self_comp, int self_molid, int excl_comp,
+                         int excl_molid, double* d_out /* n_sites*5 */);
+int launch_ewald_delta(graspa_box* b, const SiteBatch& sb, int n_old, double* d_out /* blocks*2 */, int* n_blocks);
+int launch_ewald_build(graspa_box* b, double* d_out_energy /* blocks*3 */, int* n_blocks);
+int launch_total_pairs(graspa_box* b, double* d_out /* blocks*4 */, int* n_blocks);
+
+// widom_kernel.hip
+int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
+                 uint64_t seed, uint32_t stream, int64_t first_index, int n_insert, graspa_widom_result* d_res,
+                 graspa_widom_summary* summary);
+
+// timing
+void timing_begin(graspa_box* b, const char* family);
+void timing_end(graspa_box* b, const char* family);
+
+}  // namespace graspa

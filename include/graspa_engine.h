@@ -1,0 +1,296 @@
+/*
+ * graspa_engine.h -- C-ABI of the MI355X GCMC energy engine.
+ *
+ * The engine owns one simulation box resident in HBM: framework atoms, guest
+ * molecules, the force-field table, the Ewald tables and the running structure
+ * factors.  The per-move entry points below are the calls the gRASPA Monte
+ * Carlo driver makes into its energy code; each cites the reference interface
+ * it replaces (gRASPA src_clean/ file:line).  The reference has no FFI for this
+ * path: its boundary is the set of in-process energy calls made from one host
+ * thread on the default stream, and this header is that boundary as plain C.
+ *
+ * Conventions
+ *   - Internal units of the reference: energy 10 J/mol, length Angstrom, charge e.
+ *   - All pointers are host pointers unless a name says otherwise; arrays of
+ *     positions are xyz-interleaved doubles.
+ *   - Every function returns GRASPA_OK (0) or a negative GRASPA_ERR_* code and
+ *     never exits the process (the reference prints and calls exit(); see
+ *     VDW_Coulomb.cuh:58-66).  graspa_last_error() returns the message of the
+ *     last failure on the calling thread.
+ *   - One host thread per box.  Calls on one box are ordered on the box's own
+ *     HIP stream and are synchronous unless documented otherwise.
+ *   - Component 0 is the framework; adsorbate components are numbered from 1 in
+ *     the order graspa_box_add_component returns them.
+ */
+#ifndef GRASPA_ENGINE_H
+#define GRASPA_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRASPA_OK                0
+#define GRASPA_ERR_ARG          -1
+#define GRASPA_ERR_HIP          -2
+#define GRASPA_ERR_CAPACITY     -3
+#define GRASPA_ERR_STATE        -4
+#define GRASPA_ERR_UNSUPPORTED  -5
+
+#define GRASPA_FP64   0   /* plain FP64 form: pair terms, Ewald and sums in FP64           */
+#define GRASPA_MIXED  1   /* FP32 pair terms with FP64 accumulation; FP64 Ewald and S(k)   */
+
+typedef struct graspa_box graspa_box;
+typedef struct graspa_mc graspa_mc;
+
+/* Box, force field and Ewald set-up.  Replaces the state the reference keeps in
+ * Boxsize / ForceField (data_struct.h:838-886) after read_data.cpp's parsers. */
+typedef struct {
+  double cell[9];            /* a = cell[0..2], b = cell[3..5], c = cell[6..8] (read_data.cpp:1545-1547) */
+  int    ntypes;             /* pseudo-atom types                                          */
+  const double* epsilon;     /* [ntypes*ntypes] mixed epsilon, internal units               */
+  const double* sigma;       /* [ntypes*ntypes] mixed sigma, Angstrom                       */
+  const double* shift;       /* [ntypes*ntypes] energy shift at the cutoff (0 = truncated)  */
+  double cutoff_vdw;         /* Angstrom                                                    */
+  double cutoff_coulomb;     /* Angstrom                                                    */
+  double overlap_criteria;   /* OverlapCriteria (read_data.cpp:455)                        */
+  int    use_charges;        /* ChargeMethod Ewald = 1, None = 0                            */
+  double alpha;              /* Ewald alpha, 1/Angstrom                                     */
+  int    kmax[3];
+  double reciprocal_cutoff;  /* integer k^2 cut, or |k|^2 cut when lammps_ewald            */
+  int    lammps_ewald;       /* Ewald_UseLAMMPS_Setup                                       */
+  double prefactor;          /* Coulomb prefactor 138935.483496 (read_data.cpp:611)        */
+  int    precision;          /* GRASPA_FP64 or GRASPA_MIXED                                 */
+  int    device;             /* HIP device ordinal                                          */
+} graspa_box_params;
+
+/* MoveEnergy (data_struct.h:416-442): the fields on the GCMC energy path. */
+typedef struct {
+  double hh_vdw, hg_vdw, gg_vdw;
+  double hh_real, hg_real, gg_real;
+  double hh_ewald, hg_ewald, gg_ewald;
+  double tail;
+} graspa_energy;
+
+/* Per-insertion result of a batched Widom call. */
+typedef struct {
+  double rosen;              /* Rosenbluth weight W (first bead x chain x Ewald x tail)      */
+  double energy;             /* dE of the selected configuration, all terms                 */
+  double vdw, real, ewald;   /* its parts                                                   */
+  int    first_bead;         /* selected first-bead trial                                   */
+  int    orientation;        /* selected orientation                                        */
+  int    ok;                 /* 0 if the CBMC construction failed (W = 0)                   */
+  int    _pad;
+} graspa_widom_result;
+
+/* Sums over a batched Widom call (device-side reduction). */
+typedef struct {
+  double sum_w;              /* sum of W                                                    */
+  double sum_w2;             /* sum of W^2                                                  */
+  double sum_we;             /* sum of W * dE                                               */
+  double n;                  /* insertions                                                  */
+  double n_first_bead;       /* insertions whose first bead was placed (orientations grown) */
+  double n_ok;               /* insertions with a complete construction (W > 0)             */
+} graspa_widom_summary;
+
+/* ------------------------------------------------------------------------ */
+/* Box life cycle and state                                                  */
+/* ------------------------------------------------------------------------ */
+const char* graspa_last_error(void);
+const char* graspa_version(void);
+int  graspa_device_count(int* n);
+
+/* Creates a box on params->device.  Replaces the allocation part of
+ * Initialize() (main.cpp:58-343) and Copy_ForceField_to_GPU (fxn_main.h:129-150). */
+int  graspa_box_create(const graspa_box_params* params, graspa_box** out);
+void graspa_box_destroy(graspa_box* box);
+
+/* Framework atoms (component 0, one rigid molecule): CheckFrameworkCIF output
+ * (read_data.cpp:1474-1849). */
+int  graspa_box_set_framework(graspa_box* box, int n_atoms, const double* xyz, const double* charge,
+                              const int* type);
+
+/* Adds an adsorbate species with room for capacity_molecules molecules.  The
+ * template (MoleculeDefinitionParser, read_data.cpp:2044-2195) fills slot 0
+ * until a molecule is stored there, as in the reference.  *comp_out is >= 1. */
+int  graspa_box_add_component(graspa_box* box, int molsize, int capacity_molecules, const double* template_xyz,
+                              const int* template_type, const double* template_charge, int* comp_out);
+
+/* Replaces all molecules of a component (RestartFileParser, read_data.cpp:3000-3221). */
+int  graspa_box_set_molecules(graspa_box* box, int comp, int n_mol, const double* xyz, const double* charge,
+                              const int* type, const int64_t* molid);
+/* Copies a component's atoms back to the host; arrays sized for n_mol*molsize
+ * atoms (any pointer may be NULL). */
+int  graspa_box_get_molecules(graspa_box* box, int comp, int* n_mol, double* xyz, double* charge, int* type,
+                              int64_t* molid);
+
+/* Builds the framework and adsorbate structure factors on the device from the
+ * current atoms and returns the raw Fourier sums (Ewald_Total,
+ * ewald_preparation.h:5-259; the stored guest S(k) is total minus framework). */
+int  graspa_box_init_ewald(graspa_box* box, graspa_energy* fourier);
+/* Structure factors as 2*nvec doubles (re,im) in the reference's kxyz order
+ * (Ewald_Energy_Functions.h:296-300).  Either pointer may be NULL. */
+int  graspa_box_get_structure_factors(graspa_box* box, double* sk_framework, double* sk_adsorbate);
+int  graspa_box_set_structure_factors(graspa_box* box, const double* sk_framework, const double* sk_adsorbate);
+int  graspa_box_nvec(graspa_box* box, int* nvec, int* n_active);
+
+/* ------------------------------------------------------------------------ */
+/* Per-move energies (the hot path)                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Pair energies of n_sites probe sites against every framework and guest atom,
+ * skipping atoms of molecule (self_comp, self_molid) and of the ExcludeList
+ * entry (excl_comp, excl_molid); pass -1 to disable either.
+ * out4[4*s+{0,1,2,3}] = host-guest VDW, host-guest real, guest-guest VDW,
+ * guest-guest real.  overlap[s] = 1 if a pair of site s exceeded
+ * OverlapCriteria or came within 0.1 A.
+ * Replaces Calculate_Multiple_Trial_Energy_VDWReal (VDW_Coulomb.cuh:93,
+ * VDW_Coulomb.cu:1183-1352) with its per-block partials summed on the device
+ * instead of in Host_sum_Widom_HGGG_SEPARATE (mc_widom.h:41-87). */
+int  graspa_site_energies(graspa_box* box, int n_sites, const double* xyz, const int* type, const double* charge,
+                          int self_comp, int64_t self_molid, int excl_comp, int64_t excl_molid,
+                          double* out4, uint8_t* overlap);
+
+/* CBMC trial batch: n_trials trials of `chain` sites each (site s of trial t at
+ * xyz[3*(t*chain+s)]; types and charges are per chain position).  Per-trial
+ * sums out4[4*t+...] and per-trial overlap flags, as CBMC_PairwiseInteractions
+ * (mc_widom.h:89-119) returns them. */
+int  graspa_trial_energies(graspa_box* box, int n_trials, int chain, const double* xyz, const int* type,
+                           const double* charge, int self_comp, int64_t self_molid, int excl_comp,
+                           int64_t excl_molid, double* out4, uint8_t* overlap);
+
+/* Energy change of moving molecule (comp, molid) of `molsize` atoms from old_xyz
+ * to new_xyz (HG/GG x VDW/real; Ewald fields left 0); *overlap from the new
+ * position only.  Replaces Calculate_Single_Body_Energy_VDWReal
+ * (VDW_Coulomb.cuh:91, VDW_Coulomb.cu:626-841) with the host block sum of
+ * SingleBody_Calculation (mc_single_particle.h:183-200). */
+int  graspa_single_body_delta(graspa_box* box, int comp, int64_t molid, int molsize, const double* old_xyz,
+                              const double* new_xyz, const int* type, const double* charge, graspa_energy* de,
+                              int* overlap);
+
+/* Fourier-space energy change of removing n_old charges and adding n_new
+ * (same-type = adsorbate S(k), cross = framework S(k)).  *same is the raw
+ * same-type sum and *cross the raw cross sum: the reference's
+ * GPU_EwaldDifference_General (VDW_Coulomb.cuh:112, Ewald_Energy_Functions.h:438-580)
+ * returns {same - exclusion, 2*cross}.  The updated adsorbate S(k) is staged on
+ * the device until graspa_ewald_accept. */
+int  graspa_ewald_delta(graspa_box* box, int n_old, const double* old_xyz, const double* old_charge, int n_new,
+                        const double* new_xyz, const double* new_charge, double* same, double* cross);
+/* Makes the staged S(k) current: Update_Vector_Ewald (Ewald_Energy_Functions.h:423-433). */
+int  graspa_ewald_accept(graspa_box* box);
+
+/* Commits.  update_translation_position (mc_utilities.h:285-292),
+ * Update_insertion_data_Parallel (mc_utilities.h:191-233) and
+ * Update_deletion_data_Parallel (mc_utilities.h:163-189): a deletion copies the
+ * last molecule's positions, charges and types into the hole, whose MolIDs stay. */
+int  graspa_commit_move(graspa_box* box, int comp, int mol_slot, const double* new_xyz);
+int  graspa_commit_insert(graspa_box* box, int comp, const double* xyz, const double* charge, const int* type,
+                          int64_t molid);
+int  graspa_commit_delete(graspa_box* box, int comp, int mol_slot);
+
+/* Total VDW + real-space energies (HG, GG) of the current configuration and
+ * raw Fourier sums from the running S(k): Total_VDW_Coulomb_Energy
+ * (VDW_Coulomb.cu:1580-1665) and Ewald_TotalEnergy (Ewald_Energy_Functions.h:1272-1430). */
+int  graspa_total_energy(graspa_box* box, graspa_energy* e);
+
+/* ------------------------------------------------------------------------ */
+/* Batched Widom / CBMC insertions                                           */
+/* ------------------------------------------------------------------------ */
+
+/* n_insert independent Widom test insertions of component `comp`, each a full
+ * CBMC construction (n_trial_pos first-bead trials, n_trial_orient rigid-body
+ * orientations, Ewald and tail of the selected configuration), in one
+ * persistent launch.  Insertion i draws its random numbers from the
+ * counter-based Philox4x32-10 stream (seed, stream, first_index + i), so a
+ * result does not depend on batch size or scheduling.  The semantics are
+ * Insertion_Body (mc_swap_utilities.h:3-133) on the WIDOM branch of RunMoves
+ * (axpy.cu:163-186).  excl = ExclusionIntra + ExclusionAtom of the species
+ * (ewald_preparation.h:351-366).  results may be NULL (summary only). */
+int  graspa_widom_batch(graspa_box* box, int comp, int n_trial_pos, int n_trial_orient, double beta,
+                        int vdw_real_bias, double excl, double tail_de, uint64_t seed, uint32_t stream,
+                        int64_t first_index, int n_insert, graspa_widom_result* results,
+                        graspa_widom_summary* summary);
+
+/* ------------------------------------------------------------------------ */
+/* GCMC driver in the reference's random-number order (parity mode)          */
+/* ------------------------------------------------------------------------ */
+
+/* Per-species move set-up, the Components fields RunMoves reads. */
+typedef struct {
+  double p_translation, p_rotation, p_widom, p_swap;   /* cumulative, normalised (data_struct.h:569-608) */
+  double mol_fraction, fugacity_coeff, ideal_rosen;
+  double excl_intra, excl_self;                        /* rigid exclusion (ewald_preparation.h:351-366) */
+  double max_translation[3], max_rotation[3];
+  int64_t pseudo_per_mol[16];                          /* pseudo-atom counts of one molecule            */
+} graspa_species;
+
+typedef struct {
+  int     n_species;               /* adsorbate components 1..n_species                      */
+  graspa_species species[8];
+  int     n_trial_pos, n_trial_orient;
+  int     vdw_real_bias;
+  int     use_max_step, max_step_per_cycle;
+  int     has_tail;
+  const double* tail;              /* [ntypes*ntypes] tail coefficients (may be NULL)         */
+  const int*    use_tail;          /* [ntypes*ntypes]                                         */
+  double  beta, pressure;          /* internal units                                          */
+  unsigned seed;                   /* srand seed (RandomSeed)                                  */
+  int     pool_size;               /* device random pool entries (main.cpp:145: 333334)       */
+  int64_t n_pseudo[16];            /* pseudo-atom counts of the whole box                      */
+} graspa_mc_params;
+
+/* One record per MC step. */
+typedef struct {
+  int     cycle, step, move, comp, mol, accepted, overlap, _pad;
+  double  pacc;
+  double  rosen;
+  graspa_energy de;
+} graspa_trace;
+
+#define GRASPA_MOVE_TRANSLATION 0
+#define GRASPA_MOVE_ROTATION    1
+#define GRASPA_MOVE_INSERTION   2
+#define GRASPA_MOVE_DELETION    3
+#define GRASPA_MOVE_WIDOM       4
+#define GRASPA_MOVE_NONE        5
+
+/* RandomNumber::Setup (data_struct.h:1287-1346) plus the move bookkeeping of
+ * Initialize (main.cpp:58-343).  Uses the engine's restatement of glibc rand(). */
+int  graspa_mc_create(graspa_box* box, const graspa_mc_params* params, graspa_mc** out);
+void graspa_mc_destroy(graspa_mc* mc);
+/* Run_Simulation_ForOneBox (axpy.cu:627-641) for n_cycles cycles starting at
+ * cycle index first_cycle; production != 0 records Widom statistics.  Returns the
+ * number of steps in *n_steps; up to max_trace records go to trace (may be NULL). */
+int  graspa_mc_cycles(graspa_mc* mc, int first_cycle, int n_cycles, int production, graspa_trace* trace,
+                      int max_trace, int64_t* n_steps);
+/* Running energy change (SystemComponents.deltaE), molecule counts, move
+ * acceptance counters [species][{trans_tot, trans_acc, rot_tot, rot_acc,
+ * ins_tot, ins_acc, del_tot, del_acc, widom_tot}], Widom sums. */
+int  graspa_mc_stats(graspa_mc* mc, graspa_energy* running, int* n_mol, int64_t* counters, double* widom_sum_w,
+                     double* widom_n);
+/* Per-step wall-clock latency (seconds) of the last graspa_mc_cycles call,
+ * one entry per cycle; returns the number written. */
+int  graspa_mc_cycle_times(graspa_mc* mc, double* seconds, int max_n);
+
+/* ------------------------------------------------------------------------ */
+/* Measurement                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Kernel timing with HIP events on the box's stream.  While enabled, every
+ * launch of the named kernel family ("widom", "site", "ewald") is bracketed by
+ * events; graspa_timing_read returns the summed elapsed milliseconds and the
+ * number of launches since graspa_timing_enable. */
+int  graspa_timing_enable(graspa_box* box, const char* family, int on);
+int  graspa_timing_read(graspa_box* box, double* total_ms, int64_t* launches);
+/* Blocks until all work queued on the box's stream is done. */
+int  graspa_box_sync(graspa_box* box);
+
+/* glibc-compatible rand() restatement used by the parity driver (exposed for
+ * its test against the C library): srand(seed) then n draws of rand(). */
+int  graspa_glibc_rand_sequence(unsigned seed, int n, int* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

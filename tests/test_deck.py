@@ -1,0 +1,44 @@
+"""Deck reader: synthetic decks written in gRASPA's formats read back to the
+same system; the reference's unit conversions."""
+import numpy as np
+
+from graspa_amd import deck, synth
+
+
+def test_units():
+    # fxn_main.h:115-127: beta at 300 K and 1e5 Pa in internal units
+    assert abs(deck.beta_of(300.0) - 1.0 / (0.8314462618 * 300.0 * (1.380649e-23 / 1.380649e-23))) < 1e-3 * 0.004
+    assert abs(deck.pressure_internal(1.6605402e7) - 1.0) < 1e-15
+
+
+def test_ewald_parameters_raspa_mode():
+    s = synth.config1()
+    # alpha from EwaldPrecision 1e-6 and rc 12 (read_data.cpp:693-702)
+    assert abs(s.alpha - 0.265058) < 1e-6
+    assert s.kmax == (7, 7, 7)
+    assert abs(s.recip_cut - (1.05 * 7) ** 2) < 1e-12
+
+
+def test_write_read_round_trip(tmp_path):
+    s = synth.build(n_fw=300, L=20.0, species=("CO2", "N2"), mol_fraction=[0.3, 0.7], seed=9)
+    synth.write_deck(s, str(tmp_path), cycles=10)
+    r = deck.read_deck(str(tmp_path))
+    assert r.pseudo_names == s.pseudo_names
+    assert np.allclose(r.cell, s.cell)
+    assert np.allclose(r.fw_pos, s.fw_pos, atol=1e-9)
+    assert np.allclose(r.fw_charge, s.fw_charge, atol=1e-11)
+    assert np.array_equal(r.fw_type, s.fw_type)
+    assert np.allclose(r.eps, s.eps, rtol=1e-9)
+    assert np.allclose(r.sig, s.sig, rtol=1e-9)
+    assert [c.name for c in r.components] == ["CO2", "N2"]
+    assert abs(r.components[1].mol_fraction - 0.7) < 1e-12
+    assert np.allclose(r.components[0].template_pos, s.components[0].template_pos)
+    assert r.kmax == s.kmax and abs(r.alpha - s.alpha) < 1e-12
+
+
+def test_npz_round_trip(tmp_path):
+    s = synth.config0()
+    p = str(tmp_path / "s.npz")
+    deck.save_system(p, s)
+    r = deck.load_system(p)
+    assert np.array_equal(r.fw_pos, s.fw_pos) and r.kmax == s.kmax and r.alpha == s.alpha

@@ -1,0 +1,240 @@
+"""Parity of the HIP engine with the FP64 oracle, through the C-ABI.
+
+Plain FP64 form: per-move energies to 1e-10 relative (the reference's GPU FP64
+kernels differ from its CPU FP64 code in summation order only).  Mixed form
+(FP32 pair terms, FP64 accumulation, FP64 Ewald): per-move |dE| within the
+tolerances stated in each test.  Inputs are the reference's own decks (NIST
+SPC/E, TIP4P restart, charged MFI framework) and seeded synthetic boxes.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden_system
+
+pytestmark = pytest.mark.gpu
+
+FP64_RTOL = 1e-10
+
+
+def _close(a, b, rtol=FP64_RTOL, atol=1e-9):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    return np.all(np.abs(a - b) <= atol + rtol * np.abs(b))
+
+
+@pytest.fixture(scope="module")
+def engine_mod():
+    from graspa_amd import engine
+    engine.lib()
+    return engine
+
+
+def _oracle(s):
+    from oracle.binding import OracleSystem
+    return OracleSystem(s)
+
+
+# --------------------------------------------------------------------------
+# totals and structure factors on the reference's decks
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["nist_box1", "nist_box4", "tip4p_restart"])
+def test_totals_match_oracle_and_reference(name, engine_mod, golden_values):
+    s = load_golden_system(name)
+    g = golden_values[name]
+    print("case", name, flush=True)
+    box = engine_mod.Box(s, precision=engine_mod.GRASPA_FP64)
+    f = box.init_ewald()
+    e = box.total_energy()
+    o = _oracle(s)
+    oe = o.total_vdw_real()
+    of, skf, ska, se, intra, _ = o.ewald_total()
+    assert _close(e.gg_vdw, oe.gg_vdw) and _close(e.gg_real, oe.gg_real)
+    assert _close(f.gg_ewald, of.gg_ewald)
+    assert abs(e.gg_vdw - g["gg_vdw"]) < 0.5e-5 and abs(e.gg_real - g["gg_real"]) < 0.5e-5
+    assert abs(f.gg_ewald - g["gg_fourier"]) < 0.5e-5
+    gf, ga = box.structure_factors()
+    assert np.allclose(ga, ska, rtol=1e-10, atol=1e-9)
+    box.close()
+
+
+def test_charged_framework_structure_factor(engine_mod, golden_values):
+    s = load_golden_system("co2_mfi")
+    box = engine_mod.Box(s, precision=engine_mod.GRASPA_FP64)
+    f = box.init_ewald()
+    assert abs(f.hh_ewald - golden_values["co2_mfi"]["hh_fourier"]) < 0.5e-5
+    o = _oracle(s)
+    of, skf, ska, _, _, _ = o.ewald_total()
+    gf, ga = box.structure_factors()
+    assert np.allclose(gf, skf, rtol=1e-10, atol=1e-9)
+    box.close()
+
+
+# --------------------------------------------------------------------------
+# per-move primitives
+# --------------------------------------------------------------------------
+def _synth(n_fw=600, L=24.0, seed=4, n_mol=12):
+    """Synthetic CO2 box with a few guests placed on a coarse grid."""
+    from graspa_amd import synth
+    s = synth.config0(seed=seed, n_fw=n_fw, L=L, cycles=10)
+    c = s.components[0]
+    rng = np.random.default_rng(seed)
+    pos = []
+    for m in range(n_mol):
+        cen = rng.uniform(0, L, 3)
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        pos += [cen - 1.16 * v, cen, cen + 1.16 * v]
+    c.init_pos = np.array(pos)
+    c.init_molid = np.repeat(np.arange(n_mol), 3).astype(np.int64)
+    c.init_charge = np.tile(c.charges, n_mol)
+    return s
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_site_energies(prec, engine_mod):
+    s = _synth()
+    o = _oracle(s)
+    P = engine_mod.GRASPA_FP64 if prec == "fp64" else engine_mod.GRASPA_MIXED
+    box = engine_mod.Box(s, precision=P)
+    rng = np.random.default_rng(1)
+    sites = rng.uniform(0, 24.0, (40, 3))
+    types = np.array([3, 4] * 20, dtype=np.int32)
+    q = np.where(types == 3, -0.35, 0.70)
+    for self_comp, self_molid in ((-1, -1), (1, 5)):
+        got, gov = box.site_energies(sites, types, q, self_comp, self_molid)
+        ref, rov = o.site_energies(sites, types, q, self_comp, self_molid)
+        assert np.array_equal(gov, rov)
+        keep = rov == 0
+        if prec == "fp64":
+            assert _close(got[keep], ref[keep])
+        else:
+            err = np.abs(got[keep] - ref[keep])
+            # FP32 pair terms: relative 2e-6 of the sum of |pair| magnitudes, bounded here by the values
+            assert np.all(err <= 1e-3 + 2e-6 * np.abs(ref[keep])), err.max()
+    box.close()
+
+
+def test_single_body_and_ewald_delta(engine_mod):
+    s = _synth()
+    o = _oracle(s)
+    box = engine_mod.Box(s, precision=engine_mod.GRASPA_FP64)
+    box.init_ewald()
+    of, skf, ska, _, _, _ = o.ewald_total()
+    c = s.components[0]
+    old = c.init_pos[6:9]
+    new = old + np.array([0.4, -0.2, 0.3])
+    e, ov = box.single_body_delta(1, 2, old, new, c.types, c.charges)
+    en, ovn = o.site_energies(new, c.types, c.charges, 1, 2)
+    eo, _ = o.site_energies(old, c.types, c.charges, 1, 2)
+    d = en.sum(0) - eo.sum(0)
+    assert ov == int(ovn.any())
+    assert _close([e.hg_vdw, e.hg_real, e.gg_vdw, e.gg_real], d, atol=1e-8)
+    same, cross = box.ewald_delta(old, c.charges, new, c.charges)
+    osame, ocross, sk_new = o.ewald_delta(ska, skf, old, c.charges, new, c.charges)
+    assert _close(same, osame, atol=1e-8) and _close(cross, ocross, atol=1e-8)
+    box.ewald_accept()
+    _, ga = box.structure_factors()
+    assert np.allclose(ga, sk_new, rtol=1e-10, atol=1e-9)
+    box.close()
+
+
+def test_commit_and_total_energy_consistency(engine_mod):
+    """Running sums of accepted dE equal the change of the recomputed totals."""
+    s = _synth()
+    box = engine_mod.Box(s, precision=engine_mod.GRASPA_FP64)
+    f0 = box.init_ewald()
+    e0 = box.total_energy()
+    c = s.components[0]
+    old = c.init_pos[3:6]
+    new = old + np.array([-0.5, 0.25, 0.1])
+    de, ov = box.single_body_delta(1, 1, old, new, c.types, c.charges)
+    same, cross = box.ewald_delta(old, c.charges, new, c.charges)
+    box.commit_move(1, 1, new)
+    box.ewald_accept()
+    e1 = box.total_energy()
+    f1 = box.init_ewald()
+    tot = lambda e: e.hg_vdw + e.hg_real + e.gg_vdw + e.gg_real
+    run = de.hg_vdw + de.hg_real + de.gg_vdw + de.gg_real
+    assert abs((tot(e1) - tot(e0)) - run) < 1e-7 * max(1, abs(tot(e0)))
+    fo = lambda f: f.gg_ewald + f.hg_ewald
+    assert abs((fo(f1) - fo(f0)) - (same + 2 * cross)) < 1e-7 * max(1, abs(fo(f0)))
+    # insertion then deletion of the same molecule returns the totals
+    m = np.array([[5.0, 5.0, 4.0], [5.0, 5.0, 5.16], [5.0, 5.0, 6.32]])
+    box.commit_insert(1, m, c.charges, c.types, 12)
+    box.commit_delete(1, 12)
+    e2 = box.total_energy()
+    assert abs(tot(e2) - tot(e1)) < 1e-8 * max(1, abs(tot(e1)))
+    box.close()
+
+
+# --------------------------------------------------------------------------
+# batched Widom
+# --------------------------------------------------------------------------
+def _widom_pair(n_fw, n_insert, seed, prec, engine_mod, first_index=0, stream=0):
+    from graspa_amd import synth
+    from oracle.binding import widom_records_to_numpy
+    s = synth.config1(n_fw=n_fw)
+    o = _oracle(s)
+    _, skf, ska, _, _, _ = o.ewald_total()
+    intra, selfe = o.exclusion(1)
+    ref = widom_records_to_numpy(o.widom_batch(1, ska, skf, seed=seed, n_insert=n_insert, first_index=first_index,
+                                               stream=stream))
+    box = engine_mod.Box(s, precision=prec)
+    box.init_ewald()
+    res, summ = box.widom_batch(1, n_insert, seed=seed, stream=stream, first_index=first_index, excl=intra + selfe)
+    got = engine_mod.widom_results_to_numpy(res)
+    box.close()
+    return s, ref, got, summ
+
+
+def test_widom_fp64_matches_oracle(engine_mod):
+    s, ref, got, summ = _widom_pair(400, 300, 21, engine_mod.GRASPA_FP64, engine_mod)
+    assert ref["ok"].sum() > 250
+    assert np.array_equal(got["ok"], ref["ok"])
+    assert np.array_equal(got["first_bead"][ref["ok"] == 1], ref["first_bead"][ref["ok"] == 1])
+    assert np.array_equal(got["orientation"][ref["ok"] == 1], ref["orientation"][ref["ok"] == 1])
+    assert _close(got["energy"], ref["energy"], atol=1e-8)
+    assert _close(got["rosen"], ref["rosen"], rtol=1e-9, atol=0.0)
+    assert abs(summ.sum_w - ref["rosen"].sum()) <= 1e-9 * abs(ref["rosen"].sum())
+    assert summ.n == 300 and summ.n_ok == ref["ok"].sum()
+
+
+def test_widom_mixed_within_tolerance(engine_mod):
+    """Mixed form vs FP64 oracle: same constructions (selection ties aside) and
+    |dE| per insertion within 1e-5 kJ/mol-scaled tolerance for the configurations
+    that carry weight (|dE| < 1e4 internal units)."""
+    s, ref, got, summ = _widom_pair(400, 300, 23, engine_mod.GRASPA_MIXED, engine_mod)
+    ok = (ref["ok"] == 1) & (got["ok"] == 1)
+    same_sel = ok & (got["first_bead"] == ref["first_bead"]) & (got["orientation"] == ref["orientation"])
+    assert same_sel.sum() >= 0.97 * (ref["ok"] == 1).sum()
+    err = np.abs(got["energy"][same_sel] - ref["energy"][same_sel])
+    mag = np.abs(ref["energy"][same_sel])
+    light = mag < 1e4
+    print("mixed |dE| max %.3e internal (%.3e kJ/mol), median %.3e" % (err.max(), err.max() * 0.01, np.median(err)))
+    assert np.all(err[light] <= 2e-3 + 1e-7 * mag[light])
+    w_rel = np.abs(got["rosen"][same_sel] - ref["rosen"][same_sel]) / np.maximum(ref["rosen"][same_sel], 1e-300)
+    assert np.median(w_rel) < 1e-5
+
+
+def test_widom_result_independent_of_batching(engine_mod):
+    """Insertion i gives the same result alone, inside a larger batch, or
+    offset by first_index: the counter-based streams make each result a pure
+    function of (seed, stream, index)."""
+    from graspa_amd import synth
+    s = synth.config1(n_fw=400)
+    box = engine_mod.Box(s, precision=engine_mod.GRASPA_MIXED)
+    box.init_ewald()
+    res_all, _ = box.widom_batch(1, 512, seed=5)
+    a = engine_mod.widom_results_to_numpy(res_all)
+    res_sub, _ = box.widom_batch(1, 37, seed=5, first_index=100)
+    b = engine_mod.widom_results_to_numpy(res_sub)
+    for k in ("rosen", "energy", "first_bead", "orientation", "ok"):
+        assert np.array_equal(a[k][100:137], b[k])
+    box.close()
+
+
+def test_widom_dense_config_sample(engine_mod):
+    """The stated config (1,536 atoms, 24 A) in FP64 against the oracle on a sample."""
+    s, ref, got, summ = _widom_pair(1536, 128, 99, engine_mod.GRASPA_FP64, engine_mod)
+    assert np.array_equal(got["ok"], ref["ok"])
+    assert _close(got["rosen"], ref["rosen"], rtol=1e-9, atol=1e-300)
