@@ -75,22 +75,46 @@ __device__ inline double lj_fp64(double rr, double a4e, double s2, double shift)
 // ---------------------------------------------------------------------------
 // Mixed precision: positions are 32-bit fixed-point cell fractions, so the
 // difference of two positions wraps to the minimum image exactly in integer
-// arithmetic (fraction in [-1/2, 1/2)) before one conversion to FP32.
-// ---------------------------------------------------------------------------
-__device__ inline float r2_fixed(uint32_t ax, uint32_t ay, uint32_t az, uint32_t bx, uint32_t by, uint32_t bz,
-                                 const float* h, int cubic)
+// arithmetic (fraction in [-1/2, 1/2)).
+//
+// Mixed form of one pair.  On gfx950 FP64 FMA issues at the same lane rate as
+// unpacked FP32, so the arithmetic stays FP64 and only the transcendental steps
+// use the FP32 hardware: r^2 from the exact fixed-point difference in FP64;
+// 1/r^2 and 1/r seeded by v_rcp_f32 / v_rsq_f32 and refined by one FP64
+// Newton step (relative error ~1e-14); erfc evaluated in FP32.  The FP32 LJ
+// 12th power would amplify a 1e-7 error in r^2 twelve-fold, which this avoids.
+__device__ inline double r2_fixed64(uint32_t ax, uint32_t ay, uint32_t az, uint32_t bx, uint32_t by, uint32_t bz,
+                                    const double* h, int cubic)
 {
-  const float fx = (float)(int32_t)(ax - bx);
-  const float fy = (float)(int32_t)(ay - by);
-  const float fz = (float)(int32_t)(az - bz);
+  const double fx = (double)(int32_t)(ax - bx);
+  const double fy = (double)(int32_t)(ay - by);
+  const double fz = (double)(int32_t)(az - bz);
   if (cubic) {
-    const float dx = fx * h[0], dy = fy * h[4], dz = fz * h[8];
+    const double dx = fx * h[0], dy = fy * h[4], dz = fz * h[8];
     return dx * dx + dy * dy + dz * dz;
   }
-  const float dx = h[0] * fx + h[3] * fy + h[6] * fz;
-  const float dy = h[1] * fx + h[4] * fy + h[7] * fz;
-  const float dz = h[2] * fx + h[5] * fy + h[8] * fz;
+  const double dx = h[0] * fx + h[3] * fy + h[6] * fz;
+  const double dy = h[1] * fx + h[4] * fy + h[7] * fz;
+  const double dz = h[2] * fx + h[5] * fy + h[8] * fz;
   return dx * dx + dy * dy + dz * dz;
+}
+
+__device__ inline double lj_mixed(double rr, double a4e, double s2, double shift)
+{
+  double y = (double)__builtin_amdgcn_rcpf((float)rr);   // v_rcp_f32 seed
+  y = y * (2.0 - rr * y);
+  const double sr = s2 * y;
+  const double s3 = sr * sr * sr;
+  return a4e * (s3 * (s3 - 1.0)) - shift;
+}
+
+// qq = prefactor * q_a * q_b
+__device__ inline double coulomb_mixed(double rr, double qq, double alpha)
+{
+  double y = (double)__builtin_amdgcn_rsqf((float)rr);   // v_rsq_f32 seed
+  y = y * (1.5 - 0.5 * rr * y * y);
+  const float x = (float)(alpha * (rr * y));
+  return qq * (double)erfcf(x) * y;
 }
 
 // ---------------------------------------------------------------------------

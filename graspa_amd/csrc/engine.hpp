@@ -38,6 +38,7 @@ struct BoxConst {
   int nvec;
   // mixed-precision helpers: fixed-point fraction -> Cartesian, per column
   float h32[9];                      // cell * 2^-32
+  double h64[9];                     // cell * 2^-32 (FP64)
   float cut_vdw2f, cut_coul2f, overlapf, alphaf, preff;
 };
 

@@ -246,6 +246,7 @@ int graspa_box_create(const graspa_box_params* p, graspa_box** out)
   for (int d = 0; d < 3; ++d) bc.kmax[d] = p->kmax[d];
   bc.nvec = (p->kmax[0] + 1) * (2 * p->kmax[1] + 1) * (2 * p->kmax[2] + 1);
   for (int i = 0; i < 9; ++i) bc.h32[i] = (float)(p->cell[i] * (1.0 / 4294967296.0));
+  for (int i = 0; i < 9; ++i) bc.h64[i] = p->cell[i] * (1.0 / 4294967296.0);
   bc.cut_vdw2f = (float)bc.cut_vdw2;
   bc.cut_coul2f = (float)bc.cut_coul2;
   bc.overlapf = (float)std::min(bc.overlap, 3.0e38);

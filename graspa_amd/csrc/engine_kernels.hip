@@ -109,22 +109,15 @@ __global__ __launch_bounds__(kBlock) void site_energy_kernel(
         }
       } else {
         const uint4 a = (part == 0) ? fw_fix[j] : g_fix[j];
-        const float rr = r2_fixed(a.x, a.y, a.z, sf[0], sf[1], sf[2], bc.h32, bc.cubic);
-        const float qaf = __uint_as_float(a.w);
-        if (rr < bc.cut_vdw2f) {
-          const float* f = ff32 + 4 * (ta * T + st);
-          const float sr = f[1] / rr;
-          const float s3 = sr * sr * sr;
-          const float e = f[0] * (s3 * (s3 - 1.0f)) - f[2];
-          if (e > bc.overlapf) ov = 1;
-          if (rr < 0.01f) ov = 1;
-          ev = (double)e;
+        const double rr = r2_fixed64(a.x, a.y, a.z, sf[0], sf[1], sf[2], bc.h64, bc.cubic);
+        qa = (part == 0) ? fw[3 * n_fw + j] : g[3 * guest_cap + j];
+        if (rr < bc.cut_vdw2) {
+          const double* f = ff64 + 4 * (ta * T + st);
+          ev = lj_mixed(rr, f[0], f[1], f[2]);
+          if (ev > bc.overlap) ov = 1;
+          if (rr < 0.01) ov = 1;
         }
-        if (bc.use_charges && rr < bc.cut_coul2f) {
-          const float rinv = __frsqrt_rn(rr);
-          const float r = rr * rinv;
-          er = (double)(bc.preff * qaf * (float)sq * erfcf(bc.alphaf * r) * rinv);
-        }
+        if (bc.use_charges && rr < bc.cut_coul2) er = coulomb_mixed(rr, bc.prefactor * qa * sq, bc.alpha);
       }
       if (part == 0) { acc[0] += ev; acc[1] += er; } else { acc[2] += ev; acc[3] += er; }
     }

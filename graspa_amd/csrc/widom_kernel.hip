@@ -137,13 +137,13 @@ __device__ inline void pair_pass(const WidomArgs& A, WidomShared& sh, const int 
     const int st = A.site_type[(chain == 1) ? 0 : 1 + a];
     const double sq = A.site_q[(chain == 1) ? 0 : 1 + a];
     // force-field rows and charge products of this lane's atoms for site type st
-    float4 f32[APT];
-    float qq32[APT];
+    double fa[APT], fs[APT], fh[APT], qq[APT];
     if (MIXED) {
 #pragma unroll
       for (int j = 0; j < APT; ++j) {
-        f32[j] = ffl32[ft[j] * T + st];
-        qq32[j] = bc.preff * uq[j] * (float)sq;
+        const double* f = ffl64 + 4 * (ft[j] * T + st);
+        fa[j] = f[0]; fs[j] = f[1]; fh[j] = f[2];
+        qq[j] = bc.prefactor * fq[j] * sq;
       }
     }
     for (int t = 0; t < n_trials; ++t) {
@@ -152,25 +152,17 @@ __device__ inline void pair_pass(const WidomArgs& A, WidomShared& sh, const int 
       bool ov = false;
       if (MIXED) {
         const uint32_t sx = sh.fix[s][0], sy = sh.fix[s][1], sz = sh.fix[s][2];
-        float v32 = 0.f, r32 = 0.f;
 #pragma unroll
         for (int j = 0; j < APT; ++j) {
           if (j >= n_mine) continue;
-          const float rr = r2_fixed(ux[j], uy[j], uz[j], sx, sy, sz, bc.h32, bc.cubic);
-          if (rr < bc.cut_vdw2f) {
-            const float sr2 = f32[j].y / rr;
-            const float s3 = sr2 * sr2 * sr2;
-            const float ev = f32[j].x * (s3 * (s3 - 1.0f)) - f32[j].z;
-            ov = ov || (ev > bc.overlapf) || (rr < 0.01f);
-            v32 += ev;
+          const double rr = r2_fixed64(ux[j], uy[j], uz[j], sx, sy, sz, bc.h64, bc.cubic);
+          if (rr < bc.cut_vdw2) {
+            const double ev = lj_mixed(rr, fa[j], fs[j], fh[j]);
+            ov = ov || (ev > bc.overlap) || (rr < 0.01);
+            sv += ev;
           }
-          if (rr < bc.cut_coul2f) {
-            const float rinv = __frsqrt_rn(rr);
-            r32 += qq32[j] * erfcf(bc.alphaf * (rr * rinv)) * rinv;
-          }
+          if (bc.use_charges && rr < bc.cut_coul2) sr += coulomb_mixed(rr, qq[j], bc.alpha);
         }
-        sv = (double)v32;
-        sr = (double)r32;
       } else {
         const double px = sh.pos[s][0], py = sh.pos[s][1], pz = sh.pos[s][2];
 #pragma unroll
@@ -200,19 +192,15 @@ __device__ inline void pair_pass(const WidomArgs& A, WidomShared& sh, const int 
           const int ta = A.g_type[j];
           if (MIXED) {
             const uint4 g4 = A.g_fix[j];
-            const float4 f = ffl32[ta * T + st];
-            const float rr = r2_fixed(g4.x, g4.y, g4.z, sh.fix[s][0], sh.fix[s][1], sh.fix[s][2], bc.h32, bc.cubic);
-            if (rr < bc.cut_vdw2f) {
-              const float sr2 = f.y / rr;
-              const float s3 = sr2 * sr2 * sr2;
-              const float ev = f.x * (s3 * (s3 - 1.0f)) - f.z;
-              ov = ov || (ev > bc.overlapf) || (rr < 0.01f);
-              sv += (double)ev;
+            const double* f = ffl64 + 4 * (ta * T + st);
+            const double rr = r2_fixed64(g4.x, g4.y, g4.z, sh.fix[s][0], sh.fix[s][1], sh.fix[s][2], bc.h64, bc.cubic);
+            if (rr < bc.cut_vdw2) {
+              const double ev = lj_mixed(rr, f[0], f[1], f[2]);
+              ov = ov || (ev > bc.overlap) || (rr < 0.01);
+              sv += ev;
             }
-            if (rr < bc.cut_coul2f) {
-              const float rinv = __frsqrt_rn(rr);
-              sr += (double)(bc.preff * __uint_as_float(g4.w) * (float)sq * erfcf(bc.alphaf * (rr * rinv)) * rinv);
-            }
+            if (bc.use_charges && rr < bc.cut_coul2)
+              sr += coulomb_mixed(rr, bc.prefactor * A.g[3 * A.guest_cap + j] * sq, bc.alpha);
           } else {
             const double* f = ffl64 + 4 * (ta * T + st);
             double dx = A.g[j] - sh.pos[s][0], dy = A.g[A.guest_cap + j] - sh.pos[s][1];
@@ -345,6 +333,7 @@ __global__ __launch_bounds__(kBlock, 1) void widom_kernel(const WidomArgs A)
       if (MIXED) {
         const uint4 v = A.fw_fix[a];
         ux[j] = v.x; uy[j] = v.y; uz[j] = v.z; uq[j] = __uint_as_float(v.w);
+        fq[j] = A.fw[3 * A.n_fw + a];
       } else {
         fx[j] = A.fw[a]; fy[j] = A.fw[A.n_fw + a]; fz[j] = A.fw[2 * A.n_fw + a]; fq[j] = A.fw[3 * A.n_fw + a];
       }
