@@ -143,11 +143,21 @@ def main():
     ap.add_argument("--gcmc-cycles", type=int, default=2000)
     ap.add_argument("--cpu-sample", type=int, default=15000)
     ap.add_argument("--quick", action="store_true", help="skip the secondary measurements")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run 3 plain device-to-device copies of BYTES (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: boxes and Widom batches shard across GPUs with no communication, "
               "and this build runs on a single MI355X (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration > 0:
+        import torch
+        a = torch.empty(args.copy_calibration, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     from graspa_amd import engine, synth
 
     s1 = synth.config1()
