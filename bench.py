@@ -3,8 +3,9 @@
 Headline (the timed step): one batched Widom call of the stated Widom workload
 -- 100,000 CBMC test insertions of rigid CO2 (10 trial positions x 10
 orientations each) into a synthetic 1,536-atom framework in a 24 A cubic box,
-empty box, Ewald alpha 0.265 / kmax 7, mixed precision (FP32 pair terms, FP64
-accumulation, FP64 Ewald).  value = insertions/s over K timed steps with all
+empty box, Ewald alpha 0.265 / kmax 7, in the engine's fast form (FP64 pair
+arithmetic from fixed-point positions with FP32-seeded reciprocals, an FP32
+overlap screen, FP64 Ewald).  value = insertions/s over K timed steps with all
 inputs resident in HBM.
 
 Also reported on the same line: the plain FP64 form of the same batch, a
@@ -27,18 +28,45 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector / matrix peak
 FP64_PEAK_TFLOPS = 78.6
 
 
-def widom_flops(n_fw, n_guest_atoms, K, O, chain, M, n_active_k, n_total, n_fb, n_ok, charged=True):
-    """Algorithmic FLOP of a Widom batch, per the unit costs: 52 per charged
-    pair (33 uncharged), 16 per (k-vector, site) and 12 per k-vector of the
-    Fourier term.  First-bead pairs for every insertion, orientation pairs for
-    insertions whose first bead was placed, Fourier for completed ones."""
-    pair = 52.0 if charged else 33.0
-    n_sys = n_fw + n_guest_atoms
-    f = n_total * K * n_sys * pair
-    f += n_fb * O * chain * n_sys * pair
+SCREEN_FLOP = 21.0       # difference 3 + image 12 + r^2 5 + test 1
+PAIR_FLOP = 52.0         # + LJ 10 + Coulomb 19 (erfc as 12) + accumulate 2
+PAIR_FLOP_UNCHARGED = 33.0
+
+
+def widom_flops(screened, evaluated, M, n_active_k, n_ok, charged=True):
+    """Algorithmic FLOP of a Widom batch as executed, per unit costs: 21 per
+    (trial site, atom) pair tested by the overlap screen (distance and test),
+    52 per pair with full energies (33 uncharged), 16 per (k-vector, site) and
+    12 per k-vector of the Fourier term of each completed insertion.  The pair
+    counts are the kernel's own work counters."""
+    f = screened * SCREEN_FLOP + evaluated * (PAIR_FLOP if charged else PAIR_FLOP_UNCHARGED)
     if charged:
         f += n_ok * n_active_k * (16.0 * M + 12.0)
     return f
+
+
+def reference_flops(n_sys, K, O, chain, M, n_active_k, n_total, n_fb, n_ok, charged=True):
+    """FLOP the reference's algorithm spends on the same batch: full energies of
+    all K first-bead trials, of the O orientations when the first bead is
+    placed, and the Fourier term of completed insertions."""
+    pair = PAIR_FLOP if charged else PAIR_FLOP_UNCHARGED
+    f = n_total * K * n_sys * pair + n_fb * O * chain * n_sys * pair
+    if charged:
+        f += n_ok * n_active_k * (16.0 * M + 12.0)
+    return f
+
+
+def load_traffic(kernel_prefix):
+    """HBM bytes per launch of the Widom kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy)."""
+    p = os.path.join(ROOT, "profiles", "widom_kernel_mixed_config1.json")
+    try:
+        d = json.load(open(p))
+    except (OSError, ValueError):
+        return None, None
+    if not d.get("kernel", "").startswith(kernel_prefix):
+        return None, None
+    return d.get("traffic_bytes"), d
 
 
 def run_widom(system, precision, n_insert, steps, warmup, seed=2024, timing=True):
@@ -55,7 +83,7 @@ def run_widom(system, precision, n_insert, steps, warmup, seed=2024, timing=True
     _sync_torch()
     t0 = time.perf_counter()
     summ = None
-    tot = dict(sum_w=0.0, n=0.0, n_first_bead=0.0, n_ok=0.0, sum_we=0.0)
+    tot = dict(sum_w=0.0, n=0.0, n_first_bead=0.0, n_ok=0.0, sum_we=0.0, screened_pairs=0.0, evaluated_pairs=0.0)
     for k in range(steps):
         _, summ = box.widom_batch(1, n_insert, seed=seed, first_index=(k + 1) * n_insert, excl=excl, results=False)
         for key in tot:
@@ -166,9 +194,11 @@ def main():
     r = run_widom(s1, engine.GRASPA_MIXED, args.insertions, args.steps, args.warmup)
     n_total = args.insertions * args.steps
     value = n_total / r["seconds"]
-    flops = widom_flops(n_fw, 0, K, O, M - 1, M, r["nact"], r["n"], r["n_first_bead"], r["n_ok"])
+    flops = widom_flops(r["screened_pairs"], r["evaluated_pairs"], M, r["nact"], r["n_ok"])
+    ref_flops = reference_flops(n_fw, K, O, M - 1, M, r["nact"], r["n"], r["n_first_bead"], r["n_ok"])
     avg_ms = r["kern_ms"] / max(1, r["launches"])
     achieved_tflops = flops / max(1, r["launches"]) / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else None
+    traffic, prof = load_traffic("void graspa::widom_kernel<true")
     line = {
         "metric": "Widom insertions/sec on 1 GPU (CBMC 10x10, CO2, 1536-atom framework); GCMC MC moves/sec reported alongside",
         "value": value,
@@ -180,7 +210,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair terms / fp64 accumulation and Ewald",
+        "dtype": "f64 pair terms from fixed-point positions (f32 reciprocal seeds, f32 overlap screen), f64 Ewald",
         "data": "synthetic (seeded jittered-lattice framework, Widom test insertions)",
         "config": {"workload": "Widom batch: 100,000 CO2 test insertions x (10 trial positions, 10 orientations), "
                                "1,536-atom framework, 24 A cubic, alpha 0.265, kmax 7, empty box",
@@ -190,9 +220,16 @@ def main():
                    "mean_rosenbluth_weight": r["sum_w"] / max(1, r["n"])},
         "roofline": {"bound": "mfma", "roof": "fp32 vector (VALU) peak; the pair kernel uses no MFMA",
                      "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": (achieved_tflops / FP32_PEAK_TFLOPS) if achieved_tflops else None, "traffic": None,
+                     "frac": (achieved_tflops / FP32_PEAK_TFLOPS) if achieved_tflops else None,
+                     "traffic": traffic,
+                     "traffic_source": prof.get("source") if prof else None,
                      "kernel": "widom_kernel<mixed>", "avg_launch_ms": avg_ms,
-                     "flop_per_launch": flops / max(1, r["launches"])},
+                     "flop_per_launch": flops / max(1, r["launches"]),
+                     "screened_pairs_per_launch": r["screened_pairs"] / max(1, r["launches"]),
+                     "evaluated_pairs_per_launch": r["evaluated_pairs"] / max(1, r["launches"]),
+                     "reference_algorithm_flop_per_launch": ref_flops / max(1, r["launches"]),
+                     "reference_algorithm_equivalent_tflops":
+                         ref_flops / max(1, r["launches"]) / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else None},
     }
     rho = s1.framework_density()
     from graspa_amd.deck import GAS_CONSTANT
@@ -203,7 +240,7 @@ def main():
         line["fp64_widom_insertions_per_s"] = args.insertions * max(1, args.steps // 5) / r64["seconds"]
         s_lo = synth.config1(n_fw=750)
         rl = run_widom(s_lo, engine.GRASPA_MIXED, args.insertions, max(1, args.steps // 2), 1)
-        fl = widom_flops(750, 0, K, O, M - 1, M, rl["nact"], rl["n"], rl["n_first_bead"], rl["n_ok"])
+        fl = widom_flops(rl["screened_pairs"], rl["evaluated_pairs"], M, rl["nact"], rl["n_ok"])
         al = rl["kern_ms"] / max(1, rl["launches"])
         line["low_density_variant"] = {
             "framework_atoms": 750, "insertions_per_s": args.insertions * max(1, args.steps // 2) / rl["seconds"],

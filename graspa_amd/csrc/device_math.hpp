@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "fast_math_coeffs.hpp"
+
 namespace graspa {
 
 // ---------------------------------------------------------------------------
@@ -62,6 +64,29 @@ __device__ inline void pbc_fp64(double& x, double& y, double& z, const double* c
   }
 }
 
+// Minimum image for the fast form: nearest image by rint (ties to even; the
+// reference rounds ties away from zero, which differs only at exactly half a
+// cell, where the two images are equidistant).
+template <bool CUBIC>
+__device__ inline void pbc_rint(double& x, double& y, double& z, const double* cell, const double* inv)
+{
+  if (CUBIC) {
+    x = fma(-rint(x * inv[0]), cell[0], x);
+    y = fma(-rint(y * inv[4]), cell[4], y);
+    z = fma(-rint(z * inv[8]), cell[8], z);
+  } else {
+    double s0 = inv[0] * x + inv[3] * y + inv[6] * z;
+    double s1 = inv[1] * x + inv[4] * y + inv[7] * z;
+    double s2 = inv[2] * x + inv[5] * y + inv[8] * z;
+    s0 -= rint(s0);
+    s1 -= rint(s1);
+    s2 -= rint(s2);
+    x = cell[0] * s0 + cell[3] * s1 + cell[6] * s2;
+    y = cell[1] * s0 + cell[4] * s1 + cell[7] * s2;
+    z = cell[2] * s0 + cell[5] * s1 + cell[8] * s2;
+  }
+}
+
 // 12-6 LJ at full coupling (gRASPA maths.cuh:452-494 with scaling = 1):
 // ff = {4 eps, sigma^2, shift}.
 __device__ inline double lj_fp64(double rr, double a4e, double s2, double shift)
@@ -108,13 +133,65 @@ __device__ inline double lj_mixed(double rr, double a4e, double s2, double shift
   return a4e * (s3 * (s3 - 1.0)) - shift;
 }
 
-// qq = prefactor * q_a * q_b
+// exp(y) for -708 < y <= 0: Cody-Waite reduction y = n ln2 + r with a
+// two-part ln 2, degree-11 Taylor polynomial on |r| <= ln2/2 (truncation
+// 6e-15 relative), scaled by 2^n.
+__device__ inline double exp_fast(double y)
+{
+  const double n = rint(y * fastmath::kLog2e);
+  double r = fma(-n, fastmath::kLn2Hi, y);
+  r = fma(-n, fastmath::kLn2Lo, r);
+  double p = fastmath::kExpPoly[fastmath::kExpDeg];
+#pragma unroll
+  for (int k = fastmath::kExpDeg - 1; k >= 0; --k) p = fma(p, r, fastmath::kExpPoly[k]);
+  return ldexp(p, (int)n);
+}
+
+// erfc(x) for x >= 0 to ~1e-13 relative: exp(-x^2) * t * F(t) with
+// t = c / (c + x) (reciprocal seeded by v_rcp_f32, one FP64 Newton step) and
+// F a fitted polynomial (fast_math_coeffs.hpp).  0 beyond x = 6, where
+// erfc < 2.2e-17.
+__device__ inline double erfc_fast(double x)
+{
+  const double d = fastmath::kErfcC + x;
+  double y = (double)__builtin_amdgcn_rcpf((float)d);
+  y = fma(y, fma(-d, y, 1.0), y);
+  const double t = fastmath::kErfcC * y;
+  const double u = fma(t, fastmath::kErfcUScale, fastmath::kErfcUOff);
+  double F = fastmath::kErfcPoly[fastmath::kErfcDeg];
+#pragma unroll
+  for (int k = fastmath::kErfcDeg - 1; k >= 0; --k) F = fma(F, u, fastmath::kErfcPoly[k]);
+  const double r = exp_fast(-x * x) * t * F;
+  return (x < fastmath::kErfcXMax) ? r : 0.0;
+}
+
+// qq = prefactor * q_a * q_b; 1/r seeded by v_rsq_f32 and refined by one FP64
+// Newton step (relative error ~2e-14).
 __device__ inline double coulomb_mixed(double rr, double qq, double alpha)
 {
   double y = (double)__builtin_amdgcn_rsqf((float)rr);   // v_rsq_f32 seed
   y = y * (1.5 - 0.5 * rr * y * y);
-  const float x = (float)(alpha * (rr * y));
-  return qq * (double)erfcf(x) * y;
+  return qq * erfc_fast(alpha * (rr * y)) * y;
+}
+
+// Squared distance in FP32 from fixed-point fractions, for the overlap
+// screen (relative error below 1e-6; the screen thresholds carry a 1e-5
+// margin).  h = cell * 2^-32.
+template <bool CUBIC>
+__device__ inline float r2_fixed32(uint32_t ax, uint32_t ay, uint32_t az, uint32_t bx, uint32_t by, uint32_t bz,
+                                   const float* h)
+{
+  const float fx = (float)(int32_t)(ax - bx);
+  const float fy = (float)(int32_t)(ay - by);
+  const float fz = (float)(int32_t)(az - bz);
+  if (CUBIC) {
+    const float dx = fx * h[0], dy = fy * h[4], dz = fz * h[8];
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  }
+  const float dx = fmaf(h[6], fz, fmaf(h[3], fy, h[0] * fx));
+  const float dy = fmaf(h[7], fz, fmaf(h[4], fy, h[1] * fx));
+  const float dz = fmaf(h[8], fz, fmaf(h[5], fy, h[2] * fx));
+  return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
 // ---------------------------------------------------------------------------
@@ -131,6 +208,29 @@ __device__ inline uint32_t wave_or(uint32_t v)
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v |= __shfl_xor(v, m, 64);
   return v;
+}
+__device__ inline double wave_max(double v)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
+// Value of v in lane l (l uniform across the wave).
+__device__ inline double readlane_d(double v, int l)
+{
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// Orders this wave's LDS writes before its later LDS reads by other lanes
+// (a wave executes its LDS operations in order; this keeps the compiler from
+// moving them across).
+__device__ inline void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 }  // namespace graspa

@@ -90,6 +90,7 @@ struct graspa_box {
   double* d_fw = nullptr;       // SoA [5][n_fw]: x, y, z, q, (type as double unused)
   int* d_fw_type = nullptr;
   uint4* d_fw_fix = nullptr;    // fixed-point fraction x,y,z + float q bits
+  uint4* d_fw_fixT = nullptr;   // fixed-point fraction x,y,z + type
   std::vector<double> fw_pos, fw_q;
   std::vector<int> fw_type;
 
@@ -108,7 +109,8 @@ struct graspa_box {
   double2* d_sk_fw = nullptr;
   double2* d_sk_ads = nullptr;
   double2* d_sk_stage = nullptr;
-  int* d_kact = nullptr;        // active kxyz list
+  int4* d_kact = nullptr;       // active k-vectors: kx, ky, kz, index
+  double* d_ktemp_act = nullptr;  // ktemp of the active k-vectors
   std::vector<double> ktemp;
   bool sk_staged = false;
 
@@ -119,10 +121,16 @@ struct graspa_box {
   unsigned* d_ticket = nullptr;
 
   // Widom scratch
-  double* d_wsum = nullptr;     // per-workgroup partial sums
+  double* d_wacc = nullptr;     // [0,6) batch sums, [8,10) work counters, [12] insertion counter
+  double* d_wrec_w = nullptr;   // per-insertion W, dE, flags
+  double* d_wrec_e = nullptr;
+  int* d_wrec_f = nullptr;
+  double* d_wpart = nullptr;    // reduction partials
+  size_t wrec_cap = 0;
   graspa_widom_result* d_wres = nullptr;
   size_t wres_cap = 0;
   int n_cu = 256;
+  int max_lds = 65536;          // dynamic LDS per workgroup
 
   graspa::TimingState timing;
 };
@@ -143,6 +151,9 @@ int launch_site_energies(graspa_box* b, const SiteBatch& sb, int self_comp, int 
 int launch_ewald_delta(graspa_box* b, const SiteBatch& sb, int n_old, double* d_out /* blocks*2 */, int* n_blocks);
 int launch_ewald_build(graspa_box* b, double* d_out_energy /* blocks*3 */, int* n_blocks);
 int launch_total_pairs(graspa_box* b, double* d_out /* blocks*4 */, int* n_blocks);
+
+// per-insertion record buffers of the Widom kernel (engine_api.hip)
+int ensure_widom_records(graspa_box* b, int n_insert);
 
 // widom_kernel.hip
 int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,

@@ -181,6 +181,24 @@ static int ensure_part(graspa_box* b, size_t n_doubles)
   return GRASPA_OK;
 }
 
+int graspa::ensure_widom_records(graspa_box* b, int n_insert)
+{
+  if ((size_t)n_insert <= b->wrec_cap) return GRASPA_OK;
+  hipStreamSynchronize(b->stream);
+  hipFree(b->d_wrec_w); hipFree(b->d_wrec_e); hipFree(b->d_wrec_f); hipFree(b->d_wpart);
+  b->d_wrec_w = b->d_wrec_e = b->d_wpart = nullptr;
+  b->d_wrec_f = nullptr;
+  b->wrec_cap = 0;
+  const size_t n = (size_t)n_insert;
+  const size_t nblk = (n + 2047) / 2048;
+  int rc = hip_check(hipMalloc(&b->d_wrec_w, n * sizeof(double)), "hipMalloc widom records");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_wrec_e, n * sizeof(double)), "hipMalloc widom records");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_wrec_f, n * sizeof(int)), "hipMalloc widom records");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_wpart, 6 * nblk * sizeof(double)), "hipMalloc widom records");
+  if (!rc) b->wrec_cap = n;
+  return rc;
+}
+
 #define GRASPA_CHECK_BOX(b) \
   if (!(b)) return fail(GRASPA_ERR_ARG, "null box");
 
@@ -212,7 +230,16 @@ int graspa_box_create(const graspa_box_params* p, graspa_box** out)
     return GRASPA_ERR_HIP;
   }
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, p->device) == hipSuccess) b->n_cu = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, p->device) == hipSuccess) {
+    b->n_cu = prop.multiProcessorCount;
+    b->max_lds = (int)prop.sharedMemPerBlock;
+  }
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, p->device) == hipSuccess &&
+        v > b->max_lds)
+      b->max_lds = v;
+  }
   BoxConst& bc = b->bc;
   std::memcpy(bc.cell, p->cell, sizeof(bc.cell));
   // maths.cuh:28-55
@@ -279,7 +306,7 @@ int graspa_box_create(const graspa_box_params* p, graspa_box** out)
     const double PI = 3.14159265358979323846;
     const double pref = p->prefactor * (2.0 * PI / det);
     const double alpha_sq = p->alpha * p->alpha;
-    const int kxm = p->kmax[0], kym = p->kmax[1], kzm = p->kmax[2];
+    const int kym = p->kmax[1], kzm = p->kmax[2];
     int nact = 0;
     for (int k = 0; k < b->nvec; ++k) {
       const int kz = k % (2 * kzm + 1) - kzm;
@@ -317,7 +344,32 @@ int graspa_box_create(const graspa_box_params* p, graspa_box** out)
     hipMemset(b->d_sk_ads, 0, nv * sizeof(double2));
     hipMemset(b->d_sk_stage, 0, nv * sizeof(double2));
   }
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_wsum, 8 * (size_t)(8 * b->n_cu + 2) * sizeof(double)), "hipMalloc widom");
+  {
+    // compact list of the active k-vectors for the Widom kernel's Fourier term
+    std::vector<int4> kact;
+    std::vector<double> tact;
+    const int kzm = p->kmax[2], kym = p->kmax[1];
+    for (int k = 0; k < b->nvec; ++k) {
+      if (b->ktemp[k] < 0.0) continue;
+      const int kz = k % (2 * kzm + 1) - kzm;
+      const int kxy = k / (2 * kzm + 1);
+      const int kx = kxy / (2 * kym + 1);
+      const int ky = kxy % (2 * kym + 1) - kym;
+      kact.push_back(make_int4(kx, ky, kz, k));
+      tact.push_back(b->ktemp[k]);
+    }
+    const size_t na = std::max<size_t>(kact.size(), 1);
+    rc = rc ? rc : hip_check(hipMalloc(&b->d_kact, na * sizeof(int4)), "hipMalloc ewald");
+    rc = rc ? rc : hip_check(hipMalloc(&b->d_ktemp_act, na * sizeof(double)), "hipMalloc ewald");
+    if (!rc && !kact.empty()) {
+      rc = hip_check(hipMemcpy(b->d_kact, kact.data(), kact.size() * sizeof(int4), hipMemcpyHostToDevice), "k upload");
+      rc = rc ? rc
+              : hip_check(hipMemcpy(b->d_ktemp_act, tact.data(), tact.size() * sizeof(double), hipMemcpyHostToDevice),
+                          "k upload");
+    }
+  }
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_wacc, 16 * sizeof(double)), "hipMalloc widom");
+  if (!rc) rc = hip_check(hipMemset(b->d_wacc, 0, 16 * sizeof(double)), "widom init");
   rc = rc ? rc : ensure_part(b, 4096);
   b->comps.resize(1);
   rc = rc ? rc : realloc_guests(b);
@@ -326,6 +378,7 @@ int graspa_box_create(const graspa_box_params* p, graspa_box** out)
     rc = hip_check(hipMalloc(&b->d_fw, 4 * sizeof(double)), "hipMalloc fw");
     rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_type, sizeof(int)), "hipMalloc fw");
     rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_fix, sizeof(uint4)), "hipMalloc fw");
+    rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_fixT, sizeof(uint4)), "hipMalloc fw");
   }
   if (rc) {
     graspa_box_destroy(b);
@@ -342,12 +395,14 @@ void graspa_box_destroy(graspa_box* b)
   if (b->stream) hipStreamSynchronize(b->stream);
   timing_clear(b);
   hipFree(b->d_ff64); hipFree(b->d_ff32);
-  hipFree(b->d_fw); hipFree(b->d_fw_type); hipFree(b->d_fw_fix);
+  hipFree(b->d_fw); hipFree(b->d_fw_type); hipFree(b->d_fw_fix); hipFree(b->d_fw_fixT);
   hipFree(b->d_g); hipFree(b->d_g_type); hipFree(b->d_g_molid); hipFree(b->d_g_comp); hipFree(b->d_g_fix);
   hipFree(b->d_ktemp); hipFree(b->d_sk_fw); hipFree(b->d_sk_ads); hipFree(b->d_sk_stage);
+  hipFree(b->d_kact); hipFree(b->d_ktemp_act);
   hipFree(b->d_part);
   if (b->h_part) hipHostFree(b->h_part);
-  hipFree(b->d_wsum); hipFree(b->d_wres);
+  hipFree(b->d_wacc); hipFree(b->d_wres);
+  hipFree(b->d_wrec_w); hipFree(b->d_wrec_e); hipFree(b->d_wrec_f); hipFree(b->d_wpart);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
 }
@@ -359,7 +414,9 @@ int graspa_box_set_framework(graspa_box* b, int n, const double* xyz, const doub
   for (int i = 0; i < n; ++i)
     if (type[i] < 0 || type[i] >= b->bc.ntypes) return fail(GRASPA_ERR_ARG, "framework type out of range");
   hipSetDevice(b->device);
-  hipFree(b->d_fw); hipFree(b->d_fw_type); hipFree(b->d_fw_fix);
+  hipStreamSynchronize(b->stream);
+  hipFree(b->d_fw); hipFree(b->d_fw_type); hipFree(b->d_fw_fix); hipFree(b->d_fw_fixT);
+  b->d_fw = nullptr; b->d_fw_type = nullptr; b->d_fw_fix = nullptr; b->d_fw_fixT = nullptr;
   b->n_fw = n;
   b->fw_pos.assign(xyz, xyz + 3 * (size_t)n);
   b->fw_q.assign(q, q + n);
@@ -367,6 +424,7 @@ int graspa_box_set_framework(graspa_box* b, int n, const double* xyz, const doub
   const size_t m = (size_t)std::max(n, 1);
   std::vector<double> soa(4 * m, 0.0);
   std::vector<uint4> fix(m, make_uint4(0, 0, 0, 0));
+  std::vector<uint4> fixT(m, make_uint4(0, 0, 0, 0));
   for (int i = 0; i < n; ++i) {
     soa[i] = xyz[3 * i];
     soa[n + i] = xyz[3 * i + 1];
@@ -378,10 +436,13 @@ int graspa_box_set_framework(graspa_box* b, int n, const double* xyz, const doub
     uint32_t qb;
     std::memcpy(&qb, &qf, 4);
     fix[i] = make_uint4(f[0], f[1], f[2], qb);
+    fixT[i] = make_uint4(f[0], f[1], f[2], (uint32_t)type[i]);
   }
   int rc = hip_check(hipMalloc(&b->d_fw, 4 * m * sizeof(double)), "hipMalloc fw");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_type, m * sizeof(int)), "hipMalloc fw");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_fix, m * sizeof(uint4)), "hipMalloc fw");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_fixT, m * sizeof(uint4)), "hipMalloc fw");
+  rc = rc ? rc : hip_check(hipMemcpy(b->d_fw_fixT, fixT.data(), m * sizeof(uint4), hipMemcpyHostToDevice), "fw upload");
   rc = rc ? rc : hip_check(hipMemcpy(b->d_fw, soa.data(), 4 * m * sizeof(double), hipMemcpyHostToDevice), "fw upload");
   if (!rc && n > 0) rc = hip_check(hipMemcpy(b->d_fw_type, type, n * sizeof(int), hipMemcpyHostToDevice), "fw upload");
   rc = rc ? rc : hip_check(hipMemcpy(b->d_fw_fix, fix.data(), m * sizeof(uint4), hipMemcpyHostToDevice), "fw upload");

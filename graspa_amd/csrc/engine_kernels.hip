@@ -55,21 +55,6 @@ __global__ __launch_bounds__(kBlock) void site_energy_kernel(
   int ov = 0;
   if (threadIdx.x == 0) ovs = 0;
 
-  uint32_t sf[3] = {0u, 0u, 0u};
-  if (MIXED) {
-    // fixed-point fraction of the site (same conversion as the atom store)
-    const double p[3] = {sx, sy, sz};
-    double f[3];
-    f[0] = bc.inv[0] * p[0] + bc.inv[3] * p[1] + bc.inv[6] * p[2];
-    f[1] = bc.inv[1] * p[0] + bc.inv[4] * p[1] + bc.inv[7] * p[2];
-    f[2] = bc.inv[2] * p[0] + bc.inv[5] * p[1] + bc.inv[8] * p[2];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double fr = f[d] - floor(f[d]);
-      sf[d] = (uint32_t)(int64_t)(fr * 4294967296.0);
-    }
-  }
-
   // framework (host-guest) then each guest component (guest-guest)
   for (int part = 0; part < gv.n_comp; ++part) {
     const int n = (part == 0) ? n_fw : gv.seg[part].n_atoms;
@@ -77,47 +62,39 @@ __global__ __launch_bounds__(kBlock) void site_energy_kernel(
     for (int jj = threadIdx.x; jj < n; jj += blockDim.x) {
       const int j = base + jj;
       int ta;
-      double qa;
+      double dx, dy, dz, qa;
       if (part > 0) {
         const int mid = g_molid[j];
         if (part == excl_comp && mid == excl_molid) continue;
         if (part == self_comp && mid == self_molid) continue;
         ta = g_type[j];
+        dx = g[j] - sx; dy = g[guest_cap + j] - sy; dz = g[2 * guest_cap + j] - sz; qa = g[3 * guest_cap + j];
       } else {
         ta = fw_type[j];
+        dx = fw[j] - sx; dy = fw[n_fw + j] - sy; dz = fw[2 * n_fw + j] - sz; qa = fw[3 * n_fw + j];
       }
-      double ev = 0.0, er = 0.0;
-      if (!MIXED) {
-        double dx, dy, dz;
-        if (part == 0) {
-          dx = fw[j] - sx; dy = fw[n_fw + j] - sy; dz = fw[2 * n_fw + j] - sz; qa = fw[3 * n_fw + j];
-        } else {
-          dx = g[j] - sx; dy = g[guest_cap + j] - sy; dz = g[2 * guest_cap + j] - sz; qa = g[3 * guest_cap + j];
-        }
+      if (!MIXED)
         pbc_fp64(dx, dy, dz, bc.cell, bc.inv, bc.cubic);
-        const double rr = dx * dx + dy * dy + dz * dz;
-        if (rr < bc.cut_vdw2) {
-          const double* f = ff64 + 4 * (ta * T + st);
-          ev = lj_fp64(rr, f[0], f[1], f[2]);
-          if (ev > bc.overlap) ov = 1;
-          if (rr < 0.01) ov = 1;
-        }
-        if (bc.use_charges && rr < bc.cut_coul2) {
+      else if (bc.cubic)
+        pbc_rint<true>(dx, dy, dz, bc.cell, bc.inv);
+      else
+        pbc_rint<false>(dx, dy, dz, bc.cell, bc.inv);
+      const double rr = dx * dx + dy * dy + dz * dz;
+      double ev = 0.0, er = 0.0;
+      if (rr < bc.cut_vdw2) {
+        const double* f = ff64 + 4 * (ta * T + st);
+        ev = MIXED ? lj_mixed(rr, f[0], f[1], f[2]) : lj_fp64(rr, f[0], f[1], f[2]);
+        if (ev > bc.overlap) ov = 1;
+        if (rr < 0.01) ov = 1;
+      }
+      if (bc.use_charges && rr < bc.cut_coul2) {
+        if (MIXED) {
+          er = coulomb_mixed(rr, bc.prefactor * qa * sq, bc.alpha);
+        } else {
           const double r = sqrt(rr);
           const double term = qa * sq * erfc(bc.alpha * r);
           er = bc.prefactor * 1.0 * term / r;
         }
-      } else {
-        const uint4 a = (part == 0) ? fw_fix[j] : g_fix[j];
-        const double rr = r2_fixed64(a.x, a.y, a.z, sf[0], sf[1], sf[2], bc.h64, bc.cubic);
-        qa = (part == 0) ? fw[3 * n_fw + j] : g[3 * guest_cap + j];
-        if (rr < bc.cut_vdw2) {
-          const double* f = ff64 + 4 * (ta * T + st);
-          ev = lj_mixed(rr, f[0], f[1], f[2]);
-          if (ev > bc.overlap) ov = 1;
-          if (rr < 0.01) ov = 1;
-        }
-        if (bc.use_charges && rr < bc.cut_coul2) er = coulomb_mixed(rr, bc.prefactor * qa * sq, bc.alpha);
       }
       if (part == 0) { acc[0] += ev; acc[1] += er; } else { acc[2] += ev; acc[3] += er; }
     }

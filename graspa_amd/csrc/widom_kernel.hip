@@ -1,21 +1,36 @@
 // widom_kernel.hip -- batched Widom / CBMC test insertions in one persistent
 // launch (the throughput path of the GCMC energy engine on gfx950).
 //
-// Each workgroup (4 x wave64) keeps its stripe of the framework in registers
-// for the whole launch (atom j of the stripe lives in lane j mod 256), and walks
-// insertions i = blockIdx.x, blockIdx.x + gridDim.x, ...  One insertion:
-//   1. K first-bead trial positions from the insertion's Philox stream (LDS);
-//   2. every lane evaluates its atoms against all K trials, accumulating per-trial
-//      LJ and real-space sums (FP32 pair terms in the mixed form), then one
-//      wave64 FP64 reduction per trial and a cross-wave sum in LDS;
-//   3. log-sum-exp selection of the first bead (one uniform), W_fb;
-//   4. O rigid-body orientations of the template (quaternions from 3 uniforms),
-//      the same pair pass over O x (M-1) sites, selection, W_ch;
-//   5. Fourier-space energy of the selected molecule in FP64 against the
-//      resident structure factors (one lane per k-vector);
-//   6. W = W_fb W_ch exp(-beta (dE_ewald + dE_tail)), record, block sums.
+// One wavefront carries one insertion at a time; waves take insertions from
+// an atomic counter, so the work balances itself and a result depends only on
+// the insertion's index (counter-based random numbers).  A workgroup stages
+// the framework's fixed-point coordinates and types in LDS once; after that
+// the insertion loop has no workgroup barrier.  One insertion:
+//   1. all its random draws at once, one Philox4x32-10 block per lane;
+//   2. K first-bead trial positions (per-wave LDS);
+//   3. overlap screen: FP32 squared distances from the fixed-point
+//      coordinates against per-(site type, atom type) thresholds r_ov^2 below
+//      which the pair energy certainly exceeds OverlapCriteria (or r^2 < 0.01);
+//      a trial with such a pair is rejected without computing its energies,
+//      the screen of a trial ends at its first hit, and the pass ends when all
+//      trials are rejected;
+//   4. full pair energies (LJ + real-space Coulomb, FP64 accumulation) of the
+//      trials the screen left, which also settles their overlap flags exactly;
+//   5. log-sum-exp selection (one uniform), W_fb;
+//   6. O orientations of the template about the selected first bead, the same
+//      screen + evaluation over O x (M-1) sites, selection, W_ch;
+//   7. Fourier-space energy of the selected molecule in FP64 against the
+//      resident structure factors (active k-vectors only, one lane per k);
+//   8. W = W_fb W_ch exp(-beta (dE_ewald + dE_tail)); a per-insertion record.
+// Per-insertion records are summed in index order by two small kernels, so the
+// batch sums are deterministic.
 // The semantics are Insertion_Body (mc_swap_utilities.h:3-133) with
-// Widom_Move_FirstBead_PARTIAL / Widom_Move_Chain_PARTIAL (mc_widom.h:305-614).
+// Widom_Move_FirstBead_PARTIAL / Widom_Move_Chain_PARTIAL (mc_widom.h:305-614);
+// a trial rejected by the screen is one the reference flags (VDW_Coulomb.cu:
+// 1299-1300) and leaves out of the selection (Host_sum_Widom_HGGG_SEPARATE,
+// mc_widom.h:41-87).
+#include <cstring>
+
 #include "device_math.hpp"
 #include "engine.hpp"
 
@@ -23,23 +38,36 @@ namespace graspa {
 
 constexpr int kMaxTrials = 32;   // K and O
 constexpr int kMaxMol = 8;       // sites per molecule
-constexpr int kMaxKStride = 16;  // kmax <= 15 along each axis
+constexpr int kMaxPts = 64;      // trial sites of one pass: K, or O x (M-1)
+constexpr int kMaxDraws = 3 * kMaxTrials + 1 + 3 * kMaxTrials + 1;
 constexpr int kSums = 6;         // W, W^2, W dE, n, n first bead placed, n complete
+constexpr int kRedBlock = 256;
+constexpr int kRedPer = 8;       // records per thread in the first reduction pass
+// Waves per workgroup at most: the FP64 erfc/exp coefficients of the pair term
+// keep ~190 VGPRs live, i.e. 2 waves per SIMD; 8 waves fill a CU with one workgroup.
+constexpr int kMaxWaves = 8;
+
+// Per-wave LDS workspace.  The Ewald phase reuses the draw/trial area for the
+// e^{ik.r} axis tables of the selected molecule.
+struct WaveLds {
+  double ud[kMaxDraws];          // this insertion's uniforms
+  double pos[kMaxPts][3];        // trial sites, FP64
+  uint4 pfix[kMaxPts];           // same, fixed point
+};
+constexpr size_t kWaveArea = sizeof(WaveLds);
 
 struct WidomArgs {
   BoxConst bc;
-  const double* fw;        // SoA x,y,z,q
-  const int* fw_type;
-  const uint4* fw_fix;
+  const double* fw;        // SoA x,y,z,q [4][n_fw]
+  const uint4* fw_fixT;    // fixed-point x,y,z + type
   int n_fw;
-  const double* g;
+  const double* g;         // guests SoA x,y,z,q [4][guest_cap]
   const int* g_type;
   const int* g_molid;
   const uint4* g_fix;
   int guest_cap;
   GuestView gv;
-  const double* ff64;
-  const float* ff32;
+  const double* ff64;      // {4 eps, sigma^2, shift, eps} per type pair
   int comp, new_molid;
   int M, chain;
   int site_type[kMaxMol];
@@ -54,11 +82,20 @@ struct WidomArgs {
   int64_t first_index;
   int n_insert;
   int charged;
-  const double* ktemp;
+  int n_kact;
+  const int4* kact;         // active k-vectors: kx, ky, kz, index into the S(k) arrays
+  const double* ktemp_act;  // their prefactor * exp(-k^2/4a^2)/k^2 (x2 for kx > 0)
   const double2* sk_ads;
   const double2* sk_fw;
+  int kstride;              // max kmax + 1
+  // LDS layout (bytes from the start of the dynamic area)
+  int lds_thr, lds_fw, lds_wave, wave_bytes;
   graspa_widom_result* res;   // may be null
-  double* wsum;               // [gridDim.x*kSums + kSums]
+  double* rec_w;              // [n_insert] W
+  double* rec_e;              // [n_insert] dE
+  int* rec_f;                 // [n_insert] bit 0 first bead placed, bit 1 complete
+  unsigned* ctr;              // work counter
+  unsigned long long* stat;   // screened pairs, evaluated pairs
 };
 
 // Rotate_Quaternions (mc_utilities.h:423-457)
@@ -81,17 +118,19 @@ __device__ inline void quat_rotate(double v[3], double u, double vv, double w)
   v[0] = x; v[1] = y; v[2] = z;
 }
 
-__device__ inline void to_fixed(const BoxConst& bc, const double p[3], uint32_t f[3])
+__device__ inline uint4 to_fixed4(const BoxConst& bc, const double p[3])
 {
   double s[3];
   s[0] = bc.inv[0] * p[0] + bc.inv[3] * p[1] + bc.inv[6] * p[2];
   s[1] = bc.inv[1] * p[0] + bc.inv[4] * p[1] + bc.inv[7] * p[2];
   s[2] = bc.inv[2] * p[0] + bc.inv[5] * p[1] + bc.inv[8] * p[2];
+  uint32_t f[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double fr = s[d] - floor(s[d]);
     f[d] = (uint32_t)(int64_t)(fr * 4294967296.0);
   }
+  return make_uint4(f[0], f[1], f[2], 0u);
 }
 
 __device__ inline double2 cmul2(double2 a, double2 b)
@@ -99,432 +138,550 @@ __device__ inline double2 cmul2(double2 a, double2 b)
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Shared workspace of one workgroup.
-struct WidomShared {
-  double pos[kMaxTrials * kMaxMol][3];      // trial sites (FP64)
-  uint32_t fix[kMaxTrials * kMaxMol][3];    // same, fixed point
-  double red[4][kMaxTrials][2];             // per-wave partial sums (LJ, real) per trial
-  uint32_t ov[4];                           // per-wave overlap bits
-  double tot[kMaxTrials][2];                // workgroup sums per trial
-  double logw[kMaxTrials], ew[kMaxTrials];
-  double W;
-  int ok;
-  double mol[kMaxMol][3];                   // selected molecule
-  double2 eik[kMaxMol][3][kMaxKStride];
-  double ered[4][2];
-  double e_fb[2], e_ch[2];
-  int fb, orient;
-};
+// Squared-distance threshold of the overlap screen for one (atom type, site
+// type) pair: E_LJ(r^2) = 4eps (x^2 - x) - shift with x = (sigma^2/r^2)^3
+// exceeds OverlapCriteria exactly when x > (1 + sqrt(1 + 4c))/2,
+// c = (OverlapCriteria + shift)/(4 eps) > 0, i.e. r^2 < sigma^2 / x^(1/3); the
+// reference also flags r^2 < 0.01; both only inside the VDW cutoff.  The
+// returned FP32 value is lowered by 1e-5 relative so that a screen hit is a
+// certain overlap in both precision forms; pairs near the threshold are left
+// to the exact evaluation.
+__device__ inline float screen_threshold(const double* f, double overlap, double cut2)
+{
+  double thr = 0.01;
+  const double a4e = f[0], s2 = f[1], shift = f[2];
+  if (a4e > 0.0 && s2 > 0.0) {
+    const double c = (overlap + shift) / a4e;
+    if (c > 0.0) {
+      const double xp = 0.5 * (1.0 + sqrt(1.0 + 4.0 * c));
+      thr = fmax(thr, s2 / cbrt(xp));
+    }
+  }
+  thr = fmin(thr, cut2);
+  return (float)(thr * (1.0 - 1.0e-5));
+}
 
-// Pair pass of this lane's framework atoms (registers) and of the guest atoms
-// against n_trials trials of `chain` sites (sh.pos / sh.fix).  Each trial row
-// is summed over the lane's atoms (FP32 terms in the mixed form), reduced over
-// the wave in FP64 at once, and the wave sums added in LDS; overlap bits by
-// wave vote.  Result: sh.tot[t] and sh.ov[0].
-template <bool MIXED, int APT>
-__device__ inline void pair_pass(const WidomArgs& A, WidomShared& sh, const int n_trials, const int chain,
-                                 const double (&fx)[APT], const double (&fy)[APT], const double (&fz)[APT],
-                                 const double (&fq)[APT], const uint32_t (&ux)[APT], const uint32_t (&uy)[APT],
-                                 const uint32_t (&uz)[APT], const float (&uq)[APT], const int (&ft)[APT],
-                                 const int n_mine, const float4* ffl32, const double* ffl64)
+// Overlap screen of n_trials trials x n_sites sites (trial site p = a *
+// n_trials + t in W.pfix, molecule site site0 + a).  Returns the mask of
+// trials with a certain overlap.
+template <bool CUBIC>
+__device__ inline uint64_t screen(const WidomArgs& A, const uint4* __restrict__ fwl, const float* __restrict__ thr,
+                                  const WaveLds& W, const int n_trials, const int n_sites, const int site0,
+                                  unsigned long long& stat)
 {
   const BoxConst& bc = A.bc;
   const int T = bc.ntypes;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane < n_trials) { sh.red[wave][lane][0] = 0.0; sh.red[wave][lane][1] = 0.0; }
-  uint32_t ovbits = 0u;
-  for (int a = 0; a < chain; ++a) {
-    const int st = A.site_type[(chain == 1) ? 0 : 1 + a];
-    const double sq = A.site_q[(chain == 1) ? 0 : 1 + a];
-    // force-field rows and charge products of this lane's atoms for site type st
-    double fa[APT], fs[APT], fh[APT], qq[APT];
-    if (MIXED) {
-#pragma unroll
-      for (int j = 0; j < APT; ++j) {
-        const double* f = ffl64 + 4 * (ft[j] * T + st);
-        fa[j] = f[0]; fs[j] = f[1]; fh[j] = f[2];
-        qq[j] = bc.prefactor * fq[j] * sq;
+  const int lane = threadIdx.x & 63;
+  const uint64_t all = (n_trials >= 64) ? ~0ull : ((1ull << n_trials) - 1ull);
+  uint64_t alive = all;
+  for (int c0 = 0; c0 < A.n_fw && alive; c0 += 64) {
+    const int j = c0 + lane;
+    const bool valid = j < A.n_fw;
+    const uint4 f = valid ? fwl[j] : make_uint4(0u, 0u, 0u, 0u);
+    const unsigned nval = (unsigned)min(64, A.n_fw - c0);
+    for (int a = 0; a < n_sites; ++a) {
+      const float th = valid ? thr[(site0 + a) * T + (int)f.w] : -1.0f;
+      for (int t = 0; t < n_trials; ++t) {
+        if (!((alive >> t) & 1ull)) continue;
+        const uint4 p = W.pfix[a * n_trials + t];
+        const float rr = r2_fixed32<CUBIC>(f.x, f.y, f.z, p.x, p.y, p.z, bc.h32);
+        stat += nval;
+        if (__any(rr < th)) alive &= ~(1ull << t);
       }
     }
-    for (int t = 0; t < n_trials; ++t) {
-      const int s = t * chain + a;
-      double sv = 0.0, sr = 0.0;
-      bool ov = false;
-      if (MIXED) {
-        const uint32_t sx = sh.fix[s][0], sy = sh.fix[s][1], sz = sh.fix[s][2];
-#pragma unroll
-        for (int j = 0; j < APT; ++j) {
-          if (j >= n_mine) continue;
-          const double rr = r2_fixed64(ux[j], uy[j], uz[j], sx, sy, sz, bc.h64, bc.cubic);
-          if (rr < bc.cut_vdw2) {
-            const double ev = lj_mixed(rr, fa[j], fs[j], fh[j]);
-            ov = ov || (ev > bc.overlap) || (rr < 0.01);
-            sv += ev;
-          }
-          if (bc.use_charges && rr < bc.cut_coul2) sr += coulomb_mixed(rr, qq[j], bc.alpha);
-        }
-      } else {
-        const double px = sh.pos[s][0], py = sh.pos[s][1], pz = sh.pos[s][2];
-#pragma unroll
-        for (int j = 0; j < APT; ++j) {
-          if (j >= n_mine) continue;
-          double dx = fx[j] - px, dy = fy[j] - py, dz = fz[j] - pz;
-          pbc_fp64(dx, dy, dz, bc.cell, bc.inv, bc.cubic);
-          const double rr = dx * dx + dy * dy + dz * dz;
-          if (rr < bc.cut_vdw2) {
-            const double* f = ffl64 + 4 * (ft[j] * T + st);
-            const double e = lj_fp64(rr, f[0], f[1], f[2]);
-            ov = ov || (e > bc.overlap) || (rr < 0.01);
-            sv += e;
-          }
-          if (bc.use_charges && rr < bc.cut_coul2) {
-            const double r = sqrt(rr);
-            sr += bc.prefactor * 1.0 * (fq[j] * sq * erfc(bc.alpha * r)) / r;
-          }
+  }
+  // guest molecules
+  for (int c = 1; c < A.gv.n_comp && alive; ++c) {
+    const int n = A.gv.seg[c].n_atoms, base = A.gv.seg[c].base;
+    for (int c0 = 0; c0 < n && alive; c0 += 64) {
+      const int jj = c0 + lane;
+      bool valid = jj < n;
+      uint4 f = make_uint4(0u, 0u, 0u, 0u);
+      int ty = 0;
+      if (valid) {
+        const int j = base + jj;
+        if (c == A.comp && A.g_molid[j] == A.new_molid) valid = false;
+        f = A.g_fix[j];
+        ty = A.g_type[j];
+      }
+      const unsigned nval = (unsigned)min(64, n - c0);
+      for (int a = 0; a < n_sites; ++a) {
+        const float th = valid ? thr[(site0 + a) * T + ty] : -1.0f;
+        for (int t = 0; t < n_trials; ++t) {
+          if (!((alive >> t) & 1ull)) continue;
+          const uint4 p = W.pfix[a * n_trials + t];
+          const float rr = r2_fixed32<CUBIC>(f.x, f.y, f.z, p.x, p.y, p.z, bc.h32);
+          stat += nval;
+          if (__any(rr < th)) alive &= ~(1ull << t);
         }
       }
-      // guest molecules (guest-guest), streamed from memory
-      for (int c = 1; c < A.gv.n_comp; ++c) {
-        const int n = A.gv.seg[c].n_atoms, base = A.gv.seg[c].base;
-        for (int jj = threadIdx.x; jj < n; jj += blockDim.x) {
-          const int j = base + jj;
-          if (c == A.comp && A.g_molid[j] == A.new_molid) continue;
-          const int ta = A.g_type[j];
-          if (MIXED) {
-            const uint4 g4 = A.g_fix[j];
-            const double* f = ffl64 + 4 * (ta * T + st);
-            const double rr = r2_fixed64(g4.x, g4.y, g4.z, sh.fix[s][0], sh.fix[s][1], sh.fix[s][2], bc.h64, bc.cubic);
-            if (rr < bc.cut_vdw2) {
-              const double ev = lj_mixed(rr, f[0], f[1], f[2]);
-              ov = ov || (ev > bc.overlap) || (rr < 0.01);
-              sv += ev;
-            }
-            if (bc.use_charges && rr < bc.cut_coul2)
-              sr += coulomb_mixed(rr, bc.prefactor * A.g[3 * A.guest_cap + j] * sq, bc.alpha);
-          } else {
-            const double* f = ffl64 + 4 * (ta * T + st);
-            double dx = A.g[j] - sh.pos[s][0], dy = A.g[A.guest_cap + j] - sh.pos[s][1];
-            double dz = A.g[2 * A.guest_cap + j] - sh.pos[s][2];
-            const double gq = A.g[3 * A.guest_cap + j];
-            pbc_fp64(dx, dy, dz, bc.cell, bc.inv, bc.cubic);
-            const double rr = dx * dx + dy * dy + dz * dz;
-            if (rr < bc.cut_vdw2) {
-              const double e = lj_fp64(rr, f[0], f[1], f[2]);
-              ov = ov || (e > bc.overlap) || (rr < 0.01);
-              sv += e;
-            }
-            if (bc.use_charges && rr < bc.cut_coul2) {
-              const double r = sqrt(rr);
-              sr += bc.prefactor * 1.0 * (gq * sq * erfc(bc.alpha * r)) / r;
-            }
-          }
-        }
-      }
-      sv = wave_sum(sv);
-      sr = bc.use_charges ? wave_sum(sr) : 0.0;
-      const bool any = __any(ov);
-      if (any) ovbits |= (1u << t);
-      if (lane == 0) { sh.red[wave][t][0] += sv; sh.red[wave][t][1] += sr; }
     }
   }
-  if (lane == 0) sh.ov[wave] = ovbits;
-  __syncthreads();
-  if (threadIdx.x < n_trials) {
-    const int t = threadIdx.x;
-    sh.tot[t][0] = sh.red[0][t][0] + sh.red[1][t][0] + sh.red[2][t][0] + sh.red[3][t][0];
-    sh.tot[t][1] = sh.red[0][t][1] + sh.red[1][t][1] + sh.red[2][t][1] + sh.red[3][t][1];
-  }
-  if (threadIdx.x == 0) sh.ov[0] = sh.ov[0] | sh.ov[1] | sh.ov[2] | sh.ov[3];
-  __syncthreads();
+  return all & ~alive;
 }
 
-// CBMC_FirstBead_Finish / Widom_Move_Chain_PARTIAL selection (mc_widom.h:14-39,
-// 305-383, 574-614): survivors, log-sum-exp shift, cumulative pick with the
-// uniform u, Rosenbluth sum.  Called by the whole workgroup: wave 0 evaluates
-// the exponentials in parallel, thread 0 sums in trial order as the reference
-// does.  Leaves the selected trial (or -1) in *sel and the averaged weight in
-// *W (shared variables).
-__device__ inline void select_trial(const WidomArgs& A, WidomShared& sh, const int n_trials, const double u,
-                                    double* W_out, int* sel_out)
+// Full energies of trial site p (molecule site `site`) against the framework
+// and the guests: wave sums of LJ and real-space Coulomb, and the exact
+// overlap flag (VDW_Coulomb.cu:1183-1352 for one trial site).
+template <bool MIXED, bool CUBIC>
+__device__ inline void eval_site(const WidomArgs& A, const uint4* __restrict__ fwl, const double4* __restrict__ ffl,
+                                 const WaveLds& W, const int p, const int site, double& sv, double& sr, bool& ov,
+                                 unsigned long long& stat)
 {
-  if (threadIdx.x < 64) {
-    const int lane = threadIdx.x;
-    const bool alive = lane < n_trials && !((sh.ov[0] >> lane) & 1u);
-    double lw = -INFINITY;
-    if (alive) {
-      double tot = sh.tot[lane][0];
-      if (A.bias) tot += sh.tot[lane][1];
-      lw = -A.beta * tot;
+  const BoxConst& bc = A.bc;
+  const int T = bc.ntypes;
+  const int lane = threadIdx.x & 63;
+  const double sq = A.site_q[site];
+  const bool coul = bc.use_charges && sq != 0.0;
+  const double px = W.pos[p][0], py = W.pos[p][1], pz = W.pos[p][2];
+  const double4* fr = ffl + site * T;
+  const double* fq = A.fw + 3 * (size_t)A.n_fw;
+  double v = 0.0, r = 0.0;
+  bool o = false;
+  for (int j = lane; j < A.n_fw; j += 64) {
+    const int ty = (int)fwl[j].w;
+    double dx = A.fw[j] - px, dy = A.fw[A.n_fw + j] - py, dz = A.fw[2 * A.n_fw + j] - pz;
+    if (MIXED)
+      pbc_rint<CUBIC>(dx, dy, dz, bc.cell, bc.inv);
+    else
+      pbc_fp64(dx, dy, dz, bc.cell, bc.inv, CUBIC ? 1 : 0);
+    const double rr = dx * dx + dy * dy + dz * dz;
+    if (rr < bc.cut_vdw2) {
+      const double4 ff = fr[ty];
+      const double e = MIXED ? lj_mixed(rr, ff.x, ff.y, ff.z) : lj_fp64(rr, ff.x, ff.y, ff.z);
+      o = o || (e > bc.overlap) || (rr < 0.01);
+      v += e;
     }
-    double mx = lw;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) mx = fmax(mx, __shfl_xor(mx, m, 64));
-    if (lane < n_trials) {
-      sh.logw[lane] = alive ? exp(lw - mx) : -1.0;   // shifted factors; -1 marks an overlap
-      sh.ew[lane] = alive ? exp(lw) : 0.0;
+    if (coul && rr < bc.cut_coul2) {
+      if (MIXED) {
+        r += coulomb_mixed(rr, bc.prefactor * fq[j] * sq, bc.alpha);
+      } else {
+        const double rd = sqrt(rr);
+        r += bc.prefactor * 1.0 * (fq[j] * sq * erfc(bc.alpha * rd)) / rd;
+      }
     }
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int nk = 0, sel = -1;
-    double ssum = 0.0, rb = 0.0;
-    for (int t = 0; t < n_trials; ++t)
-      if (sh.logw[t] >= 0.0) { ssum += sh.logw[t]; rb += sh.ew[t]; ++nk; }
-    if (nk > 0) {
-      const double ws = u * ssum;
-      int t = 0;
-      while (sh.logw[t] < 0.0) ++t;
-      double cumw = sh.logw[t];
+  unsigned long long npair = (unsigned long long)A.n_fw;
+  for (int c = 1; c < A.gv.n_comp; ++c) {
+    const int n = A.gv.seg[c].n_atoms, base = A.gv.seg[c].base;
+    npair += (unsigned long long)n;
+    for (int jj = lane; jj < n; jj += 64) {
+      const int j = base + jj;
+      if (c == A.comp && A.g_molid[j] == A.new_molid) continue;
+      const int ta = A.g_type[j];
+      const double gq = A.g[3 * A.guest_cap + j];
+      double dx = A.g[j] - px, dy = A.g[A.guest_cap + j] - py, dz = A.g[2 * A.guest_cap + j] - pz;
+      if (MIXED)
+        pbc_rint<CUBIC>(dx, dy, dz, bc.cell, bc.inv);
+      else
+        pbc_fp64(dx, dy, dz, bc.cell, bc.inv, CUBIC ? 1 : 0);
+      const double rr = dx * dx + dy * dy + dz * dz;
+      if (rr < bc.cut_vdw2) {
+        const double4 ff = fr[ta];
+        const double e = MIXED ? lj_mixed(rr, ff.x, ff.y, ff.z) : lj_fp64(rr, ff.x, ff.y, ff.z);
+        o = o || (e > bc.overlap) || (rr < 0.01);
+        v += e;
+      }
+      if (coul && rr < bc.cut_coul2) {
+        if (MIXED) {
+          r += coulomb_mixed(rr, bc.prefactor * gq * sq, bc.alpha);
+        } else {
+          const double rd = sqrt(rr);
+          r += bc.prefactor * 1.0 * (gq * sq * erfc(bc.alpha * rd)) / rd;
+        }
+      }
+    }
+  }
+  stat += npair;
+  sv = wave_sum(v);
+  sr = coul ? wave_sum(r) : 0.0;
+  ov = __any(o) != 0;
+}
+
+// SelectTrialPosition / CBMC_FirstBead_Finish / Widom_Move_Chain_PARTIAL
+// (mc_widom.h:14-39, 305-383, 574-614): lane t < n holds trial t's LJ (ev) and
+// real-space (er) sums; ovm marks overlapping trials.  Survivors, log-sum-exp
+// shift, cumulative pick with the uniform u in trial order, Rosenbluth sum
+// averaged over n, and the LJ-only bias correction.  Returns the selected
+// trial (or -1) and W, both uniform.
+__device__ inline void wave_select(double ev, double er, uint64_t ovm, int n, double u, double beta, int bias,
+                                   int& sel_out, double& W_out)
+{
+  const int lane = threadIdx.x & 63;
+  const bool alive = lane < n && !((ovm >> lane) & 1ull);
+  double lw = -INFINITY;
+  if (alive) {
+    double tot = ev;
+    if (bias) tot += er;
+    lw = -beta * tot;
+  }
+  const double mx = wave_max(lw);
+  const double shf = alive ? exp(lw - mx) : -1.0;   // shifted factors; -1 marks an overlap
+  const double ew = alive ? exp(lw) : 0.0;
+  int nk = 0;
+  double ssum = 0.0, rb = 0.0;
+  for (int t = 0; t < n; ++t) {
+    const double s = readlane_d(shf, t);
+    if (s >= 0.0) {
+      ssum += s;
+      rb += readlane_d(ew, t);
+      ++nk;
+    }
+  }
+  int sel = -1;
+  if (nk > 0) {
+    const double ws = u * ssum;
+    int t = 0;
+    while (readlane_d(shf, t) < 0.0) ++t;
+    double cumw = readlane_d(shf, t);
+    sel = t;
+    int left = nk - 1;
+    while (cumw < ws && left > 0) {
+      ++t;
+      while (readlane_d(shf, t) < 0.0) ++t;
+      cumw += readlane_d(shf, t);
       sel = t;
-      int left = nk - 1;
-      while (cumw < ws && left > 0) {
-        ++t;
-        while (sh.logw[t] < 0.0) ++t;
-        cumw += sh.logw[t];
-        sel = t;
-        --left;
-      }
+      --left;
     }
-    double W = 0.0;
-    if (nk > 0 && !(rb < 1e-150)) {
-      W = rb / (double)n_trials;
-      if (!A.bias) W *= exp(-A.beta * sh.tot[sel][1]);
-    } else {
-      sel = -1;
-    }
-    *W_out = W;
-    *sel_out = sel;
   }
-  __syncthreads();
+  double W = 0.0;
+  if (nk > 0 && !(rb < 1e-150)) {
+    W = rb / (double)n;
+    if (!bias) W *= exp(-beta * readlane_d(er, sel));
+  } else {
+    sel = -1;
+  }
+  sel_out = sel;
+  W_out = W;
 }
 
-template <bool MIXED, int APT>
-__global__ __launch_bounds__(kBlock, 1) void widom_kernel(const WidomArgs A)
+template <bool MIXED, bool FW_LDS, bool CUBIC>
+__global__ __launch_bounds__(kMaxWaves * 64) void widom_kernel(const WidomArgs A)
 {
-  __shared__ WidomShared sh;
-  __shared__ double sel_w;
-  __shared__ int sel_t;
-  extern __shared__ char dyn[];
+  extern __shared__ __align__(16) char dyn[];
   const BoxConst& bc = A.bc;
   const int T = bc.ntypes;
-  // force-field rows in LDS
-  float4* ffl32 = reinterpret_cast<float4*>(dyn);
-  double* ffl64 = reinterpret_cast<double*>(dyn + (size_t)T * T * sizeof(float4));
-  for (int i = threadIdx.x; i < T * T; i += blockDim.x) {
-    ffl32[i] = make_float4(A.ff32[4 * i], A.ff32[4 * i + 1], A.ff32[4 * i + 2], 0.f);
-    for (int k = 0; k < 4; ++k) ffl64[4 * i + k] = A.ff64[4 * i + k];
+  const int M = A.M, chain = A.chain, K = A.K, O = A.O;
+  // per-(site, atom type) force-field rows and screen thresholds
+  double4* ffl = reinterpret_cast<double4*>(dyn);
+  float* thr = reinterpret_cast<float*>(dyn + A.lds_thr);
+  for (int i = threadIdx.x; i < M * T; i += blockDim.x) {
+    const int a = i / T, ty = i % T;
+    const double* f = A.ff64 + 4 * (ty * T + A.site_type[a]);
+    ffl[i] = make_double4(f[0], f[1], f[2], 0.0);
+    thr[i] = screen_threshold(f, bc.overlap, bc.cut_vdw2);
   }
-  // this lane's framework stripe, resident in registers for the whole launch
-  int n_mine = 0;
-  double fx[APT], fy[APT], fz[APT], fq[APT];
-  uint32_t ux[APT], uy[APT], uz[APT];
-  float uq[APT];
-  int ft[APT];
-#pragma unroll
-  for (int j = 0; j < APT; ++j) {
-    const int a = j * kBlock + threadIdx.x;
-    fx[j] = fy[j] = fz[j] = fq[j] = 0.0;
-    ux[j] = uy[j] = uz[j] = 0u;
-    uq[j] = 0.f;
-    ft[j] = 0;
-    if (a < A.n_fw) {
-      ++n_mine;
-      ft[j] = A.fw_type[a];
-      if (MIXED) {
-        const uint4 v = A.fw_fix[a];
-        ux[j] = v.x; uy[j] = v.y; uz[j] = v.z; uq[j] = __uint_as_float(v.w);
-        fq[j] = A.fw[3 * A.n_fw + a];
-      } else {
-        fx[j] = A.fw[a]; fy[j] = A.fw[A.n_fw + a]; fz[j] = A.fw[2 * A.n_fw + a]; fq[j] = A.fw[3 * A.n_fw + a];
-      }
-    }
+  const uint4* fwl;
+  if (FW_LDS) {
+    uint4* s = reinterpret_cast<uint4*>(dyn + A.lds_fw);
+    for (int i = threadIdx.x; i < A.n_fw; i += blockDim.x) s[i] = A.fw_fixT[i];
+    fwl = s;
+  } else {
+    fwl = A.fw_fixT;
   }
   __syncthreads();
 
-  double blk_w = 0.0, blk_w2 = 0.0, blk_we = 0.0, blk_n = 0.0, blk_fb = 0.0, blk_ok = 0.0;
-  const int K = A.K, O = A.O, chain = A.chain;
-  for (int ins = blockIdx.x; ins < A.n_insert; ins += gridDim.x) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  WaveLds& W = *reinterpret_cast<WaveLds*>(dyn + A.lds_wave + (size_t)wv * A.wave_bytes);
+  double2* eik = reinterpret_cast<double2*>(&W);   // Ewald phase only
+  double (*mol)[3] = reinterpret_cast<double (*)[3]>(dyn + A.lds_wave + (size_t)wv * A.wave_bytes + A.wave_bytes -
+                                                     kMaxMol * 3 * sizeof(double));
+  unsigned long long st_screen = 0, st_full = 0;
+  const int nd = 3 * K + 1 + (chain > 0 ? 3 * O + 1 : 0);
+
+  unsigned next = 0;
+  if (lane == 0) next = atomicAdd(A.ctr, 1u);
+  for (;;) {
+    const int ins = __builtin_amdgcn_readfirstlane((int)next);
+    if (ins >= A.n_insert) break;
+    if (lane == 0) next = atomicAdd(A.ctr, 1u);
     const uint64_t idx = (uint64_t)(A.first_index + ins);
-    // 1. first-bead trial positions: diag(cell) * U^3 (get_random_trial_position, mc_widom.h:122-213)
-    if (threadIdx.x < K) {
-      const int k = threadIdx.x;
-      double p[3];
-      p[0] = bc.cell[0] * philox_uniform(A.seed, A.stream, idx, 3 * k);
-      p[1] = bc.cell[4] * philox_uniform(A.seed, A.stream, idx, 3 * k + 1);
-      p[2] = bc.cell[8] * philox_uniform(A.seed, A.stream, idx, 3 * k + 2);
-      sh.pos[k][0] = p[0]; sh.pos[k][1] = p[1]; sh.pos[k][2] = p[2];
-      if (MIXED) to_fixed(bc, p, sh.fix[k]);
-    }
-    __syncthreads();
-    // 2. pair pass over the K trials, 3. first-bead selection
-    pair_pass<MIXED, APT>(A, sh, K, 1, fx, fy, fz, fq, ux, uy, uz, uq, ft, n_mine, ffl32, ffl64);
-    select_trial(A, sh, K, philox_uniform(A.seed, A.stream, idx, 3 * K), &sel_w, &sel_t);
-    if (threadIdx.x == 0) {
-      const int sel = sel_t;
-      sh.W = sel_w;
-      sh.ok = (sel >= 0 && sel_w > 1e-150) ? 1 : 0;
-      sh.fb = sel;
-      sh.e_fb[0] = sel >= 0 ? sh.tot[sel][0] : 0.0;
-      sh.e_fb[1] = sel >= 0 ? sh.tot[sel][1] : 0.0;
-      if (sel >= 0) {
-        sh.mol[0][0] = sh.pos[sel][0]; sh.mol[0][1] = sh.pos[sel][1]; sh.mol[0][2] = sh.pos[sel][2];
+
+    // 1. draws: block b holds draws 2b and 2b+1 (philox_uniform's layout)
+    for (int b = lane; 2 * b < nd; b += 64) {
+      uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)b, A.stream};
+      philox4x32_10(c, (uint32_t)A.seed, (uint32_t)(A.seed >> 32));
+      const uint64_t x0 = (((uint64_t)c[0] << 32) | c[1]) >> 11;
+      W.ud[2 * b] = ((double)x0 + 0.5) * (1.0 / 9007199254740992.0);
+      if (2 * b + 1 < nd) {
+        const uint64_t x1 = (((uint64_t)c[2] << 32) | c[3]) >> 11;
+        W.ud[2 * b + 1] = ((double)x1 + 0.5) * (1.0 / 9007199254740992.0);
       }
-      sh.e_ch[0] = sh.e_ch[1] = 0.0;
-      sh.orient = 0;
     }
-    __syncthreads();
-    if (threadIdx.x == 0 && sh.ok) blk_fb += 1.0;
-    if (sh.ok && chain > 0) {
-      // 4. orientations of the template about the selected first bead
-      //    (get_random_trial_orientation, mc_widom.h:215-303)
-      if (threadIdx.x < O) {
-        const int o = threadIdx.x;
-        const double u = philox_uniform(A.seed, A.stream, idx, 3 * K + 1 + 3 * o);
-        const double v = philox_uniform(A.seed, A.stream, idx, 3 * K + 2 + 3 * o);
-        const double w = philox_uniform(A.seed, A.stream, idx, 3 * K + 3 + 3 * o);
+    wave_sync();
+    // 2. first-bead trial positions: diag(cell) * U^3 (get_random_trial_position, mc_widom.h:122-213)
+    if (lane < K) {
+      double p[3];
+      p[0] = bc.cell[0] * W.ud[3 * lane];
+      p[1] = bc.cell[4] * W.ud[3 * lane + 1];
+      p[2] = bc.cell[8] * W.ud[3 * lane + 2];
+      W.pos[lane][0] = p[0]; W.pos[lane][1] = p[1]; W.pos[lane][2] = p[2];
+      W.pfix[lane] = to_fixed4(bc, p);
+    }
+    wave_sync();
+    // 3.-5. first bead
+    uint64_t ovm = screen<CUBIC>(A, fwl, thr, W, K, 1, 0, st_screen);
+    double ev = 0.0, er = 0.0;
+    for (int t = 0; t < K; ++t) {
+      if ((ovm >> t) & 1ull) continue;
+      double sv, sr;
+      bool o;
+      eval_site<MIXED, CUBIC>(A, fwl, ffl, W, t, 0, sv, sr, o, st_full);
+      if (o) ovm |= 1ull << t;
+      if (lane == t) { ev = sv; er = sr; }
+    }
+    int sel;
+    double Wfb;
+    wave_select(ev, er, ovm, K, W.ud[3 * K], A.beta, A.bias, sel, Wfb);
+    const bool fb_ok = sel >= 0 && Wfb > 1e-150;
+    bool ok = fb_ok;
+    double Wtot = Wfb, e_v = 0.0, e_r = 0.0;
+    int orient = 0;
+    if (fb_ok) {
+      e_v = readlane_d(ev, sel);
+      e_r = readlane_d(er, sel);
+      if (lane < 3) mol[0][lane] = W.pos[sel][lane];
+    }
+    wave_sync();
+    // 6. orientations of the template about the selected first bead
+    //    (get_random_trial_orientation, mc_widom.h:215-303); trial site of
+    //    orientation o and chain atom a at index a * O + o
+    if (fb_ok && chain > 0) {
+      if (lane < O) {
+        const double u = W.ud[3 * K + 1 + 3 * lane];
+        const double v = W.ud[3 * K + 2 + 3 * lane];
+        const double w = W.ud[3 * K + 3 + 3 * lane];
         for (int a = 0; a < chain; ++a) {
           double vec[3] = {A.tvec[a][0], A.tvec[a][1], A.tvec[a][2]};
           quat_rotate(vec, u, v, w);
-          const int s = o * chain + a;
-          double p[3] = {sh.mol[0][0] + vec[0], sh.mol[0][1] + vec[1], sh.mol[0][2] + vec[2]};
-          sh.pos[s][0] = p[0]; sh.pos[s][1] = p[1]; sh.pos[s][2] = p[2];
-          if (MIXED) to_fixed(bc, p, sh.fix[s]);
+          const int s = a * O + lane;
+          double p[3] = {mol[0][0] + vec[0], mol[0][1] + vec[1], mol[0][2] + vec[2]};
+          W.pos[s][0] = p[0]; W.pos[s][1] = p[1]; W.pos[s][2] = p[2];
+          W.pfix[s] = to_fixed4(bc, p);
         }
       }
-      __syncthreads();
-      pair_pass<MIXED, APT>(A, sh, O, chain, fx, fy, fz, fq, ux, uy, uz, uq, ft, n_mine, ffl32, ffl64);
-      select_trial(A, sh, O, philox_uniform(A.seed, A.stream, idx, 3 * K + 3 * O + 1), &sel_w, &sel_t);
-      if (threadIdx.x == 0) {
-        const int sel = sel_t;
-        if (sel < 0 || !(sh.W * sel_w > 1e-150)) {
-          sh.ok = 0;
-        } else {
-          sh.W *= sel_w;
-          sh.orient = sel;
-          sh.e_ch[0] = sh.tot[sel][0]; sh.e_ch[1] = sh.tot[sel][1];
-          for (int a = 0; a < chain; ++a) {
-            const int s = sel * chain + a;
-            sh.mol[1 + a][0] = sh.pos[s][0]; sh.mol[1 + a][1] = sh.pos[s][1]; sh.mol[1 + a][2] = sh.pos[s][2];
-          }
+      wave_sync();
+      uint64_t ovo = screen<CUBIC>(A, fwl, thr, W, O, chain, 1, st_screen);
+      double cv = 0.0, cr = 0.0;
+      for (int o = 0; o < O; ++o) {
+        if ((ovo >> o) & 1ull) continue;
+        double tv = 0.0, tr = 0.0;
+        bool dead = false;
+        for (int a = 0; a < chain && !dead; ++a) {
+          double sv, sr;
+          bool ob;
+          eval_site<MIXED, CUBIC>(A, fwl, ffl, W, a * O + o, 1 + a, sv, sr, ob, st_full);
+          tv += sv;
+          tr += sr;
+          dead = ob;
+        }
+        if (dead) ovo |= 1ull << o;
+        if (lane == o) { cv = tv; cr = tr; }
+      }
+      int so;
+      double Wch;
+      wave_select(cv, cr, ovo, O, W.ud[3 * K + 3 * O + 1], A.beta, A.bias, so, Wch);
+      if (so < 0 || !(Wfb * Wch > 1e-150)) {
+        ok = false;
+      } else {
+        Wtot = Wfb * Wch;
+        orient = so;
+        e_v += readlane_d(cv, so);
+        e_r += readlane_d(cr, so);
+        for (int i = lane; i < 3 * chain; i += 64) {
+          const int a = i / 3, d = i % 3;
+          mol[1 + a][d] = W.pos[a * O + so][d];
         }
       }
-      __syncthreads();
+      wave_sync();
     }
-    // 5. Fourier-space energy of the selected molecule, FP64 (Fourier_Ewald_Diff,
+    // 7. Fourier-space energy of the selected molecule, FP64 (Fourier_Ewald_Diff,
     //    Ewald_Energy_Functions.h:280-397, with Oldsize = 0)
     double ew = 0.0;
-    if (sh.ok && A.charged) {
-      for (int t = threadIdx.x; t < A.M * 3; t += blockDim.x) {
+    if (ok && A.charged) {
+      const int KS = A.kstride;
+      for (int t = lane; t < M * 3; t += 64) {
         const int site = t / 3, axis = t % 3;
-        double s = bc.inv[axis] * sh.mol[site][0] + bc.inv[3 + axis] * sh.mol[site][1] + bc.inv[6 + axis] * sh.mol[site][2];
+        double s = bc.inv[axis] * mol[site][0] + bc.inv[3 + axis] * mol[site][1] + bc.inv[6 + axis] * mol[site][2];
         s *= 2 * 3.14159265358979323846;
         double sn, cs;
         sincos(s, &sn, &cs);
         const double2 e1 = make_double2(cs, sn);
-        double2* e = sh.eik[site][axis];
+        double2* e = eik + (size_t)t * KS;
         e[0] = make_double2(1.0, 0.0);
         if (bc.kmax[axis] >= 1) e[1] = e1;
-        for (int k = 2; k <= bc.kmax[axis]; ++k) e[k] = cmul2(e[k - 1], e1);
-      }
-      __syncthreads();
-      double es = 0.0, ec = 0.0;
-      const int kzm = bc.kmax[2], kym = bc.kmax[1];
-      for (int k = threadIdx.x; k < bc.nvec; k += blockDim.x) {
-        const double t = A.ktemp[k];
-        if (t < 0.0) continue;
-        const int kz = k % (2 * kzm + 1) - kzm;
-        const int kxy = k / (2 * kzm + 1);
-        const int kx = kxy / (2 * kym + 1);
-        const int ky = kxy % (2 * kym + 1) - kym;
-        double2 cn = make_double2(0.0, 0.0);
-        for (int i = 0; i < A.M; ++i) {
-          double2 ey = sh.eik[i][1][abs(ky)];
-          if (ky < 0) ey.y = -ey.y;
-          const double2 exy = cmul2(sh.eik[i][0][kx], ey);
-          double2 ez = sh.eik[i][2][abs(kz)];
-          if (kz < 0) ez.y = -ez.y;
-          const double2 p = cmul2(exy, ez);
-          cn.x += A.site_q[i] * p.x;
-          cn.y += A.site_q[i] * p.y;
+        double2 cur = e1;
+        for (int k = 2; k <= bc.kmax[axis]; ++k) {
+          cur = cmul2(cur, e1);
+          e[k] = cur;
         }
-        const double2 ov = A.sk_ads[k];
+      }
+      wave_sync();
+      double es = 0.0, ec = 0.0;
+      for (int kk = lane; kk < A.n_kact; kk += 64) {
+        const int4 kv = A.kact[kk];
+        const double t = A.ktemp_act[kk];
+        const int aky = abs(kv.y), akz = abs(kv.z);
+        double2 cn = make_double2(0.0, 0.0);
+        for (int i = 0; i < M; ++i) {
+          const double2* e = eik + (size_t)(3 * i) * KS;
+          double2 ey = e[KS + aky];
+          if (kv.y < 0) ey.y = -ey.y;
+          const double2 exy = cmul2(e[kv.x], ey);
+          double2 ez = e[2 * KS + akz];
+          if (kv.z < 0) ez.y = -ez.y;
+          const double2 pr = cmul2(exy, ez);
+          cn.x += A.site_q[i] * pr.x;
+          cn.y += A.site_q[i] * pr.y;
+        }
+        const double2 ov = A.sk_ads[kv.w];
         const double2 nv = make_double2(ov.x + cn.x, ov.y + cn.y);
         es += t * (nv.x * nv.x + nv.y * nv.y) - t * (ov.x * ov.x + ov.y * ov.y);
-        const double2 cv = A.sk_fw[k];
-        ec += t * (cv.x * cn.x + cv.y * cn.y);
+        const double2 cvk = A.sk_fw[kv.w];
+        ec += t * (cvk.x * cn.x + cvk.y * cn.y);
       }
       es = wave_sum(es);
       ec = wave_sum(ec);
-      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-      if (lane == 0) { sh.ered[wave][0] = es; sh.ered[wave][1] = ec; }
-      __syncthreads();
-      const double same = sh.ered[0][0] + sh.ered[1][0] + sh.ered[2][0] + sh.ered[3][0];
-      const double cross = sh.ered[0][1] + sh.ered[1][1] + sh.ered[2][1] + sh.ered[3][1];
-      ew = (same - A.excl) + 2.0 * cross;
+      ew = (es - A.excl) + 2.0 * ec;
+      wave_sync();
     }
-    // 6. record
-    if (threadIdx.x == 0) {
-      double W = 0.0, vdw = 0.0, real = 0.0, etot = 0.0;
-      const int ok = sh.ok;
+    // 8. record
+    if (lane == 0) {
+      double Wf = 0.0, etot = 0.0;
       if (ok) {
-        W = sh.W * exp(-A.beta * ew) * exp(-A.beta * A.tail_de);
-        vdw = sh.e_fb[0] + sh.e_ch[0];
-        real = sh.e_fb[1] + sh.e_ch[1];
-        etot = vdw + real + ew + A.tail_de;
+        Wf = Wtot * exp(-A.beta * ew) * exp(-A.beta * A.tail_de);
+        etot = e_v + e_r + ew + A.tail_de;
       }
-      blk_w += W; blk_w2 += W * W; blk_we += W * etot; blk_n += 1.0; blk_ok += ok ? 1.0 : 0.0;
+      A.rec_w[ins] = Wf;
+      A.rec_e[ins] = etot;
+      A.rec_f[ins] = (fb_ok ? 1 : 0) | (ok ? 2 : 0);
       if (A.res) {
         graspa_widom_result r;
-        r.rosen = W; r.energy = etot; r.vdw = vdw; r.real = real; r.ewald = ok ? ew : 0.0;
-        r.first_bead = ok ? sh.fb : -1; r.orientation = ok ? sh.orient : -1; r.ok = ok; r._pad = 0;
+        r.rosen = Wf;
+        r.energy = etot;
+        r.vdw = ok ? e_v : 0.0;
+        r.real = ok ? e_r : 0.0;
+        r.ewald = ok ? ew : 0.0;
+        r.first_bead = ok ? sel : -1;
+        r.orientation = ok ? orient : -1;
+        r.ok = ok ? 1 : 0;
+        r._pad = 0;
         A.res[ins] = r;
       }
     }
+  }
+  if (lane == 0) {
+    atomicAdd(&A.stat[0], st_screen);
+    atomicAdd(&A.stat[1], st_full);
+  }
+}
+
+__global__ void widom_reset_kernel(unsigned* ctr, unsigned long long* stat)
+{
+  if (threadIdx.x == 0) {
+    *ctr = 0u;
+    stat[0] = 0ull;
+    stat[1] = 0ull;
+  }
+}
+
+// Sums of the per-insertion records over fixed index ranges, in a fixed order
+// (deterministic): block b covers records [b*2048, (b+1)*2048).
+__global__ __launch_bounds__(kRedBlock) void widom_reduce_kernel(const double* rw, const double* re, const int* rf,
+                                                                 int n, double* part)
+{
+  __shared__ double s[kSums][kRedBlock];
+  double acc[kSums] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int base = blockIdx.x * kRedBlock * kRedPer;
+  for (int k = 0; k < kRedPer; ++k) {
+    const int i = base + k * kRedBlock + threadIdx.x;
+    if (i < n) {
+      const double w = rw[i];
+      const int f = rf[i];
+      acc[0] += w;
+      acc[1] += w * w;
+      acc[2] += w * re[i];
+      acc[3] += 1.0;
+      acc[4] += (f & 1) ? 1.0 : 0.0;
+      acc[5] += (f & 2) ? 1.0 : 0.0;
+    }
+  }
+  for (int q = 0; q < kSums; ++q) s[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int h = kRedBlock / 2; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int q = 0; q < kSums; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + h];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    double* o = A.wsum + kSums * blockIdx.x;
-    o[0] = blk_w; o[1] = blk_w2; o[2] = blk_we; o[3] = blk_n; o[4] = blk_fb; o[5] = blk_ok;
-  }
+  if (threadIdx.x < kSums) part[kSums * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
-// Sums the per-workgroup partials in workgroup order (deterministic).
-__global__ void widom_sum_kernel(double* wsum, int nblk)
+__global__ __launch_bounds__(kRedBlock) void widom_final_kernel(const double* part, int nblk, double* out)
 {
-  const int i = threadIdx.x;
-  if (i < kSums) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += wsum[kSums * b + i];
-    wsum[kSums * nblk + i] = s;
+  __shared__ double s[kSums][kRedBlock];
+  double acc[kSums] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nblk; b += kRedBlock)
+    for (int q = 0; q < kSums; ++q) acc[q] += part[kSums * b + q];
+  for (int q = 0; q < kSums; ++q) s[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int h = kRedBlock / 2; h >= 1; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int q = 0; q < kSums; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + h];
+    __syncthreads();
   }
+  if (threadIdx.x < kSums) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
-template <bool MIXED, int N>
-static int launch_one(graspa_box* b, const WidomArgs& A, int n_insert, size_t lds)
+// Waves per workgroup and workgroups per CU for one instantiation: the most
+// resident waves, ties to the larger workgroup (fewer framework copies).
+template <bool MIXED, bool FW_LDS, bool CUBIC>
+static int launch_one(graspa_box* b, const WidomArgs& A0, size_t fixed_lds, size_t wave_bytes)
 {
-  int occ = 1;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, widom_kernel<MIXED, N>, kBlock, lds) != hipSuccess || occ < 1)
-    occ = 1;
-  const int grid = std::max(1, std::min(n_insert, occ * b->n_cu));
-  hipLaunchKernelGGL((widom_kernel<MIXED, N>), dim3(grid), dim3(kBlock), lds, b->stream, A);
-  if (hip_check(hipGetLastError(), "widom_kernel launch")) return -1;
-  return grid;
+  auto kern = widom_kernel<MIXED, FW_LDS, CUBIC>;
+  int best_nw = 0, best_blocks = 0, best_waves = 0;
+  size_t best_lds = 0;
+  for (int nw : {4, kMaxWaves}) {
+    const size_t lds = fixed_lds + (size_t)nw * wave_bytes;
+    if (lds > (size_t)b->max_lds) continue;
+    if (lds > 65536) {
+      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      (void)hipGetLastError();
+    }
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, 64 * nw, lds) != hipSuccess || blocks < 1) {
+      (void)hipGetLastError();
+      continue;
+    }
+    if (blocks * nw >= best_waves) {
+      best_waves = blocks * nw;
+      best_nw = nw;
+      best_blocks = blocks;
+      best_lds = lds;
+    }
+  }
+  if (best_nw == 0) return fail(GRASPA_ERR_UNSUPPORTED, "widom: no launch shape fits the LDS");
+  WidomArgs A = A0;
+  const long long waves_needed = (A.n_insert + 0LL);
+  long long grid = (long long)best_blocks * b->n_cu;
+  grid = std::max(1LL, std::min(grid, (waves_needed + best_nw - 1) / best_nw));
+  timing_begin(b, "widom");
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * best_nw), best_lds, b->stream, A);
+  timing_end(b, "widom");
+  if (hip_check(hipGetLastError(), "widom_kernel launch")) return GRASPA_ERR_HIP;
+  return GRASPA_OK;
 }
 
 template <bool MIXED>
-static int launch_apt(graspa_box* b, const WidomArgs& A, int apt, int n_insert, size_t lds)
+static int launch_mixed(graspa_box* b, const WidomArgs& A, bool fw_lds, size_t fixed_lds, size_t wave_bytes)
 {
-  if (apt <= 1) return launch_one<MIXED, 1>(b, A, n_insert, lds);
-  if (apt <= 2) return launch_one<MIXED, 2>(b, A, n_insert, lds);
-  if (apt <= 4) return launch_one<MIXED, 4>(b, A, n_insert, lds);
-  if (apt <= 6) return launch_one<MIXED, 6>(b, A, n_insert, lds);
-  if (apt <= 8) return launch_one<MIXED, 8>(b, A, n_insert, lds);
-  if (apt <= 12) return launch_one<MIXED, 12>(b, A, n_insert, lds);
-  if (apt <= 16) return launch_one<MIXED, 16>(b, A, n_insert, lds);
-  if (apt <= 24) return launch_one<MIXED, 24>(b, A, n_insert, lds);
-  fail(GRASPA_ERR_UNSUPPORTED, "framework too large for the register-resident Widom kernel (> 6144 atoms)");
-  return -1;
+  if (fw_lds)
+    return A.bc.cubic ? launch_one<MIXED, true, true>(b, A, fixed_lds, wave_bytes)
+                      : launch_one<MIXED, true, false>(b, A, fixed_lds, wave_bytes);
+  return A.bc.cubic ? launch_one<MIXED, false, true>(b, A, fixed_lds, wave_bytes)
+                    : launch_one<MIXED, false, false>(b, A, fixed_lds, wave_bytes);
 }
+
+static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
                  uint64_t seed, uint32_t stream, int64_t first_index, int n_insert, graspa_widom_result* d_res,
@@ -534,19 +691,19 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   const Component& c = b->comps[comp];
   if (K < 1 || K > kMaxTrials || O < 1 || O > kMaxTrials) return fail(GRASPA_ERR_ARG, "widom: 1 <= K,O <= 32");
   if (c.molsize < 1 || c.molsize > kMaxMol) return fail(GRASPA_ERR_UNSUPPORTED, "widom: molecule larger than 8 sites");
-  if (std::max(b->bc.kmax[0], std::max(b->bc.kmax[1], b->bc.kmax[2])) >= kMaxKStride)
-    return fail(GRASPA_ERR_UNSUPPORTED, "widom: kmax above 15");
+  if (O * (c.molsize - 1) > kMaxPts) return fail(GRASPA_ERR_UNSUPPORTED, "widom: orientations x chain sites above 64");
   if (b->bc.ntypes > kMaxTypes) return fail(GRASPA_ERR_UNSUPPORTED, "widom: too many pseudo-atom types");
   if (n_insert <= 0) {
-    if (summary) *summary = graspa_widom_summary{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (summary) *summary = graspa_widom_summary{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     return GRASPA_OK;
   }
+  if (ensure_widom_records(b, n_insert)) return GRASPA_ERR_HIP;
   WidomArgs A{};
   A.bc = b->bc;
-  A.fw = b->d_fw; A.fw_type = b->d_fw_type; A.fw_fix = b->d_fw_fix; A.n_fw = b->n_fw;
+  A.fw = b->d_fw; A.fw_fixT = b->d_fw_fixT; A.n_fw = b->n_fw;
   A.g = b->d_g; A.g_type = b->d_g_type; A.g_molid = b->d_g_molid; A.g_fix = b->d_g_fix; A.guest_cap = b->guest_cap;
   guest_view(b, &A.gv);
-  A.ff64 = b->d_ff64; A.ff32 = b->d_ff32;
+  A.ff64 = b->d_ff64;
   A.comp = comp;
   A.new_molid = c.n_mol;
   A.M = c.molsize;
@@ -562,31 +719,62 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   A.charged = (b->bc.use_charges && qabs > 1e-6) ? 1 : 0;
   A.K = K; A.O = O; A.beta = beta; A.bias = bias; A.excl = excl; A.tail_de = tail_de;
   A.seed = seed; A.stream = stream; A.first_index = first_index; A.n_insert = n_insert;
-  A.ktemp = b->d_ktemp; A.sk_ads = b->d_sk_ads; A.sk_fw = b->d_sk_fw;
+  A.n_kact = b->n_active;
+  A.kact = b->d_kact;
+  A.ktemp_act = b->d_ktemp_act;
+  A.sk_ads = b->d_sk_ads; A.sk_fw = b->d_sk_fw;
+  A.kstride = 1 + std::max(b->bc.kmax[0], std::max(b->bc.kmax[1], b->bc.kmax[2]));
   A.res = d_res;
-  A.wsum = b->d_wsum;
-  const int apt = (b->n_fw + kBlock - 1) / kBlock;
-  const int Tp = A.bc.ntypes;
-  const size_t lds = (size_t)Tp * Tp * (sizeof(float4) + 4 * sizeof(double));
-  timing_begin(b, "widom");
-  const int grid = (b->precision == GRASPA_MIXED) ? launch_apt<true>(b, A, apt, n_insert, lds)
-                                                  : launch_apt<false>(b, A, apt, n_insert, lds);
-  timing_end(b, "widom");
-  if (grid < 0) return GRASPA_ERR_HIP;
-  hipLaunchKernelGGL(widom_sum_kernel, dim3(1), dim3(64), 0, b->stream, b->d_wsum, grid);
-  if (hip_check(hipGetLastError(), "widom_sum launch")) return GRASPA_ERR_HIP;
+  A.rec_w = b->d_wrec_w; A.rec_e = b->d_wrec_e; A.rec_f = b->d_wrec_f;
+  A.ctr = reinterpret_cast<unsigned*>(b->d_wacc + 12);
+  A.stat = reinterpret_cast<unsigned long long*>(b->d_wacc + 8);
+  // LDS layout: FF rows | thresholds | framework (optional) | per-wave areas
+  const int T = A.bc.ntypes, M = A.M;
+  size_t off = align16((size_t)M * T * sizeof(double4));
+  A.lds_thr = (int)off;
+  off = align16(off + (size_t)M * T * sizeof(float));
+  const size_t eik_bytes = (size_t)M * 3 * A.kstride * sizeof(double2);
+  const size_t wave_bytes = align16(std::max(kWaveArea, eik_bytes)) + kMaxMol * 3 * sizeof(double);
+  A.wave_bytes = (int)wave_bytes;
+  const size_t fw_bytes = (size_t)b->n_fw * sizeof(uint4);
+  A.lds_fw = (int)off;
+  const size_t base_lds = off;
+
+  hipLaunchKernelGGL(widom_reset_kernel, dim3(1), dim3(64), 0, b->stream, A.ctr, A.stat);
+  if (hip_check(hipGetLastError(), "widom_reset launch")) return GRASPA_ERR_HIP;
+  // framework in LDS when it fits beside four wave areas, else read through the caches
+  int rc = GRASPA_ERR_UNSUPPORTED;
+  for (int pass = 0; pass < 2 && rc == GRASPA_ERR_UNSUPPORTED; ++pass) {
+    const bool fw_lds = pass == 0;
+    if (fw_lds && base_lds + align16(fw_bytes) + 4 * wave_bytes > (size_t)b->max_lds) continue;
+    const size_t fixed_lds = fw_lds ? base_lds + align16(fw_bytes) : base_lds;
+    A.lds_wave = (int)fixed_lds;
+    rc = (b->precision == GRASPA_MIXED) ? launch_mixed<true>(b, A, fw_lds, fixed_lds, wave_bytes)
+                                        : launch_mixed<false>(b, A, fw_lds, fixed_lds, wave_bytes);
+  }
+  if (rc) return rc;
+  const int nblk = (n_insert + kRedBlock * kRedPer - 1) / (kRedBlock * kRedPer);
+  hipLaunchKernelGGL(widom_reduce_kernel, dim3(nblk), dim3(kRedBlock), 0, b->stream, A.rec_w, A.rec_e, A.rec_f,
+                     n_insert, b->d_wpart);
+  if (hip_check(hipGetLastError(), "widom_reduce launch")) return GRASPA_ERR_HIP;
+  hipLaunchKernelGGL(widom_final_kernel, dim3(1), dim3(kRedBlock), 0, b->stream, b->d_wpart, nblk, b->d_wacc);
+  if (hip_check(hipGetLastError(), "widom_final launch")) return GRASPA_ERR_HIP;
   if (summary) {
-    double s4[kSums];
-    if (hip_check(hipMemcpyAsync(s4, b->d_wsum + kSums * grid, kSums * sizeof(double), hipMemcpyDeviceToHost, b->stream),
+    double acc[12];
+    if (hip_check(hipMemcpyAsync(acc, b->d_wacc, 12 * sizeof(double), hipMemcpyDeviceToHost, b->stream),
                   "widom summary copy"))
       return GRASPA_ERR_HIP;
     if (hip_check(hipStreamSynchronize(b->stream), "widom sync")) return GRASPA_ERR_HIP;
-    summary->sum_w = s4[0];
-    summary->sum_w2 = s4[1];
-    summary->sum_we = s4[2];
-    summary->n = s4[3];
-    summary->n_first_bead = s4[4];
-    summary->n_ok = s4[5];
+    unsigned long long st[2];
+    std::memcpy(st, acc + 8, sizeof(st));
+    summary->sum_w = acc[0];
+    summary->sum_w2 = acc[1];
+    summary->sum_we = acc[2];
+    summary->n = acc[3];
+    summary->n_first_bead = acc[4];
+    summary->n_ok = acc[5];
+    summary->screened_pairs = (double)st[0];
+    summary->evaluated_pairs = (double)st[1];
   }
   return GRASPA_OK;
 }

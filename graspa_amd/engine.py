@@ -58,7 +58,8 @@ class WidomResult(C.Structure):
 
 class WidomSummary(C.Structure):
     _fields_ = [("sum_w", C.c_double), ("sum_w2", C.c_double), ("sum_we", C.c_double), ("n", C.c_double),
-                ("n_first_bead", C.c_double), ("n_ok", C.c_double)]
+                ("n_first_bead", C.c_double), ("n_ok", C.c_double), ("screened_pairs", C.c_double),
+                ("evaluated_pairs", C.c_double)]
 
 
 class Species(C.Structure):

@@ -92,6 +92,8 @@ typedef struct {
   double n;                  /* insertions                                                  */
   double n_first_bead;       /* insertions whose first bead was placed (orientations grown) */
   double n_ok;               /* insertions with a complete construction (W > 0)             */
+  double screened_pairs;     /* work: (trial site, atom) pairs tested by the overlap screen */
+  double evaluated_pairs;    /* work: (trial site, atom) pairs with full energies           */
 } graspa_widom_summary;
 
 /* ------------------------------------------------------------------------ */

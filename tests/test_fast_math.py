@@ -1,0 +1,82 @@
+"""The FP64 erfc/exp of the fast pair term (graspa_amd/csrc/device_math.hpp,
+coefficients in fast_math_coeffs.hpp), re-evaluated step by step in numpy and
+checked against scipy's erfc/exp -- libraries independent of the fit -- and
+against the regeneration of the coefficient header."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+from scipy.special import erfc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "graspa_amd", "csrc")
+
+
+def _coeffs():
+    txt = open(os.path.join(CSRC, "fast_math_coeffs.hpp")).read()
+    scal = {m.group(1): float(m.group(2)) for m in re.finditer(r"constexpr (?:double|int) (\w+) = ([-0-9.e+]+);", txt)}
+    arrs = {}
+    for m in re.finditer(r"constexpr double (\w+)\[\d+\] = \{([^}]*)\};", txt, re.S):
+        arrs[m.group(1)] = [float(v) for v in m.group(2).replace("\n", " ").split(",") if v.strip()]
+    return scal, arrs
+
+
+def _exp_fast(y, s, a):
+    n = np.rint(y * s["kLog2e"])
+    r = y - n * s["kLn2Hi"]
+    r = r - n * s["kLn2Lo"]
+    p = np.full_like(y, a["kExpPoly"][-1])
+    for c in reversed(a["kExpPoly"][:-1]):
+        p = p * r + c
+    return np.ldexp(p, n.astype(np.int64))
+
+
+def _erfc_fast(x, s, a):
+    d = s["kErfcC"] + x
+    y = (1.0 / d.astype(np.float32)).astype(np.float32).astype(np.float64)   # FP32 reciprocal seed
+    y = y + y * (1.0 - d * y)                                                # one Newton step
+    t = s["kErfcC"] * y
+    u = t * s["kErfcUScale"] + s["kErfcUOff"]
+    F = np.full_like(x, a["kErfcPoly"][-1])
+    for c in reversed(a["kErfcPoly"][:-1]):
+        F = F * u + c
+    r = _exp_fast(-x * x, s, a) * t * F
+    return np.where(x < s["kErfcXMax"], r, 0.0)
+
+
+def test_exp_fast_accuracy():
+    s, a = _coeffs()
+    y = np.linspace(-40.0, 0.0, 400001)
+    rel = np.abs(_exp_fast(y, s, a) / np.exp(y) - 1.0)
+    assert rel.max() < 2e-14
+
+
+def test_erfc_fast_accuracy():
+    s, a = _coeffs()
+    x = np.concatenate([np.linspace(0.0, 6.0, 600001)[:-1], [1e-12, 0.5, 1.0, 2.0, 3.18, 5.999999]])
+    ref = erfc(x)
+    got = _erfc_fast(x, s, a)
+    rel = np.abs(got / ref - 1.0)
+    assert rel.max() < 5e-13
+    # beyond the fitted range the term is dropped: erfc(6) = 2.15e-17
+    assert _erfc_fast(np.array([6.0, 7.5]), s, a).tolist() == [0.0, 0.0]
+    assert erfc(6.0) < 2.2e-17
+
+
+def test_erfc_fast_pins_a_dropped_term():
+    """A coefficient dropped from the fit breaks the accuracy bound."""
+    s, a = _coeffs()
+    x = np.linspace(0.0, 6.0, 60001)
+    a2 = dict(a)
+    a2["kErfcPoly"] = a["kErfcPoly"][:-1]
+    rel = np.abs(_erfc_fast(x, s, a2) / erfc(x) - 1.0)
+    assert rel.max() > 1e-10
+
+
+def test_coefficient_header_is_regenerated_identically():
+    out = subprocess.run([sys.executable, os.path.join(CSRC, "gen_fast_math.py")], capture_output=True, text=True,
+                         check=True).stdout
+    assert out.strip() == open(os.path.join(CSRC, "fast_math_coeffs.hpp")).read().strip()

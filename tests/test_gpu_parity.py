@@ -109,8 +109,9 @@ def test_site_energies(prec, engine_mod):
             assert _close(got[keep], ref[keep])
         else:
             err = np.abs(got[keep] - ref[keep])
-            # FP32 pair terms: relative 2e-6 of the sum of |pair| magnitudes, bounded here by the values
-            assert np.all(err <= 1e-3 + 2e-6 * np.abs(ref[keep])), err.max()
+            # FP64 arithmetic with FP32-seeded reciprocals and the fitted erfc (~1e-13 relative)
+            print("mixed site energies: max |err| %.3e" % err.max())
+            assert np.all(err <= 1e-6 + 1e-9 * np.abs(ref[keep])), err.max()
     box.close()
 
 
@@ -170,10 +171,11 @@ def test_commit_and_total_energy_consistency(engine_mod):
 # --------------------------------------------------------------------------
 # batched Widom
 # --------------------------------------------------------------------------
-def _widom_pair(n_fw, n_insert, seed, prec, engine_mod, first_index=0, stream=0):
+def _widom_pair(n_fw, n_insert, seed, prec, engine_mod, first_index=0, stream=0, system=None):
     from graspa_amd import synth
     from oracle.binding import widom_records_to_numpy
-    s = synth.config1(n_fw=n_fw)
+    s = synth.config1(n_fw=n_fw) if system is None else system
+    print("widom case n_fw=%d n=%d seed=%d prec=%d" % (len(s.fw_type), n_insert, seed, prec), flush=True)
     o = _oracle(s)
     _, skf, ska, _, _, _ = o.ewald_total()
     intra, selfe = o.exclusion(1)
@@ -199,21 +201,28 @@ def test_widom_fp64_matches_oracle(engine_mod):
     assert summ.n == 300 and summ.n_ok == ref["ok"].sum()
 
 
-def test_widom_mixed_within_tolerance(engine_mod):
-    """Mixed form vs FP64 oracle: same constructions (selection ties aside) and
-    |dE| per insertion within 1e-5 kJ/mol-scaled tolerance for the configurations
-    that carry weight (|dE| < 1e4 internal units)."""
-    s, ref, got, summ = _widom_pair(400, 300, 23, engine_mod.GRASPA_MIXED, engine_mod)
+MIXED_DE_TOL = 1e-3    # internal units (10 J/mol): 1e-5 kJ/mol per insertion
+
+
+def _check_mixed(ref, got, min_agree=0.97):
     ok = (ref["ok"] == 1) & (got["ok"] == 1)
     same_sel = ok & (got["first_bead"] == ref["first_bead"]) & (got["orientation"] == ref["orientation"])
-    assert same_sel.sum() >= 0.97 * (ref["ok"] == 1).sum()
+    assert same_sel.sum() >= min_agree * (ref["ok"] == 1).sum()
     err = np.abs(got["energy"][same_sel] - ref["energy"][same_sel])
-    mag = np.abs(ref["energy"][same_sel])
-    light = mag < 1e4
-    print("mixed |dE| max %.3e internal (%.3e kJ/mol), median %.3e" % (err.max(), err.max() * 0.01, np.median(err)))
-    assert np.all(err[light] <= 2e-3 + 1e-7 * mag[light])
+    print("mixed |dE| max %.3e internal (%.3e kJ/mol), median %.3e over %d" %
+          (err.max(), err.max() * 0.01, np.median(err), err.size))
+    assert np.all(err <= MIXED_DE_TOL)
     w_rel = np.abs(got["rosen"][same_sel] - ref["rosen"][same_sel]) / np.maximum(ref["rosen"][same_sel], 1e-300)
-    assert np.median(w_rel) < 1e-5
+    assert np.max(w_rel) < 1e-4
+    return err
+
+
+def test_widom_mixed_within_tolerance(engine_mod):
+    """Mixed form vs FP64 oracle: the same constructions (selection ties aside)
+    and |dE| per insertion within 1e-5 kJ/mol."""
+    s, ref, got, summ = _widom_pair(400, 300, 23, engine_mod.GRASPA_MIXED, engine_mod)
+    err = _check_mixed(ref, got)
+    assert np.median(err) < 1e-6
 
 
 def test_widom_result_independent_of_batching(engine_mod):
@@ -238,3 +247,82 @@ def test_widom_dense_config_sample(engine_mod):
     s, ref, got, summ = _widom_pair(1536, 128, 99, engine_mod.GRASPA_FP64, engine_mod)
     assert np.array_equal(got["ok"], ref["ok"])
     assert _close(got["rosen"], ref["rosen"], rtol=1e-9, atol=1e-300)
+    # the overlap screen did the rejecting: far fewer full pair evaluations than tests
+    assert summ.screened_pairs > 0 and summ.evaluated_pairs < summ.screened_pairs
+    assert summ.evaluated_pairs % 1536 == 0
+
+
+def test_widom_dense_config_mixed(engine_mod):
+    s, ref, got, summ = _widom_pair(1536, 2000, 98, engine_mod.GRASPA_MIXED, engine_mod)
+    assert np.array_equal(got["ok"] == 1, ref["ok"] == 1)
+    k = ref["ok"] == 1
+    assert np.array_equal(got["first_bead"][k], ref["first_bead"][k])
+    if (ref["ok"] == 1).any():
+        _check_mixed(ref, got, min_agree=1.0)
+    assert summ.n == 2000 and summ.n_first_bead >= summ.n_ok == (ref["ok"] == 1).sum()
+
+
+def _triclinic_system(n_fw=400):
+    from graspa_amd import synth
+    s = synth.config1(n_fw=n_fw)
+    s.cell = s.cell.copy()
+    s.cell[3] = 3.5     # b tilted along x
+    s.cell[6] = -2.0    # c tilted along x
+    s.cell[7] = 1.5     # c tilted along y
+    s.setup_ewald()
+    assert not s.cubic
+    return s
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_widom_triclinic_cell(prec, engine_mod):
+    """Triclinic cell: minimum image through fractional coordinates in both
+    forms; trial positions use diag(cell) as the reference does (mc_widom.h:122-213)."""
+    P = engine_mod.GRASPA_FP64 if prec == "fp64" else engine_mod.GRASPA_MIXED
+    s, ref, got, summ = _widom_pair(0, 300, 31, P, engine_mod, system=_triclinic_system())
+    assert (ref["ok"] == 1).sum() > 100
+    if prec == "fp64":
+        assert np.array_equal(got["ok"], ref["ok"])
+        k = ref["ok"] == 1
+        assert np.array_equal(got["first_bead"][k], ref["first_bead"][k])
+        assert _close(got["energy"], ref["energy"], atol=1e-8)
+        assert _close(got["rosen"], ref["rosen"], rtol=1e-9, atol=0.0)
+    else:
+        _check_mixed(ref, got)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_widom_large_framework_streams_from_cache(prec, engine_mod):
+    """12,000 framework atoms (192 KB of coordinates) do not fit in LDS: the
+    kernel reads them through the caches instead; same results."""
+    from graspa_amd import synth
+    s = synth.config1(n_fw=12000, L=48.0)
+    P = engine_mod.GRASPA_FP64 if prec == "fp64" else engine_mod.GRASPA_MIXED
+    s, ref, got, summ = _widom_pair(0, 64, 41, P, engine_mod, system=s)
+    if prec == "fp64":
+        assert np.array_equal(got["ok"], ref["ok"])
+        assert _close(got["rosen"], ref["rosen"], rtol=1e-9, atol=1e-300)
+        assert _close(got["energy"], ref["energy"], atol=1e-7)
+    else:
+        k = ref["ok"] == 1
+        assert np.array_equal(got["first_bead"][k], ref["first_bead"][k])
+        if k.any():
+            _check_mixed(ref, got, min_agree=0.9)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_widom_with_guests(prec, engine_mod):
+    """Widom insertions into a loaded box: guest-guest pairs, the guests in the
+    overlap screen and in the energies."""
+    s = _synth(n_fw=400, n_mol=20)
+    s.n_trial_pos, s.n_trial_orient = 10, 10
+    P = engine_mod.GRASPA_FP64 if prec == "fp64" else engine_mod.GRASPA_MIXED
+    s, ref, got, summ = _widom_pair(0, 300, 51, P, engine_mod, system=s)
+    assert (ref["ok"] == 1).sum() > 50
+    if prec == "fp64":
+        assert np.array_equal(got["ok"], ref["ok"])
+        assert _close(got["energy"], ref["energy"], atol=1e-8)
+        assert _close(got["rosen"], ref["rosen"], rtol=1e-9, atol=0.0)
+    else:
+        _check_mixed(ref, got)
+    assert summ.evaluated_pairs % (400 + 60) == 0
