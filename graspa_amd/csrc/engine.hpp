@@ -91,6 +91,17 @@ struct graspa_box {
   int* d_fw_type = nullptr;
   uint4* d_fw_fix = nullptr;    // fixed-point fraction x,y,z + float q bits
   uint4* d_fw_fixT = nullptr;   // fixed-point fraction x,y,z + type
+  // Widom copy of the framework in Morton order of its screen cells, and the
+  // screen's cell table (cap slots per cell, empty slots have type ~0u)
+  double* d_fws = nullptr;      // SoA [4][n_fw] x, y, z, q
+  int* d_fws_type = nullptr;
+  uint4* d_ctab = nullptr;      // [n_cells * cap] fixed-point x,y,z + type
+  int cell_n[3] = {0, 0, 0};
+  int cell_cap = 0;
+  int use_cells = 0;
+  // real-space Coulomb table of the fast form (coulomb_table.cpp)
+  double* d_coul = nullptr;
+  int coul_n = 0;               // doubles
   std::vector<double> fw_pos, fw_q;
   std::vector<int> fw_type;
 
@@ -154,6 +165,37 @@ int launch_total_pairs(graspa_box* b, double* d_out /* blocks*4 */, int* n_block
 
 // per-insertion record buffers of the Widom kernel (engine_api.hip)
 int ensure_widom_records(graspa_box* b, int n_insert);
+
+// Squared-distance threshold below which a pair of these force-field
+// parameters certainly overlaps (energy above the criterion or r^2 < 0.01,
+// inside the VDW cutoff); used by the Widom overlap screen and its cell grid.
+__host__ __device__ inline double overlap_r2(double a4e, double s2, double shift, double overlap, double cut2)
+{
+  double thr = 0.01;
+  if (a4e > 0.0 && s2 > 0.0) {
+    const double c = (overlap + shift) / a4e;
+    if (c > 0.0) {
+      const double xp = 0.5 * (1.0 + sqrt(1.0 + 4.0 * c));
+      const double r2 = s2 / cbrt(xp);
+      thr = thr > r2 ? thr : r2;
+    }
+  }
+  return thr < cut2 ? thr : cut2;
+}
+
+// screen cell grid of the framework (engine_api.hip)
+int build_screen_cells(graspa_box* b);
+
+// Real-space Coulomb table g(r^2) = erfc(alpha r)/r of the fast pair form
+// (coulomb_table.cpp): kCoulTabM intervals per octave of r^2 from
+// 2^kCoulTabELo, a degree-kCoulTabDeg polynomial in r^2 - r_i^2 per interval.
+constexpr int kCoulTabELo = -7;
+constexpr int kCoulTabMBits = 6;
+constexpr int kCoulTabM = 1 << kCoulTabMBits;
+constexpr int kCoulTabDeg = 5;
+constexpr int kCoulTabStride = 6;    // doubles per interval
+int coulomb_table_octaves(double cut2);
+void build_coulomb_table(double alpha, double cut2, std::vector<double>& tab);
 
 // widom_kernel.hip
 int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,

@@ -199,6 +199,101 @@ int graspa::ensure_widom_records(graspa_box* b, int n_insert)
   return rc;
 }
 
+static uint32_t spread3(uint32_t v)
+{
+  uint32_t r = 0;
+  for (int i = 0; i < 10; ++i) r |= ((v >> i) & 1u) << (3 * i);
+  return r;
+}
+
+// Cell grid of the Widom overlap screen.  Cells are at least as wide
+// (perpendicular to each face, in fractional coordinates) as the largest
+// overlap distance of any type pair, so every atom that can overlap a point
+// lies in the 27 cells around the point's cell: testing those is a complete
+// overlap test.  The table holds cap slots per cell (fixed-point x,y,z +
+// type, empty slots typed ~0u).  The Widom kernel's FP64 copy of the
+// framework is sorted by the Morton code of the atoms' cells, so that 64
+// consecutive atoms are close in space.  Without a usable grid (fewer than
+// three cells along an axis, or more than kCellCapMax atoms in a cell) the
+// screen scans all atoms.
+int graspa::build_screen_cells(graspa_box* b)
+{
+  constexpr int kCellCapMax = 4;
+  hipFree(b->d_fws); hipFree(b->d_fws_type); hipFree(b->d_ctab);
+  b->d_fws = nullptr; b->d_fws_type = nullptr; b->d_ctab = nullptr;
+  b->use_cells = 0;
+  b->cell_cap = 0;
+  b->cell_n[0] = b->cell_n[1] = b->cell_n[2] = 0;
+  const BoxConst& bc = b->bc;
+  const int n = b->n_fw, T = bc.ntypes;
+  double thr_max = 0.01;
+  for (int i = 0; i < T * T; ++i)
+    thr_max = std::max(thr_max, overlap_r2(4.0 * b->eps[i], b->sig[i] * b->sig[i], b->shift[i], bc.overlap, bc.cut_vdw2));
+  const double d = std::sqrt(thr_max) * (1.0 + 1e-3);
+  const double* c = bc.cell;
+  auto cross_norm = [](const double* u, const double* v) {
+    const double x = u[1] * v[2] - u[2] * v[1], y = u[2] * v[0] - u[0] * v[2], z = u[0] * v[1] - u[1] * v[0];
+    return std::sqrt(x * x + y * y + z * z);
+  };
+  const double w[3] = {b->volume / cross_norm(c + 3, c + 6), b->volume / cross_norm(c + 6, c), b->volume / cross_norm(c, c + 3)};
+  int nc[3];
+  bool grid_ok = n > 0;
+  for (int a = 0; a < 3; ++a) {
+    nc[a] = (int)std::min(64.0, std::floor(w[a] / d));
+    if (nc[a] < 3) grid_ok = false;
+  }
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::vector<int> cell_of(n, 0);
+  std::vector<uint32_t> key(n, 0);
+  std::vector<uint32_t> fx(3 * (size_t)std::max(n, 1));
+  for (int i = 0; i < n; ++i) fixed_from_cart(bc, &b->fw_pos[3 * i], &fx[3 * i]);
+  int cap = 0;
+  if (grid_ok) {
+    std::vector<int> count((size_t)nc[0] * nc[1] * nc[2], 0);
+    for (int i = 0; i < n; ++i) {
+      uint32_t ci[3];
+      for (int a = 0; a < 3; ++a) ci[a] = (uint32_t)(((uint64_t)fx[3 * i + a] * (uint64_t)nc[a]) >> 32);
+      cell_of[i] = (int)(ci[0] + nc[0] * (ci[1] + nc[1] * ci[2]));
+      key[i] = spread3(ci[0]) | (spread3(ci[1]) << 1) | (spread3(ci[2]) << 2);
+      cap = std::max(cap, ++count[cell_of[i]]);
+    }
+    if (cap > kCellCapMax) grid_ok = false;
+  }
+  if (grid_ok) std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key[x] < key[y]; });
+  const size_t m = (size_t)std::max(n, 1);
+  std::vector<double> soa(4 * m, 0.0);
+  std::vector<int> ty(m, 0);
+  for (int k = 0; k < n; ++k) {
+    const int i = order[k];
+    soa[k] = b->fw_pos[3 * i];
+    soa[n + k] = b->fw_pos[3 * i + 1];
+    soa[2 * n + k] = b->fw_pos[3 * i + 2];
+    soa[3 * n + k] = b->fw_q[i];
+    ty[k] = b->fw_type[i];
+  }
+  int rc = hip_check(hipMalloc(&b->d_fws, 4 * m * sizeof(double)), "hipMalloc fw");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_fws_type, m * sizeof(int)), "hipMalloc fw");
+  rc = rc ? rc : hip_check(hipMemcpy(b->d_fws, soa.data(), 4 * m * sizeof(double), hipMemcpyHostToDevice), "fw upload");
+  rc = rc ? rc : hip_check(hipMemcpy(b->d_fws_type, ty.data(), m * sizeof(int), hipMemcpyHostToDevice), "fw upload");
+  if (rc || !grid_ok) return rc;
+  const size_t ncell = (size_t)nc[0] * nc[1] * nc[2];
+  std::vector<uint4> tab(ncell * cap, make_uint4(0u, 0u, 0u, ~0u));
+  std::vector<int> fill(ncell, 0);
+  for (int k = 0; k < n; ++k) {
+    const int i = order[k];
+    const int cl = cell_of[i];
+    tab[(size_t)cl * cap + fill[cl]++] = make_uint4(fx[3 * i], fx[3 * i + 1], fx[3 * i + 2], (uint32_t)b->fw_type[i]);
+  }
+  rc = hip_check(hipMalloc(&b->d_ctab, tab.size() * sizeof(uint4)), "hipMalloc cells");
+  rc = rc ? rc : hip_check(hipMemcpy(b->d_ctab, tab.data(), tab.size() * sizeof(uint4), hipMemcpyHostToDevice), "cells upload");
+  if (rc) return rc;
+  b->use_cells = 1;
+  b->cell_cap = cap;
+  for (int a = 0; a < 3; ++a) b->cell_n[a] = nc[a];
+  return GRASPA_OK;
+}
+
 #define GRASPA_CHECK_BOX(b) \
   if (!(b)) return fail(GRASPA_ERR_ARG, "null box");
 
@@ -368,6 +463,16 @@ int graspa_box_create(const graspa_box_params* p, graspa_box** out)
                           "k upload");
     }
   }
+  {
+    // real-space Coulomb table of the fast form
+    std::vector<double> ct(kCoulTabStride, 0.0);
+    if (bc.use_charges && bc.alpha > 0.0 && bc.cut_coul2 > 0.0) build_coulomb_table(bc.alpha, bc.cut_coul2, ct);
+    b->coul_n = (int)ct.size();
+    rc = rc ? rc : hip_check(hipMalloc(&b->d_coul, ct.size() * sizeof(double)), "hipMalloc coulomb table");
+    rc = rc ? rc
+            : hip_check(hipMemcpy(b->d_coul, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice),
+                        "coulomb table upload");
+  }
   rc = rc ? rc : hip_check(hipMalloc(&b->d_wacc, 16 * sizeof(double)), "hipMalloc widom");
   if (!rc) rc = hip_check(hipMemset(b->d_wacc, 0, 16 * sizeof(double)), "widom init");
   rc = rc ? rc : ensure_part(b, 4096);
@@ -379,6 +484,7 @@ int graspa_box_create(const graspa_box_params* p, graspa_box** out)
     rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_type, sizeof(int)), "hipMalloc fw");
     rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_fix, sizeof(uint4)), "hipMalloc fw");
     rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_fixT, sizeof(uint4)), "hipMalloc fw");
+    rc = rc ? rc : build_screen_cells(b);
   }
   if (rc) {
     graspa_box_destroy(b);
@@ -396,9 +502,10 @@ void graspa_box_destroy(graspa_box* b)
   timing_clear(b);
   hipFree(b->d_ff64); hipFree(b->d_ff32);
   hipFree(b->d_fw); hipFree(b->d_fw_type); hipFree(b->d_fw_fix); hipFree(b->d_fw_fixT);
+  hipFree(b->d_fws); hipFree(b->d_fws_type); hipFree(b->d_ctab);
   hipFree(b->d_g); hipFree(b->d_g_type); hipFree(b->d_g_molid); hipFree(b->d_g_comp); hipFree(b->d_g_fix);
   hipFree(b->d_ktemp); hipFree(b->d_sk_fw); hipFree(b->d_sk_ads); hipFree(b->d_sk_stage);
-  hipFree(b->d_kact); hipFree(b->d_ktemp_act);
+  hipFree(b->d_kact); hipFree(b->d_ktemp_act); hipFree(b->d_coul);
   hipFree(b->d_part);
   if (b->h_part) hipHostFree(b->h_part);
   hipFree(b->d_wacc); hipFree(b->d_wres);
@@ -446,6 +553,7 @@ int graspa_box_set_framework(graspa_box* b, int n, const double* xyz, const doub
   rc = rc ? rc : hip_check(hipMemcpy(b->d_fw, soa.data(), 4 * m * sizeof(double), hipMemcpyHostToDevice), "fw upload");
   if (!rc && n > 0) rc = hip_check(hipMemcpy(b->d_fw_type, type, n * sizeof(int), hipMemcpyHostToDevice), "fw upload");
   rc = rc ? rc : hip_check(hipMemcpy(b->d_fw_fix, fix.data(), m * sizeof(uint4), hipMemcpyHostToDevice), "fw upload");
+  rc = rc ? rc : build_screen_cells(b);
   return rc;
 }
 

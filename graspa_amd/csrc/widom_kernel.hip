@@ -4,16 +4,16 @@
 // One wavefront carries one insertion at a time; waves take insertions from
 // an atomic counter, so the work balances itself and a result depends only on
 // the insertion's index (counter-based random numbers).  A workgroup stages
-// the framework's fixed-point coordinates and types in LDS once; after that
-// the insertion loop has no workgroup barrier.  One insertion:
+// the overlap screen's cell table in LDS once (when it fits); after that the
+// insertion loop has no workgroup barrier.  One insertion:
 //   1. all its random draws at once, one Philox4x32-10 block per lane;
 //   2. K first-bead trial positions (per-wave LDS);
-//   3. overlap screen: FP32 squared distances from the fixed-point
-//      coordinates against per-(site type, atom type) thresholds r_ov^2 below
-//      which the pair energy certainly exceeds OverlapCriteria (or r^2 < 0.01);
-//      a trial with such a pair is rejected without computing its energies,
-//      the screen of a trial ends at its first hit, and the pass ends when all
-//      trials are rejected;
+//   3. overlap screen: FP32 squared distances from fixed-point coordinates
+//      against per-(site type, atom type) thresholds r_ov^2 below which the
+//      pair energy certainly exceeds OverlapCriteria (or r^2 < 0.01), over the
+//      atoms of the 27 grid cells around the trial site (cells at least r_ov
+//      wide, so this finds every overlapping atom); a trial with a hit is
+//      rejected without computing its energies;
 //   4. full pair energies (LJ + real-space Coulomb, FP64 accumulation) of the
 //      trials the screen left, which also settles their overlap flags exactly;
 //   5. log-sum-exp selection (one uniform), W_fb;
@@ -58,9 +58,14 @@ constexpr size_t kWaveArea = sizeof(WaveLds);
 
 struct WidomArgs {
   BoxConst bc;
-  const double* fw;        // SoA x,y,z,q [4][n_fw]
-  const uint4* fw_fixT;    // fixed-point x,y,z + type
+  const double* fw;        // SoA x,y,z,q [4][n_fw], Morton order of the screen cells
+  const int* fw_type;      // same order
+  const uint4* fw_fixT;    // fixed-point x,y,z + type (scan without a cell grid)
   int n_fw;
+  int use_cells;
+  int cell_n[3];
+  int cell_cap;
+  const uint4* ctab;       // [cells * cell_cap] fixed-point x,y,z + type, ~0u = empty
   const double* g;         // guests SoA x,y,z,q [4][guest_cap]
   const int* g_type;
   const int* g_molid;
@@ -88,8 +93,10 @@ struct WidomArgs {
   const double2* sk_ads;
   const double2* sk_fw;
   int kstride;              // max kmax + 1
+  const double* coul;       // real-space Coulomb table (fast form)
+  int coul_n;               // doubles
   // LDS layout (bytes from the start of the dynamic area)
-  int lds_thr, lds_fw, lds_wave, wave_bytes;
+  int lds_thr, lds_coul, lds_tab, lds_wave, wave_bytes;
   graspa_widom_result* res;   // may be null
   double* rec_w;              // [n_insert] W
   double* rec_e;              // [n_insert] dE
@@ -142,30 +149,34 @@ __device__ inline double2 cmul2(double2 a, double2 b)
 // type) pair: E_LJ(r^2) = 4eps (x^2 - x) - shift with x = (sigma^2/r^2)^3
 // exceeds OverlapCriteria exactly when x > (1 + sqrt(1 + 4c))/2,
 // c = (OverlapCriteria + shift)/(4 eps) > 0, i.e. r^2 < sigma^2 / x^(1/3); the
-// reference also flags r^2 < 0.01; both only inside the VDW cutoff.  The
-// returned FP32 value is lowered by 1e-5 relative so that a screen hit is a
-// certain overlap in both precision forms; pairs near the threshold are left
-// to the exact evaluation.
+// reference also flags r^2 < 0.01; both only inside the VDW cutoff
+// (overlap_r2).  The FP32 value is lowered by 1e-5 relative so that a screen
+// hit is a certain overlap in both precision forms; pairs near the threshold
+// are left to the exact evaluation.
 __device__ inline float screen_threshold(const double* f, double overlap, double cut2)
 {
-  double thr = 0.01;
-  const double a4e = f[0], s2 = f[1], shift = f[2];
-  if (a4e > 0.0 && s2 > 0.0) {
-    const double c = (overlap + shift) / a4e;
-    if (c > 0.0) {
-      const double xp = 0.5 * (1.0 + sqrt(1.0 + 4.0 * c));
-      thr = fmax(thr, s2 / cbrt(xp));
-    }
-  }
-  thr = fmin(thr, cut2);
-  return (float)(thr * (1.0 - 1.0e-5));
+  return (float)(overlap_r2(f[0], f[1], f[2], overlap, cut2) * (1.0 - 1.0e-5));
+}
+
+// Tests of trial site P (molecule site `site`) against one candidate atom per
+// lane; true on a lane whose atom certainly overlaps the site.
+template <bool CUBIC>
+__device__ inline bool screen_hit(const BoxConst& bc, const float* __restrict__ thr, const int T, const int site,
+                                  const uint4 at, const bool valid, const uint4 P)
+{
+  const float th = valid ? thr[site * T + (int)at.w] : -1.0f;
+  const float rr = r2_fixed32<CUBIC>(at.x, at.y, at.z, P.x, P.y, P.z, bc.h32);
+  return rr < th;
 }
 
 // Overlap screen of n_trials trials x n_sites sites (trial site p = a *
-// n_trials + t in W.pfix, molecule site site0 + a).  Returns the mask of
-// trials with a certain overlap.
+// n_trials + t in W.pfix, molecule site site0 + a).  With the cell grid, a
+// trial site is tested against the atoms of the 27 cells around its own cell,
+// which hold every atom that can overlap it; otherwise against all atoms.
+// Guests are always scanned.  Returns the mask of trials with a certain
+// overlap.
 template <bool CUBIC>
-__device__ inline uint64_t screen(const WidomArgs& A, const uint4* __restrict__ fwl, const float* __restrict__ thr,
+__device__ inline uint64_t screen(const WidomArgs& A, const uint4* __restrict__ tab, const float* __restrict__ thr,
                                   const WaveLds& W, const int n_trials, const int n_sites, const int site0,
                                   unsigned long long& stat)
 {
@@ -174,19 +185,71 @@ __device__ inline uint64_t screen(const WidomArgs& A, const uint4* __restrict__ 
   const int lane = threadIdx.x & 63;
   const uint64_t all = (n_trials >= 64) ? ~0ull : ((1ull << n_trials) - 1ull);
   uint64_t alive = all;
-  for (int c0 = 0; c0 < A.n_fw && alive; c0 += 64) {
-    const int j = c0 + lane;
-    const bool valid = j < A.n_fw;
-    const uint4 f = valid ? fwl[j] : make_uint4(0u, 0u, 0u, 0u);
-    const unsigned nval = (unsigned)min(64, A.n_fw - c0);
+  if (A.use_cells) {
+    // this lane's (neighbour cell, slot) in each of the two lane groups
+    const int C = A.cell_cap, nslot = 27 * C;
+    int off[2][3], slot[2];
+    bool lv[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int l = g * 64 + lane;
+      lv[g] = l < nslot;
+      const int nb = lv[g] ? l / C : 0;
+      slot[g] = lv[g] ? l - nb * C : 0;
+      off[g][0] = nb % 3 - 1;
+      off[g][1] = (nb / 3) % 3 - 1;
+      off[g][2] = nb / 9 - 1;
+    }
+    const int ngroups = nslot > 64 ? 2 : 1;
+    const int n0 = A.cell_n[0], n1 = A.cell_n[1], n2 = A.cell_n[2];
+    // four trial sites per step, so that their LDS gathers are in flight together
+    constexpr int kStep = 4;
     for (int a = 0; a < n_sites; ++a) {
-      const float th = valid ? thr[(site0 + a) * T + (int)f.w] : -1.0f;
-      for (int t = 0; t < n_trials; ++t) {
-        if (!((alive >> t) & 1ull)) continue;
-        const uint4 p = W.pfix[a * n_trials + t];
-        const float rr = r2_fixed32<CUBIC>(f.x, f.y, f.z, p.x, p.y, p.z, bc.h32);
-        stat += nval;
-        if (__any(rr < th)) alive &= ~(1ull << t);
+      for (int t0 = 0; t0 < n_trials; t0 += kStep) {
+        if (!((alive >> t0) & 0xFull)) continue;
+        uint4 P[kStep];
+        int cell[kStep][3];
+#pragma unroll
+        for (int g = 0; g < kStep; ++g) {
+          P[g] = W.pfix[a * n_trials + min(t0 + g, n_trials - 1)];
+          cell[g][0] = (int)(((uint64_t)P[g].x * (uint64_t)n0) >> 32);
+          cell[g][1] = (int)(((uint64_t)P[g].y * (uint64_t)n1) >> 32);
+          cell[g][2] = (int)(((uint64_t)P[g].z * (uint64_t)n2) >> 32);
+        }
+        for (int gg = 0; gg < ngroups; ++gg) {
+          uint4 at[kStep];
+#pragma unroll
+          for (int g = 0; g < kStep; ++g) {
+            int x = cell[g][0] + off[gg][0], y = cell[g][1] + off[gg][1], z = cell[g][2] + off[gg][2];
+            x = x < 0 ? x + n0 : (x >= n0 ? x - n0 : x);
+            y = y < 0 ? y + n1 : (y >= n1 ? y - n1 : y);
+            z = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
+            at[g] = tab[(x + n0 * (y + n1 * z)) * C + slot[gg]];
+          }
+#pragma unroll
+          for (int g = 0; g < kStep; ++g) {
+            const int t = t0 + g;
+            const bool act = t < n_trials && ((alive >> t) & 1ull);
+            const bool valid = lv[gg] && at[g].w != ~0u;
+            if (act) stat += (unsigned long long)__popcll(__ballot(valid));
+            if (__any(act && screen_hit<CUBIC>(bc, thr, T, site0 + a, at[g], valid, P[g]))) alive &= ~(1ull << t);
+          }
+        }
+      }
+    }
+  } else {
+    for (int c0 = 0; c0 < A.n_fw && alive; c0 += 64) {
+      const int j = c0 + lane;
+      const bool valid = j < A.n_fw;
+      const uint4 f = valid ? A.fw_fixT[j] : make_uint4(0u, 0u, 0u, 0u);
+      const unsigned nval = (unsigned)min(64, A.n_fw - c0);
+      for (int a = 0; a < n_sites; ++a) {
+        for (int t = 0; t < n_trials; ++t) {
+          if (!((alive >> t) & 1ull)) continue;
+          stat += nval;
+          if (__any(screen_hit<CUBIC>(bc, thr, T, site0 + a, f, valid, W.pfix[a * n_trials + t])))
+            alive &= ~(1ull << t);
+        }
       }
     }
   }
@@ -197,22 +260,19 @@ __device__ inline uint64_t screen(const WidomArgs& A, const uint4* __restrict__ 
       const int jj = c0 + lane;
       bool valid = jj < n;
       uint4 f = make_uint4(0u, 0u, 0u, 0u);
-      int ty = 0;
       if (valid) {
         const int j = base + jj;
         if (c == A.comp && A.g_molid[j] == A.new_molid) valid = false;
         f = A.g_fix[j];
-        ty = A.g_type[j];
+        f.w = (uint32_t)A.g_type[j];
       }
       const unsigned nval = (unsigned)min(64, n - c0);
       for (int a = 0; a < n_sites; ++a) {
-        const float th = valid ? thr[(site0 + a) * T + ty] : -1.0f;
         for (int t = 0; t < n_trials; ++t) {
           if (!((alive >> t) & 1ull)) continue;
-          const uint4 p = W.pfix[a * n_trials + t];
-          const float rr = r2_fixed32<CUBIC>(f.x, f.y, f.z, p.x, p.y, p.z, bc.h32);
           stat += nval;
-          if (__any(rr < th)) alive &= ~(1ull << t);
+          if (__any(screen_hit<CUBIC>(bc, thr, T, site0 + a, f, valid, W.pfix[a * n_trials + t])))
+            alive &= ~(1ull << t);
         }
       }
     }
@@ -220,27 +280,51 @@ __device__ inline uint64_t screen(const WidomArgs& A, const uint4* __restrict__ 
   return all & ~alive;
 }
 
+// erfc(alpha r)/r from the per-box table (coulomb_table.cpp): the interval is
+// read from the exponent and top mantissa bits of r^2 as FP32, the value is a
+// degree-5 polynomial in r^2 - s_i.  Clamped indices only occur for r^2 below
+// 2^-7, i.e. pairs the overlap rule already rejects.
+__device__ inline double coul_table(const double2* __restrict__ ct, const int nint, const double rr)
+{
+  constexpr int kShift = 23 - kCoulTabMBits;
+  const uint32_t bits = __float_as_uint((float)rr);
+  int idx = (int)(bits >> kShift) - ((127 + kCoulTabELo) << kCoulTabMBits);
+  idx = min(max(idx, 0), nint - 1);
+  const double si = (double)__uint_as_float(bits & ~((1u << kShift) - 1u));
+  const double d = rr - si;
+  const double2* c = ct + 3 * idx;
+  const double2 c01 = c[0], c23 = c[1], c45 = c[2];
+  double g = fma(c45.y, d, c45.x);
+  g = fma(g, d, c23.y);
+  g = fma(g, d, c23.x);
+  g = fma(g, d, c01.y);
+  return fma(g, d, c01.x);
+}
+
 // Full energies of trial site p (molecule site `site`) against the framework
 // and the guests: wave sums of LJ and real-space Coulomb, and the exact
-// overlap flag (VDW_Coulomb.cu:1183-1352 for one trial site).
+// overlap flag (VDW_Coulomb.cu:1183-1352 for one trial site).  The framework
+// is read in the Morton order of the screen cells, so the cutoff branches of
+// a wave's 64 atoms mostly agree.
 template <bool MIXED, bool CUBIC>
-__device__ inline void eval_site(const WidomArgs& A, const uint4* __restrict__ fwl, const double4* __restrict__ ffl,
+__device__ inline void eval_site(const WidomArgs& A, const double4* __restrict__ ffl, const double2* __restrict__ ct,
                                  const WaveLds& W, const int p, const int site, double& sv, double& sr, bool& ov,
                                  unsigned long long& stat)
 {
   const BoxConst& bc = A.bc;
+  const int nint = A.coul_n / kCoulTabStride;
   const int T = bc.ntypes;
   const int lane = threadIdx.x & 63;
   const double sq = A.site_q[site];
   const bool coul = bc.use_charges && sq != 0.0;
   const double px = W.pos[p][0], py = W.pos[p][1], pz = W.pos[p][2];
   const double4* fr = ffl + site * T;
+  const double* fx = A.fw;
   const double* fq = A.fw + 3 * (size_t)A.n_fw;
   double v = 0.0, r = 0.0;
   bool o = false;
-  for (int j = lane; j < A.n_fw; j += 64) {
-    const int ty = (int)fwl[j].w;
-    double dx = A.fw[j] - px, dy = A.fw[A.n_fw + j] - py, dz = A.fw[2 * A.n_fw + j] - pz;
+  auto pair = [&](double ax, double ay, double az, double aq, int ty, double& v_, double& r_, bool& o_) {
+    double dx = ax - px, dy = ay - py, dz = az - pz;
     if (MIXED)
       pbc_rint<CUBIC>(dx, dy, dz, bc.cell, bc.inv);
     else
@@ -249,18 +333,47 @@ __device__ inline void eval_site(const WidomArgs& A, const uint4* __restrict__ f
     if (rr < bc.cut_vdw2) {
       const double4 ff = fr[ty];
       const double e = MIXED ? lj_mixed(rr, ff.x, ff.y, ff.z) : lj_fp64(rr, ff.x, ff.y, ff.z);
-      o = o || (e > bc.overlap) || (rr < 0.01);
-      v += e;
+      o_ = o_ || (e > bc.overlap) || (rr < 0.01);
+      v_ += e;
     }
     if (coul && rr < bc.cut_coul2) {
       if (MIXED) {
-        r += coulomb_mixed(rr, bc.prefactor * fq[j] * sq, bc.alpha);
+        r_ += (bc.prefactor * aq * sq) * coul_table(ct, nint, rr);
       } else {
         const double rd = sqrt(rr);
-        r += bc.prefactor * 1.0 * (fq[j] * sq * erfc(bc.alpha * rd)) / rd;
+        r_ += bc.prefactor * 1.0 * (aq * sq * erfc(bc.alpha * rd)) / rd;
       }
     }
+  };
+  // Two atoms per lane per iteration (independent chains), and the next pair
+  // of atoms is loaded one iteration ahead: the framework stream does not fit
+  // the L1 cache, so its loads are served from L2.
+  const int n = A.n_fw;
+  double nx0 = 0.0, ny0 = 0.0, nz0 = 0.0, nq0 = 0.0, nx1 = 0.0, ny1 = 0.0, nz1 = 0.0, nq1 = 0.0;
+  int nt0 = 0, nt1 = 0;
+  if (lane < n) { nx0 = fx[lane]; ny0 = fx[n + lane]; nz0 = fx[2 * n + lane]; nq0 = fq[lane]; nt0 = A.fw_type[lane]; }
+  if (lane + 64 < n) {
+    const int k = lane + 64;
+    nx1 = fx[k]; ny1 = fx[n + k]; nz1 = fx[2 * n + k]; nq1 = fq[k]; nt1 = A.fw_type[k];
   }
+  double v1 = 0.0, r1 = 0.0;
+  bool o1 = false;
+  for (int j = lane; j < n; j += 128) {
+    const double ax0 = nx0, ay0 = ny0, az0 = nz0, aq0 = nq0, ax1 = nx1, ay1 = ny1, az1 = nz1, aq1 = nq1;
+    const int ty0 = nt0, ty1 = nt1;
+    const bool has1 = j + 64 < n;
+    const int jn = j + 128;
+    if (jn < n) { nx0 = fx[jn]; ny0 = fx[n + jn]; nz0 = fx[2 * n + jn]; nq0 = fq[jn]; nt0 = A.fw_type[jn]; }
+    if (jn + 64 < n) {
+      const int k = jn + 64;
+      nx1 = fx[k]; ny1 = fx[n + k]; nz1 = fx[2 * n + k]; nq1 = fq[k]; nt1 = A.fw_type[k];
+    }
+    pair(ax0, ay0, az0, aq0, ty0, v, r, o);
+    if (has1) pair(ax1, ay1, az1, aq1, ty1, v1, r1, o1);
+  }
+  v += v1;
+  r += r1;
+  o = o || o1;
   unsigned long long npair = (unsigned long long)A.n_fw;
   for (int c = 1; c < A.gv.n_comp; ++c) {
     const int n = A.gv.seg[c].n_atoms, base = A.gv.seg[c].base;
@@ -284,7 +397,7 @@ __device__ inline void eval_site(const WidomArgs& A, const uint4* __restrict__ f
       }
       if (coul && rr < bc.cut_coul2) {
         if (MIXED) {
-          r += coulomb_mixed(rr, bc.prefactor * gq * sq, bc.alpha);
+          r += (bc.prefactor * gq * sq) * coul_table(ct, nint, rr);
         } else {
           const double rd = sqrt(rr);
           r += bc.prefactor * 1.0 * (gq * sq * erfc(bc.alpha * rd)) / rd;
@@ -355,7 +468,7 @@ __device__ inline void wave_select(double ev, double er, uint64_t ovm, int n, do
   W_out = W;
 }
 
-template <bool MIXED, bool FW_LDS, bool CUBIC>
+template <bool MIXED, bool TAB_LDS, bool CUBIC>
 __global__ __launch_bounds__(kMaxWaves * 64) void widom_kernel(const WidomArgs A)
 {
   extern __shared__ __align__(16) char dyn[];
@@ -371,13 +484,19 @@ __global__ __launch_bounds__(kMaxWaves * 64) void widom_kernel(const WidomArgs A
     ffl[i] = make_double4(f[0], f[1], f[2], 0.0);
     thr[i] = screen_threshold(f, bc.overlap, bc.cut_vdw2);
   }
-  const uint4* fwl;
-  if (FW_LDS) {
-    uint4* s = reinterpret_cast<uint4*>(dyn + A.lds_fw);
-    for (int i = threadIdx.x; i < A.n_fw; i += blockDim.x) s[i] = A.fw_fixT[i];
-    fwl = s;
+  // real-space Coulomb table of the fast form
+  double2* ctl = reinterpret_cast<double2*>(dyn + A.lds_coul);
+  if (MIXED)
+    for (int i = threadIdx.x; i < A.coul_n / 2; i += blockDim.x) ctl[i] = reinterpret_cast<const double2*>(A.coul)[i];
+  // the screen's cell table, in LDS when it fits
+  const uint4* tab;
+  if (TAB_LDS) {
+    uint4* s = reinterpret_cast<uint4*>(dyn + A.lds_tab);
+    const int n = A.cell_n[0] * A.cell_n[1] * A.cell_n[2] * A.cell_cap;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = A.ctab[i];
+    tab = s;
   } else {
-    fwl = A.fw_fixT;
+    tab = A.ctab;
   }
   __syncthreads();
 
@@ -420,13 +539,13 @@ __global__ __launch_bounds__(kMaxWaves * 64) void widom_kernel(const WidomArgs A
     }
     wave_sync();
     // 3.-5. first bead
-    uint64_t ovm = screen<CUBIC>(A, fwl, thr, W, K, 1, 0, st_screen);
+    uint64_t ovm = screen<CUBIC>(A, tab, thr, W, K, 1, 0, st_screen);
     double ev = 0.0, er = 0.0;
     for (int t = 0; t < K; ++t) {
       if ((ovm >> t) & 1ull) continue;
       double sv, sr;
       bool o;
-      eval_site<MIXED, CUBIC>(A, fwl, ffl, W, t, 0, sv, sr, o, st_full);
+      eval_site<MIXED, CUBIC>(A, ffl, ctl, W, t, 0, sv, sr, o, st_full);
       if (o) ovm |= 1ull << t;
       if (lane == t) { ev = sv; er = sr; }
     }
@@ -461,7 +580,7 @@ __global__ __launch_bounds__(kMaxWaves * 64) void widom_kernel(const WidomArgs A
         }
       }
       wave_sync();
-      uint64_t ovo = screen<CUBIC>(A, fwl, thr, W, O, chain, 1, st_screen);
+      uint64_t ovo = screen<CUBIC>(A, tab, thr, W, O, chain, 1, st_screen);
       double cv = 0.0, cr = 0.0;
       for (int o = 0; o < O; ++o) {
         if ((ovo >> o) & 1ull) continue;
@@ -470,7 +589,7 @@ __global__ __launch_bounds__(kMaxWaves * 64) void widom_kernel(const WidomArgs A
         for (int a = 0; a < chain && !dead; ++a) {
           double sv, sr;
           bool ob;
-          eval_site<MIXED, CUBIC>(A, fwl, ffl, W, a * O + o, 1 + a, sv, sr, ob, st_full);
+          eval_site<MIXED, CUBIC>(A, ffl, ctl, W, a * O + o, 1 + a, sv, sr, ob, st_full);
           tv += sv;
           tr += sr;
           dead = ob;
@@ -633,11 +752,11 @@ __global__ __launch_bounds__(kRedBlock) void widom_final_kernel(const double* pa
 }
 
 // Waves per workgroup and workgroups per CU for one instantiation: the most
-// resident waves, ties to the larger workgroup (fewer framework copies).
-template <bool MIXED, bool FW_LDS, bool CUBIC>
+// resident waves, ties to the larger workgroup (fewer cell-table copies).
+template <bool MIXED, bool TAB_LDS, bool CUBIC>
 static int launch_one(graspa_box* b, const WidomArgs& A0, size_t fixed_lds, size_t wave_bytes)
 {
-  auto kern = widom_kernel<MIXED, FW_LDS, CUBIC>;
+  auto kern = widom_kernel<MIXED, TAB_LDS, CUBIC>;
   int best_nw = 0, best_blocks = 0, best_waves = 0;
   size_t best_lds = 0;
   for (int nw : {4, kMaxWaves}) {
@@ -672,9 +791,9 @@ static int launch_one(graspa_box* b, const WidomArgs& A0, size_t fixed_lds, size
 }
 
 template <bool MIXED>
-static int launch_mixed(graspa_box* b, const WidomArgs& A, bool fw_lds, size_t fixed_lds, size_t wave_bytes)
+static int launch_mixed(graspa_box* b, const WidomArgs& A, bool tab_lds, size_t fixed_lds, size_t wave_bytes)
 {
-  if (fw_lds)
+  if (tab_lds)
     return A.bc.cubic ? launch_one<MIXED, true, true>(b, A, fixed_lds, wave_bytes)
                       : launch_one<MIXED, true, false>(b, A, fixed_lds, wave_bytes);
   return A.bc.cubic ? launch_one<MIXED, false, true>(b, A, fixed_lds, wave_bytes)
@@ -700,7 +819,11 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   if (ensure_widom_records(b, n_insert)) return GRASPA_ERR_HIP;
   WidomArgs A{};
   A.bc = b->bc;
-  A.fw = b->d_fw; A.fw_fixT = b->d_fw_fixT; A.n_fw = b->n_fw;
+  A.fw = b->d_fws; A.fw_type = b->d_fws_type; A.fw_fixT = b->d_fw_fixT; A.n_fw = b->n_fw;
+  A.use_cells = b->use_cells;
+  for (int d = 0; d < 3; ++d) A.cell_n[d] = b->cell_n[d];
+  A.cell_cap = b->cell_cap;
+  A.ctab = b->d_ctab;
   A.g = b->d_g; A.g_type = b->d_g_type; A.g_molid = b->d_g_molid; A.g_fix = b->d_g_fix; A.guest_cap = b->guest_cap;
   guest_view(b, &A.gv);
   A.ff64 = b->d_ff64;
@@ -728,29 +851,34 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   A.rec_w = b->d_wrec_w; A.rec_e = b->d_wrec_e; A.rec_f = b->d_wrec_f;
   A.ctr = reinterpret_cast<unsigned*>(b->d_wacc + 12);
   A.stat = reinterpret_cast<unsigned long long*>(b->d_wacc + 8);
-  // LDS layout: FF rows | thresholds | framework (optional) | per-wave areas
+  A.coul = b->d_coul;
+  A.coul_n = b->coul_n;
+  // LDS layout: FF rows | thresholds | Coulomb table (fast form) | cell table (optional) | per-wave areas
   const int T = A.bc.ntypes, M = A.M;
   size_t off = align16((size_t)M * T * sizeof(double4));
   A.lds_thr = (int)off;
   off = align16(off + (size_t)M * T * sizeof(float));
+  A.lds_coul = (int)off;
+  if (b->precision == GRASPA_MIXED) off = align16(off + (size_t)b->coul_n * sizeof(double));
   const size_t eik_bytes = (size_t)M * 3 * A.kstride * sizeof(double2);
   const size_t wave_bytes = align16(std::max(kWaveArea, eik_bytes)) + kMaxMol * 3 * sizeof(double);
   A.wave_bytes = (int)wave_bytes;
-  const size_t fw_bytes = (size_t)b->n_fw * sizeof(uint4);
-  A.lds_fw = (int)off;
+  const size_t tab_bytes =
+      b->use_cells ? (size_t)b->cell_n[0] * b->cell_n[1] * b->cell_n[2] * b->cell_cap * sizeof(uint4) : 0;
+  A.lds_tab = (int)off;
   const size_t base_lds = off;
 
   hipLaunchKernelGGL(widom_reset_kernel, dim3(1), dim3(64), 0, b->stream, A.ctr, A.stat);
   if (hip_check(hipGetLastError(), "widom_reset launch")) return GRASPA_ERR_HIP;
-  // framework in LDS when it fits beside four wave areas, else read through the caches
+  // cell table in LDS when it fits beside four wave areas, else read through the caches
   int rc = GRASPA_ERR_UNSUPPORTED;
   for (int pass = 0; pass < 2 && rc == GRASPA_ERR_UNSUPPORTED; ++pass) {
-    const bool fw_lds = pass == 0;
-    if (fw_lds && base_lds + align16(fw_bytes) + 4 * wave_bytes > (size_t)b->max_lds) continue;
-    const size_t fixed_lds = fw_lds ? base_lds + align16(fw_bytes) : base_lds;
+    const bool tab_lds = pass == 0;
+    if (tab_lds && (!b->use_cells || base_lds + align16(tab_bytes) + 4 * wave_bytes > (size_t)b->max_lds)) continue;
+    const size_t fixed_lds = tab_lds ? base_lds + align16(tab_bytes) : base_lds;
     A.lds_wave = (int)fixed_lds;
-    rc = (b->precision == GRASPA_MIXED) ? launch_mixed<true>(b, A, fw_lds, fixed_lds, wave_bytes)
-                                        : launch_mixed<false>(b, A, fw_lds, fixed_lds, wave_bytes);
+    rc = (b->precision == GRASPA_MIXED) ? launch_mixed<true>(b, A, tab_lds, fixed_lds, wave_bytes)
+                                        : launch_mixed<false>(b, A, tab_lds, fixed_lds, wave_bytes);
   }
   if (rc) return rc;
   const int nblk = (n_insert + kRedBlock * kRedPer - 1) / (kRedBlock * kRedPer);

@@ -125,6 +125,7 @@ _SIGS = {
     "graspa_timing_read": (C.c_int, [C.c_void_p, _f64p, _i64p]),
     "graspa_box_sync": (C.c_int, [C.c_void_p]),
     "graspa_glibc_rand_sequence": (C.c_int, [C.c_uint, C.c_int, _i32p]),
+    "graspa_coulomb_table": (C.c_int, [C.c_double, C.c_double, _f64p, C.c_int]),
 }
 
 

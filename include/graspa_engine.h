@@ -292,6 +292,14 @@ int  graspa_box_sync(graspa_box* box);
  * its test against the C library): srand(seed) then n draws of rand(). */
 int  graspa_glibc_rand_sequence(unsigned seed, int n, int* out);
 
+/* Host helper with no counterpart in the reference: the table of
+ * erfc(alpha r)/r over r^2 that the fast (GRASPA_MIXED) Widom kernel uses for
+ * the real-space Coulomb term (64 intervals per octave of r^2 from 2^-7, a
+ * degree-5 polynomial in r^2 - r_i^2 each, 6 doubles per interval), exposed
+ * so that it can be checked without a GPU.  Returns the number of doubles
+ * written, the number needed when cap is too small, or -1 for bad arguments. */
+int  graspa_coulomb_table(double alpha, double cut2, double* out, int cap);
+
 #ifdef __cplusplus
 }
 #endif

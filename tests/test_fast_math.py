@@ -76,6 +76,61 @@ def test_erfc_fast_pins_a_dropped_term():
     assert rel.max() > 1e-10
 
 
+def _coulomb_table(alpha, cut2):
+    import ctypes as C
+    from graspa_amd import engine
+    L = engine.lib()
+    n = L.graspa_coulomb_table(alpha, cut2, None, 0)
+    assert n > 0 and n % 6 == 0
+    buf = (C.c_double * n)()
+    assert L.graspa_coulomb_table(alpha, cut2, buf, n) == n
+    return np.frombuffer(buf, dtype=np.float64).copy()
+
+
+def _coul_table_eval(tab, rr):
+    """erfc(alpha r)/r the way the fast Widom kernel reads the table: interval
+    from the exponent and top 6 mantissa bits of r^2 as FP32, degree-5
+    polynomial in r^2 - s_i."""
+    bits = rr.astype(np.float32).view(np.uint32)
+    idx = (bits >> 17).astype(np.int64) - ((127 - 7) << 6)
+    idx = np.clip(idx, 0, tab.size // 6 - 1)
+    si = (bits & np.uint32(0xFFFE0000)).view(np.float32).astype(np.float64)
+    d = rr - si
+    c = tab.reshape(-1, 6)[idx]
+    g = c[:, 5]
+    for k in (4, 3, 2, 1, 0):
+        g = g * d + c[:, k]
+    return g
+
+
+@pytest.mark.parametrize("alpha,cut", [(0.265058, 12.0), (0.285, 10.0), (0.2, 16.0)])
+def test_coulomb_table_accuracy(alpha, cut):
+    """The fast form's real-space table against scipy's erfc: absolute error
+    of erfc(alpha r)/r below 1e-13 (1e-9 internal units for the largest charge
+    products) for every r^2 the overlap rule lets through (0.01 .. cutoff^2)."""
+    tab = _coulomb_table(alpha, cut * cut)
+    rng = np.random.default_rng(3)
+    rr = np.concatenate([np.geomspace(0.01, cut * cut * (1 - 1e-12), 400001),
+                         rng.uniform(0.01, cut * cut, 200000),
+                         2.0 ** np.arange(-6, int(np.log2(cut * cut)) + 1)])   # octave edges
+    rr = rr[rr < cut * cut]
+    r = np.sqrt(rr)
+    ref = erfc(alpha * r) / r
+    got = _coul_table_eval(tab, rr)
+    err = np.abs(got - ref)
+    assert err.max() < 1e-13, err.max()
+    assert np.max(err / ref) < 1e-9
+
+
+def test_coulomb_table_pins_a_wrong_interval():
+    """Reading the neighbouring interval's polynomial breaks the bound."""
+    tab = _coulomb_table(0.265058, 144.0)
+    rr = np.geomspace(0.5, 100.0, 1001)
+    shifted = np.roll(tab.reshape(-1, 6), 1, axis=0).ravel()
+    r = np.sqrt(rr)
+    assert np.abs(_coul_table_eval(shifted, rr) - erfc(0.265058 * r) / r).max() > 1e-6
+
+
 def test_coefficient_header_is_regenerated_identically():
     out = subprocess.run([sys.executable, os.path.join(CSRC, "gen_fast_math.py")], capture_output=True, text=True,
                          check=True).stdout
