@@ -138,6 +138,16 @@ struct graspa_box {
   int* d_wrec_f = nullptr;
   double* d_wpart = nullptr;    // reduction partials
   size_t wrec_cap = 0;
+  // Widom pipeline buffers (trial masks, energies, work lists, per-insertion state)
+  void* d_wmask1 = nullptr;
+  void* d_wmask2 = nullptr;
+  void* d_we1 = nullptr;
+  void* d_we2 = nullptr;
+  void* d_wlist1 = nullptr;
+  void* d_wlist2 = nullptr;
+  void* d_wstate = nullptr;
+  void* d_wcounts = nullptr;    // sub-list lengths of the work lists
+  size_t wp_n = 0, wp_e1 = 0, wp_e2 = 0;
   graspa_widom_result* d_wres = nullptr;
   size_t wres_cap = 0;
   int n_cu = 256;

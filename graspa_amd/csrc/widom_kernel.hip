@@ -1,34 +1,36 @@
-// widom_kernel.hip -- batched Widom / CBMC test insertions in one persistent
-// launch (the throughput path of the GCMC energy engine on gfx950).
+// widom_kernel.hip -- batched Widom / CBMC test insertions (the throughput path
+// of the GCMC energy engine on gfx950).
 //
-// One wavefront carries one insertion at a time; waves take insertions from
-// an atomic counter, so the work balances itself and a result depends only on
-// the insertion's index (counter-based random numbers).  A workgroup stages
-// the overlap screen's cell table in LDS once (when it fits); after that the
-// insertion loop has no workgroup barrier.  One insertion:
-//   1. all its random draws at once, one Philox4x32-10 block per lane;
-//   2. K first-bead trial positions (per-wave LDS);
-//   3. overlap screen: FP32 squared distances from fixed-point coordinates
-//      against per-(site type, atom type) thresholds r_ov^2 below which the
-//      pair energy certainly exceeds OverlapCriteria (or r^2 < 0.01), over the
-//      atoms of the 27 grid cells around the trial site (cells at least r_ov
-//      wide, so this finds every overlapping atom); a trial with a hit is
-//      rejected without computing its energies;
-//   4. full pair energies (LJ + real-space Coulomb, FP64 accumulation) of the
-//      trials the screen left, which also settles their overlap flags exactly;
-//   5. log-sum-exp selection (one uniform), W_fb;
-//   6. O orientations of the template about the selected first bead, the same
-//      screen + evaluation over O x (M-1) sites, selection, W_ch;
-//   7. Fourier-space energy of the selected molecule in FP64 against the
-//      resident structure factors (active k-vectors only, one lane per k);
-//   8. W = W_fb W_ch exp(-beta (dE_ewald + dE_tail)); a per-insertion record.
-// Per-insertion records are summed in index order by two small kernels, so the
-// batch sums are deterministic.
+// A batch runs as a short pipeline of small kernels on the box's stream,
+// linked by compact work lists in HBM, so that each kernel keeps few registers
+// live and runs at high occupancy:
+//   A  (one wave per insertion)  K first-bead trial positions from the
+//      insertion's Philox stream; overlap screen; the surviving trial sites
+//      are appended to list 1;
+//   B  (one wave per trial site) full pair energies (LJ + real-space Coulomb,
+//      FP64 accumulation) of every site in the list, with the force-field rows,
+//      the real-space table and the framework staged in LDS; exact overlap
+//      flags;
+//   C  (one wave per insertion)  first-bead selection; O orientations of the
+//      template about the selected bead; screen; survivors to list 2;
+//   D  = B on list 2;
+//   E  (one wave per insertion)  orientation selection, Fourier-space energy
+//      of the selected molecule against the resident structure factors, W and
+//      the per-insertion record.
+// Two small kernels then sum the records in index order (deterministic sums).
+// Every random number of insertion i comes from the counter-based stream
+// (seed, stream, first_index + i), so a result depends on nothing but i.
+//
+// The overlap screen: FP32 squared distances from fixed-point coordinates
+// against per-(site type, atom type) thresholds r_ov^2 below which the pair
+// energy certainly exceeds OverlapCriteria (or r^2 < 0.01), over the atoms of
+// the 27 grid cells around the trial site (cells at least r_ov wide, so this
+// finds every overlapping atom); a trial with a hit is one the reference flags
+// (VDW_Coulomb.cu:1299-1300) and leaves out of the selection
+// (Host_sum_Widom_HGGG_SEPARATE, mc_widom.h:41-87), so its energies are not
+// computed.
 // The semantics are Insertion_Body (mc_swap_utilities.h:3-133) with
-// Widom_Move_FirstBead_PARTIAL / Widom_Move_Chain_PARTIAL (mc_widom.h:305-614);
-// a trial rejected by the screen is one the reference flags (VDW_Coulomb.cu:
-// 1299-1300) and leaves out of the selection (Host_sum_Widom_HGGG_SEPARATE,
-// mc_widom.h:41-87).
+// Widom_Move_FirstBead_PARTIAL / Widom_Move_Chain_PARTIAL (mc_widom.h:305-614).
 #include <cstring>
 
 #include "device_math.hpp"
@@ -36,25 +38,43 @@
 
 namespace graspa {
 
-constexpr int kMaxTrials = 32;   // K and O
+constexpr int kMaxTrials = 32;   // K and O (trial masks are 32-bit)
 constexpr int kMaxMol = 8;       // sites per molecule
-constexpr int kMaxPts = 64;      // trial sites of one pass: K, or O x (M-1)
-constexpr int kMaxDraws = 3 * kMaxTrials + 1 + 3 * kMaxTrials + 1;
+constexpr int kMaxPts = 64;      // trial sites of one screen: K, or O x (M-1)
 constexpr int kSums = 6;         // W, W^2, W dE, n, n first bead placed, n complete
 constexpr int kRedBlock = 256;
 constexpr int kRedPer = 8;       // records per thread in the first reduction pass
-// Waves per workgroup at most: the FP64 erfc/exp coefficients of the pair term
-// keep ~190 VGPRs live, i.e. 2 waves per SIMD; 8 waves fill a CU with one workgroup.
-constexpr int kMaxWaves = 8;
+constexpr int kInsWaves = 4;     // waves per workgroup of the per-insertion kernels
+constexpr int kEvalWaves = 16;   // waves per workgroup of the pair-energy kernel
 
-// Per-wave LDS workspace.  The Ewald phase reuses the draw/trial area for the
-// e^{ik.r} axis tables of the selected molecule.
+// Per-wave LDS workspace of the screening kernels.
+constexpr int kMaxDraws = 3 * kMaxTrials + 1 + 3 * kMaxTrials + 1;
 struct WaveLds {
-  double ud[kMaxDraws];          // this insertion's uniforms
   double pos[kMaxPts][3];        // trial sites, FP64
   uint4 pfix[kMaxPts];           // same, fixed point
+  double ud[kMaxDraws + 2];      // this insertion's uniforms
 };
-constexpr size_t kWaveArea = sizeof(WaveLds);
+// Work lists are appended per workgroup of the screening kernels: workgroup g
+// of a grid of G owns insertions [g*chunk, (g+1)*chunk) and the sub-list at
+// g*chunk*cap_per_insertion; its length goes to counts[g].
+constexpr int kMaxScreenGroups = 1024;
+
+// State of one insertion between the kernels.
+struct InsState {
+  double fb[3];                  // selected first-bead position
+  double wfb, ev, er;            // first-bead Rosenbluth weight, its LJ and real-space energies
+  int sel;                       // selected first-bead trial, -1 if none
+  int ok;                        // first bead placed
+};
+
+// One trial site to evaluate.
+struct TrialPt {
+  double x, y, z;
+  int site;                      // molecule site (0 = first bead, 1 + a = chain atom a)
+  int slot;                      // index of its energy slot
+  int ins;                       // insertion (trial-mask word)
+  int bit;                       // trial bit in that word
+};
 
 struct WidomArgs {
   BoxConst bc;
@@ -95,15 +115,49 @@ struct WidomArgs {
   int kstride;              // max kmax + 1
   const double* coul;       // real-space Coulomb table (fast form)
   int coul_n;               // doubles
-  // LDS layout (bytes from the start of the dynamic area)
-  int lds_thr, lds_coul, lds_tab, lds_wave, wave_bytes;
+  // pipeline buffers
+  uint32_t* mask1;          // [n_insert] first-bead trials with an overlap
+  double2* e1;              // [n_insert * K] first-bead (LJ, real) energies
+  TrialPt* list1;
+  InsState* state;
+  uint32_t* mask2;          // [n_insert] orientations with an overlap
+  double2* e2;              // [n_insert * O * chain] per chain site
+  TrialPt* list2;
+  unsigned* counts1;        // [groups] sub-list lengths of list 1
+  unsigned* counts2;        // [groups] of list 2
+  int groups, chunk;        // screening grid: workgroups and insertions per workgroup
+  // outputs
   graspa_widom_result* res;   // may be null
   double* rec_w;              // [n_insert] W
   double* rec_e;              // [n_insert] dE
   int* rec_f;                 // [n_insert] bit 0 first bead placed, bit 1 complete
-  unsigned* ctr;              // work counter
   unsigned long long* stat;   // screened pairs, evaluated pairs
+  // LDS layout (bytes from the start of the dynamic area)
+  int lds_thr, lds_tab, lds_wave;       // screening kernels
+  int lds_ffl, lds_coul, lds_fw;        // pair-energy kernel
 };
+
+__device__ inline double draw(const WidomArgs& A, uint64_t idx, uint32_t d)
+{
+  return philox_uniform(A.seed, A.stream, idx, d);
+}
+
+// Uniforms d0 .. d0+nd-1 of insertion idx into ud[0 .. nd), one Philox block
+// (two uniforms, philox_uniform's layout) per lane.
+__device__ inline void fill_draws(const WidomArgs& A, uint64_t idx, int d0, int nd, double* ud)
+{
+  const int lane = threadIdx.x & 63;
+  const int b0 = d0 >> 1, b1 = (d0 + nd - 1) >> 1;
+  for (int b = b0 + lane; b <= b1; b += 64) {
+    uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)b, A.stream};
+    philox4x32_10(c, (uint32_t)A.seed, (uint32_t)(A.seed >> 32));
+    const int d = 2 * b;
+    if (d >= d0 && d < d0 + nd)
+      ud[d - d0] = ((double)((((uint64_t)c[0] << 32) | c[1]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    if (d + 1 >= d0 && d + 1 < d0 + nd)
+      ud[d + 1 - d0] = ((double)((((uint64_t)c[2] << 32) | c[3]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  }
+}
 
 // Rotate_Quaternions (mc_utilities.h:423-457)
 __device__ inline void quat_rotate(double v[3], double u, double vv, double w)
@@ -123,6 +177,30 @@ __device__ inline void quat_rotate(double v[3], double u, double vv, double w)
   const double y = v[0] * r3 + v[1] * r4 + v[2] * r5;
   const double z = v[0] * r6 + v[1] * r7 + v[2] * r8;
   v[0] = x; v[1] = y; v[2] = z;
+}
+
+// First-bead trial position t: diag(cell) * U^3 (get_random_trial_position,
+// mc_widom.h:122-213), draws 3t .. 3t+2.
+__device__ inline void first_bead_pos(const WidomArgs& A, uint64_t idx, int t, double p[3])
+{
+  p[0] = A.bc.cell[0] * draw(A, idx, 3 * t);
+  p[1] = A.bc.cell[4] * draw(A, idx, 3 * t + 1);
+  p[2] = A.bc.cell[8] * draw(A, idx, 3 * t + 2);
+}
+
+// Chain atom a of orientation o about the first bead fb (get_random_trial_orientation,
+// mc_widom.h:215-303): the template vector rotated by the quaternion from draws
+// 3K+1+3o .. 3K+3+3o.
+__device__ inline void orient_pos(const WidomArgs& A, uint64_t idx, int o, int a, const double fb[3], double p[3])
+{
+  const double u = draw(A, idx, 3 * A.K + 1 + 3 * o);
+  const double v = draw(A, idx, 3 * A.K + 2 + 3 * o);
+  const double w = draw(A, idx, 3 * A.K + 3 + 3 * o);
+  double vec[3] = {A.tvec[a][0], A.tvec[a][1], A.tvec[a][2]};
+  quat_rotate(vec, u, v, w);
+  p[0] = fb[0] + vec[0];
+  p[1] = fb[1] + vec[1];
+  p[2] = fb[2] + vec[2];
 }
 
 __device__ inline uint4 to_fixed4(const BoxConst& bc, const double p[3])
@@ -158,8 +236,6 @@ __device__ inline float screen_threshold(const double* f, double overlap, double
   return (float)(overlap_r2(f[0], f[1], f[2], overlap, cut2) * (1.0 - 1.0e-5));
 }
 
-// Tests of trial site P (molecule site `site`) against one candidate atom per
-// lane; true on a lane whose atom certainly overlaps the site.
 template <bool CUBIC>
 __device__ inline bool screen_hit(const BoxConst& bc, const float* __restrict__ thr, const int T, const int site,
                                   const uint4 at, const bool valid, const uint4 P)
@@ -176,15 +252,15 @@ __device__ inline bool screen_hit(const BoxConst& bc, const float* __restrict__ 
 // Guests are always scanned.  Returns the mask of trials with a certain
 // overlap.
 template <bool CUBIC>
-__device__ inline uint64_t screen(const WidomArgs& A, const uint4* __restrict__ tab, const float* __restrict__ thr,
+__device__ inline uint32_t screen(const WidomArgs& A, const uint4* __restrict__ tab, const float* __restrict__ thr,
                                   const WaveLds& W, const int n_trials, const int n_sites, const int site0,
                                   unsigned long long& stat)
 {
   const BoxConst& bc = A.bc;
   const int T = bc.ntypes;
   const int lane = threadIdx.x & 63;
-  const uint64_t all = (n_trials >= 64) ? ~0ull : ((1ull << n_trials) - 1ull);
-  uint64_t alive = all;
+  const uint32_t all = (n_trials >= 32) ? ~0u : ((1u << n_trials) - 1u);
+  uint32_t alive = all;
   if (A.use_cells) {
     // this lane's (neighbour cell, slot) in each of the two lane groups
     const int C = A.cell_cap, nslot = 27 * C;
@@ -206,15 +282,15 @@ __device__ inline uint64_t screen(const WidomArgs& A, const uint4* __restrict__ 
     constexpr int kStep = 4;
     for (int a = 0; a < n_sites; ++a) {
       for (int t0 = 0; t0 < n_trials; t0 += kStep) {
-        if (!((alive >> t0) & 0xFull)) continue;
+        if (!((alive >> t0) & 0xFu)) continue;
         uint4 P[kStep];
         int cell[kStep][3];
 #pragma unroll
         for (int g = 0; g < kStep; ++g) {
           P[g] = W.pfix[a * n_trials + min(t0 + g, n_trials - 1)];
-          cell[g][0] = (int)(((uint64_t)P[g].x * (uint64_t)n0) >> 32);
-          cell[g][1] = (int)(((uint64_t)P[g].y * (uint64_t)n1) >> 32);
-          cell[g][2] = (int)(((uint64_t)P[g].z * (uint64_t)n2) >> 32);
+          cell[g][0] = __builtin_amdgcn_readfirstlane((int)(((uint64_t)P[g].x * (uint64_t)n0) >> 32));
+          cell[g][1] = __builtin_amdgcn_readfirstlane((int)(((uint64_t)P[g].y * (uint64_t)n1) >> 32));
+          cell[g][2] = __builtin_amdgcn_readfirstlane((int)(((uint64_t)P[g].z * (uint64_t)n2) >> 32));
         }
         for (int gg = 0; gg < ngroups; ++gg) {
           uint4 at[kStep];
@@ -224,15 +300,15 @@ __device__ inline uint64_t screen(const WidomArgs& A, const uint4* __restrict__ 
             x = x < 0 ? x + n0 : (x >= n0 ? x - n0 : x);
             y = y < 0 ? y + n1 : (y >= n1 ? y - n1 : y);
             z = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
-            at[g] = tab[(x + n0 * (y + n1 * z)) * C + slot[gg]];
+            at[g] = tab[__mul24(x + __mul24(n0, y + __mul24(n1, z)), C) + slot[gg]];
           }
 #pragma unroll
           for (int g = 0; g < kStep; ++g) {
             const int t = t0 + g;
-            const bool act = t < n_trials && ((alive >> t) & 1ull);
+            const bool act = t < n_trials && ((alive >> t) & 1u);
             const bool valid = lv[gg] && at[g].w != ~0u;
             if (act) stat += (unsigned long long)__popcll(__ballot(valid));
-            if (__any(act && screen_hit<CUBIC>(bc, thr, T, site0 + a, at[g], valid, P[g]))) alive &= ~(1ull << t);
+            if (__any(act && screen_hit<CUBIC>(bc, thr, T, site0 + a, at[g], valid, P[g]))) alive &= ~(1u << t);
           }
         }
       }
@@ -245,10 +321,10 @@ __device__ inline uint64_t screen(const WidomArgs& A, const uint4* __restrict__ 
       const unsigned nval = (unsigned)min(64, A.n_fw - c0);
       for (int a = 0; a < n_sites; ++a) {
         for (int t = 0; t < n_trials; ++t) {
-          if (!((alive >> t) & 1ull)) continue;
+          if (!((alive >> t) & 1u)) continue;
           stat += nval;
           if (__any(screen_hit<CUBIC>(bc, thr, T, site0 + a, f, valid, W.pfix[a * n_trials + t])))
-            alive &= ~(1ull << t);
+            alive &= ~(1u << t);
         }
       }
     }
@@ -269,10 +345,10 @@ __device__ inline uint64_t screen(const WidomArgs& A, const uint4* __restrict__ 
       const unsigned nval = (unsigned)min(64, n - c0);
       for (int a = 0; a < n_sites; ++a) {
         for (int t = 0; t < n_trials; ++t) {
-          if (!((alive >> t) & 1ull)) continue;
+          if (!((alive >> t) & 1u)) continue;
           stat += nval;
           if (__any(screen_hit<CUBIC>(bc, thr, T, site0 + a, f, valid, W.pfix[a * n_trials + t])))
-            alive &= ~(1ull << t);
+            alive &= ~(1u << t);
         }
       }
     }
@@ -301,127 +377,17 @@ __device__ inline double coul_table(const double2* __restrict__ ct, const int ni
   return fma(g, d, c01.x);
 }
 
-// Full energies of trial site p (molecule site `site`) against the framework
-// and the guests: wave sums of LJ and real-space Coulomb, and the exact
-// overlap flag (VDW_Coulomb.cu:1183-1352 for one trial site).  The framework
-// is read in the Morton order of the screen cells, so the cutoff branches of
-// a wave's 64 atoms mostly agree.
-template <bool MIXED, bool CUBIC>
-__device__ inline void eval_site(const WidomArgs& A, const double4* __restrict__ ffl, const double2* __restrict__ ct,
-                                 const WaveLds& W, const int p, const int site, double& sv, double& sr, bool& ov,
-                                 unsigned long long& stat)
-{
-  const BoxConst& bc = A.bc;
-  const int nint = A.coul_n / kCoulTabStride;
-  const int T = bc.ntypes;
-  const int lane = threadIdx.x & 63;
-  const double sq = A.site_q[site];
-  const bool coul = bc.use_charges && sq != 0.0;
-  const double px = W.pos[p][0], py = W.pos[p][1], pz = W.pos[p][2];
-  const double4* fr = ffl + site * T;
-  const double* fx = A.fw;
-  const double* fq = A.fw + 3 * (size_t)A.n_fw;
-  double v = 0.0, r = 0.0;
-  bool o = false;
-  auto pair = [&](double ax, double ay, double az, double aq, int ty, double& v_, double& r_, bool& o_) {
-    double dx = ax - px, dy = ay - py, dz = az - pz;
-    if (MIXED)
-      pbc_rint<CUBIC>(dx, dy, dz, bc.cell, bc.inv);
-    else
-      pbc_fp64(dx, dy, dz, bc.cell, bc.inv, CUBIC ? 1 : 0);
-    const double rr = dx * dx + dy * dy + dz * dz;
-    if (rr < bc.cut_vdw2) {
-      const double4 ff = fr[ty];
-      const double e = MIXED ? lj_mixed(rr, ff.x, ff.y, ff.z) : lj_fp64(rr, ff.x, ff.y, ff.z);
-      o_ = o_ || (e > bc.overlap) || (rr < 0.01);
-      v_ += e;
-    }
-    if (coul && rr < bc.cut_coul2) {
-      if (MIXED) {
-        r_ += (bc.prefactor * aq * sq) * coul_table(ct, nint, rr);
-      } else {
-        const double rd = sqrt(rr);
-        r_ += bc.prefactor * 1.0 * (aq * sq * erfc(bc.alpha * rd)) / rd;
-      }
-    }
-  };
-  // Two atoms per lane per iteration (independent chains), and the next pair
-  // of atoms is loaded one iteration ahead: the framework stream does not fit
-  // the L1 cache, so its loads are served from L2.
-  const int n = A.n_fw;
-  double nx0 = 0.0, ny0 = 0.0, nz0 = 0.0, nq0 = 0.0, nx1 = 0.0, ny1 = 0.0, nz1 = 0.0, nq1 = 0.0;
-  int nt0 = 0, nt1 = 0;
-  if (lane < n) { nx0 = fx[lane]; ny0 = fx[n + lane]; nz0 = fx[2 * n + lane]; nq0 = fq[lane]; nt0 = A.fw_type[lane]; }
-  if (lane + 64 < n) {
-    const int k = lane + 64;
-    nx1 = fx[k]; ny1 = fx[n + k]; nz1 = fx[2 * n + k]; nq1 = fq[k]; nt1 = A.fw_type[k];
-  }
-  double v1 = 0.0, r1 = 0.0;
-  bool o1 = false;
-  for (int j = lane; j < n; j += 128) {
-    const double ax0 = nx0, ay0 = ny0, az0 = nz0, aq0 = nq0, ax1 = nx1, ay1 = ny1, az1 = nz1, aq1 = nq1;
-    const int ty0 = nt0, ty1 = nt1;
-    const bool has1 = j + 64 < n;
-    const int jn = j + 128;
-    if (jn < n) { nx0 = fx[jn]; ny0 = fx[n + jn]; nz0 = fx[2 * n + jn]; nq0 = fq[jn]; nt0 = A.fw_type[jn]; }
-    if (jn + 64 < n) {
-      const int k = jn + 64;
-      nx1 = fx[k]; ny1 = fx[n + k]; nz1 = fx[2 * n + k]; nq1 = fq[k]; nt1 = A.fw_type[k];
-    }
-    pair(ax0, ay0, az0, aq0, ty0, v, r, o);
-    if (has1) pair(ax1, ay1, az1, aq1, ty1, v1, r1, o1);
-  }
-  v += v1;
-  r += r1;
-  o = o || o1;
-  unsigned long long npair = (unsigned long long)A.n_fw;
-  for (int c = 1; c < A.gv.n_comp; ++c) {
-    const int n = A.gv.seg[c].n_atoms, base = A.gv.seg[c].base;
-    npair += (unsigned long long)n;
-    for (int jj = lane; jj < n; jj += 64) {
-      const int j = base + jj;
-      if (c == A.comp && A.g_molid[j] == A.new_molid) continue;
-      const int ta = A.g_type[j];
-      const double gq = A.g[3 * A.guest_cap + j];
-      double dx = A.g[j] - px, dy = A.g[A.guest_cap + j] - py, dz = A.g[2 * A.guest_cap + j] - pz;
-      if (MIXED)
-        pbc_rint<CUBIC>(dx, dy, dz, bc.cell, bc.inv);
-      else
-        pbc_fp64(dx, dy, dz, bc.cell, bc.inv, CUBIC ? 1 : 0);
-      const double rr = dx * dx + dy * dy + dz * dz;
-      if (rr < bc.cut_vdw2) {
-        const double4 ff = fr[ta];
-        const double e = MIXED ? lj_mixed(rr, ff.x, ff.y, ff.z) : lj_fp64(rr, ff.x, ff.y, ff.z);
-        o = o || (e > bc.overlap) || (rr < 0.01);
-        v += e;
-      }
-      if (coul && rr < bc.cut_coul2) {
-        if (MIXED) {
-          r += (bc.prefactor * gq * sq) * coul_table(ct, nint, rr);
-        } else {
-          const double rd = sqrt(rr);
-          r += bc.prefactor * 1.0 * (gq * sq * erfc(bc.alpha * rd)) / rd;
-        }
-      }
-    }
-  }
-  stat += npair;
-  sv = wave_sum(v);
-  sr = coul ? wave_sum(r) : 0.0;
-  ov = __any(o) != 0;
-}
-
 // SelectTrialPosition / CBMC_FirstBead_Finish / Widom_Move_Chain_PARTIAL
 // (mc_widom.h:14-39, 305-383, 574-614): lane t < n holds trial t's LJ (ev) and
 // real-space (er) sums; ovm marks overlapping trials.  Survivors, log-sum-exp
 // shift, cumulative pick with the uniform u in trial order, Rosenbluth sum
 // averaged over n, and the LJ-only bias correction.  Returns the selected
 // trial (or -1) and W, both uniform.
-__device__ inline void wave_select(double ev, double er, uint64_t ovm, int n, double u, double beta, int bias,
+__device__ inline void wave_select(double ev, double er, uint32_t ovm, int n, double u, double beta, int bias,
                                    int& sel_out, double& W_out)
 {
   const int lane = threadIdx.x & 63;
-  const bool alive = lane < n && !((ovm >> lane) & 1ull);
+  const bool alive = lane < n && !((ovm >> lane) & 1u);
   double lw = -INFINITY;
   if (alive) {
     double tot = ev;
@@ -468,103 +434,325 @@ __device__ inline void wave_select(double ev, double er, uint64_t ovm, int n, do
   W_out = W;
 }
 
-template <bool MIXED, bool TAB_LDS, bool CUBIC>
-__global__ __launch_bounds__(kMaxWaves * 64) void widom_kernel(const WidomArgs A)
+// Screen thresholds and the cell table of a screening workgroup, in LDS.
+template <bool TAB_LDS>
+__device__ inline const uint4* stage_screen(const WidomArgs& A, char* dyn, float*& thr)
 {
-  extern __shared__ __align__(16) char dyn[];
-  const BoxConst& bc = A.bc;
-  const int T = bc.ntypes;
-  const int M = A.M, chain = A.chain, K = A.K, O = A.O;
-  // per-(site, atom type) force-field rows and screen thresholds
-  double4* ffl = reinterpret_cast<double4*>(dyn);
-  float* thr = reinterpret_cast<float*>(dyn + A.lds_thr);
-  for (int i = threadIdx.x; i < M * T; i += blockDim.x) {
+  const int T = A.bc.ntypes;
+  thr = reinterpret_cast<float*>(dyn + A.lds_thr);
+  for (int i = threadIdx.x; i < A.M * T; i += blockDim.x) {
     const int a = i / T, ty = i % T;
-    const double* f = A.ff64 + 4 * (ty * T + A.site_type[a]);
-    ffl[i] = make_double4(f[0], f[1], f[2], 0.0);
-    thr[i] = screen_threshold(f, bc.overlap, bc.cut_vdw2);
+    thr[i] = screen_threshold(A.ff64 + 4 * (ty * T + A.site_type[a]), A.bc.overlap, A.bc.cut_vdw2);
   }
-  // real-space Coulomb table of the fast form
-  double2* ctl = reinterpret_cast<double2*>(dyn + A.lds_coul);
-  if (MIXED)
-    for (int i = threadIdx.x; i < A.coul_n / 2; i += blockDim.x) ctl[i] = reinterpret_cast<const double2*>(A.coul)[i];
-  // the screen's cell table, in LDS when it fits
-  const uint4* tab;
   if (TAB_LDS) {
     uint4* s = reinterpret_cast<uint4*>(dyn + A.lds_tab);
     const int n = A.cell_n[0] * A.cell_n[1] * A.cell_n[2] * A.cell_cap;
     for (int i = threadIdx.x; i < n; i += blockDim.x) s[i] = A.ctab[i];
-    tab = s;
-  } else {
-    tab = A.ctab;
+    return s;
   }
-  __syncthreads();
+  return A.ctab;
+}
 
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  WaveLds& W = *reinterpret_cast<WaveLds*>(dyn + A.lds_wave + (size_t)wv * A.wave_bytes);
-  double2* eik = reinterpret_cast<double2*>(&W);   // Ewald phase only
-  double (*mol)[3] = reinterpret_cast<double (*)[3]>(dyn + A.lds_wave + (size_t)wv * A.wave_bytes + A.wave_bytes -
-                                                     kMaxMol * 3 * sizeof(double));
-  unsigned long long st_screen = 0, st_full = 0;
-  const int nd = 3 * K + 1 + (chain > 0 ? 3 * O + 1 : 0);
-
-  unsigned next = 0;
-  if (lane == 0) next = atomicAdd(A.ctr, 1u);
-  for (;;) {
-    const int ins = __builtin_amdgcn_readfirstlane((int)next);
-    if (ins >= A.n_insert) break;
-    if (lane == 0) next = atomicAdd(A.ctr, 1u);
-    const uint64_t idx = (uint64_t)(A.first_index + ins);
-
-    // 1. draws: block b holds draws 2b and 2b+1 (philox_uniform's layout)
-    for (int b = lane; 2 * b < nd; b += 64) {
-      uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)b, A.stream};
-      philox4x32_10(c, (uint32_t)A.seed, (uint32_t)(A.seed >> 32));
-      const uint64_t x0 = (((uint64_t)c[0] << 32) | c[1]) >> 11;
-      W.ud[2 * b] = ((double)x0 + 0.5) * (1.0 / 9007199254740992.0);
-      if (2 * b + 1 < nd) {
-        const uint64_t x1 = (((uint64_t)c[2] << 32) | c[3]) >> 11;
-        W.ud[2 * b + 1] = ((double)x1 + 0.5) * (1.0 / 9007199254740992.0);
-      }
+// Appends this wave's surviving trials (bits of `alive`, n_sites sites each)
+// to its workgroup's sub-list (LDS counter wg_cnt); the sites of trial t are
+// W.pos[a * n_trials + t].
+__device__ inline void append_points(const WaveLds& W, TrialPt* sub, unsigned* wg_cnt, int ins, uint32_t alive,
+                                     int n_trials, int n_sites, int site0, int slot0)
+{
+  const int lane = threadIdx.x & 63;
+  const int n_alive = __popc(alive);
+  if (n_alive == 0) return;
+  int base = 0;
+  if (lane == 0) base = (int)atomicAdd(wg_cnt, (unsigned)(n_alive * n_sites));
+  base = __builtin_amdgcn_readfirstlane(base);
+  if (lane < n_trials && ((alive >> lane) & 1u)) {
+    const int rank = __popc(alive & ((1u << lane) - 1u));
+    for (int a = 0; a < n_sites; ++a) {
+      const int p = a * n_trials + lane;
+      TrialPt q;
+      q.x = W.pos[p][0]; q.y = W.pos[p][1]; q.z = W.pos[p][2];
+      q.site = site0 + a;
+      q.slot = (slot0 + lane) * n_sites + a;
+      q.ins = ins;
+      q.bit = lane;
+      sub[base + rank * n_sites + a] = q;
     }
+  }
+}
+
+// A: first-bead trial positions and their overlap screen.  Workgroup g owns
+// insertions [g*chunk, (g+1)*chunk).
+template <bool CUBIC, bool TAB_LDS>
+__global__ __launch_bounds__(kInsWaves * 64) void widom_first_bead_kernel(const WidomArgs A)
+{
+  extern __shared__ __align__(16) char dyn[];
+  __shared__ unsigned wg_cnt;
+  if (threadIdx.x == 0) wg_cnt = 0u;
+  float* thr;
+  const uint4* tab = stage_screen<TAB_LDS>(A, dyn, thr);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  WaveLds& W = *reinterpret_cast<WaveLds*>(dyn + A.lds_wave + (size_t)wv * sizeof(WaveLds));
+  const int K = A.K;
+  const int i0 = blockIdx.x * A.chunk, i1 = min(A.n_insert, i0 + A.chunk);
+  TrialPt* sub = A.list1 + (size_t)i0 * K;
+  const uint32_t allK = (K >= 32) ? ~0u : ((1u << K) - 1u);
+  unsigned long long st = 0;
+  for (int ins = i0 + wv; ins < i1; ins += kInsWaves) {
+    const uint64_t idx = (uint64_t)(A.first_index + ins);
+    fill_draws(A, idx, 0, 3 * K, W.ud);
     wave_sync();
-    // 2. first-bead trial positions: diag(cell) * U^3 (get_random_trial_position, mc_widom.h:122-213)
     if (lane < K) {
       double p[3];
-      p[0] = bc.cell[0] * W.ud[3 * lane];
-      p[1] = bc.cell[4] * W.ud[3 * lane + 1];
-      p[2] = bc.cell[8] * W.ud[3 * lane + 2];
+      p[0] = A.bc.cell[0] * W.ud[3 * lane];
+      p[1] = A.bc.cell[4] * W.ud[3 * lane + 1];
+      p[2] = A.bc.cell[8] * W.ud[3 * lane + 2];
       W.pos[lane][0] = p[0]; W.pos[lane][1] = p[1]; W.pos[lane][2] = p[2];
-      W.pfix[lane] = to_fixed4(bc, p);
+      W.pfix[lane] = to_fixed4(A.bc, p);
     }
     wave_sync();
-    // 3.-5. first bead
-    uint64_t ovm = screen<CUBIC>(A, tab, thr, W, K, 1, 0, st_screen);
-    double ev = 0.0, er = 0.0;
-    for (int t = 0; t < K; ++t) {
-      if ((ovm >> t) & 1ull) continue;
-      double sv, sr;
-      bool o;
-      eval_site<MIXED, CUBIC>(A, ffl, ctl, W, t, 0, sv, sr, o, st_full);
-      if (o) ovm |= 1ull << t;
-      if (lane == t) { ev = sv; er = sr; }
+    const uint32_t dead = screen<CUBIC>(A, tab, thr, W, K, 1, 0, st);
+    if (lane == 0) A.mask1[ins] = dead;
+    append_points(W, sub, &wg_cnt, ins, allK & ~dead, K, 1, 0, ins * K);
+    wave_sync();
+  }
+  if (lane == 0) atomicAdd(&A.stat[0], st);
+  __syncthreads();
+  if (threadIdx.x == 0) A.counts1[blockIdx.x] = wg_cnt;
+}
+
+// B / D: full energies of the trial sites of a work list (VDW_Coulomb.cu:
+// 1183-1352 for one trial site): wave sums of LJ and real-space Coulomb and
+// the exact overlap flag; one wave per site, 64 framework atoms per step.
+// The framework (x, y, z, q and type) is staged in LDS when it fits.
+template <bool MIXED, bool CUBIC, bool FW_LDS>
+__global__ __launch_bounds__(kEvalWaves * 64) void widom_eval_kernel(const WidomArgs A, const TrialPt* list,
+                                                                   const unsigned* counts, int per_ins,
+                                                                   double2* energy, uint32_t* mask)
+{
+  extern __shared__ __align__(16) char dyn[];
+  // exclusive prefix sums of the screening workgroups' sub-list lengths
+  __shared__ unsigned pre[kMaxScreenGroups + 1];
+  const int G = A.groups;
+  for (int i = threadIdx.x; i < G; i += blockDim.x) pre[i + 1] = counts[i];
+  if (threadIdx.x == 0) pre[0] = 0u;
+  __syncthreads();
+  for (int h = 1; h < G; h <<= 1) {
+    unsigned add[kMaxScreenGroups / (kEvalWaves * 64)];
+    for (int k = 0, i = threadIdx.x; i < G; ++k, i += blockDim.x) add[k] = (i >= h) ? pre[i + 1 - h] : 0u;
+    __syncthreads();
+    for (int k = 0, i = threadIdx.x; i < G; ++k, i += blockDim.x) pre[i + 1] += add[k];
+    __syncthreads();
+  }
+  const BoxConst& bc = A.bc;
+  const int T = bc.ntypes, n = A.n_fw;
+  double4* ffl = reinterpret_cast<double4*>(dyn + A.lds_ffl);
+  for (int i = threadIdx.x; i < A.M * T; i += blockDim.x) {
+    const int a = i / T, ty = i % T;
+    const double* f = A.ff64 + 4 * (ty * T + A.site_type[a]);
+    ffl[i] = make_double4(f[0], f[1], f[2], 0.0);
+  }
+  double2* ct = reinterpret_cast<double2*>(dyn + A.lds_coul);
+  if (MIXED)
+    for (int i = threadIdx.x; i < A.coul_n / 2; i += blockDim.x) ct[i] = reinterpret_cast<const double2*>(A.coul)[i];
+  double4* fw4 = reinterpret_cast<double4*>(dyn + A.lds_fw);
+  int* fwt = reinterpret_cast<int*>(dyn + A.lds_fw + (size_t)n * sizeof(double4));
+  if (FW_LDS) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      fw4[i] = make_double4(A.fw[i], A.fw[n + i], A.fw[2 * n + i], A.fw[3 * n + i]);
+      fwt[i] = A.fw_type[i];
     }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int nint = A.coul_n / kCoulTabStride;
+  const int npts = (int)pre[G];
+  unsigned long long n_eval = 0;
+  for (int pi = blockIdx.x * kEvalWaves + wv; pi < npts; pi += gridDim.x * kEvalWaves) {
+    // sub-list g holding point pi: the last g with pre[g] <= pi
+    int lo = 0, hi = G - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if ((int)pre[mid] <= pi) lo = mid; else hi = mid - 1;
+    }
+    const TrialPt P = list[(size_t)lo * A.chunk * per_ins + (pi - (int)pre[lo])];
+    const double px = P.x, py = P.y, pz = P.z;
+    const double sq = A.site_q[P.site];
+    const bool coul = bc.use_charges && sq != 0.0;
+    const double4* fr = ffl + P.site * T;
+    double v = 0.0, r = 0.0;
+    bool o = false;
+    if (MIXED) {
+      // straight-line pair terms with selects for the cutoffs, two atoms per
+      // lane per step (independent chains) under one wave-uniform branch that
+      // skips atom pairs beyond both cutoffs for all lanes
+      const double cmax = coul ? fmax(bc.cut_vdw2, bc.cut_coul2) : bc.cut_vdw2;
+      const double qs = bc.prefactor * sq;
+      auto pair_fast = [&](double rr, double aq, int ty, double& v_, double& r_, bool& o_) {
+        const double4 ff = fr[ty];
+        const double e = lj_mixed(rr, ff.x, ff.y, ff.z);
+        const bool inv = rr < bc.cut_vdw2;
+        v_ += inv ? e : 0.0;
+        o_ = o_ || (inv && ((e > bc.overlap) || (rr < 0.01)));
+        if (coul) {
+          const double g = coul_table(ct, nint, rr);
+          r_ += (rr < bc.cut_coul2) ? (qs * aq) * g : 0.0;
+        }
+      };
+      double v1 = 0.0, r1 = 0.0;
+      bool o1 = false;
+      for (int j = lane; j < n; j += 128) {
+        const int k = j + 64;
+        const bool has1 = k < n;
+        double ax0, ay0, az0, aq0, ax1 = 0.0, ay1 = 0.0, az1 = 0.0, aq1 = 0.0;
+        int ty0, ty1 = 0;
+        if (FW_LDS) {
+          const double4 a0 = fw4[j];
+          ax0 = a0.x; ay0 = a0.y; az0 = a0.z; aq0 = a0.w; ty0 = fwt[j];
+          if (has1) {
+            const double4 a1 = fw4[k];
+            ax1 = a1.x; ay1 = a1.y; az1 = a1.z; aq1 = a1.w; ty1 = fwt[k];
+          }
+        } else {
+          ax0 = A.fw[j]; ay0 = A.fw[n + j]; az0 = A.fw[2 * n + j]; aq0 = A.fw[3 * n + j]; ty0 = A.fw_type[j];
+          if (has1) {
+            ax1 = A.fw[k]; ay1 = A.fw[n + k]; az1 = A.fw[2 * n + k]; aq1 = A.fw[3 * n + k]; ty1 = A.fw_type[k];
+          }
+        }
+        double dx0 = ax0 - px, dy0 = ay0 - py, dz0 = az0 - pz;
+        double dx1 = ax1 - px, dy1 = ay1 - py, dz1 = az1 - pz;
+        pbc_rint<CUBIC>(dx0, dy0, dz0, bc.cell, bc.inv);
+        pbc_rint<CUBIC>(dx1, dy1, dz1, bc.cell, bc.inv);
+        const double rr0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+        const double rr1 = has1 ? dx1 * dx1 + dy1 * dy1 + dz1 * dz1 : 1e300;   // 1e300: contributes nothing
+        if (__any(rr0 < cmax || rr1 < cmax)) {
+          pair_fast(rr0, aq0, ty0, v, r, o);
+          pair_fast(rr1, aq1, ty1, v1, r1, o1);
+        }
+      }
+      v += v1;
+      r += r1;
+      o = o || o1;
+    } else {
+      // plain FP64 form: the reference's expressions, one atom at a time
+      for (int j = lane; j < n; j += 64) {
+        double ax, ay, az, aq;
+        int ty;
+        if (FW_LDS) {
+          const double4 a0 = fw4[j];
+          ax = a0.x; ay = a0.y; az = a0.z; aq = a0.w; ty = fwt[j];
+        } else {
+          ax = A.fw[j]; ay = A.fw[n + j]; az = A.fw[2 * n + j]; aq = A.fw[3 * n + j]; ty = A.fw_type[j];
+        }
+        double dx = ax - px, dy = ay - py, dz = az - pz;
+        pbc_fp64(dx, dy, dz, bc.cell, bc.inv, CUBIC ? 1 : 0);
+        const double rr = dx * dx + dy * dy + dz * dz;
+        if (rr < bc.cut_vdw2) {
+          const double4 ff = fr[ty];
+          const double e = lj_fp64(rr, ff.x, ff.y, ff.z);
+          o = o || (e > bc.overlap) || (rr < 0.01);
+          v += e;
+        }
+        if (coul && rr < bc.cut_coul2) {
+          const double rd = sqrt(rr);
+          r += bc.prefactor * 1.0 * (aq * sq * erfc(bc.alpha * rd)) / rd;
+        }
+      }
+    }
+    // guest molecules (guest-guest), from memory
+    unsigned long long npair = (unsigned long long)n;
+    for (int c = 1; c < A.gv.n_comp; ++c) {
+      const int ng = A.gv.seg[c].n_atoms, base = A.gv.seg[c].base;
+      npair += (unsigned long long)ng;
+      for (int jj = lane; jj < ng; jj += 64) {
+        const int j = base + jj;
+        if (c == A.comp && A.g_molid[j] == A.new_molid) continue;
+        const int ta = A.g_type[j];
+        const double gq = A.g[3 * A.guest_cap + j];
+        double dx = A.g[j] - px, dy = A.g[A.guest_cap + j] - py, dz = A.g[2 * A.guest_cap + j] - pz;
+        if (MIXED)
+          pbc_rint<CUBIC>(dx, dy, dz, bc.cell, bc.inv);
+        else
+          pbc_fp64(dx, dy, dz, bc.cell, bc.inv, CUBIC ? 1 : 0);
+        const double rr = dx * dx + dy * dy + dz * dz;
+        if (rr < bc.cut_vdw2) {
+          const double4 ff = fr[ta];
+          const double e = MIXED ? lj_mixed(rr, ff.x, ff.y, ff.z) : lj_fp64(rr, ff.x, ff.y, ff.z);
+          o = o || (e > bc.overlap) || (rr < 0.01);
+          v += e;
+        }
+        if (coul && rr < bc.cut_coul2) {
+          if (MIXED) {
+            r += (bc.prefactor * gq * sq) * coul_table(ct, nint, rr);
+          } else {
+            const double rd = sqrt(rr);
+            r += bc.prefactor * 1.0 * (gq * sq * erfc(bc.alpha * rd)) / rd;
+          }
+        }
+      }
+    }
+    n_eval += npair;
+    const double sv = wave_sum(v);
+    const double sr = coul ? wave_sum(r) : 0.0;
+    const bool ov = __any(o) != 0;
+    if (lane == 0) {
+      energy[P.slot] = make_double2(sv, sr);
+      if (ov) atomicOr(mask + P.ins, 1u << P.bit);
+    }
+  }
+  if (lane == 0 && n_eval) atomicAdd(&A.stat[1], n_eval);
+}
+
+// C: first-bead selection; orientations about the selected bead and their
+// overlap screen.  Workgroup g owns insertions [g*chunk, (g+1)*chunk).
+template <bool CUBIC, bool TAB_LDS>
+__global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const WidomArgs A)
+{
+  extern __shared__ __align__(16) char dyn[];
+  __shared__ unsigned wg_cnt;
+  if (threadIdx.x == 0) wg_cnt = 0u;
+  float* thr;
+  const uint4* tab = stage_screen<TAB_LDS>(A, dyn, thr);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  WaveLds& W = *reinterpret_cast<WaveLds*>(dyn + A.lds_wave + (size_t)wv * sizeof(WaveLds));
+  const int K = A.K, O = A.O, chain = A.chain;
+  const int i0 = blockIdx.x * A.chunk, i1 = min(A.n_insert, i0 + A.chunk);
+  TrialPt* sub = A.list2 + (size_t)i0 * O * chain;
+  const uint32_t allO = (O >= 32) ? ~0u : ((1u << O) - 1u);
+  unsigned long long st = 0;
+  for (int ins = i0 + wv; ins < i1; ins += kInsWaves) {
+    const uint64_t idx = (uint64_t)(A.first_index + ins);
+    // draws 0 .. 3K+3O: first-bead trials, first-bead selection, orientations
+    fill_draws(A, idx, 0, 3 * K + 1 + 3 * O, W.ud);
+    const uint32_t dead = A.mask1[ins];
+    double ev = 0.0, er = 0.0;
+    if (lane < K && !((dead >> lane) & 1u)) {
+      const double2 e = A.e1[(size_t)ins * K + lane];
+      ev = e.x;
+      er = e.y;
+    }
+    wave_sync();
     int sel;
     double Wfb;
-    wave_select(ev, er, ovm, K, W.ud[3 * K], A.beta, A.bias, sel, Wfb);
+    wave_select(ev, er, dead, K, W.ud[3 * K], A.beta, A.bias, sel, Wfb);
     const bool fb_ok = sel >= 0 && Wfb > 1e-150;
-    bool ok = fb_ok;
-    double Wtot = Wfb, e_v = 0.0, e_r = 0.0;
-    int orient = 0;
+    InsState S;
+    S.sel = fb_ok ? sel : -1;
+    S.ok = fb_ok ? 1 : 0;
+    S.wfb = Wfb;
+    S.ev = fb_ok ? readlane_d(ev, sel) : 0.0;
+    S.er = fb_ok ? readlane_d(er, sel) : 0.0;
+    S.fb[0] = S.fb[1] = S.fb[2] = 0.0;
     if (fb_ok) {
-      e_v = readlane_d(ev, sel);
-      e_r = readlane_d(er, sel);
-      if (lane < 3) mol[0][lane] = W.pos[sel][lane];
+      S.fb[0] = A.bc.cell[0] * W.ud[3 * sel];
+      S.fb[1] = A.bc.cell[4] * W.ud[3 * sel + 1];
+      S.fb[2] = A.bc.cell[8] * W.ud[3 * sel + 2];
     }
-    wave_sync();
-    // 6. orientations of the template about the selected first bead
-    //    (get_random_trial_orientation, mc_widom.h:215-303); trial site of
-    //    orientation o and chain atom a at index a * O + o
+    if (lane == 0) {
+      A.state[ins] = S;
+      A.mask2[ins] = 0u;
+    }
     if (fb_ok && chain > 0) {
       if (lane < O) {
         const double u = W.ud[3 * K + 1 + 3 * lane];
@@ -573,52 +761,73 @@ __global__ __launch_bounds__(kMaxWaves * 64) void widom_kernel(const WidomArgs A
         for (int a = 0; a < chain; ++a) {
           double vec[3] = {A.tvec[a][0], A.tvec[a][1], A.tvec[a][2]};
           quat_rotate(vec, u, v, w);
+          const double p[3] = {S.fb[0] + vec[0], S.fb[1] + vec[1], S.fb[2] + vec[2]};
           const int s = a * O + lane;
-          double p[3] = {mol[0][0] + vec[0], mol[0][1] + vec[1], mol[0][2] + vec[2]};
           W.pos[s][0] = p[0]; W.pos[s][1] = p[1]; W.pos[s][2] = p[2];
-          W.pfix[s] = to_fixed4(bc, p);
+          W.pfix[s] = to_fixed4(A.bc, p);
         }
       }
       wave_sync();
-      uint64_t ovo = screen<CUBIC>(A, tab, thr, W, O, chain, 1, st_screen);
+      const uint32_t dead2 = screen<CUBIC>(A, tab, thr, W, O, chain, 1, st);
+      if (lane == 0) A.mask2[ins] = dead2;
+      append_points(W, sub, &wg_cnt, ins, allO & ~dead2, O, chain, 1, ins * O);
+    }
+    wave_sync();
+  }
+  if (lane == 0) atomicAdd(&A.stat[0], st);
+  __syncthreads();
+  if (threadIdx.x == 0) A.counts2[blockIdx.x] = wg_cnt;
+}
+
+// E: orientation selection, Fourier-space energy of the selected molecule
+// (Fourier_Ewald_Diff, Ewald_Energy_Functions.h:280-397, with Oldsize = 0)
+// and the record: W = W_fb W_ch exp(-beta (dE_ewald + dE_tail)).
+__global__ __launch_bounds__(kInsWaves * 64) void widom_finish_kernel(const WidomArgs A)
+{
+  extern __shared__ __align__(16) char dyn[];
+  const BoxConst& bc = A.bc;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int K = A.K, O = A.O, chain = A.chain, M = A.M, KS = A.kstride;
+  const size_t wave_bytes = (size_t)M * 3 * KS * sizeof(double2) + kMaxMol * 3 * sizeof(double);
+  double2* eik = reinterpret_cast<double2*>(dyn + (size_t)wv * wave_bytes);
+  double (*mol)[3] = reinterpret_cast<double (*)[3]>(dyn + (size_t)wv * wave_bytes + (size_t)M * 3 * KS * sizeof(double2));
+  for (int ins = blockIdx.x * kInsWaves + wv; ins < A.n_insert; ins += gridDim.x * kInsWaves) {
+    const uint64_t idx = (uint64_t)(A.first_index + ins);
+    const InsState S = A.state[ins];
+    bool ok = S.ok != 0;
+    double Wtot = S.wfb, e_v = S.ev, e_r = S.er;
+    int orient = 0;
+    if (lane < 3) mol[0][lane] = S.fb[lane];
+    if (ok && chain > 0) {
+      const uint32_t dead2 = A.mask2[ins];
       double cv = 0.0, cr = 0.0;
-      for (int o = 0; o < O; ++o) {
-        if ((ovo >> o) & 1ull) continue;
-        double tv = 0.0, tr = 0.0;
-        bool dead = false;
-        for (int a = 0; a < chain && !dead; ++a) {
-          double sv, sr;
-          bool ob;
-          eval_site<MIXED, CUBIC>(A, ffl, ctl, W, a * O + o, 1 + a, sv, sr, ob, st_full);
-          tv += sv;
-          tr += sr;
-          dead = ob;
+      if (lane < O && !((dead2 >> lane) & 1u)) {
+        for (int a = 0; a < chain; ++a) {
+          const double2 e = A.e2[((size_t)ins * O + lane) * chain + a];
+          cv += e.x;
+          cr += e.y;
         }
-        if (dead) ovo |= 1ull << o;
-        if (lane == o) { cv = tv; cr = tr; }
       }
       int so;
       double Wch;
-      wave_select(cv, cr, ovo, O, W.ud[3 * K + 3 * O + 1], A.beta, A.bias, so, Wch);
-      if (so < 0 || !(Wfb * Wch > 1e-150)) {
+      wave_select(cv, cr, dead2, O, draw(A, idx, 3 * K + 3 * O + 1), A.beta, A.bias, so, Wch);
+      if (so < 0 || !(S.wfb * Wch > 1e-150)) {
         ok = false;
       } else {
-        Wtot = Wfb * Wch;
+        Wtot = S.wfb * Wch;
         orient = so;
         e_v += readlane_d(cv, so);
         e_r += readlane_d(cr, so);
-        for (int i = lane; i < 3 * chain; i += 64) {
-          const int a = i / 3, d = i % 3;
-          mol[1 + a][d] = W.pos[a * O + so][d];
+        if (lane < chain) {
+          double p[3];
+          orient_pos(A, idx, so, lane, S.fb, p);
+          mol[1 + lane][0] = p[0]; mol[1 + lane][1] = p[1]; mol[1 + lane][2] = p[2];
         }
       }
-      wave_sync();
     }
-    // 7. Fourier-space energy of the selected molecule, FP64 (Fourier_Ewald_Diff,
-    //    Ewald_Energy_Functions.h:280-397, with Oldsize = 0)
+    wave_sync();
     double ew = 0.0;
     if (ok && A.charged) {
-      const int KS = A.kstride;
       for (int t = lane; t < M * 3; t += 64) {
         const int site = t / 3, axis = t % 3;
         double s = bc.inv[axis] * mol[site][0] + bc.inv[3 + axis] * mol[site][1] + bc.inv[6 + axis] * mol[site][2];
@@ -662,9 +871,7 @@ __global__ __launch_bounds__(kMaxWaves * 64) void widom_kernel(const WidomArgs A
       es = wave_sum(es);
       ec = wave_sum(ec);
       ew = (es - A.excl) + 2.0 * ec;
-      wave_sync();
     }
-    // 8. record
     if (lane == 0) {
       double Wf = 0.0, etot = 0.0;
       if (ok) {
@@ -673,7 +880,7 @@ __global__ __launch_bounds__(kMaxWaves * 64) void widom_kernel(const WidomArgs A
       }
       A.rec_w[ins] = Wf;
       A.rec_e[ins] = etot;
-      A.rec_f[ins] = (fb_ok ? 1 : 0) | (ok ? 2 : 0);
+      A.rec_f[ins] = (S.ok ? 1 : 0) | (ok ? 2 : 0);
       if (A.res) {
         graspa_widom_result r;
         r.rosen = Wf;
@@ -681,24 +888,20 @@ __global__ __launch_bounds__(kMaxWaves * 64) void widom_kernel(const WidomArgs A
         r.vdw = ok ? e_v : 0.0;
         r.real = ok ? e_r : 0.0;
         r.ewald = ok ? ew : 0.0;
-        r.first_bead = ok ? sel : -1;
+        r.first_bead = ok ? S.sel : -1;
         r.orientation = ok ? orient : -1;
         r.ok = ok ? 1 : 0;
         r._pad = 0;
         A.res[ins] = r;
       }
     }
-  }
-  if (lane == 0) {
-    atomicAdd(&A.stat[0], st_screen);
-    atomicAdd(&A.stat[1], st_full);
+    wave_sync();
   }
 }
 
-__global__ void widom_reset_kernel(unsigned* ctr, unsigned long long* stat)
+__global__ void widom_reset_kernel(unsigned long long* stat)
 {
   if (threadIdx.x == 0) {
-    *ctr = 0u;
     stat[0] = 0ull;
     stat[1] = 0ull;
   }
@@ -751,56 +954,108 @@ __global__ __launch_bounds__(kRedBlock) void widom_final_kernel(const double* pa
   if (threadIdx.x < kSums) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
-// Waves per workgroup and workgroups per CU for one instantiation: the most
-// resident waves, ties to the larger workgroup (fewer cell-table copies).
-template <bool MIXED, bool TAB_LDS, bool CUBIC>
-static int launch_one(graspa_box* b, const WidomArgs& A0, size_t fixed_lds, size_t wave_bytes)
-{
-  auto kern = widom_kernel<MIXED, TAB_LDS, CUBIC>;
-  int best_nw = 0, best_blocks = 0, best_waves = 0;
-  size_t best_lds = 0;
-  for (int nw : {4, kMaxWaves}) {
-    const size_t lds = fixed_lds + (size_t)nw * wave_bytes;
-    if (lds > (size_t)b->max_lds) continue;
-    if (lds > 65536) {
-      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      (void)hipGetLastError();
-    }
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, 64 * nw, lds) != hipSuccess || blocks < 1) {
-      (void)hipGetLastError();
-      continue;
-    }
-    if (blocks * nw >= best_waves) {
-      best_waves = blocks * nw;
-      best_nw = nw;
-      best_blocks = blocks;
-      best_lds = lds;
-    }
-  }
-  if (best_nw == 0) return fail(GRASPA_ERR_UNSUPPORTED, "widom: no launch shape fits the LDS");
-  WidomArgs A = A0;
-  const long long waves_needed = (A.n_insert + 0LL);
-  long long grid = (long long)best_blocks * b->n_cu;
-  grid = std::max(1LL, std::min(grid, (waves_needed + best_nw - 1) / best_nw));
-  timing_begin(b, "widom");
-  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * best_nw), best_lds, b->stream, A);
-  timing_end(b, "widom");
-  if (hip_check(hipGetLastError(), "widom_kernel launch")) return GRASPA_ERR_HIP;
-  return GRASPA_OK;
-}
-
-template <bool MIXED>
-static int launch_mixed(graspa_box* b, const WidomArgs& A, bool tab_lds, size_t fixed_lds, size_t wave_bytes)
-{
-  if (tab_lds)
-    return A.bc.cubic ? launch_one<MIXED, true, true>(b, A, fixed_lds, wave_bytes)
-                      : launch_one<MIXED, true, false>(b, A, fixed_lds, wave_bytes);
-  return A.bc.cubic ? launch_one<MIXED, false, true>(b, A, fixed_lds, wave_bytes)
-                    : launch_one<MIXED, false, false>(b, A, fixed_lds, wave_bytes);
-}
-
 static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// Resident workgroups per CU for a kernel, workgroup size and dynamic LDS;
+// 0 if it cannot run.
+template <typename Kern>
+static int occupancy(graspa_box* b, Kern kern, int threads, size_t lds)
+{
+  if (lds > (size_t)b->max_lds) return 0;
+  if (lds > 65536) {
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipGetLastError();
+  }
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kern, threads, lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return blocks;
+}
+
+// Launches a per-insertion kernel on a grid of resident workgroups.
+template <typename Kern>
+static int launch_ins(graspa_box* b, Kern kern, const WidomArgs& A, size_t lds, const char* what, int fixed_grid = 0)
+{
+  const int occ = occupancy(b, kern, kInsWaves * 64, lds);
+  if (occ < 1) return fail(GRASPA_ERR_UNSUPPORTED, std::string("widom: no launch shape fits the LDS (") + what + ")");
+  const long long need = ((long long)A.n_insert + kInsWaves - 1) / kInsWaves;
+  const long long grid = fixed_grid > 0 ? fixed_grid : std::max(1LL, std::min(need, (long long)occ * b->n_cu));
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kInsWaves * 64), lds, b->stream, A);
+  return hip_check(hipGetLastError(), what);
+}
+
+// Per-box buffers of the pipeline, grown on demand.
+static int ensure_pipeline(graspa_box* b, int n_insert, int K, int OC)
+{
+  const size_t n = (size_t)n_insert;
+  const size_t need_e1 = n * K, need_e2 = n * std::max(OC, 1);
+  if (n <= b->wp_n && need_e1 <= b->wp_e1 && need_e2 <= b->wp_e2) return GRASPA_OK;
+  const size_t cn = std::max(n, b->wp_n), c1 = std::max(need_e1, b->wp_e1), c2 = std::max(need_e2, b->wp_e2);
+  hipStreamSynchronize(b->stream);
+  hipFree(b->d_wmask1); hipFree(b->d_wmask2); hipFree(b->d_we1); hipFree(b->d_we2);
+  hipFree(b->d_wlist1); hipFree(b->d_wlist2); hipFree(b->d_wstate);
+  b->d_wmask1 = b->d_wmask2 = nullptr;
+  b->d_we1 = b->d_we2 = b->d_wlist1 = b->d_wlist2 = b->d_wstate = nullptr;
+  b->wp_n = b->wp_e1 = b->wp_e2 = 0;
+  int rc = hip_check(hipMalloc(&b->d_wmask1, cn * sizeof(uint32_t)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_wmask2, cn * sizeof(uint32_t)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_wstate, cn * sizeof(InsState)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_we1, c1 * sizeof(double2)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_wlist1, c1 * sizeof(TrialPt)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_we2, c2 * sizeof(double2)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_wlist2, c2 * sizeof(TrialPt)), "hipMalloc widom pipeline");
+  if (!rc && !b->d_wcounts)
+    rc = hip_check(hipMalloc(&b->d_wcounts, 2 * kMaxScreenGroups * sizeof(unsigned)), "hipMalloc widom pipeline");
+  if (!rc) {
+    b->wp_n = cn;
+    b->wp_e1 = c1;
+    b->wp_e2 = c2;
+  }
+  return rc;
+}
+
+template <bool MIXED, bool CUBIC>
+static int launch_eval(graspa_box* b, const WidomArgs& A, const TrialPt* list, const unsigned* counts, int per_ins,
+                       double2* energy, uint32_t* mask)
+{
+  const size_t fw_bytes = (size_t)A.n_fw * (sizeof(double4) + sizeof(int));
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool fw_lds = pass == 0;
+    const size_t lds = (size_t)A.lds_fw + (fw_lds ? align16(fw_bytes) : 0);
+    auto kern = fw_lds ? widom_eval_kernel<MIXED, CUBIC, true> : widom_eval_kernel<MIXED, CUBIC, false>;
+    const int occ = occupancy(b, kern, kEvalWaves * 64, lds);
+    if (occ < 1) continue;
+    timing_begin(b, "widom");
+    hipLaunchKernelGGL(kern, dim3((unsigned)(occ * b->n_cu)), dim3(kEvalWaves * 64), lds, b->stream, A, list, counts,
+                       per_ins, energy, mask);
+    timing_end(b, "widom");
+    return hip_check(hipGetLastError(), "widom_eval_kernel launch");
+  }
+  return fail(GRASPA_ERR_UNSUPPORTED, "widom: no launch shape fits the LDS (pair energies)");
+}
+
+template <bool CUBIC, bool TAB_LDS>
+static int launch_screens(graspa_box* b, const WidomArgs& A, size_t lds, int stage)
+{
+  if (stage == 0)
+    return launch_ins(b, widom_first_bead_kernel<CUBIC, TAB_LDS>, A, lds, "widom_first_bead_kernel", A.groups);
+  return launch_ins(b, widom_chain_kernel<CUBIC, TAB_LDS>, A, lds, "widom_chain_kernel", A.groups);
+}
+
+template <bool CUBIC>
+static int launch_screen_stage(graspa_box* b, WidomArgs& A, size_t base_lds, size_t tab_bytes, int stage)
+{
+  const size_t waves = kInsWaves * sizeof(WaveLds);
+  if (b->use_cells && base_lds + align16(tab_bytes) + waves <= (size_t)b->max_lds) {
+    A.lds_wave = (int)(base_lds + align16(tab_bytes));
+    const int rc = launch_screens<CUBIC, true>(b, A, base_lds + align16(tab_bytes) + waves, stage);
+    if (rc != GRASPA_ERR_UNSUPPORTED) return rc;
+  }
+  A.lds_wave = (int)base_lds;
+  return launch_screens<CUBIC, false>(b, A, base_lds + waves, stage);
+}
 
 int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
                  uint64_t seed, uint32_t stream, int64_t first_index, int n_insert, graspa_widom_result* d_res,
@@ -817,6 +1072,8 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
     return GRASPA_OK;
   }
   if (ensure_widom_records(b, n_insert)) return GRASPA_ERR_HIP;
+  const int chain = c.molsize - 1;
+  if (ensure_pipeline(b, n_insert, K, O * chain)) return GRASPA_ERR_HIP;
   WidomArgs A{};
   A.bc = b->bc;
   A.fw = b->d_fws; A.fw_type = b->d_fws_type; A.fw_fixT = b->d_fw_fixT; A.n_fw = b->n_fw;
@@ -830,7 +1087,7 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   A.comp = comp;
   A.new_molid = c.n_mol;
   A.M = c.molsize;
-  A.chain = c.molsize - 1;
+  A.chain = chain;
   for (int a = 0; a < c.molsize; ++a) {
     A.site_type[a] = c.type[a];
     A.site_q[a] = c.q[a];
@@ -847,39 +1104,60 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   A.ktemp_act = b->d_ktemp_act;
   A.sk_ads = b->d_sk_ads; A.sk_fw = b->d_sk_fw;
   A.kstride = 1 + std::max(b->bc.kmax[0], std::max(b->bc.kmax[1], b->bc.kmax[2]));
-  A.res = d_res;
-  A.rec_w = b->d_wrec_w; A.rec_e = b->d_wrec_e; A.rec_f = b->d_wrec_f;
-  A.ctr = reinterpret_cast<unsigned*>(b->d_wacc + 12);
-  A.stat = reinterpret_cast<unsigned long long*>(b->d_wacc + 8);
   A.coul = b->d_coul;
   A.coul_n = b->coul_n;
-  // LDS layout: FF rows | thresholds | Coulomb table (fast form) | cell table (optional) | per-wave areas
+  A.mask1 = static_cast<uint32_t*>(b->d_wmask1);
+  A.mask2 = static_cast<uint32_t*>(b->d_wmask2);
+  A.e1 = static_cast<double2*>(b->d_we1);
+  A.e2 = static_cast<double2*>(b->d_we2);
+  A.list1 = static_cast<TrialPt*>(b->d_wlist1);
+  A.list2 = static_cast<TrialPt*>(b->d_wlist2);
+  A.state = static_cast<InsState*>(b->d_wstate);
+  A.counts1 = static_cast<unsigned*>(b->d_wcounts);
+  A.counts2 = A.counts1 + kMaxScreenGroups;
+  // screening grid: up to kMaxScreenGroups workgroups, each owning a contiguous range of insertions
+  A.groups = (int)std::min<long long>(kMaxScreenGroups, std::max<long long>(1, ((long long)n_insert + 31) / 32));
+  A.chunk = (n_insert + A.groups - 1) / A.groups;
+  A.res = d_res;
+  A.rec_w = b->d_wrec_w; A.rec_e = b->d_wrec_e; A.rec_f = b->d_wrec_f;
+  A.stat = reinterpret_cast<unsigned long long*>(b->d_wacc + 8);
   const int T = A.bc.ntypes, M = A.M;
-  size_t off = align16((size_t)M * T * sizeof(double4));
-  A.lds_thr = (int)off;
-  off = align16(off + (size_t)M * T * sizeof(float));
-  A.lds_coul = (int)off;
-  if (b->precision == GRASPA_MIXED) off = align16(off + (size_t)b->coul_n * sizeof(double));
-  const size_t eik_bytes = (size_t)M * 3 * A.kstride * sizeof(double2);
-  const size_t wave_bytes = align16(std::max(kWaveArea, eik_bytes)) + kMaxMol * 3 * sizeof(double);
-  A.wave_bytes = (int)wave_bytes;
+  // screening kernels: thresholds | cell table (optional) | per-wave areas
+  A.lds_thr = 0;
+  const size_t scr_base = align16((size_t)M * T * sizeof(float));
+  A.lds_tab = (int)scr_base;
   const size_t tab_bytes =
       b->use_cells ? (size_t)b->cell_n[0] * b->cell_n[1] * b->cell_n[2] * b->cell_cap * sizeof(uint4) : 0;
-  A.lds_tab = (int)off;
-  const size_t base_lds = off;
+  // pair-energy kernel: FF rows | Coulomb table (fast form) | framework (optional)
+  A.lds_ffl = 0;
+  size_t off = align16((size_t)M * T * sizeof(double4));
+  A.lds_coul = (int)off;
+  if (b->precision == GRASPA_MIXED) off = align16(off + (size_t)b->coul_n * sizeof(double));
+  A.lds_fw = (int)off;
+  const bool mixed = b->precision == GRASPA_MIXED, cubic = A.bc.cubic != 0;
 
-  hipLaunchKernelGGL(widom_reset_kernel, dim3(1), dim3(64), 0, b->stream, A.ctr, A.stat);
+  hipLaunchKernelGGL(widom_reset_kernel, dim3(1), dim3(64), 0, b->stream, A.stat);
   if (hip_check(hipGetLastError(), "widom_reset launch")) return GRASPA_ERR_HIP;
-  // cell table in LDS when it fits beside four wave areas, else read through the caches
-  int rc = GRASPA_ERR_UNSUPPORTED;
-  for (int pass = 0; pass < 2 && rc == GRASPA_ERR_UNSUPPORTED; ++pass) {
-    const bool tab_lds = pass == 0;
-    if (tab_lds && (!b->use_cells || base_lds + align16(tab_bytes) + 4 * wave_bytes > (size_t)b->max_lds)) continue;
-    const size_t fixed_lds = tab_lds ? base_lds + align16(tab_bytes) : base_lds;
-    A.lds_wave = (int)fixed_lds;
-    rc = (b->precision == GRASPA_MIXED) ? launch_mixed<true>(b, A, tab_lds, fixed_lds, wave_bytes)
-                                        : launch_mixed<false>(b, A, tab_lds, fixed_lds, wave_bytes);
+  int rc = cubic ? launch_screen_stage<true>(b, A, scr_base, tab_bytes, 0)
+                 : launch_screen_stage<false>(b, A, scr_base, tab_bytes, 0);
+  if (rc) return rc;
+  auto eval = [&](const TrialPt* list, const unsigned* counts, int per_ins, double2* energy, uint32_t* mask) {
+    if (mixed) return cubic ? launch_eval<true, true>(b, A, list, counts, per_ins, energy, mask)
+                            : launch_eval<true, false>(b, A, list, counts, per_ins, energy, mask);
+    return cubic ? launch_eval<false, true>(b, A, list, counts, per_ins, energy, mask)
+                 : launch_eval<false, false>(b, A, list, counts, per_ins, energy, mask);
+  };
+  rc = eval(A.list1, A.counts1, K, A.e1, A.mask1);
+  if (rc) return rc;
+  rc = cubic ? launch_screen_stage<true>(b, A, scr_base, tab_bytes, 1)
+             : launch_screen_stage<false>(b, A, scr_base, tab_bytes, 1);
+  if (rc) return rc;
+  if (chain > 0) {
+    rc = eval(A.list2, A.counts2, O * chain, A.e2, A.mask2);
+    if (rc) return rc;
   }
+  const size_t fin_lds = (size_t)kInsWaves * ((size_t)M * 3 * A.kstride * sizeof(double2) + kMaxMol * 3 * sizeof(double));
+  rc = launch_ins(b, widom_finish_kernel, A, fin_lds, "widom_finish_kernel");
   if (rc) return rc;
   const int nblk = (n_insert + kRedBlock * kRedPer - 1) / (kRedBlock * kRedPer);
   hipLaunchKernelGGL(widom_reduce_kernel, dim3(nblk), dim3(kRedBlock), 0, b->stream, A.rec_w, A.rec_e, A.rec_f,
