@@ -1,0 +1,1296 @@
+// batch_mc.hip -- device-resident GCMC of many independent boxes in one
+// persistent launch on gfx950: one workgroup (16 waves) owns one box and runs
+// its Monte Carlo steps with no host round trip.
+//
+// One step, as RunMoves (axpy.cu:102-298) does it:
+//   selection of component, molecule and move (Select_Box_Component_Molecule,
+//   axpy.cu:74-101);
+//   translation / rotation (SingleBodyMove, mc_single_particle.h:10-314;
+//   get_new_position, mc_utilities.h:485-606): pair energy change of the
+//   moved molecule, overlap on the new position, Ewald change, Metropolis;
+//   insertion / deletion / Widom (Insertion_Body / Deletion_Body,
+//   mc_swap_utilities.h:3-225, with the CBMC trial functions of mc_widom.h:14-614
+//   and GetPrefactor, mc_utilities.h:315-351): K first-bead trials, selection,
+//   O orientations of the template about the selected bead, selection, Ewald
+//   and tail terms, Rosenbluth weight, acceptance;
+//   commit (mc_utilities.h:163-292, 357-417): positions, the last molecule
+//   into a deleted molecule's hole, and the structure factors by flipping the
+//   box's double buffer (Update_Vector_Ewald, Ewald_Energy_Functions.h:423-433);
+// cycles of max(20, N_molecules + 1) steps with the max-step update every 500
+// cycles (axpy.cu:378-400, 580-591).
+//
+// Work inside a step: the pair energies of up to 64 probe sites (trial sites,
+// or the old and new sites of a moved molecule) against every framework and
+// guest atom run over all 1024 lanes (lanes over atoms, up to 8 sites per pass
+// held in registers, FP64 sums reduced by a transposing butterfly across the
+// wave, then in wave order across the workgroup, so every sum is
+// deterministic); the Ewald change runs one lane per active k-vector with
+// e^{ik.r} axis tables of the moved sites in LDS; selections, the Metropolis
+// test and bookkeeping run on wave 0.  The force-field rows and, in the fast
+// form, the real-space Coulomb table are staged in LDS once per launch.
+//
+// Random numbers: move t of a box uses philox_uniform(seed, stream, t, d) with
+// the draw layout documented in graspa_engine.h, so a box's trajectory does
+// not depend on the other boxes of the batch or on the launch boundaries.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device_math.hpp"
+#include "engine.hpp"
+#include "move_math.hpp"
+
+namespace graspa {
+
+constexpr int kBT = 1024;              // threads of a box workgroup
+constexpr int kBW = kBT / 64;          // waves
+constexpr int kBMaxTrial = 32;         // K and O
+constexpr int kBMaxMol = 8;            // atoms per molecule
+constexpr int kBMaxSites = 64;         // probe sites of one pair pass
+constexpr int kBGroup = 4;             // sites per pass over the atoms
+constexpr int kBDrawTrial = 16;        // first trial draw
+constexpr int kBMaxDraws = kBDrawTrial + 6 * kBMaxTrial;
+
+enum { kMvTrans = GRASPA_MOVE_TRANSLATION, kMvRot = GRASPA_MOVE_ROTATION, kMvIns = GRASPA_MOVE_INSERTION,
+       kMvDel = GRASPA_MOVE_DELETION, kMvWidom = GRASPA_MOVE_WIDOM, kMvNone = GRASPA_MOVE_NONE };
+
+// Per-species set-up of one box (the Components fields RunMoves reads).
+struct BSpecies {
+  double p_trans, p_rot, p_widom, p_swap;     // cumulative
+  double mol_fraction, fugacity, ideal_rosen, excl;   // excl = ExclusionIntra + ExclusionAtom
+  int base, molsize, capacity, charged;
+  int pseudo[16];
+};
+
+// Read-only description of one box.
+struct BBox {
+  BoxConst bc;
+  double volume;
+  const double* fw;                 // SoA x, y, z, q with stride n_fw
+  const int* fw_type;
+  int n_fw;
+  int guest_cap;
+  double* g;                        // SoA x, y, z, q with stride guest_cap
+  int* g_type;
+  int* g_molid;
+  int* g_comp;
+  const double* ff64;               // {4 eps, sigma^2, shift, eps} per type pair
+  const double* coul;
+  int coul_n;
+  int kstride;
+  const int4* kact;                 // active k-vectors {kx, ky, kz, index}
+  const double* ktemp_act;
+  int n_active;
+  int has_tail;
+  const double2* sk_fw;
+  double2* sk[2];                   // adsorbate S(k) double buffer
+  const double* tail;               // [T*T]
+  const int* use_tail;
+  int n_comp, n_valid;
+  int valid[kMaxComps];             // species with SwapProbability > 1e-10
+  BSpecies sp[kMaxComps];
+  int K, O, bias, use_max_step;
+  int max_step, trace_cap;
+  int cyc_cap, _pad;
+  double beta, pressure;
+  uint64_t seed;
+  uint32_t stream, _pad2;
+  graspa_trace* trace;
+  float* cyc;                       // per-cycle ticks (ring of cyc_cap)
+};
+
+// Mutable state of one box; held in LDS during a launch.
+struct BState {
+  int64_t move, cycle, target;
+  int step, steps;
+  int sk_cur, error, production, _pad;
+  int n_mol[kMaxComps];
+  int64_t total_mol;
+  int64_t n_pseudo[16];
+  double max_trans[kMaxComps][3], max_rot[kMaxComps][3];
+  int64_t cnt[kMaxComps][9];
+  int64_t upd_tot[kMaxComps][2], upd_acc[kMaxComps][2];
+  double widom_w[kMaxComps], widom_n[kMaxComps];
+  double running[10];
+  double load_sum[kMaxComps], load_sum2[kMaxComps];
+  int64_t load_n;
+  uint64_t cycle_t0;
+  int64_t trace_n;
+  int64_t cyc_first;                // first cycle of the current run
+};
+static_assert(sizeof(BState) % 8 == 0, "BState is copied as 64-bit words");
+static_assert(sizeof(BBox) % 8 == 0, "BBox is copied as 64-bit words");
+
+struct BSite {
+  double x, y, z, q;
+  int type, _p;
+};
+
+// LDS workspace of one step.
+struct BMove {
+  BSite site[kBMaxSites];
+  double e[kBMaxSites][4];          // hg vdw, hg real, gg vdw, gg real per site
+  double part[2][kBW][2 * kBGroup];
+  double ud[kBMaxDraws];
+  double lw[kBMaxTrial], shf[kBMaxTrial], ew[kBMaxTrial], t4[kBMaxTrial][4];
+  double molp[kBMaxMol][3], molq[kBMaxMol];
+  int molt[kBMaxMol];
+  double ewp[kBW][2];
+  double en[10];                    // graspa_energy of the move
+  double W, pacc, same, cross;
+  unsigned long long ovl;
+  int type, tmove, comp, mol, M, start, charged;
+  int fail, accept, overlap;
+};
+
+// ---------------------------------------------------------------------------
+// pair energies of the probe sites L.site[0 .. n_sites)
+// ---------------------------------------------------------------------------
+
+// Wave sums of the 2 * kBGroup values of every lane (LJ and real-space sums
+// of the pass's sites): halving exchanges, each lane keeping the half its lane
+// bit selects, until each group of 64 / NV lanes holds one value, then plain
+// exchanges inside the group.  The first lane of group g writes value g.
+__device__ inline void wave_sum_group(const double av[kBGroup], const double ar[kBGroup], double* out)
+{
+  constexpr int NV = 2 * kBGroup;
+  static_assert(NV >= 2 && NV <= 64 && (NV & (NV - 1)) == 0, "power-of-two value count");
+  const int lane = threadIdx.x & 63;
+  double v[NV];
+#pragma unroll
+  for (int j = 0; j < kBGroup; ++j) { v[2 * j] = av[j]; v[2 * j + 1] = ar[j]; }
+  int idx = 0;
+#pragma unroll
+  for (int n = NV, m = 32; n > 1; n >>= 1, m >>= 1) {
+    const bool hi = (lane & m) != 0;
+#pragma unroll
+    for (int i = 0; i < n / 2; ++i) {
+      const double keep = hi ? v[n / 2 + i] : v[i];
+      const double send = hi ? v[i] : v[n / 2 + i];
+      v[i] = keep + __shfl_xor(send, m, 64);
+    }
+    if (hi) idx += n / 2;
+  }
+#pragma unroll
+  for (int m = 64 / NV / 2; m >= 1; m >>= 1) v[0] += __shfl_xor(v[0], m, 64);
+  if ((lane & (64 / NV - 1)) == 0) out[idx] = v[0];
+}
+
+__device__ inline unsigned long long wave_or64(unsigned long long v)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v |= __shfl_xor(v, m, 64);
+  return v;
+}
+
+// Semantics of one pair: Calculate_Multiple_Trial_Energy_VDWReal
+// (VDW_Coulomb.cu:1183-1352) and the single-body kernel (VDW_Coulomb.cu:626-841):
+// LJ inside the VDW cutoff with the overlap flag (E > OverlapCriteria or
+// r^2 < 0.01), erfc real space inside the Coulomb cutoff.
+template <bool MIXED, bool CUBIC>
+__device__ inline void pair_site(const BoxConst& bc, const double* __restrict__ ffl, const double2* __restrict__ ct,
+                                 const int nint, const int T, const BSite& s, const double ax, const double ay,
+                                 const double az, const double aq, const int at, double& ev, double& er, uint32_t& ov,
+                                 const uint32_t bit)
+{
+  double dx = ax - s.x, dy = ay - s.y, dz = az - s.z;
+  if (MIXED) pbc_rint<CUBIC>(dx, dy, dz, bc.cell, bc.inv);
+  else pbc_fp64(dx, dy, dz, bc.cell, bc.inv, CUBIC ? 1 : 0);
+  const double rr = dx * dx + dy * dy + dz * dz;
+  if (rr < bc.cut_vdw2) {
+    const double* f = ffl + 4 * (at * T + s.type);
+    const double e = MIXED ? lj_mixed(rr, f[0], f[1], f[2]) : lj_fp64(rr, f[0], f[1], f[2]);
+    ev += e;
+    if (e > bc.overlap || rr < 0.01) ov |= bit;
+  }
+  if (bc.use_charges && rr < bc.cut_coul2) {
+    if (MIXED) {
+      er += (bc.prefactor * aq * s.q) * coul_table(ct, nint, rr);
+    } else {
+      const double r = sqrt(rr);
+      const double term = aq * s.q * erfc(bc.alpha * r);
+      er += bc.prefactor * 1.0 * term / r;
+    }
+  }
+}
+
+// Energies of sites [0, n_sites) into L.e and their overlap flags into
+// L.ovl (bit s), against the framework (host-guest) and every guest atom
+// except the index range [ex_lo, ex_hi) (the molecule itself).
+template <bool MIXED, bool CUBIC>
+__device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double* __restrict__ ffl,
+                          const double2* __restrict__ ct, const int nint, const int n_sites, const int ex_lo,
+                          const int ex_hi)
+{
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const BoxConst& bc = B.bc;
+  const int T = bc.ntypes;
+  const int ngrp = (n_sites + kBGroup - 1) / kBGroup;
+  const int per = (n_sites + ngrp - 1) / ngrp;
+  const int nf = B.n_fw, cap = B.guest_cap;
+  if (tid == 0) L.ovl = 0ull;
+  unsigned long long ovl = 0ull;
+  for (int g0 = 0; g0 < n_sites; g0 += per) {
+    const int cnt = min(per, n_sites - g0);
+#pragma unroll 1
+    for (int part = 0; part < 2; ++part) {
+      double av[kBGroup], ar[kBGroup];
+      uint32_t ovb = 0;
+#pragma unroll
+      for (int j = 0; j < kBGroup; ++j) { av[j] = 0.0; ar[j] = 0.0; }
+      if (part == 0) {
+#pragma unroll 1
+        for (int i = tid; i < nf; i += kBT) {
+          const double ax = B.fw[i], ay = B.fw[nf + i], az = B.fw[2 * nf + i], aq = B.fw[3 * nf + i];
+          const int at = B.fw_type[i];
+#pragma unroll
+          for (int j = 0; j < kBGroup; ++j)
+            if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[g0 + j], ax, ay, az, aq, at, av[j], ar[j],
+                                                 ovb, 1u << j);
+        }
+      } else {
+        for (int c = 1; c < B.n_comp; ++c) {
+          const int lo = B.sp[c].base, hi = lo + S.n_mol[c] * B.sp[c].molsize;
+#pragma unroll 1
+          for (int i = lo + tid; i < hi; i += kBT) {
+            if (i >= ex_lo && i < ex_hi) continue;
+            const double ax = B.g[i], ay = B.g[cap + i], az = B.g[2 * cap + i], aq = B.g[3 * cap + i];
+            const int at = B.g_type[i];
+#pragma unroll
+            for (int j = 0; j < kBGroup; ++j)
+              if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[g0 + j], ax, ay, az, aq, at, av[j],
+                                                   ar[j], ovb, 1u << j);
+          }
+        }
+      }
+      wave_sum_group(av, ar, L.part[part][wave]);
+      ovl |= (unsigned long long)ovb << g0;
+    }
+    __syncthreads();
+    if (tid < 4 * cnt) {
+      const int j = tid >> 2, t = tid & 3, part = t >> 1, k = t & 1;
+      double s = 0.0;
+      for (int w = 0; w < kBW; ++w) s += L.part[part][w][2 * j + k];
+      L.e[g0 + j][t] = s;
+    }
+    __syncthreads();
+  }
+  ovl = wave_or64(ovl);
+  if (lane == 0 && ovl) atomicOr(&L.ovl, ovl);
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Ewald: Fourier energy change of removing sites [0, n_old) and adding sites
+// [n_old, n_old + n_new) of L.site (GPU_EwaldDifference_General,
+// Ewald_Energy_Functions.h:438-580 with Fourier_Ewald_Diff :280-397): raw
+// same-type and cross sums into L.same / L.cross; the updated adsorbate S(k)
+// goes to the box's other buffer.
+// ---------------------------------------------------------------------------
+__device__ void ewald_pass(const BBox& B, const BState& S, BMove& L, double2* __restrict__ eik, const int n_old,
+                           const int n_new)
+{
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int ns = n_old + n_new, ks = B.kstride;
+  for (int t = tid; t < ns * 3; t += kBT) {
+    const int site = t / 3, axis = t - 3 * site;
+    axis_table(B.bc, L.site[site].x, L.site[site].y, L.site[site].z, axis, eik + (site * 3 + axis) * ks);
+  }
+  __syncthreads();
+  const double2* __restrict__ cur = B.sk[S.sk_cur];
+  double2* __restrict__ stg = B.sk[S.sk_cur ^ 1];
+  double a0 = 0.0, a1 = 0.0;
+  for (int j = tid; j < B.n_active; j += kBT) {
+    const int4 kv = B.kact[j];
+    const double t = B.ktemp_act[j];
+    double2 co = make_double2(0.0, 0.0), cn = make_double2(0.0, 0.0);
+    for (int i = 0; i < ns; ++i) {
+      const double2* e = eik + i * 3 * ks;
+      double2 ey = e[ks + abs(kv.y)];
+      if (kv.y < 0) ey.y = -ey.y;
+      double2 ez = e[2 * ks + abs(kv.z)];
+      if (kv.z < 0) ez.y = -ez.y;
+      const double2 p = cmul2(cmul2(e[kv.x], ey), ez);
+      const double q = L.site[i].q;
+      if (i < n_old) { co.x += q * p.x; co.y += q * p.y; }
+      else { cn.x += q * p.x; cn.y += q * p.y; }
+    }
+    const double2 ov = cur[kv.w];
+    const double2 nv = make_double2(ov.x + cn.x - co.x, ov.y + cn.y - co.y);
+    a0 += t * (nv.x * nv.x + nv.y * nv.y) - t * (ov.x * ov.x + ov.y * ov.y);
+    const double2 cv = B.sk_fw[kv.w];
+    a1 += t * (cv.x * (cn.x - co.x) + cv.y * (cn.y - co.y));
+    stg[kv.w] = nv;
+  }
+  a0 = wave_sum(a0);
+  a1 = wave_sum(a1);
+  if (lane == 0) { L.ewp[wave][0] = a0; L.ewp[wave][1] = a1; }
+  __syncthreads();
+  if (tid == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int w = 0; w < kBW; ++w) { s0 += L.ewp[w][0]; s1 += L.ewp[w][1]; }
+    L.same = s0;
+    L.cross = s1;
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// CBMC selection on wave 0 (SelectTrialPosition, mc_widom.h:14-39;
+// CBMC_FirstBead_Finish :305-383; Widom_Move_Chain_PARTIAL :574-614): trial
+// sums L.t4[0..n) with overlap bits ovm.  Survivors in trial order, log-sum-exp
+// shift, cumulative pick with u (insertion) or the first survivor
+// (deletion), Rosenbluth sum over n, the real-space correction when the bias
+// is LJ only.  Returns the selected trial (-1 on failure) and multiplies *W.
+// ---------------------------------------------------------------------------
+__device__ int select_wave0(const BBox& B, BMove& L, const int n, const uint32_t ovm, const double u, const bool del,
+                            const bool chain, double* W)
+{
+  const int lane = threadIdx.x & 63;
+  const bool alive = lane < n && !((ovm >> lane) & 1u);
+  double lw = -INFINITY;
+  if (alive) {
+    double tot = L.t4[lane][0] + L.t4[lane][2];
+    if (B.bias) tot += L.t4[lane][1] + L.t4[lane][3];
+    lw = -B.beta * tot;
+  }
+  const double mx = wave_max(lw);
+  if (lane < n) {
+    L.shf[lane] = alive ? exp(lw - mx) : -1.0;
+    L.ew[lane] = alive ? exp(lw) : -1.0;
+  }
+  wave_sync();
+  int sel = -1;
+  if (lane == 0) {
+    int nk = 0, first = -1;
+    double ssum = 0.0, rb = 0.0;
+    for (int t = 0; t < n; ++t)
+      if (L.shf[t] >= 0.0) {
+        if (first < 0) first = t;
+        ssum += L.shf[t];
+        rb += L.ew[t];
+        ++nk;
+      }
+    bool fail = false;
+    if (!del) {
+      if (nk == 0) {
+        fail = true;
+      } else {
+        const double ws = u * ssum;
+        int t = first;
+        double cumw = L.shf[t];
+        int left = nk - 1;
+        while (cumw < ws && left > 0) {
+          ++t;
+          while (L.shf[t] < 0.0) ++t;
+          cumw += L.shf[t];
+          --left;
+        }
+        sel = t;
+        if (rb < 1e-150) fail = true;
+      }
+    } else {
+      sel = first;
+      if (chain && nk == 0) fail = true;
+    }
+    if (!fail) {
+      double w = rb / (double)n;
+      if (sel >= 0 && !B.bias) w *= exp(-B.beta * (L.t4[sel][1] + L.t4[sel][3]));
+      w *= *W;
+      if (w <= 1e-150) fail = true;
+      *W = w;
+    }
+    if (fail) sel = -1;
+  }
+  return sel;
+}
+
+// TailCorrectionDifference (TailCorrection_Energy_Functions.h:36-82)
+__device__ double tail_delta(const BBox& B, const BState& S, const int comp, const int sign)
+{
+  const int T = B.bc.ntypes;
+  double e = 0.0;
+  for (int i = 0; i < T && i < 16; ++i) {
+    const int64_t di = (int64_t)B.sp[comp].pseudo[i] * sign;
+    for (int j = i; j < T && j < 16; ++j) {
+      const int64_t dj = (int64_t)B.sp[comp].pseudo[j] * sign;
+      const int ij = i * T + j;
+      if (!B.use_tail[ij]) continue;
+      const int64_t dn = S.n_pseudo[i] * dj + S.n_pseudo[j] * di + di * dj;
+      double de = B.tail[ij] * (double)dn;
+      if (i != j) de *= 2.0;
+      e += de;
+    }
+  }
+  return e / B.volume;
+}
+
+__device__ inline void add_energy(double* run, const double* en)
+{
+  for (int i = 0; i < 10; ++i) run[i] += en[i];
+}
+// graspa_energy order: hh_vdw, hg_vdw, gg_vdw, hh_real, hg_real, gg_real,
+// hh_ewald, hg_ewald, gg_ewald, tail
+enum { kHGV = 1, kGGV = 2, kHGR = 4, kGGR = 5, kHGE = 7, kGGE = 8, kTail = 9 };
+
+// ---------------------------------------------------------------------------
+// the moves
+// ---------------------------------------------------------------------------
+template <bool MIXED, bool CUBIC>
+__device__ void single_body(const BBox& B, BState& S, BMove& L, const double* ffl, const double2* ct, int nint,
+                            double2* eik)
+{
+  const int tid = threadIdx.x;
+  const int M = L.M, start = L.start, cap = B.guest_cap, comp = L.comp;
+  if (tid < M) {
+    const int j = start + tid;
+    BSite s;
+    s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j]; s.q = B.g[3 * cap + j];
+    s.type = B.g_type[j]; s._p = 0;
+    L.site[tid] = s;
+  }
+  __syncthreads();
+  if (tid < M) {
+    const BSite o = L.site[tid];
+    BSite s = o;
+    if (L.type == kMvTrans) {
+      s.x = o.x + S.max_trans[comp][0] * 2.0 * (L.ud[8] - 0.5);
+      s.y = o.y + S.max_trans[comp][1] * 2.0 * (L.ud[9] - 0.5);
+      s.z = o.z + S.max_trans[comp][2] * 2.0 * (L.ud[10] - 0.5);
+    } else {
+      const BSite a0 = L.site[0];
+      double p[3] = {o.x - a0.x, o.y - a0.y, o.z - a0.z};
+      rotate_axis(p, S.max_rot[comp][0] * 2.0 * (L.ud[8] - 0.5), 1.0, 0.0, 0.0);
+      rotate_axis(p, S.max_rot[comp][1] * 2.0 * (L.ud[9] - 0.5), 0.0, 1.0, 0.0);
+      rotate_axis(p, S.max_rot[comp][2] * 2.0 * (L.ud[10] - 0.5), 0.0, 0.0, 1.0);
+      s.x = p[0] + a0.x; s.y = p[1] + a0.y; s.z = p[2] + a0.z;
+    }
+    L.site[M + tid] = s;
+  }
+  __syncthreads();
+  pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, 2 * M, start, start + M);
+  if (tid == 0) {
+    const unsigned long long newmask = ((1ull << M) - 1ull) << M;
+    L.overlap = (L.ovl & newmask) ? 1 : 0;
+    if (!L.overlap) {
+      for (int a = 0; a < M; ++a) {
+        L.en[kHGV] += L.e[M + a][0] - L.e[a][0];
+        L.en[kHGR] += L.e[M + a][1] - L.e[a][1];
+        L.en[kGGV] += L.e[M + a][2] - L.e[a][2];
+        L.en[kGGR] += L.e[M + a][3] - L.e[a][3];
+      }
+    }
+  }
+  __syncthreads();
+  if (L.overlap) return;
+  if (L.charged) {
+    ewald_pass(B, S, L, eik, M, M);
+    if (tid == 0) {
+      L.en[kGGE] = L.same;
+      L.en[kHGE] = 2.0 * L.cross;
+    }
+  }
+  if (tid == 0) {
+    double tot = 0.0;
+    for (int i = 0; i < 10; ++i) tot += L.en[i];
+    L.pacc = 1.0 * exp(-B.beta * tot);
+    L.accept = (L.ud[4] < L.pacc) ? 1 : 0;
+    const int k = (L.type == kMvTrans) ? 0 : 1;
+    if (L.accept) {
+      S.cnt[comp][2 * k + 1]++;
+      S.upd_acc[comp][k]++;
+      add_energy(S.running, L.en);
+      if (L.charged) S.sk_cur ^= 1;
+    }
+  }
+  __syncthreads();
+  if (L.accept && tid < M) {
+    const int j = start + tid;
+    const BSite s = L.site[M + tid];
+    B.g[j] = s.x; B.g[cap + j] = s.y; B.g[2 * cap + j] = s.z;
+  }
+}
+
+template <bool MIXED, bool CUBIC>
+__device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, const double2* ct, int nint,
+                     double2* eik)
+{
+  const int tid = threadIdx.x;
+  const int M = L.M, comp = L.comp, cap = B.guest_cap;
+  const int K = B.K, O = B.O, chain = M - 1;
+  const bool del = (L.type == kMvDel);
+  const BSpecies& sp = B.sp[comp];
+  const int tstart = del ? L.start : sp.base;   // types and charges of the molecule's atoms
+  const int ex_lo = del ? L.start : 0, ex_hi = del ? L.start + M : 0;
+  // first-bead trials (get_random_trial_position, mc_widom.h:122-213)
+  if (tid < K) {
+    BSite s;
+    if (del && tid == 0) {
+      s.x = B.g[L.start]; s.y = B.g[cap + L.start]; s.z = B.g[2 * cap + L.start];
+    } else {
+      s.x = B.bc.cell[0] * L.ud[kBDrawTrial + 3 * tid];
+      s.y = B.bc.cell[4] * L.ud[kBDrawTrial + 3 * tid + 1];
+      s.z = B.bc.cell[8] * L.ud[kBDrawTrial + 3 * tid + 2];
+    }
+    s.q = B.g[3 * cap + tstart];
+    s.type = B.g_type[tstart];
+    s._p = 0;
+    L.site[tid] = s;
+  }
+  __syncthreads();
+  pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, K, ex_lo, ex_hi);
+  if (tid < K) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) L.t4[tid][t] = L.e[tid][t];
+  }
+  __syncthreads();
+  if (tid < 64) {
+    double W = 1.0;
+    const int sel = select_wave0(B, L, K, (uint32_t)L.ovl, L.ud[5], del, false, &W);
+    if (tid == 0) {
+      if (sel < 0) {
+        L.fail = 1;
+      } else {
+        L.W = W;
+        L.en[kHGV] = L.t4[sel][0]; L.en[kHGR] = L.t4[sel][1];
+        L.en[kGGV] = L.t4[sel][2]; L.en[kGGR] = L.t4[sel][3];
+        L.molp[0][0] = L.site[sel].x; L.molp[0][1] = L.site[sel].y; L.molp[0][2] = L.site[sel].z;
+        L.molq[0] = L.site[sel].q;
+        L.molt[0] = L.site[sel].type;
+      }
+    }
+  }
+  __syncthreads();
+  if (L.fail) return;
+  if (chain > 0) {
+    // orientations (get_random_trial_orientation, mc_widom.h:215-303): the
+    // template vectors of the component's first slot as it is now, rotated
+    for (int t = tid; t < O * chain; t += kBT) {
+      const int o = t / chain, a = t - o * chain;
+      BSite s;
+      if (del && o == 0) {
+        const int j = L.start + 1 + a;
+        s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j];
+      } else {
+        const int j0 = sp.base, j = sp.base + 1 + a;
+        double v[3] = {B.g[j] - B.g[j0], B.g[cap + j] - B.g[cap + j0], B.g[2 * cap + j] - B.g[2 * cap + j0]};
+        const int d = kBDrawTrial + 3 * K + 3 * o;
+        quat_rotate(v, L.ud[d], L.ud[d + 1], L.ud[d + 2]);
+        s.x = L.molp[0][0] + v[0]; s.y = L.molp[0][1] + v[1]; s.z = L.molp[0][2] + v[2];
+      }
+      s.q = B.g[3 * cap + tstart + 1 + a];
+      s.type = B.g_type[tstart + 1 + a];
+      s._p = 0;
+      L.site[t] = s;
+    }
+    __syncthreads();
+    pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, O * chain, ex_lo, ex_hi);
+    if (tid < O) {
+      double e4[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int a = 0; a < chain; ++a)
+        for (int t = 0; t < 4; ++t) e4[t] += L.e[tid * chain + a][t];
+      for (int t = 0; t < 4; ++t) L.t4[tid][t] = e4[t];
+    }
+    uint32_t ovo = 0;
+    const unsigned long long amask = (chain >= 64) ? ~0ull : ((1ull << chain) - 1ull);
+    for (int o = 0; o < O; ++o)
+      if ((L.ovl >> (o * chain)) & amask) ovo |= 1u << o;
+    __syncthreads();
+    if (tid < 64) {
+      double W = L.W;
+      const int sel = select_wave0(B, L, O, ovo, L.ud[6], del, true, &W);
+      if (tid == 0) {
+        if (sel < 0) {
+          L.fail = 1;
+        } else {
+          L.W = W;
+          L.en[kHGV] += L.t4[sel][0]; L.en[kHGR] += L.t4[sel][1];
+          L.en[kGGV] += L.t4[sel][2]; L.en[kGGR] += L.t4[sel][3];
+          for (int a = 0; a < chain; ++a) {
+            const BSite& s = L.site[sel * chain + a];
+            L.molp[1 + a][0] = s.x; L.molp[1 + a][1] = s.y; L.molp[1 + a][2] = s.z;
+            L.molq[1 + a] = s.q;
+            L.molt[1 + a] = s.type;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (L.fail) return;
+  }
+  // Ewald of the whole molecule: the new one, or the real one for a deletion
+  if (L.charged) {
+    if (tid < M) {
+      BSite s;
+      if (del) {
+        const int j = L.start + tid;
+        s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j]; s.q = B.g[3 * cap + j]; s.type = B.g_type[j];
+      } else {
+        s.x = L.molp[tid][0]; s.y = L.molp[tid][1]; s.z = L.molp[tid][2]; s.q = L.molq[tid]; s.type = L.molt[tid];
+      }
+      s._p = 0;
+      L.site[tid] = s;
+    }
+    __syncthreads();
+    ewald_pass(B, S, L, eik, del ? M : 0, del ? 0 : M);
+  }
+  if (tid == 0) {
+    double W = L.W;
+    const double N = (double)S.n_mol[comp];
+    const double bz = B.beta * sp.mol_fraction * B.pressure * sp.fugacity * B.volume;
+    const double pref = del ? N / bz : bz / (1.0 + N);
+    if (L.charged) {
+      double same = L.same;
+      if (!del) {
+        same -= sp.excl * 1.0;
+        L.en[kGGE] = same;
+        L.en[kHGE] = 2.0 * L.cross;
+        W *= exp(-B.beta * (same + 2.0 * L.cross));
+      } else {
+        same -= sp.excl * (0.0 - 1.0);
+        W /= exp(-B.beta * (same + 2.0 * L.cross));
+        L.en[kGGE] = -1.0 * same;
+        L.en[kHGE] = -1.0 * (2.0 * L.cross);
+      }
+    }
+    if (B.has_tail) {
+      const double te = tail_delta(B, S, comp, del ? -1 : 1);
+      if (!del) { W *= exp(-B.beta * te); L.en[kTail] = te; }
+      else { W /= exp(-B.beta * te); L.en[kTail] = -te; }
+    }
+    L.W = W;
+    if (L.type == kMvWidom) {
+      if (S.production) { S.widom_w[comp] += W; S.widom_n[comp] += 1.0; }
+    } else {
+      double pacc = del ? pref * sp.ideal_rosen / W : pref * W / sp.ideal_rosen;
+      if (!del && W <= 1e-150) pacc = 0.0;
+      L.pacc = pacc;
+      int acc = (L.ud[4] < pacc) ? 1 : 0;
+      if (acc && !del && S.n_mol[comp] + 1 > sp.capacity) {
+        S.error = GRASPA_ERR_CAPACITY;
+        acc = 0;
+      }
+      L.accept = acc;
+      if (acc) {
+        if (!del) {
+          S.total_mol++;
+          for (int t = 0; t < 16; ++t) S.n_pseudo[t] += sp.pseudo[t];
+          S.cnt[comp][5]++;
+        } else {
+          S.total_mol--;
+          for (int t = 0; t < 16; ++t) S.n_pseudo[t] -= sp.pseudo[t];
+          S.cnt[comp][7]++;
+          for (int i = 0; i < 10; ++i) L.en[i] = -L.en[i];
+        }
+        add_energy(S.running, L.en);
+        if (L.charged) S.sk_cur ^= 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (L.accept) {
+    const int N = S.n_mol[comp];
+    if (!del) {
+      if (tid < M) {
+        const int j = sp.base + N * M + tid;
+        B.g[j] = L.molp[tid][0]; B.g[cap + j] = L.molp[tid][1]; B.g[2 * cap + j] = L.molp[tid][2];
+        B.g[3 * cap + j] = L.molq[tid];
+        B.g_type[j] = L.molt[tid];
+        B.g_molid[j] = N;
+        B.g_comp[j] = comp;
+      }
+    } else {
+      // the last molecule moves into the hole, whose MolIDs stay
+      // (Update_deletion_data_Parallel, mc_utilities.h:163-189)
+      const int last = sp.base + (N - 1) * M;
+      if (L.start != last && tid < M) {
+        const int j = L.start + tid, k = last + tid;
+        B.g[j] = B.g[k]; B.g[cap + j] = B.g[cap + k]; B.g[2 * cap + j] = B.g[2 * cap + k];
+        B.g[3 * cap + j] = B.g[3 * cap + k];
+        B.g_type[j] = B.g_type[k];
+      }
+    }
+    __syncthreads();
+    if (tid == 0) S.n_mol[comp] += del ? -1 : 1;
+  }
+}
+
+// Update_Max_Translation / Update_Max_Rotation (mc_utilities.h:612-665)
+__device__ inline void update_max(double* v, int64_t& acc, int64_t& tot, double hi)
+{
+  const double ratio = (double)acc / (double)tot;
+  for (int d = 0; d < 3; ++d) {
+    v[d] *= (ratio > 0.5) ? 1.05 : 0.95;
+    if (v[d] < 0.01) v[d] = 0.01;
+    if (v[d] > hi) v[d] = hi;
+  }
+  acc = 0;
+  tot = 0;
+}
+
+template <bool MIXED, bool CUBIC>
+__global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ boxes, BState* __restrict__ states,
+                                                       const int max_moves)
+{
+  extern __shared__ double4 dyn4[];
+  __shared__ BState S;
+  __shared__ BMove L;
+  __shared__ BBox B;
+  double* dyn = reinterpret_cast<double*>(dyn4);
+  const int tid = threadIdx.x;
+  {
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&states[blockIdx.x]);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(&S);
+    for (int i = tid; i < (int)(sizeof(BState) / 8); i += kBT) dst[i] = src[i];
+    const uint64_t* bsrc = reinterpret_cast<const uint64_t*>(&boxes[blockIdx.x]);
+    uint64_t* bdst = reinterpret_cast<uint64_t*>(&B);
+    for (int i = tid; i < (int)(sizeof(BBox) / 8); i += kBT) bdst[i] = bsrc[i];
+  }
+  __syncthreads();
+  const int T = B.bc.ntypes;
+  const int nff = 4 * T * T;
+  const int nff4 = (nff + 3) & ~3;
+  double* ffl = dyn;
+  double2* ct = reinterpret_cast<double2*>(dyn + nff4);
+  const int nct = MIXED ? ((B.coul_n + 3) & ~3) : 0;
+  double2* eik = reinterpret_cast<double2*>(dyn + nff4 + nct);
+  const int nint = B.coul_n / kCoulTabStride;
+  for (int i = tid; i < nff; i += kBT) ffl[i] = B.ff64[i];
+  if (MIXED)
+    for (int i = tid; i < B.coul_n; i += kBT) dyn[nff4 + i] = B.coul[i];
+  __syncthreads();
+  const int nd = kBDrawTrial + 3 * B.K + 3 * B.O;
+  for (int mv = 0; mv < max_moves; ++mv) {
+    if (S.cycle >= S.target || S.error) break;
+    // this move's uniforms, one Philox block (two uniforms) per thread
+    if (2 * tid < nd) {
+      uint32_t c[4] = {(uint32_t)S.move, (uint32_t)((uint64_t)S.move >> 32), (uint32_t)tid, B.stream};
+      philox4x32_10(c, (uint32_t)B.seed, (uint32_t)(B.seed >> 32));
+      L.ud[2 * tid] = ((double)((((uint64_t)c[0] << 32) | c[1]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+      L.ud[2 * tid + 1] = ((double)((((uint64_t)c[2] << 32) | c[3]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    }
+    if (tid == 0 && S.step == 0) {
+      // Determine_Number_Of_Steps (axpy.cu:580-591)
+      int64_t steps = 20;
+      if (steps < S.total_mol) steps = S.total_mol;
+      if (B.use_max_step && steps > B.max_step) steps = B.max_step;
+      S.steps = (int)steps;
+      S.cycle_t0 = wall_clock64();
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const double* u = L.ud;
+      int vi = (int)(u[0] * (double)B.n_valid);
+      if (vi >= B.n_valid) vi = B.n_valid - 1;
+      const int comp = B.valid[vi];
+      const int n = S.n_mol[comp];
+      const int mol = (int)(u[1] * (double)n);
+      const double r = u[2];
+      const BSpecies& sp = B.sp[comp];
+      int tm = kMvNone, work = 0;
+      if (r < sp.p_trans) { tm = kMvTrans; work = n > 0; }
+      else if (r < sp.p_rot) { tm = kMvRot; work = n > 0; }
+      else if (r < sp.p_widom) { tm = kMvWidom; work = 1; }
+      else if (r < sp.p_swap) {
+        if (u[3] < 0.5) { tm = kMvIns; work = 1; }
+        else { tm = kMvDel; work = n > 0; }
+      }
+      L.tmove = tm;
+      L.type = work ? tm : kMvNone;
+      L.comp = comp;
+      L.mol = mol;
+      L.M = sp.molsize;
+      L.start = sp.base + mol * sp.molsize;
+      L.charged = sp.charged;
+      for (int i = 0; i < 10; ++i) L.en[i] = 0.0;
+      L.W = 0.0; L.pacc = 0.0; L.same = 0.0; L.cross = 0.0;
+      L.fail = 0; L.accept = 0; L.overlap = 0;
+      if (work) {
+        if (tm == kMvTrans) { S.cnt[comp][0]++; S.upd_tot[comp][0]++; }
+        else if (tm == kMvRot) { S.cnt[comp][2]++; S.upd_tot[comp][1]++; }
+        else if (tm == kMvIns) S.cnt[comp][4]++;
+        else if (tm == kMvDel) S.cnt[comp][6]++;
+        else if (tm == kMvWidom) S.cnt[comp][8]++;
+      }
+    }
+    __syncthreads();
+    const int type = L.type;
+    if (type == kMvTrans || type == kMvRot) single_body<MIXED, CUBIC>(B, S, L, ffl, ct, nint, eik);
+    else if (type != kMvNone) cbmc<MIXED, CUBIC>(B, S, L, ffl, ct, nint, eik);
+    __syncthreads();
+    if (tid == 0) {
+      const int comp = L.comp;
+      if (L.fail && L.type == kMvWidom && S.production) S.widom_n[comp] += 1.0;
+      if (B.trace && S.trace_n < B.trace_cap) {
+        graspa_trace& tr = B.trace[(size_t)blockIdx.x * B.trace_cap + S.trace_n];
+        tr.cycle = (int)S.cycle; tr.step = S.step; tr.move = L.tmove; tr.comp = comp; tr.mol = L.mol;
+        tr.accepted = L.accept; tr.overlap = L.overlap; tr._pad = 0;
+        tr.pacc = L.pacc;
+        tr.rosen = L.fail ? 0.0 : L.W;
+        const bool keep = (L.accept || L.type == kMvWidom) && !L.fail;
+        double* de = &tr.de.hh_vdw;
+        for (int i = 0; i < 10; ++i) de[i] = keep ? L.en[i] : 0.0;
+      }
+      S.trace_n++;
+      S.move++;
+      S.step++;
+      if (S.step >= S.steps) {
+        // end of cycle: loadings, max-step update (axpy.cu:378-400), latency
+        if (S.production) {
+          for (int c = 1; c < B.n_comp; ++c) {
+            const double nm = (double)S.n_mol[c];
+            S.load_sum[c] += nm;
+            S.load_sum2[c] += nm * nm;
+          }
+          S.load_n++;
+        }
+        if (S.cycle % 500 == 0) {
+          for (int c = 1; c < B.n_comp; ++c) {
+            if (S.upd_tot[c][0] > 0) update_max(S.max_trans[c], S.upd_acc[c][0], S.upd_tot[c][0], 5.0);
+            if (S.upd_tot[c][1] > 0) update_max(S.max_rot[c], S.upd_acc[c][1], S.upd_tot[c][1], 3.14);
+          }
+        }
+        if (B.cyc) B.cyc[S.cycle % B.cyc_cap] = (float)(wall_clock64() - S.cycle_t0);
+        S.cycle++;
+        S.step = 0;
+      }
+    }
+    __syncthreads();
+  }
+  {
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&S);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(&states[blockIdx.x]);
+    for (int i = tid; i < (int)(sizeof(BState) / 8); i += kBT) dst[i] = src[i];
+  }
+}
+
+}  // namespace graspa
+
+using namespace graspa;
+
+struct graspa_batch {
+  int device = 0;
+  int precision = GRASPA_FP64;
+  int cubic = 1;
+  hipStream_t stream = nullptr;
+  int n = 0;
+  std::vector<graspa_box*> boxes;
+  std::vector<BBox> hb;
+  std::vector<BState> hs;
+  std::vector<std::vector<double>> tails;
+  std::vector<std::vector<int>> use_tails;
+  BBox* d_box = nullptr;
+  BState* d_state = nullptr;
+  double* d_tail = nullptr;
+  int* d_use_tail = nullptr;
+  graspa_trace* d_trace = nullptr;
+  int trace_cap = 0;
+  float* d_cyc = nullptr;
+  int cyc_cap = 0;
+  size_t lds = 0;
+  double tick_hz = 1.0e8;
+  double kernel_ms = 0.0;
+  int64_t launches = 0;
+};
+
+namespace {
+
+int upload_batch(graspa_batch* bt)
+{
+  if (hip_check(hipMemcpyAsync(bt->d_box, bt->hb.data(), bt->n * sizeof(BBox), hipMemcpyHostToDevice, bt->stream),
+                "batch upload"))
+    return GRASPA_ERR_HIP;
+  if (hip_check(hipMemcpyAsync(bt->d_state, bt->hs.data(), bt->n * sizeof(BState), hipMemcpyHostToDevice, bt->stream),
+                "batch upload"))
+    return GRASPA_ERR_HIP;
+  return hip_check(hipStreamSynchronize(bt->stream), "batch upload");
+}
+
+int download_states(graspa_batch* bt)
+{
+  if (hip_check(hipMemcpyAsync(bt->hs.data(), bt->d_state, bt->n * sizeof(BState), hipMemcpyDeviceToHost, bt->stream),
+                "batch state copy"))
+    return GRASPA_ERR_HIP;
+  return hip_check(hipStreamSynchronize(bt->stream), "batch state copy");
+}
+
+void free_batch(graspa_batch* bt)
+{
+  if (!bt) return;
+  hipSetDevice(bt->device);
+  if (bt->stream) hipStreamSynchronize(bt->stream);
+  hipFree(bt->d_box); hipFree(bt->d_state); hipFree(bt->d_tail); hipFree(bt->d_use_tail);
+  hipFree(bt->d_trace); hipFree(bt->d_cyc);
+  if (bt->stream) hipStreamDestroy(bt->stream);
+  delete bt;
+}
+
+template <bool MIXED, bool CUBIC>
+int launch_batch(graspa_batch* bt, int max_moves)
+{
+  auto kern = batch_mc_kernel<MIXED, CUBIC>;
+  (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bt->lds);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, bt->stream);
+  hipLaunchKernelGGL(kern, dim3(bt->n), dim3(kBT), bt->lds, bt->stream, bt->d_box, bt->d_state, max_moves);
+  hipError_t le = hipGetLastError();
+  hipEventRecord(e1, bt->stream);
+  hipError_t se = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (le == hipSuccess && se == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (hip_check(le, "batch_mc_kernel launch")) return GRASPA_ERR_HIP;
+  if (hip_check(se, "batch_mc_kernel")) return GRASPA_ERR_HIP;
+  bt->kernel_ms += ms;
+  bt->launches++;
+  return GRASPA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int graspa_batch_create(const graspa_batch_box* boxes, int n_boxes, int trace_cap, int cycle_cap, graspa_batch** out)
+{
+  if (!boxes || n_boxes < 1 || !out || trace_cap < 0 || cycle_cap < 0) return fail(GRASPA_ERR_ARG, "bad arguments");
+  *out = nullptr;
+  graspa_batch* bt = new graspa_batch();
+  bt->n = n_boxes;
+  bt->device = boxes[0].box ? boxes[0].box->device : 0;
+  bt->precision = boxes[0].box ? boxes[0].box->precision : GRASPA_FP64;
+  bt->trace_cap = trace_cap;
+  bt->cyc_cap = cycle_cap;
+  bt->hb.resize(n_boxes);
+  bt->hs.resize(n_boxes);
+  bt->tails.resize(n_boxes);
+  bt->use_tails.resize(n_boxes);
+  int max_t = 1, max_coul = 0, max_ks = 1;
+  size_t tail_total = 0;
+  for (int i = 0; i < n_boxes; ++i) {
+    const graspa_batch_box& in = boxes[i];
+    graspa_box* b = in.box;
+    const graspa_mc_params* p = in.params;
+    if (!b || !p) { free_batch(bt); return fail(GRASPA_ERR_ARG, "null box or params"); }
+    if (b->device != bt->device || b->precision != bt->precision) {
+      free_batch(bt);
+      return fail(GRASPA_ERR_ARG, "all boxes of a batch must share the device and the precision form");
+    }
+    const int ncomp = (int)b->comps.size();
+    if (p->n_species != ncomp - 1 || ncomp < 2) {
+      free_batch(bt);
+      return fail(GRASPA_ERR_ARG, "n_species must match the box's adsorbate components");
+    }
+    if (p->n_trial_pos < 1 || p->n_trial_pos > kBMaxTrial || p->n_trial_orient < 1 || p->n_trial_orient > kBMaxTrial) {
+      free_batch(bt);
+      return fail(GRASPA_ERR_ARG, "trial counts must be in 1..32");
+    }
+    if (b->bc.ntypes > kMaxTypes) { free_batch(bt); return fail(GRASPA_ERR_ARG, "too many atom types"); }
+    if (b->bc.use_charges && b->nvec <= 0) {
+      free_batch(bt);
+      return fail(GRASPA_ERR_STATE, "charged box without Ewald tables");
+    }
+    if (p->has_tail && (!p->tail || !p->use_tail)) {
+      free_batch(bt);
+      return fail(GRASPA_ERR_ARG, "tail tables required when has_tail");
+    }
+    hipSetDevice(b->device);
+    hipStreamSynchronize(b->stream);
+    BBox& h = bt->hb[i];
+    std::memset(&h, 0, sizeof(h));
+    h.bc = b->bc;
+    h.volume = b->volume;
+    h.fw = b->d_fw; h.fw_type = b->d_fw_type; h.n_fw = b->n_fw;
+    h.guest_cap = b->guest_cap;
+    h.g = b->d_g; h.g_type = b->d_g_type; h.g_molid = b->d_g_molid; h.g_comp = b->d_g_comp;
+    h.ff64 = b->d_ff64;
+    h.coul = b->d_coul; h.coul_n = b->coul_n;
+    h.kstride = 1 + std::max(b->bc.kmax[0], std::max(b->bc.kmax[1], b->bc.kmax[2]));
+    h.kact = b->d_kact; h.ktemp_act = b->d_ktemp_act; h.n_active = b->bc.use_charges ? b->n_active : 0;
+    h.sk_fw = b->d_sk_fw; h.sk[0] = b->d_sk_ads; h.sk[1] = b->d_sk_stage;
+    h.n_comp = ncomp;
+    h.n_valid = 0;
+    for (int c = 1; c < ncomp; ++c) {
+      const graspa_species& s = p->species[c - 1];
+      const Component& cc = b->comps[c];
+      if (cc.molsize > kBMaxMol) { free_batch(bt); return fail(GRASPA_ERR_ARG, "molecules of at most 8 atoms"); }
+      if ((cc.molsize - 1) * p->n_trial_orient > kBMaxSites) {
+        free_batch(bt);
+        return fail(GRASPA_ERR_ARG, "n_trial_orient * (molsize - 1) must be <= 64");
+      }
+      BSpecies& d = h.sp[c];
+      d.p_trans = s.p_translation; d.p_rot = s.p_rotation; d.p_widom = s.p_widom; d.p_swap = s.p_swap;
+      d.mol_fraction = s.mol_fraction; d.fugacity = s.fugacity_coeff; d.ideal_rosen = s.ideal_rosen;
+      d.excl = s.excl_intra + s.excl_self;
+      d.base = cc.base; d.molsize = cc.molsize; d.capacity = cc.capacity;
+      double qabs = 0.0;
+      for (int a = 0; a < cc.molsize; ++a) qabs += std::fabs(cc.q[a]);
+      d.charged = (b->bc.use_charges && qabs > 1e-6) ? 1 : 0;   // hasPartialCharge (read_data.cpp:2160-2164)
+      for (int t = 0; t < 16; ++t) d.pseudo[t] = (int)s.pseudo_per_mol[t];
+      if (s.p_swap > 1e-10) h.valid[h.n_valid++] = c;
+    }
+    if (h.n_valid == 0) { free_batch(bt); return fail(GRASPA_ERR_ARG, "no species with a move probability"); }
+    h.K = p->n_trial_pos; h.O = p->n_trial_orient; h.bias = p->vdw_real_bias;
+    h.use_max_step = p->use_max_step; h.max_step = p->max_step_per_cycle;
+    h.beta = p->beta; h.pressure = p->pressure;
+    h.seed = in.seed; h.stream = in.stream;
+    h.has_tail = p->has_tail;
+    h.trace_cap = trace_cap;
+    h.cyc_cap = cycle_cap;
+    const int T = b->bc.ntypes;
+    if (p->has_tail) {
+      bt->tails[i].assign(p->tail, p->tail + T * T);
+      bt->use_tails[i].assign(p->use_tail, p->use_tail + T * T);
+      tail_total += (size_t)T * T;
+    }
+    max_t = std::max(max_t, T);
+    max_coul = std::max(max_coul, b->coul_n);
+    max_ks = std::max(max_ks, h.kstride);
+
+    BState& s = bt->hs[i];
+    std::memset(&s, 0, sizeof(s));
+    s.total_mol = 1;
+    for (int c = 1; c < ncomp; ++c) {
+      s.n_mol[c] = b->comps[c].n_mol;
+      s.total_mol += b->comps[c].n_mol;
+      for (int d = 0; d < 3; ++d) {
+        s.max_trans[c][d] = p->species[c - 1].max_translation[d];
+        s.max_rot[c][d] = p->species[c - 1].max_rotation[d];
+      }
+    }
+    for (int t = 0; t < 16; ++t) s.n_pseudo[t] = p->n_pseudo[t];
+  }
+  hipSetDevice(bt->device);
+  int rc = hip_check(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking), "batch stream");
+  // LDS: force field, real-space table (fast form), e^{ik.r} tables of 2 x 8 sites
+  const size_t nff4 = ((size_t)4 * max_t * max_t + 3) & ~(size_t)3;
+  const size_t nct = bt->precision == GRASPA_MIXED ? (((size_t)max_coul + 3) & ~(size_t)3) : 0;
+  bt->lds = (nff4 + nct) * sizeof(double) + (size_t)2 * kBMaxMol * 3 * max_ks * sizeof(double2);
+  int max_lds = 65536;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, bt->device) == hipSuccess) {
+      max_lds = (int)prop.sharedMemPerBlock;
+      int v = 0;
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, bt->device) == hipSuccess && v > max_lds)
+        max_lds = v;
+    }
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, bt->device) == hipSuccess && khz > 0)
+      bt->tick_hz = 1000.0 * khz;
+  }
+  const size_t static_lds = sizeof(BState) + sizeof(BMove);
+  if (!rc && bt->lds + static_lds > (size_t)max_lds) rc = fail(GRASPA_ERR_UNSUPPORTED, "batch workspace exceeds LDS");
+  for (int i = 0; i < n_boxes && !rc; ++i) bt->boxes.push_back(boxes[i].box);
+  if (!rc && tail_total) {
+    rc = hip_check(hipMalloc(&bt->d_tail, tail_total * sizeof(double)), "hipMalloc batch");
+    rc = rc ? rc : hip_check(hipMalloc(&bt->d_use_tail, tail_total * sizeof(int)), "hipMalloc batch");
+    size_t off = 0;
+    for (int i = 0; i < n_boxes && !rc; ++i) {
+      if (bt->tails[i].empty()) continue;
+      const size_t m = bt->tails[i].size();
+      rc = hip_check(hipMemcpy(bt->d_tail + off, bt->tails[i].data(), m * sizeof(double), hipMemcpyHostToDevice),
+                     "tail upload");
+      rc = rc ? rc : hip_check(hipMemcpy(bt->d_use_tail + off, bt->use_tails[i].data(), m * sizeof(int),
+                                         hipMemcpyHostToDevice), "tail upload");
+      bt->hb[i].tail = bt->d_tail + off;
+      bt->hb[i].use_tail = bt->d_use_tail + off;
+      off += m;
+    }
+  }
+  if (!rc && trace_cap > 0) {
+    rc = hip_check(hipMalloc(&bt->d_trace, (size_t)n_boxes * trace_cap * sizeof(graspa_trace)), "hipMalloc batch");
+    for (int i = 0; i < n_boxes; ++i) bt->hb[i].trace = bt->d_trace;
+  }
+  if (!rc && cycle_cap > 0) {
+    rc = hip_check(hipMalloc(&bt->d_cyc, (size_t)n_boxes * cycle_cap * sizeof(float)), "hipMalloc batch");
+    for (int i = 0; i < n_boxes; ++i) bt->hb[i].cyc = bt->d_cyc + (size_t)i * cycle_cap;
+  }
+  rc = rc ? rc : hip_check(hipMalloc(&bt->d_box, n_boxes * sizeof(BBox)), "hipMalloc batch");
+  rc = rc ? rc : hip_check(hipMalloc(&bt->d_state, n_boxes * sizeof(BState)), "hipMalloc batch");
+  // both S(k) buffers start equal, so entries a move never writes (inactive
+  // k-vectors) agree whichever buffer is current
+  for (int i = 0; i < n_boxes && !rc; ++i) {
+    graspa_box* b = boxes[i].box;
+    if (b->bc.use_charges && b->nvec > 0)
+      rc = hip_check(hipMemcpyAsync(b->d_sk_stage, b->d_sk_ads, b->nvec * sizeof(double2), hipMemcpyDeviceToDevice,
+                                    bt->stream), "sk copy");
+  }
+  rc = rc ? rc : upload_batch(bt);
+  if (rc) { free_batch(bt); return rc; }
+  bt->cubic = 1;
+  for (int i = 0; i < n_boxes; ++i) bt->cubic &= bt->hb[i].bc.cubic ? 1 : 0;
+  *out = bt;
+  return GRASPA_OK;
+}
+
+void graspa_batch_destroy(graspa_batch* bt) { free_batch(bt); }
+
+int graspa_batch_run(graspa_batch* bt, int n_cycles, int production, int moves_per_launch, int64_t* moves)
+{
+  if (!bt || n_cycles < 0) return fail(GRASPA_ERR_ARG, "bad arguments");
+  hipSetDevice(bt->device);
+  if (int rc = download_states(bt)) return rc;
+  int64_t before = 0;
+  for (int i = 0; i < bt->n; ++i) {
+    BState& s = bt->hs[i];
+    if (s.error) return fail(s.error, "a box of the batch stopped on an error (molecule capacity)");
+    s.target = s.cycle + n_cycles;
+    s.production = production ? 1 : 0;
+    s.trace_n = 0;
+    s.cyc_first = s.cycle;
+    before += s.move;
+  }
+  if (int rc = hip_check(hipMemcpyAsync(bt->d_state, bt->hs.data(), bt->n * sizeof(BState), hipMemcpyHostToDevice,
+                                        bt->stream), "batch state upload"))
+    return rc;
+  const int mpl = moves_per_launch > 0 ? moves_per_launch : 4096;
+  int64_t last = before;
+  for (;;) {
+    int rc;
+    if (bt->precision == GRASPA_MIXED)
+      rc = bt->cubic ? launch_batch<true, true>(bt, mpl) : launch_batch<true, false>(bt, mpl);
+    else
+      rc = bt->cubic ? launch_batch<false, true>(bt, mpl) : launch_batch<false, false>(bt, mpl);
+    if (rc) return rc;
+    if ((rc = download_states(bt))) return rc;
+    bool done = true;
+    int err = 0;
+    for (int i = 0; i < bt->n; ++i) {
+      const BState& s = bt->hs[i];
+      if (s.error) err = s.error;
+      else if (s.cycle < s.target) done = false;
+    }
+    if (err) return fail(err, "a box of the batch stopped on an error (molecule capacity)");
+    if (done) break;
+    int64_t now = 0;
+    for (int i = 0; i < bt->n; ++i) now += bt->hs[i].move;
+    if (now == last) return fail(GRASPA_ERR_STATE, "batch launch made no progress");
+    last = now;
+  }
+  int64_t after = 0;
+  for (int i = 0; i < bt->n; ++i) after += bt->hs[i].move;
+  if (moves) *moves = after - before;
+  return GRASPA_OK;
+}
+
+int graspa_batch_box_stats(graspa_batch* bt, int i, graspa_energy* running, int* n_mol, int64_t* counters,
+                           double* widom_sum_w, double* widom_n, double* load_sum, double* load_sum2,
+                           int64_t* n_load_cycles, int64_t* moves, int64_t* cycles)
+{
+  if (!bt || i < 0 || i >= bt->n) return fail(GRASPA_ERR_ARG, "bad box index");
+  hipSetDevice(bt->device);
+  if (int rc = download_states(bt)) return rc;
+  const BState& s = bt->hs[i];
+  const int ncomp = bt->hb[i].n_comp;
+  if (running) std::memcpy(running, s.running, sizeof(graspa_energy));
+  for (int c = 1; c < ncomp; ++c) {
+    if (n_mol) n_mol[c - 1] = s.n_mol[c];
+    if (counters)
+      for (int k = 0; k < 9; ++k) counters[9 * (c - 1) + k] = s.cnt[c][k];
+    if (widom_sum_w) widom_sum_w[c - 1] = s.widom_w[c];
+    if (widom_n) widom_n[c - 1] = s.widom_n[c];
+    if (load_sum) load_sum[c - 1] = s.load_sum[c];
+    if (load_sum2) load_sum2[c - 1] = s.load_sum2[c];
+  }
+  if (n_load_cycles) *n_load_cycles = s.load_n;
+  if (moves) *moves = s.move;
+  if (cycles) *cycles = s.cycle;
+  return GRASPA_OK;
+}
+
+int graspa_batch_cycle_times(graspa_batch* bt, int i, double* seconds, int max_n)
+{
+  if (!bt || i < 0 || i >= bt->n || max_n < 0 || (max_n > 0 && !seconds)) return fail(GRASPA_ERR_ARG, "bad arguments");
+  if (!bt->d_cyc) return 0;
+  hipSetDevice(bt->device);
+  if (int rc = download_states(bt)) return rc;
+  const BState& s = bt->hs[i];
+  const int64_t first = std::max(s.cyc_first, s.cycle - (int64_t)bt->cyc_cap);
+  const int n = (int)std::min<int64_t>(max_n, s.cycle - first);
+  std::vector<float> ring(bt->cyc_cap);
+  if (hip_check(hipMemcpy(ring.data(), bt->d_cyc + (size_t)i * bt->cyc_cap, bt->cyc_cap * sizeof(float),
+                          hipMemcpyDeviceToHost), "cycle times copy"))
+    return GRASPA_ERR_HIP;
+  for (int k = 0; k < n; ++k) seconds[k] = (double)ring[(first + k) % bt->cyc_cap] / bt->tick_hz;
+  return n;
+}
+
+int graspa_batch_trace(graspa_batch* bt, int i, graspa_trace* out, int max_n)
+{
+  if (!bt || i < 0 || i >= bt->n || max_n < 0 || (max_n > 0 && !out)) return fail(GRASPA_ERR_ARG, "bad arguments");
+  if (!bt->d_trace) return 0;
+  hipSetDevice(bt->device);
+  if (int rc = download_states(bt)) return rc;
+  const int n = (int)std::min<int64_t>(std::min<int64_t>(max_n, bt->trace_cap), bt->hs[i].trace_n);
+  if (n > 0 && hip_check(hipMemcpy(out, bt->d_trace + (size_t)i * bt->trace_cap, n * sizeof(graspa_trace),
+                                   hipMemcpyDeviceToHost), "trace copy"))
+    return GRASPA_ERR_HIP;
+  return n;
+}
+
+int graspa_batch_kernel_time(graspa_batch* bt, double* total_ms, int64_t* launches)
+{
+  if (!bt) return fail(GRASPA_ERR_ARG, "null batch");
+  if (total_ms) *total_ms = bt->kernel_ms;
+  if (launches) *launches = bt->launches;
+  return GRASPA_OK;
+}
+
+int graspa_batch_sync(graspa_batch* bt)
+{
+  if (!bt) return fail(GRASPA_ERR_ARG, "null batch");
+  hipSetDevice(bt->device);
+  if (int rc = download_states(bt)) return rc;
+  bool flipped = false;
+  for (int i = 0; i < bt->n; ++i) {
+    graspa_box* b = bt->boxes[i];
+    BState& s = bt->hs[i];
+    const int cap = b->guest_cap;
+    std::vector<double> g(4 * (size_t)std::max(cap, 1));
+    std::vector<int> gt(std::max(cap, 1)), gm(std::max(cap, 1));
+    if (cap > 0) {
+      if (hip_check(hipMemcpy(g.data(), b->d_g, 4 * (size_t)cap * sizeof(double), hipMemcpyDeviceToHost), "sync"))
+        return GRASPA_ERR_HIP;
+      if (hip_check(hipMemcpy(gt.data(), b->d_g_type, (size_t)cap * sizeof(int), hipMemcpyDeviceToHost), "sync"))
+        return GRASPA_ERR_HIP;
+      if (hip_check(hipMemcpy(gm.data(), b->d_g_molid, (size_t)cap * sizeof(int), hipMemcpyDeviceToHost), "sync"))
+        return GRASPA_ERR_HIP;
+    }
+    for (size_t c = 1; c < b->comps.size(); ++c) {
+      Component& cc = b->comps[c];
+      cc.n_mol = s.n_mol[c];
+      const int na = std::max(cc.n_mol, 1) * cc.molsize;
+      for (int a = 0; a < na; ++a) {
+        const int j = cc.base + a;
+        cc.pos[3 * a] = g[j];
+        cc.pos[3 * a + 1] = g[cap + j];
+        cc.pos[3 * a + 2] = g[2 * (size_t)cap + j];
+        cc.q[a] = g[3 * (size_t)cap + j];
+        cc.type[a] = gt[j];
+        cc.molid[a] = gm[j];
+      }
+      // refreshes the fixed-point copy the screening kernels read
+      if (int rc = upload_guest_atoms(b, (int)c, 0, na)) return rc;
+    }
+    if (s.sk_cur) {
+      std::swap(b->d_sk_ads, b->d_sk_stage);
+      bt->hb[i].sk[0] = b->d_sk_ads;
+      bt->hb[i].sk[1] = b->d_sk_stage;
+      s.sk_cur = 0;
+      flipped = true;
+    }
+    b->sk_staged = false;
+    if (hip_check(hipStreamSynchronize(b->stream), "sync")) return GRASPA_ERR_HIP;
+  }
+  if (flipped) return upload_batch(bt);
+  return GRASPA_OK;
+}
+
+}  // extern "C"

@@ -173,6 +173,10 @@ int launch_ewald_delta(graspa_box* b, const SiteBatch& sb, int n_old, double* d_
 int launch_ewald_build(graspa_box* b, double* d_out_energy /* blocks*3 */, int* n_blocks);
 int launch_total_pairs(graspa_box* b, double* d_out /* blocks*4 */, int* n_blocks);
 
+// writes atoms [first_atom, first_atom + n_atoms) of a component from the host
+// mirror into the device guest store (engine_api.hip)
+int upload_guest_atoms(graspa_box* b, int comp, int first_atom, int n_atoms);
+
 // per-insertion record buffers of the Widom kernel (engine_api.hip)
 int ensure_widom_records(graspa_box* b, int n_insert);
 

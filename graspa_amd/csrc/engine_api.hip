@@ -105,7 +105,7 @@ __global__ void patch_atoms_kernel(const AtomPatch p, double* g, int cap, int* g
 
 using namespace graspa;
 
-static int upload_guest_atoms(graspa_box* b, int comp, int first_atom, int n_atoms)
+int graspa::upload_guest_atoms(graspa_box* b, int comp, int first_atom, int n_atoms)
 {
   // one launch per kMaxSites atoms, from the host mirror
   const Component& c = b->comps[comp];

@@ -9,6 +9,7 @@
 // host in one copy.  The throughput path is the fused Widom kernel.
 #include "device_math.hpp"
 #include "engine.hpp"
+#include "move_math.hpp"
 
 namespace graspa {
 
@@ -129,32 +130,6 @@ int launch_site_energies(graspa_box* b, const SiteBatch& sb, int self_comp, int 
   return hip_check(hipGetLastError(), "site_energy_kernel launch");
 }
 
-// ---------------------------------------------------------------------------
-// Per-site e^{i n s} along each cell axis for n = 0..kmax, by the complex
-// recurrence of the reference (Ewald_Energy_Functions.h:96-104).
-// ---------------------------------------------------------------------------
-__device__ inline double2 cmul(double2 a, double2 b)
-{
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-__device__ inline void axis_table(const BoxConst& bc, double px, double py, double pz, int axis, double2* out)
-{
-  const double* a = bc.inv;
-  double s;
-  if (axis == 0) s = a[0] * px + a[3] * py + a[6] * pz;
-  else if (axis == 1) s = a[1] * px + a[4] * py + a[7] * pz;
-  else s = a[2] * px + a[5] * py + a[8] * pz;
-  s *= 2 * 3.14159265358979323846;
-  const int km = bc.kmax[axis];
-  out[0] = make_double2(1.0, 0.0);
-  double sn, cs;
-  sincos(s, &sn, &cs);
-  const double2 e1 = make_double2(cs, sn);
-  if (km >= 1) out[1] = e1;
-  for (int k = 2; k <= km; ++k) out[k] = cmul(out[k - 1], e1);
-}
-
 // Fourier energy change of one move (Fourier_Ewald_Diff, Ewald_Energy_Functions.h:280-397):
 // one thread per k-vector; per-site axis tables in LDS.
 __global__ __launch_bounds__(kBlock) void ewald_delta_kernel(const BoxConst bc, const SiteBatch sb, const int n_old,
@@ -188,10 +163,10 @@ __global__ __launch_bounds__(kBlock) void ewald_delta_kernel(const BoxConst bc, 
         const double2* e = eik + i * 3 * kstride;
         double2 ey = e[kstride + abs(ky)];
         if (ky < 0) ey.y = -ey.y;
-        const double2 exy = cmul(e[kx], ey);
+        const double2 exy = cmul2(e[kx], ey);
         double2 ez = e[2 * kstride + abs(kz)];
         if (kz < 0) ez.y = -ez.y;
-        const double2 p = cmul(exy, ez);
+        const double2 p = cmul2(exy, ez);
         const double q = sb.q[i];
         if (i < n_old) { co.x += q * p.x; co.y += q * p.y; }
         else { cn.x += q * p.x; cn.y += q * p.y; }
@@ -274,10 +249,10 @@ __global__ __launch_bounds__(kBlock) void sk_build_kernel(const BoxConst bc, con
           const int j = base + jj;
           double2 ey = tab[((size_t)1 * kstride + abs(ky)) * tn + j];
           if (ky < 0) ey.y = -ey.y;
-          const double2 exy = cmul(tab[((size_t)0 * kstride + kx) * tn + j], ey);
+          const double2 exy = cmul2(tab[((size_t)0 * kstride + kx) * tn + j], ey);
           double2 ez = tab[((size_t)2 * kstride + abs(kz)) * tn + j];
           if (kz < 0) ez.y = -ez.y;
-          const double2 p = cmul(exy, ez);
+          const double2 p = cmul2(exy, ez);
           s.x += q[j] * p.x;
           s.y += q[j] * p.y;
         }

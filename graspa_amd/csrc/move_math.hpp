@@ -1,0 +1,95 @@
+// move_math.hpp -- rigid-body rotations, e^{ik.r} axis tables and the
+// real-space Coulomb table lookup shared by the Widom and batched-GCMC kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_math.hpp"
+#include "engine.hpp"
+
+namespace graspa {
+
+__device__ inline double2 cmul2(double2 a, double2 b)
+{
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Rotate_Quaternions (mc_utilities.h:423-457): uniform random rotation from
+// three uniforms (Shoemake's quaternion).
+__device__ inline void quat_rotate(double v[3], double u, double vv, double w)
+{
+  const double pi = 3.14159265358979323846;
+  const double q0 = sqrt(1 - u) * sin(2 * pi * vv);
+  const double q1 = sqrt(1 - u) * cos(2 * pi * vv);
+  const double q2 = sqrt(u) * sin(2 * pi * w);
+  const double q3 = sqrt(u) * cos(2 * pi * w);
+  const double a01 = q0 * q1, a02 = q0 * q2, a03 = q0 * q3;
+  const double a11 = q1 * q1, a12 = q1 * q2, a13 = q1 * q3;
+  const double a22 = q2 * q2, a23 = q2 * q3, a33 = q3 * q3;
+  const double r0 = 1.0 - 2.0 * (a22 + a33), r1 = 2.0 * (a12 - a03), r2 = 2.0 * (a13 + a02);
+  const double r3 = 2.0 * (a12 + a03), r4 = 1.0 - 2.0 * (a11 + a33), r5 = 2.0 * (a23 - a01);
+  const double r6 = 2.0 * (a13 - a02), r7 = 2.0 * (a23 + a01), r8 = 1.0 - 2.0 * (a11 + a22);
+  const double x = v[0] * r0 + v[1] * r1 + v[2] * r2;
+  const double y = v[0] * r3 + v[1] * r4 + v[2] * r5;
+  const double z = v[0] * r6 + v[1] * r7 + v[2] * r8;
+  v[0] = x; v[1] = y; v[2] = z;
+}
+
+// RotationAroundAxis (mc_utilities.h:459-483): rotation of p by theta about
+// the unit axis (ax, ay, az).
+__device__ inline void rotate_axis(double p[3], double theta, double ax, double ay, double az)
+{
+  double s, c;
+  sincos(theta, &s, &c);
+  const double w = 1.0 - c;
+  const double r0 = ax * ax * w + c, r1 = ax * ay * w + az * s, r2 = ax * az * w - ay * s;
+  const double r3 = ax * ay * w - az * s, r4 = ay * ay * w + c, r5 = ay * az * w + ax * s;
+  const double r6 = ax * az * w + ay * s, r7 = ay * az * w - ax * s, r8 = az * az * w + c;
+  const double x = p[0] * r0 + p[1] * r1 + p[2] * r2;
+  const double y = p[0] * r3 + p[1] * r4 + p[2] * r5;
+  const double z = p[0] * r6 + p[1] * r7 + p[2] * r8;
+  p[0] = x; p[1] = y; p[2] = z;
+}
+
+// e^{i n s} along one cell axis for n = 0..kmax, s = 2 pi (inverse cell . r)
+// on that axis, by the complex recurrence of the reference
+// (Ewald_Energy_Functions.h:96-104).
+__device__ inline void axis_table(const BoxConst& bc, double px, double py, double pz, int axis, double2* out)
+{
+  const double* a = bc.inv;
+  double s;
+  if (axis == 0) s = a[0] * px + a[3] * py + a[6] * pz;
+  else if (axis == 1) s = a[1] * px + a[4] * py + a[7] * pz;
+  else s = a[2] * px + a[5] * py + a[8] * pz;
+  s *= 2 * 3.14159265358979323846;
+  const int km = bc.kmax[axis];
+  out[0] = make_double2(1.0, 0.0);
+  double sn, cs;
+  sincos(s, &sn, &cs);
+  const double2 e1 = make_double2(cs, sn);
+  if (km >= 1) out[1] = e1;
+  for (int k = 2; k <= km; ++k) out[k] = cmul2(out[k - 1], e1);
+}
+
+// erfc(alpha r)/r from the per-box table (coulomb_table.cpp): the interval is
+// read from the exponent and top mantissa bits of r^2 as FP32, the value is a
+// degree-5 polynomial in r^2 - s_i.  Clamped indices only occur for r^2 below
+// 2^-7, i.e. pairs the overlap rule already rejects.
+__device__ inline double coul_table(const double2* __restrict__ ct, const int nint, const double rr)
+{
+  constexpr int kShift = 23 - kCoulTabMBits;
+  const uint32_t bits = __float_as_uint((float)rr);
+  int idx = (int)(bits >> kShift) - ((127 + kCoulTabELo) << kCoulTabMBits);
+  idx = min(max(idx, 0), nint - 1);
+  const double si = (double)__uint_as_float(bits & ~((1u << kShift) - 1u));
+  const double d = rr - si;
+  const double2* c = ct + 3 * idx;
+  const double2 c01 = c[0], c23 = c[1], c45 = c[2];
+  double g = fma(c45.y, d, c45.x);
+  g = fma(g, d, c23.y);
+  g = fma(g, d, c23.x);
+  g = fma(g, d, c01.y);
+  return fma(g, d, c01.x);
+}
+
+}  // namespace graspa

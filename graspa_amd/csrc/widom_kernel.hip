@@ -35,6 +35,7 @@
 
 #include "device_math.hpp"
 #include "engine.hpp"
+#include "move_math.hpp"
 
 namespace graspa {
 
@@ -159,26 +160,6 @@ __device__ inline void fill_draws(const WidomArgs& A, uint64_t idx, int d0, int 
   }
 }
 
-// Rotate_Quaternions (mc_utilities.h:423-457)
-__device__ inline void quat_rotate(double v[3], double u, double vv, double w)
-{
-  const double pi = 3.14159265358979323846;
-  const double q0 = sqrt(1 - u) * sin(2 * pi * vv);
-  const double q1 = sqrt(1 - u) * cos(2 * pi * vv);
-  const double q2 = sqrt(u) * sin(2 * pi * w);
-  const double q3 = sqrt(u) * cos(2 * pi * w);
-  const double a01 = q0 * q1, a02 = q0 * q2, a03 = q0 * q3;
-  const double a11 = q1 * q1, a12 = q1 * q2, a13 = q1 * q3;
-  const double a22 = q2 * q2, a23 = q2 * q3, a33 = q3 * q3;
-  const double r0 = 1.0 - 2.0 * (a22 + a33), r1 = 2.0 * (a12 - a03), r2 = 2.0 * (a13 + a02);
-  const double r3 = 2.0 * (a12 + a03), r4 = 1.0 - 2.0 * (a11 + a33), r5 = 2.0 * (a23 - a01);
-  const double r6 = 2.0 * (a13 - a02), r7 = 2.0 * (a23 + a01), r8 = 1.0 - 2.0 * (a11 + a22);
-  const double x = v[0] * r0 + v[1] * r1 + v[2] * r2;
-  const double y = v[0] * r3 + v[1] * r4 + v[2] * r5;
-  const double z = v[0] * r6 + v[1] * r7 + v[2] * r8;
-  v[0] = x; v[1] = y; v[2] = z;
-}
-
 // First-bead trial position t: diag(cell) * U^3 (get_random_trial_position,
 // mc_widom.h:122-213), draws 3t .. 3t+2.
 __device__ inline void first_bead_pos(const WidomArgs& A, uint64_t idx, int t, double p[3])
@@ -216,11 +197,6 @@ __device__ inline uint4 to_fixed4(const BoxConst& bc, const double p[3])
     f[d] = (uint32_t)(int64_t)(fr * 4294967296.0);
   }
   return make_uint4(f[0], f[1], f[2], 0u);
-}
-
-__device__ inline double2 cmul2(double2 a, double2 b)
-{
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
 // Squared-distance threshold of the overlap screen for one (atom type, site
@@ -354,27 +330,6 @@ __device__ inline uint32_t screen(const WidomArgs& A, const uint4* __restrict__ 
     }
   }
   return all & ~alive;
-}
-
-// erfc(alpha r)/r from the per-box table (coulomb_table.cpp): the interval is
-// read from the exponent and top mantissa bits of r^2 as FP32, the value is a
-// degree-5 polynomial in r^2 - s_i.  Clamped indices only occur for r^2 below
-// 2^-7, i.e. pairs the overlap rule already rejects.
-__device__ inline double coul_table(const double2* __restrict__ ct, const int nint, const double rr)
-{
-  constexpr int kShift = 23 - kCoulTabMBits;
-  const uint32_t bits = __float_as_uint((float)rr);
-  int idx = (int)(bits >> kShift) - ((127 + kCoulTabELo) << kCoulTabMBits);
-  idx = min(max(idx, 0), nint - 1);
-  const double si = (double)__uint_as_float(bits & ~((1u << kShift) - 1u));
-  const double d = rr - si;
-  const double2* c = ct + 3 * idx;
-  const double2 c01 = c[0], c23 = c[1], c45 = c[2];
-  double g = fma(c45.y, d, c45.x);
-  g = fma(g, d, c23.y);
-  g = fma(g, d, c23.x);
-  g = fma(g, d, c01.y);
-  return fma(g, d, c01.x);
 }
 
 // SelectTrialPosition / CBMC_FirstBead_Finish / Widom_Move_Chain_PARTIAL

@@ -78,6 +78,11 @@ class MCParams(C.Structure):
                 ("n_pseudo", C.c_int64 * 16)]
 
 
+class BatchBox(C.Structure):
+    _fields_ = [("box", C.c_void_p), ("params", C.POINTER(MCParams)), ("seed", C.c_uint64), ("stream", C.c_uint32),
+                ("_pad", C.c_int)]
+
+
 class Trace(C.Structure):
     _fields_ = [("cycle", C.c_int), ("step", C.c_int), ("move", C.c_int), ("comp", C.c_int), ("mol", C.c_int),
                 ("accepted", C.c_int), ("overlap", C.c_int), ("_pad", C.c_int), ("pacc", C.c_double),
@@ -126,6 +131,15 @@ _SIGS = {
     "graspa_box_sync": (C.c_int, [C.c_void_p]),
     "graspa_glibc_rand_sequence": (C.c_int, [C.c_uint, C.c_int, _i32p]),
     "graspa_coulomb_table": (C.c_int, [C.c_double, C.c_double, _f64p, C.c_int]),
+    "graspa_batch_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "graspa_batch_destroy": (None, [C.c_void_p]),
+    "graspa_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _i64p]),
+    "graspa_batch_box_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Energy), _i32p, _i64p, _f64p, _f64p, _f64p,
+                                         _f64p, _i64p, _i64p, _i64p]),
+    "graspa_batch_cycle_times": (C.c_int, [C.c_void_p, C.c_int, _f64p, C.c_int]),
+    "graspa_batch_trace": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
+    "graspa_batch_kernel_time": (C.c_int, [C.c_void_p, _f64p, _i64p]),
+    "graspa_batch_sync": (C.c_int, [C.c_void_p]),
 }
 
 
@@ -447,6 +461,93 @@ class MonteCarlo:
     def close(self):
         if self.h:
             lib().graspa_mc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchStats:
+    """State of one box of a batch (graspa_batch_box_stats)."""
+
+    def __init__(self, running, n_mol, counters, widom_w, widom_n, load_sum, load_sum2, n_load, moves, cycles):
+        self.running, self.n_mol, self.counters = running, n_mol, counters
+        self.widom_w, self.widom_n = widom_w, widom_n
+        self.load_sum, self.load_sum2, self.n_load = load_sum, load_sum2, n_load
+        self.moves, self.cycles = moves, cycles
+
+    def loading(self):
+        """Mean molecules per box of each species over the production cycles."""
+        return self.load_sum / max(self.n_load, 1)
+
+
+class Batch:
+    """Device-resident GCMC of many boxes (graspa_batch_*): box i draws its
+    moves from the Philox stream (seed_i, stream_i, move)."""
+
+    def __init__(self, boxes, seeds, streams=None, trace_cap=0, cycle_cap=0, params=None):
+        self.boxes = list(boxes)
+        n = len(self.boxes)
+        streams = list(range(n)) if streams is None else list(streams)
+        self.params = list(params) if params is not None else [make_mc_params(b.system, b) for b in self.boxes]
+        arr = (BatchBox * n)()
+        for i, b in enumerate(self.boxes):
+            arr[i].box = b.h
+            arr[i].params = C.pointer(self.params[i])
+            arr[i].seed = int(seeds[i])
+            arr[i].stream = int(streams[i])
+        self._arr = arr
+        self.trace_cap = trace_cap
+        h = C.c_void_p()
+        _check(lib().graspa_batch_create(C.cast(arr, C.c_void_p), n, trace_cap, cycle_cap, C.byref(h)),
+               "graspa_batch_create")
+        self.h = h
+
+    def run(self, n_cycles, production=True, moves_per_launch=0):
+        m = C.c_int64()
+        _check(lib().graspa_batch_run(self.h, n_cycles, int(production), moves_per_launch, C.byref(m)),
+               "graspa_batch_run")
+        return m.value
+
+    def stats(self, i):
+        ns = len(self.boxes[i].system.components)
+        e = Energy()
+        nm = np.zeros(ns, dtype=np.int32)
+        cnt = np.zeros(9 * ns, dtype=np.int64)
+        ww, wn, ls, ls2 = np.zeros(ns), np.zeros(ns), np.zeros(ns), np.zeros(ns)
+        nl, mv, cy = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().graspa_batch_box_stats(self.h, i, C.byref(e), _p(nm, _i32p), _p(cnt, _i64p), _p(ww), _p(wn),
+                                            _p(ls), _p(ls2), C.byref(nl), C.byref(mv), C.byref(cy)), "batch_stats")
+        return BatchStats(e, nm, cnt.reshape(ns, 9), ww, wn, ls, ls2, nl.value, mv.value, cy.value)
+
+    def cycle_times(self, i, n):
+        out = np.zeros(n)
+        k = lib().graspa_batch_cycle_times(self.h, i, _p(out), n)
+        if k < 0:
+            _check(k, "batch_cycle_times")
+        return out[:k]
+
+    def trace(self, i):
+        buf = (Trace * max(self.trace_cap, 1))()
+        k = lib().graspa_batch_trace(self.h, i, C.cast(buf, C.c_void_p), self.trace_cap)
+        if k < 0:
+            _check(k, "batch_trace")
+        return buf[:k]
+
+    def kernel_time(self):
+        t, n = C.c_double(), C.c_int64()
+        _check(lib().graspa_batch_kernel_time(self.h, C.byref(t), C.byref(n)), "batch_kernel_time")
+        return t.value, n.value
+
+    def sync(self):
+        _check(lib().graspa_batch_sync(self.h), "batch_sync")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().graspa_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -68,9 +68,18 @@ SPECIES = {"CO2": _co2, "N2": _n2, "butane": _butane}
 
 
 def build(n_fw=1536, L=24.0, species=("CO2",), mol_fraction=None, seed=0, T=300.0, P=1e5, cutoff=12.0,
-          charges=True, K=10, O=10, probs=None, cycles=2000, fw_seed=None):
-    """A cubic box with a synthetic framework and the given adsorbate species."""
+          charges=True, K=10, O=10, probs=None, cycles=2000, fw_seed=None, ucells=1):
+    """A cubic box with a synthetic framework and the given adsorbate species.
+    ucells > 1 replicates the framework cell ucells times along each axis (the
+    reference's UnitCells keyword), so the box is ucells * L wide."""
     fw_pos, fw_q, fw_t, fw_eps, fw_sig = framework(n_fw, L, seed if fw_seed is None else fw_seed)
+    if ucells > 1:
+        shifts = np.stack(np.meshgrid(*([np.arange(ucells)] * 3), indexing="ij"), -1).reshape(-1, 3) * L
+        fw_pos = (fw_pos[None, :, :] + shifts[:, None, :]).reshape(-1, 3)
+        fw_q = np.tile(fw_q, len(shifts))
+        fw_t = np.tile(fw_t, len(shifts))
+        n_fw = len(fw_t)
+        L = L * ucells
     names = ["Fw_a", "Fw_b", "Fw_c"]   # CIF labels lose trailing digits (read_data.cpp:1652)
     eps_k = list(fw_eps)
     sigma = list(fw_sig)
@@ -136,16 +145,20 @@ def config2(seed=2000, n_fw=3000, L=30.0, P=1e4):
                  probs=dict(translation=1.0, rotation=1.0, swap=2.0))
 
 
-def config3_box(b, seed_base=3000):
+def config3_box(b, seed_base=3000, cutoff=12.0):
     """Box b of the 256-box config: N ~ U{500..6000}, L ~ U(18,40), CO2/N2 50:50,
-    T ~ U(250,350), P log-U(1e3,1e7)."""
+    T ~ U(250,350), P log-U(1e3,1e7).  A cell narrower than twice the cutoff
+    is replicated (UnitCells 2 2 2), as a reference deck must do for the
+    minimum-image convention, so N and L are those of the unit cell."""
     rng = np.random.default_rng(seed_base + b)
     n = int(rng.integers(500, 6001))
     L = float(rng.uniform(18.0, 40.0))
     T = float(rng.uniform(250.0, 350.0))
     P = float(10 ** rng.uniform(3.0, 7.0))
+    uc = int(math.ceil(2.0 * cutoff / L - 1e-12))
     return build(n_fw=n, L=L, species=("CO2", "N2"), mol_fraction=[0.5, 0.5], seed=seed_base + b, T=T, P=P,
-                 cycles=1000, probs=dict(translation=1.0, rotation=1.0, swap=2.0))
+                 cycles=1000, probs=dict(translation=1.0, rotation=1.0, swap=2.0), ucells=max(uc, 1),
+                 cutoff=cutoff)
 
 
 def write_deck(system: System, dirpath: str, framework_name="synth", cycles=None):

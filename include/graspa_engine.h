@@ -276,6 +276,67 @@ int  graspa_mc_stats(graspa_mc* mc, graspa_energy* running, int* n_mol, int64_t*
 int  graspa_mc_cycle_times(graspa_mc* mc, double* seconds, int max_n);
 
 /* ------------------------------------------------------------------------ */
+/* Device-resident GCMC of many boxes (batched form)                         */
+/* ------------------------------------------------------------------------ */
+
+/* A batch runs the Monte Carlo steps of many independent boxes in one
+ * persistent launch, one workgroup per box: component / molecule / move
+ * selection, translation and rotation proposals, CBMC trial batches with their
+ * selection, the Ewald energy change, the Metropolis test, the commit and the
+ * structure-factor flip all run on the device.  It replaces RunMoves
+ * (axpy.cu:102-298) and Run_Simulation_ForOneBox (axpy.cu:378-400, 580-641)
+ * for jobs that do not need the reference's random-number sequence: move t of
+ * a box draws from the Philox4x32-10 stream (seed, stream, t), so a box's
+ * trajectory is the same alone and in any batch.  Draw layout of a move:
+ * d0 component (uniform over the species with SwapProbability > 0), d1
+ * molecule, d2 move type, d3 insertion / deletion coin, d4 Metropolis, d5
+ * first-bead selection, d6 orientation selection, d8..d10 translation or
+ * rotation, d16+3k.. first-bead trial k, d16+3K+3o.. orientation o.
+ *
+ * Every box of a batch is on one device and has one precision form.  Between
+ * graspa_batch_create and graspa_batch_destroy the boxes belong to the batch:
+ * after graspa_batch_sync their host-side state (molecules, counts, structure
+ * factors) is current and the read-only box calls may be used. */
+typedef struct graspa_batch graspa_batch;
+
+typedef struct {
+  graspa_box* box;
+  const graspa_mc_params* params;  /* species, trials, bias, beta, pressure, tail, max step
+                                      (seed and pool_size are not used)                          */
+  uint64_t seed;                   /* Philox key of this box's moves                            */
+  uint32_t stream;                 /* Philox stream of this box                                 */
+  int      _pad;
+} graspa_batch_box;
+
+/* trace_cap: move records kept per box and run (0 = none); cycle_cap:
+ * per-cycle latencies kept per box (ring, 0 = none). */
+int  graspa_batch_create(const graspa_batch_box* boxes, int n_boxes, int trace_cap, int cycle_cap,
+                         graspa_batch** out);
+void graspa_batch_destroy(graspa_batch* batch);
+/* Advances every box by n_cycles cycles (production != 0 records Widom and
+ * loading statistics); launches of at most moves_per_launch moves per box
+ * (<= 0: a default) until all boxes are done.  *moves = moves made. */
+int  graspa_batch_run(graspa_batch* batch, int n_cycles, int production, int moves_per_launch, int64_t* moves);
+/* State of box i: running energy change, molecule counts [n_species], move
+ * counters [n_species][9] (graspa_mc_stats layout), Widom sums [n_species],
+ * per-cycle loading sums sum N and sum N^2 [n_species] over the production
+ * cycles (*n_load_cycles), moves and cycles made since creation.  Any output
+ * pointer may be NULL. */
+int  graspa_batch_box_stats(graspa_batch* batch, int i, graspa_energy* running, int* n_mol, int64_t* counters,
+                            double* widom_sum_w, double* widom_n, double* load_sum, double* load_sum2,
+                            int64_t* n_load_cycles, int64_t* moves, int64_t* cycles);
+/* Per-cycle wall-clock latency (seconds, device clock) of box i's cycles in
+ * the last graspa_batch_run, oldest first; returns the number written. */
+int  graspa_batch_cycle_times(graspa_batch* batch, int i, double* seconds, int max_n);
+/* Move records of box i from the last graspa_batch_run (graspa_trace layout,
+ * cycle and step counted from the batch's creation); returns the number. */
+int  graspa_batch_trace(graspa_batch* batch, int i, graspa_trace* out, int max_n);
+/* Summed kernel time (HIP events) and launches of the batch since creation. */
+int  graspa_batch_kernel_time(graspa_batch* batch, double* total_ms, int64_t* launches);
+/* Copies the device state of every box back to its graspa_box. */
+int  graspa_batch_sync(graspa_batch* batch);
+
+/* ------------------------------------------------------------------------ */
 /* Measurement                                                               */
 /* ------------------------------------------------------------------------ */
 

@@ -746,7 +746,16 @@ static void energy_add(or_energy* a, const or_energy* b)
 
 typedef struct {
   const or_box* b; or_atoms* comps; int ncomp; or_mc* mc; double* sk_ads; double* sk_fw; double* sk_tmp;
+  /* batched form: the uniforms of move `move` come from the Philox stream
+   * (seed, stream, move) by draw index instead of the shared pool */
+  int philox; uint64_t seed, stream, move;
 } mc_ctx;
+
+static double mc_draw(const mc_ctx* x, int d) { return or_philox_uniform(x->seed, x->stream, x->move, (uint32_t)d); }
+
+/* Draw layout of one move in the batched form (the engine's batch_mc.hip). */
+enum { D_COMP = 0, D_MOL = 1, D_MOVE = 2, D_COIN = 3, D_ACC = 4, D_SEL_FB = 5, D_SEL_OR = 6, D_VEC = 8,
+       D_TRIAL = 16 };
 
 static int comp_has_charge(const mc_ctx* x, int c) { return x->mc->has_charge[c] && !x->b->no_charges; }
 
@@ -759,9 +768,15 @@ static void single_body(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
   int M = A->molsize;
   int start = mol * M;
   if (move == OR_TRANSLATION) mc->trans_total[comp]++; else mc->rot_total[comp]++;
-  pool_check((size_t)M);
-  const double* R = &g_pool[3 * g_pool_offset];
-  g_pool_offset += (size_t)M;
+  double Rp[3];
+  const double* R = Rp;
+  if (x->philox) {
+    for (int d = 0; d < 3; d++) Rp[d] = mc_draw(x, D_VEC + d);
+  } else {
+    pool_check((size_t)M);
+    R = &g_pool[3 * g_pool_offset];
+    g_pool_offset += (size_t)M;
+  }
   double oldp[3 * 64], newp[3 * 64], q[64];
   int ty[64];
   for (int i = 0; i < M; i++) {
@@ -811,7 +826,7 @@ static void single_body(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
   }
   int acc = 0;
   if (!overlap) {
-    double u = or_rand_uniform();
+    double u = x->philox ? mc_draw(x, D_ACC) : or_rand_uniform();
     if (u < pacc) acc = 1;
   }
   if (acc) {
@@ -857,9 +872,12 @@ static void cbmc_move(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
 
   /* first bead: Widom_Move_FirstBead_PARTIAL (mc_widom.h:385-507),
    * get_random_trial_position (mc_widom.h:122-213) */
-  pool_check((size_t)K);
+  if (!x->philox) pool_check((size_t)K);
   for (int k = 0; k < K; k++) {
-    const double* R = &g_pool[3 * (g_pool_offset + k)];
+    double Rp[3];
+    const double* R = Rp;
+    if (x->philox) for (int d = 0; d < 3; d++) Rp[d] = mc_draw(x, D_TRIAL + 3 * k + d);
+    else R = &g_pool[3 * (g_pool_offset + k)];
     if (del && k == 0) {
       sp[0] = A->pos[3 * start]; sp[1] = A->pos[3 * start + 1]; sp[2] = A->pos[3 * start + 2];
     } else {
@@ -867,7 +885,7 @@ static void cbmc_move(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
     }
     st[k] = A->type[start]; sq[k] = A->charge[start];
   }
-  g_pool_offset += (size_t)K;
+  if (!x->philox) g_pool_offset += (size_t)K;
   or_site_energies(b, x->comps, x->ncomp, K, sp, st, sq, comp, sel_molid, -1, -1, s4, sov);
   sum_trials(s4, sov, K, 1, beta, mc->vdw_real_bias, &ts);
   /* CBMC_FirstBead_Finish (mc_widom.h:305-383) */
@@ -875,7 +893,7 @@ static void cbmc_move(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
   double rb;
   if (!del) {
     if (ts.n_keep == 0) goto fail;
-    sel = select_trial(ts.rosen, ts.n_keep, or_rand_uniform());
+    sel = select_trial(ts.rosen, ts.n_keep, x->philox ? mc_draw(x, D_SEL_FB) : or_rand_uniform());
     rb = sum_exp(&ts);
     if (rb < 1e-150) goto fail;
   } else {
@@ -895,9 +913,12 @@ static void cbmc_move(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
     /* Widom_Move_Chain_PARTIAL (mc_widom.h:509-614), get_random_trial_orientation
      * (mc_widom.h:215-303): template vectors from the component's first
      * molecule slot as it is now. */
-    pool_check((size_t)O);
+    if (!x->philox) pool_check((size_t)O);
     for (int t = 0; t < O; t++) {
-      const double* R = &g_pool[3 * (g_pool_offset + t)];
+      double Rp[3];
+      const double* R = Rp;
+      if (x->philox) for (int d = 0; d < 3; d++) Rp[d] = mc_draw(x, D_TRIAL + 3 * K + 3 * t + d);
+      else R = &g_pool[3 * (g_pool_offset + t)];
       for (int a = 0; a < chain; a++) {
         int s = t * chain + a;
         if (del && t == 0) {
@@ -911,14 +932,14 @@ static void cbmc_move(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
         st[s] = A->type[start + 1 + a]; sq[s] = A->charge[start + 1 + a];
       }
     }
-    g_pool_offset += (size_t)O;
+    if (!x->philox) g_pool_offset += (size_t)O;
     or_site_energies(b, x->comps, x->ncomp, O * chain, sp, st, sq, comp, sel_molid, -1, -1, s4, sov);
     sum_trials(s4, sov, O, chain, beta, mc->vdw_real_bias, &ts);
     int osel = 0;
     double rc;
     if (!del) {
       if (ts.n_keep == 0) goto fail;
-      osel = select_trial(ts.rosen, ts.n_keep, or_rand_uniform());
+      osel = select_trial(ts.rosen, ts.n_keep, x->philox ? mc_draw(x, D_SEL_OR) : or_rand_uniform());
       rc = sum_exp(&ts);
       if (rc < 1e-150) goto fail;
     } else {
@@ -978,7 +999,7 @@ static void cbmc_move(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
   double pacc = del ? pref * mc->ideal_rosen[comp] / W : pref * W / mc->ideal_rosen[comp];
   if (!del && W <= 1e-150) pacc = 0.0;
   tr->pacc = pacc;
-  double u = or_rand_uniform();
+  double u = x->philox ? mc_draw(x, D_ACC) : or_rand_uniform();
   if (!(u < pacc)) return;  /* rejected: energy zero */
   /* accepted */
   tr->accepted = 1;
@@ -1040,6 +1061,7 @@ int or_mc_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
 {
   if (!g_pool) return -1;
   mc_ctx x;
+  memset(&x, 0, sizeof(x));
   x.b = b; x.comps = comps; x.ncomp = ncomp; x.mc = mc; x.sk_ads = sk_ads; x.sk_fw = sk_fw;
   x.sk_tmp = (double*)calloc((size_t)2 * or_nvec(b), sizeof(double));
   int nt = 0;
@@ -1079,6 +1101,74 @@ int or_mc_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
       nt++;
     }
     /* GatherStatisticsDuringSimulation (axpy.cu:378-400) */
+    if (cyc % 500 == 0) {
+      for (int c = 1; c < ncomp; c++) {
+        if (mc->trans_total[c] > 0) update_max(mc->max_translation[c], &mc->trans_acc[c], &mc->trans_total[c], 5.0);
+        if (mc->rot_total[c] > 0) update_max(mc->max_rotation[c], &mc->rot_acc[c], &mc->rot_total[c], 3.14);
+      }
+    }
+  }
+  free(x.sk_tmp);
+  return nt;
+}
+
+/* Batched form of the same driver: one box, the uniforms of move t from the
+ * Philox stream (seed, stream, t) with the draw layout of mc_ctx (D_*), the
+ * component drawn uniformly over the species with SwapProbability > 1e-10
+ * (the reference's rejection loop over [0, ncomp) at axpy.cu:94-97 has the
+ * same distribution).  *move_index is the box's move counter and advances by
+ * one per step.  Cycle numbering, steps per cycle and the max-step update
+ * follow or_mc_cycles. */
+int or_batch_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
+                    double* sk_ads, double* sk_fw, uint64_t seed, uint64_t stream, int64_t* move_index,
+                    int first_cycle, int n_cycles, or_trace* trace, int max_trace)
+{
+  mc_ctx x;
+  memset(&x, 0, sizeof(x));
+  x.b = b; x.comps = comps; x.ncomp = ncomp; x.mc = mc; x.sk_ads = sk_ads; x.sk_fw = sk_fw;
+  x.philox = 1; x.seed = seed; x.stream = stream;
+  x.sk_tmp = (double*)calloc((size_t)2 * or_nvec(b), sizeof(double));
+  int valid[8], n_valid = 0;
+  for (int c = 1; c < ncomp && c < 8; c++)
+    if (mc->p_swap[c] > 1e-10) valid[n_valid++] = c;
+  if (n_valid == 0) { free(x.sk_tmp); return -1; }
+  int nt = 0;
+  for (int cyc = first_cycle; cyc < first_cycle + n_cycles; cyc++) {
+    int64_t steps = 20;
+    if (steps < mc->total_molecules) steps = mc->total_molecules;
+    if (mc->use_max_step && steps > mc->max_step_per_cycle) steps = mc->max_step_per_cycle;
+    for (int64_t s = 0; s < steps; s++) {
+      x.move = (uint64_t)(*move_index);
+      int vi = (int)(mc_draw(&x, D_COMP) * (double)n_valid);
+      if (vi >= n_valid) vi = n_valid - 1;
+      int comp = valid[vi];
+      int mol = (int)(mc_draw(&x, D_MOL) * (double)comps[comp].n_mol);
+      double r = mc_draw(&x, D_MOVE);
+      or_trace tr;
+      memset(&tr, 0, sizeof(tr));
+      tr.cycle = cyc; tr.step = (int)s; tr.comp = comp; tr.mol = mol; tr.move = OR_NOMOVE;
+      if (r < mc->p_translation[comp]) {
+        tr.move = OR_TRANSLATION;
+        if (comps[comp].n_mol > 0) single_body(&x, comp, mol, OR_TRANSLATION, &tr);
+      } else if (r < mc->p_rotation[comp]) {
+        tr.move = OR_ROTATION;
+        if (comps[comp].n_mol > 0) single_body(&x, comp, mol, OR_ROTATION, &tr);
+      } else if (r < mc->p_widom[comp]) {
+        tr.move = OR_WIDOM;
+        cbmc_move(&x, comp, mol, OR_WIDOM, &tr);
+      } else if (r < mc->p_swap[comp]) {
+        if (mc_draw(&x, D_COIN) < 0.5) {
+          tr.move = OR_INSERTION;
+          cbmc_move(&x, comp, mol, OR_INSERTION, &tr);
+        } else {
+          tr.move = OR_DELETION;
+          if (comps[comp].n_mol > 0) cbmc_move(&x, comp, mol, OR_DELETION, &tr);
+        }
+      }
+      if (nt < max_trace) trace[nt] = tr;
+      nt++;
+      (*move_index)++;
+    }
     if (cyc % 500 == 0) {
       for (int c = 1; c < ncomp; c++) {
         if (mc->trans_total[c] > 0) update_max(mc->max_translation[c], &mc->trans_acc[c], &mc->trans_total[c], 5.0);

@@ -168,6 +168,15 @@ int  or_mc_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
                   double* sk_ads, double* sk_fw, int first_cycle, int n_cycles,
                   or_trace* trace, int max_trace);
 
+/* Batched form of the driver (the engine's device-resident GCMC): move t of
+ * the box draws uniform d from or_philox_uniform(seed, stream, t, d) --
+ * d0 component, d1 molecule, d2 move, d3 insertion/deletion coin, d4
+ * Metropolis, d5 first-bead selection, d6 orientation selection, d8..d10
+ * translation / rotation, d16+3k.. trial k, d16+3K+3o.. orientation o. */
+int  or_batch_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
+                     double* sk_ads, double* sk_fw, uint64_t seed, uint64_t stream, int64_t* move_index,
+                     int first_cycle, int n_cycles, or_trace* trace, int max_trace);
+
 /* ---- batched Widom with per-insertion counter-based streams ---- */
 typedef struct {
   double rosen;          /* Rosenbluth weight W of the insertion          */
