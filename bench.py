@@ -151,6 +151,41 @@ def run_gcmc(system, cycles):
                 accepted=int(cnt[0][1] + cnt[0][3] + cnt[0][5] + cnt[0][7]))
 
 
+def run_batch_gcmc(n_boxes, cycles, precision, warm_cycles=1, capacity=2048):
+    """Config-3 style batch: n_boxes independent CO2/N2 boxes (frameworks of
+    500-6000 atoms per unit cell, 18-40 A, T 250-350 K, P 1e3-1e7 Pa), all on
+    the device in one persistent launch; moves/s over `cycles` cycles per box
+    and the per-cycle latency distribution over every box's cycles."""
+    import numpy as np
+    from graspa_amd import engine, synth
+    systems = [synth.config3_box(b) for b in range(n_boxes)]
+    boxes = []
+    for s in systems:
+        bx = engine.Box(s, precision=precision, capacity=capacity)
+        bx.init_ewald()
+        boxes.append(bx)
+    bt = engine.Batch(boxes, [3000 + b for b in range(n_boxes)], list(range(n_boxes)), cycle_cap=max(cycles, 1))
+    if warm_cycles:
+        bt.run(warm_cycles, production=False)
+    k0, l0 = bt.kernel_time()
+    t0 = time.perf_counter()
+    moves = bt.run(cycles, production=True)
+    t1 = time.perf_counter()
+    k1, l1 = bt.kernel_time()
+    lat = np.concatenate([bt.cycle_times(i, cycles) for i in range(n_boxes)])
+    n_mol = np.array([bt.stats(i).n_mol for i in range(n_boxes)])
+    atoms = np.array([len(s.fw_type) for s in systems])
+    out = dict(boxes=n_boxes, cycles_per_box=cycles, moves=int(moves), seconds=t1 - t0,
+               moves_per_s=moves / (t1 - t0), kernel_ms=k1 - k0, launches=int(l1 - l0),
+               cycle_p50_ms=float(np.percentile(lat, 50) * 1e3), cycle_p99_ms=float(np.percentile(lat, 99) * 1e3),
+               framework_atoms_min=int(atoms.min()), framework_atoms_max=int(atoms.max()),
+               mean_molecules_per_box=float(n_mol.sum(1).mean()), max_molecules_per_box=int(n_mol.sum(1).max()))
+    bt.close()
+    for bx in boxes:
+        bx.close()
+    return out
+
+
 def cpu_baseline(system, n_insert):
     from oracle import binding as ob
     o = ob.OracleSystem(system)
@@ -172,6 +207,8 @@ def main():
     ap.add_argument("--gcmc-cycles", type=int, default=2000)
     ap.add_argument("--cpu-sample", type=int, default=15000)
     ap.add_argument("--quick", action="store_true", help="skip the secondary measurements")
+    ap.add_argument("--batch-boxes", type=int, default=256)
+    ap.add_argument("--batch-cycles", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run 3 plain device-to-device copies of BYTES (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -243,6 +280,8 @@ def main():
     from graspa_amd.deck import GAS_CONSTANT
     w = r["sum_w"] / max(1, r["n"])
     line["config"]["henry_mol_per_kg_pa"] = w / (GAS_CONSTANT * s1.temperature * rho)
+    if args.batch_boxes > 0:
+        line["gcmc_batched"] = run_batch_gcmc(args.batch_boxes, args.batch_cycles, engine.GRASPA_MIXED)
     if not args.quick:
         r64 = run_widom(s1, engine.GRASPA_FP64, args.insertions, max(1, args.steps // 5), 1, timing=False)
         line["fp64_widom_insertions_per_s"] = args.insertions * max(1, args.steps // 5) / r64["seconds"]
@@ -258,6 +297,8 @@ def main():
         s0 = synth.config0(cycles=args.gcmc_cycles)
         g = run_gcmc(s0, args.gcmc_cycles)
         line["gcmc_fp64_single_box"] = g
+        if "gcmc_batched" in line:
+            line["gcmc_batched"]["over_single_box_fp64"] = line["gcmc_batched"]["moves_per_s"] / g["moves_per_s"]
         line["cpu_baseline"] = cpu_baseline(s1, args.cpu_sample)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line))

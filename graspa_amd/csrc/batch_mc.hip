@@ -88,6 +88,9 @@ struct BBox {
   int has_tail;
   const double2* sk_fw;
   double2* sk[2];                   // adsorbate S(k) double buffer
+  const uint4* ctab;                // overlap-screen cells: cap slots of {fx, fy, fz, type}
+  int cell_n[3];
+  int cell_cap;                     // 0: no cell grid
   const double* tail;               // [T*T]
   const int* use_tail;
   int n_comp, n_valid;
@@ -133,6 +136,8 @@ struct BSite {
 // LDS workspace of one step.
 struct BMove {
   BSite site[kBMaxSites];
+  uint4 pfix[kBMaxSites];           // sites as fixed-point cell fractions (screen)
+  unsigned char idx[kBMaxSites];    // sites of the current pair pass
   double e[kBMaxSites][4];          // hg vdw, hg real, gg vdw, gg real per site
   double part[2][kBW][2 * kBGroup];
   double ud[kBMaxDraws];
@@ -142,8 +147,8 @@ struct BMove {
   double ewp[kBW][2];
   double en[10];                    // graspa_energy of the move
   double W, pacc, same, cross;
-  unsigned long long ovl;
-  int type, tmove, comp, mol, M, start, charged;
+  unsigned long long ovl, smask;
+  int type, tmove, comp, mol, M, start, charged, n_eval;
   int fail, accept, overlap;
 };
 
@@ -218,9 +223,9 @@ __device__ inline void pair_site(const BoxConst& bc, const double* __restrict__ 
   }
 }
 
-// Energies of sites [0, n_sites) into L.e and their overlap flags into
-// L.ovl (bit s), against the framework (host-guest) and every guest atom
-// except the index range [ex_lo, ex_hi) (the molecule itself).
+// Energies of the sites L.idx[0 .. n_sites) into L.e[site] and their overlap
+// flags into L.ovl (bit site), against the framework (host-guest) and every
+// guest atom except the index range [ex_lo, ex_hi) (the molecule itself).
 template <bool MIXED, bool CUBIC>
 __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double* __restrict__ ffl,
                           const double2* __restrict__ ct, const int nint, const int n_sites, const int ex_lo,
@@ -232,14 +237,13 @@ __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double
   const int ngrp = (n_sites + kBGroup - 1) / kBGroup;
   const int per = (n_sites + ngrp - 1) / ngrp;
   const int nf = B.n_fw, cap = B.guest_cap;
-  if (tid == 0) L.ovl = 0ull;
   unsigned long long ovl = 0ull;
   for (int g0 = 0; g0 < n_sites; g0 += per) {
     const int cnt = min(per, n_sites - g0);
+    uint32_t ovb = 0;
 #pragma unroll 1
     for (int part = 0; part < 2; ++part) {
       double av[kBGroup], ar[kBGroup];
-      uint32_t ovb = 0;
 #pragma unroll
       for (int j = 0; j < kBGroup; ++j) { av[j] = 0.0; ar[j] = 0.0; }
       if (part == 0) {
@@ -249,8 +253,8 @@ __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double
           const int at = B.fw_type[i];
 #pragma unroll
           for (int j = 0; j < kBGroup; ++j)
-            if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[g0 + j], ax, ay, az, aq, at, av[j], ar[j],
-                                                 ovb, 1u << j);
+            if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j],
+                                                 ar[j], ovb, 1u << j);
         }
       } else {
         for (int c = 1; c < B.n_comp; ++c) {
@@ -262,25 +266,70 @@ __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double
             const int at = B.g_type[i];
 #pragma unroll
             for (int j = 0; j < kBGroup; ++j)
-              if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[g0 + j], ax, ay, az, aq, at, av[j],
-                                                   ar[j], ovb, 1u << j);
+              if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at,
+                                                   av[j], ar[j], ovb, 1u << j);
           }
         }
       }
       wave_sum_group(av, ar, L.part[part][wave]);
-      ovl |= (unsigned long long)ovb << g0;
     }
+#pragma unroll
+    for (int j = 0; j < kBGroup; ++j)
+      if (j < cnt && ((ovb >> j) & 1u)) ovl |= 1ull << L.idx[g0 + j];
     __syncthreads();
     if (tid < 4 * cnt) {
       const int j = tid >> 2, t = tid & 3, part = t >> 1, k = t & 1;
       double s = 0.0;
       for (int w = 0; w < kBW; ++w) s += L.part[part][w][2 * j + k];
-      L.e[g0 + j][t] = s;
+      L.e[L.idx[g0 + j]][t] = s;
     }
     __syncthreads();
   }
   ovl = wave_or64(ovl);
   if (lane == 0 && ovl) atomicOr(&L.ovl, ovl);
+  __syncthreads();
+}
+
+// Overlap screen of sites [first, first + n): FP32 squared distances from
+// fixed-point cell fractions to the framework atoms of the 27 grid cells
+// around each site (the cells are at least as wide as the largest overlap
+// distance, so they hold every framework atom that can overlap the site)
+// against thresholds r_ov^2 lowered by 1e-5 relative: a hit is a pair the
+// exact evaluation flags as an overlap in either precision form.  Hits go to
+// L.smask (bit site).  Needs a cell grid (B.cell_cap > 0).
+template <bool CUBIC>
+__device__ void screen_pass(const BBox& B, BMove& L, const float* __restrict__ thr, const int first, const int n)
+{
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int T = B.bc.ntypes, C = B.cell_cap;
+  const int n0 = B.cell_n[0], n1 = B.cell_n[1], n2 = B.cell_n[2];
+  if (tid < n) {
+    const BSite& st = L.site[first + tid];
+    const double p[3] = {st.x, st.y, st.z};
+    L.pfix[first + tid] = to_fixed4(B.bc, p);
+  }
+  if (tid == 0) L.smask = 0ull;
+  __syncthreads();
+  const int per = 27 * C, total = n * per;
+  unsigned long long m = 0ull;
+  for (int t = tid; t < total; t += kBT) {
+    const int k = t / per, r = t - k * per, nb = r / C, slot = r - nb * C;
+    const int site = first + k;
+    const uint4 P = L.pfix[site];
+    int x = (int)(((uint64_t)P.x * (uint64_t)n0) >> 32) + nb % 3 - 1;
+    int y = (int)(((uint64_t)P.y * (uint64_t)n1) >> 32) + (nb / 3) % 3 - 1;
+    int z = (int)(((uint64_t)P.z * (uint64_t)n2) >> 32) + nb / 9 - 1;
+    x = x < 0 ? x + n0 : (x >= n0 ? x - n0 : x);
+    y = y < 0 ? y + n1 : (y >= n1 ? y - n1 : y);
+    z = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
+    const uint4 at = B.ctab[(size_t)(x + n0 * (y + n1 * z)) * C + slot];
+    if (at.w != ~0u) {
+      const float th = thr[L.site[site].type * T + (int)at.w];
+      if (r2_fixed32<CUBIC>(at.x, at.y, at.z, P.x, P.y, P.z, B.bc.h32) < th) m |= 1ull << site;
+    }
+  }
+  m = wave_or64(m);
+  if (lane == 0 && m) atomicOr(&L.smask, m);
   __syncthreads();
 }
 
@@ -441,8 +490,8 @@ enum { kHGV = 1, kGGV = 2, kHGR = 4, kGGR = 5, kHGE = 7, kGGE = 8, kTail = 9 };
 // the moves
 // ---------------------------------------------------------------------------
 template <bool MIXED, bool CUBIC>
-__device__ void single_body(const BBox& B, BState& S, BMove& L, const double* ffl, const double2* ct, int nint,
-                            double2* eik)
+__device__ void single_body(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr, const double2* ct,
+                            int nint, double2* eik)
 {
   const int tid = threadIdx.x;
   const int M = L.M, start = L.start, cap = B.guest_cap, comp = L.comp;
@@ -471,6 +520,17 @@ __device__ void single_body(const BBox& B, BState& S, BMove& L, const double* ff
     }
     L.site[M + tid] = s;
   }
+  __syncthreads();
+  if (B.cell_cap > 0) {
+    // a certain overlap of the new position rejects the move (no energies needed)
+    screen_pass<CUBIC>(B, L, thr, M, M);
+    if (L.smask) {
+      if (tid == 0) L.overlap = 1;
+      return;
+    }
+  }
+  if (tid < 2 * M) L.idx[tid] = (unsigned char)tid;
+  if (tid == 0) L.ovl = 0ull;
   __syncthreads();
   pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, 2 * M, start, start + M);
   if (tid == 0) {
@@ -515,9 +575,24 @@ __device__ void single_body(const BBox& B, BState& S, BMove& L, const double* ff
   }
 }
 
+// Survivors of the screen among sites [0, n) in groups of g consecutive sites
+// (a trial is dead if any of its sites is screened out) become the pass's
+// site list; the screened sites' overlap bits are set.  Thread 0.
+__device__ inline void list_survivors(BMove& L, const int n, const int g)
+{
+  const unsigned long long gm = (g >= 64) ? ~0ull : ((1ull << g) - 1ull);
+  int k = 0;
+  for (int t0 = 0; t0 < n; t0 += g) {
+    if ((L.smask >> t0) & gm) continue;
+    for (int a = 0; a < g; ++a) L.idx[k++] = (unsigned char)(t0 + a);
+  }
+  L.n_eval = k;
+  L.ovl = L.smask;
+}
+
 template <bool MIXED, bool CUBIC>
-__device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, const double2* ct, int nint,
-                     double2* eik)
+__device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr, const double2* ct,
+                     int nint, double2* eik)
 {
   const int tid = threadIdx.x;
   const int M = L.M, comp = L.comp, cap = B.guest_cap;
@@ -542,7 +617,11 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
     L.site[tid] = s;
   }
   __syncthreads();
-  pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, K, ex_lo, ex_hi);
+  if (B.cell_cap > 0) screen_pass<CUBIC>(B, L, thr, 0, K);
+  else if (tid == 0) L.smask = 0ull;
+  if (tid == 0) list_survivors(L, K, 1);
+  __syncthreads();
+  if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, L.n_eval, ex_lo, ex_hi);
   if (tid < K) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) L.t4[tid][t] = L.e[tid][t];
@@ -588,7 +667,11 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
       L.site[t] = s;
     }
     __syncthreads();
-    pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, O * chain, ex_lo, ex_hi);
+    if (B.cell_cap > 0) screen_pass<CUBIC>(B, L, thr, 0, O * chain);
+    else if (tid == 0) L.smask = 0ull;
+    if (tid == 0) list_survivors(L, O * chain, chain);
+    __syncthreads();
+    if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, L.n_eval, ex_lo, ex_hi);
     if (tid < O) {
       double e4[4] = {0.0, 0.0, 0.0, 0.0};
       for (int a = 0; a < chain; ++a)
@@ -754,12 +837,18 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
   const int T = B.bc.ntypes;
   const int nff = 4 * T * T;
   const int nff4 = (nff + 3) & ~3;
+  const int nthr = ((T * T + 7) & ~7) / 2;     // screen thresholds, in doubles
   double* ffl = dyn;
-  double2* ct = reinterpret_cast<double2*>(dyn + nff4);
+  float* thr = reinterpret_cast<float*>(dyn + nff4);
+  double2* ct = reinterpret_cast<double2*>(dyn + nff4 + nthr);
   const int nct = MIXED ? ((B.coul_n + 3) & ~3) : 0;
-  double2* eik = reinterpret_cast<double2*>(dyn + nff4 + nct);
+  double2* eik = reinterpret_cast<double2*>(dyn + nff4 + nthr + nct);
   const int nint = B.coul_n / kCoulTabStride;
   for (int i = tid; i < nff; i += kBT) ffl[i] = B.ff64[i];
+  for (int i = tid; i < T * T; i += kBT) {
+    const int st = i / T, at = i - st * T;
+    thr[i] = screen_threshold(B.ff64 + 4 * (at * T + st), B.bc.overlap, B.bc.cut_vdw2);
+  }
   if (MIXED)
     for (int i = tid; i < B.coul_n; i += kBT) dyn[nff4 + i] = B.coul[i];
   __syncthreads();
@@ -819,8 +908,8 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
     }
     __syncthreads();
     const int type = L.type;
-    if (type == kMvTrans || type == kMvRot) single_body<MIXED, CUBIC>(B, S, L, ffl, ct, nint, eik);
-    else if (type != kMvNone) cbmc<MIXED, CUBIC>(B, S, L, ffl, ct, nint, eik);
+    if (type == kMvTrans || type == kMvRot) single_body<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, eik);
+    else if (type != kMvNone) cbmc<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, eik);
     __syncthreads();
     if (tid == 0) {
       const int comp = L.comp;
@@ -878,20 +967,19 @@ struct graspa_batch {
   int cubic = 1;
   hipStream_t stream = nullptr;
   int n = 0;
-  std::vector<graspa_box*> boxes;
+  std::vector<graspa_box*> boxes;   // slot -> box (nullptr: empty slot)
   std::vector<BBox> hb;
   std::vector<BState> hs;
-  std::vector<std::vector<double>> tails;
-  std::vector<std::vector<int>> use_tails;
   BBox* d_box = nullptr;
   BState* d_state = nullptr;
-  double* d_tail = nullptr;
+  double* d_tail = nullptr;         // kMaxTypes^2 per slot
   int* d_use_tail = nullptr;
   graspa_trace* d_trace = nullptr;
   int trace_cap = 0;
   float* d_cyc = nullptr;
   int cyc_cap = 0;
   size_t lds = 0;
+  int max_lds = 65536;
   double tick_hz = 1.0e8;
   double kernel_ms = 0.0;
   int64_t launches = 0;
@@ -929,6 +1017,123 @@ void free_batch(graspa_batch* bt)
   delete bt;
 }
 
+// Dynamic LDS of one box: force field, real-space table (fast form), e^{ik.r}
+// tables of up to 2 x 8 sites.
+size_t box_lds(const graspa_box* b, int precision)
+{
+  const int T = b->bc.ntypes;
+  const int ks = 1 + std::max(b->bc.kmax[0], std::max(b->bc.kmax[1], b->bc.kmax[2]));
+  const size_t nff4 = ((size_t)4 * T * T + 3) & ~(size_t)3;
+  const size_t nthr = (((size_t)T * T + 7) & ~(size_t)7) / 2;
+  const size_t nct = precision == GRASPA_MIXED ? (((size_t)b->coul_n + 3) & ~(size_t)3) : 0;
+  return (nff4 + nthr + nct) * sizeof(double) + (size_t)2 * kBMaxMol * 3 * ks * sizeof(double2);
+}
+
+// Validates one box and fills slot i's descriptor and initial state (host
+// side); the caller uploads.
+int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
+{
+  graspa_box* b = in.box;
+  const graspa_mc_params* p = in.params;
+  if (!b || !p) return fail(GRASPA_ERR_ARG, "null box or params");
+  if (b->device != bt->device || b->precision != bt->precision)
+    return fail(GRASPA_ERR_ARG, "all boxes of a batch must share the device and the precision form");
+  const int ncomp = (int)b->comps.size();
+  if (p->n_species != ncomp - 1 || ncomp < 2)
+    return fail(GRASPA_ERR_ARG, "n_species must match the box's adsorbate components");
+  if (p->n_trial_pos < 1 || p->n_trial_pos > kBMaxTrial || p->n_trial_orient < 1 || p->n_trial_orient > kBMaxTrial)
+    return fail(GRASPA_ERR_ARG, "trial counts must be in 1..32");
+  const int T = b->bc.ntypes;
+  if (T > kMaxTypes) return fail(GRASPA_ERR_ARG, "too many atom types");
+  if (b->bc.use_charges && b->nvec <= 0) return fail(GRASPA_ERR_STATE, "charged box without Ewald tables");
+  if (p->has_tail && (!p->tail || !p->use_tail)) return fail(GRASPA_ERR_ARG, "tail tables required when has_tail");
+  const size_t need = box_lds(b, bt->precision);
+  if (need + sizeof(BState) + sizeof(BMove) + sizeof(BBox) > (size_t)bt->max_lds)
+    return fail(GRASPA_ERR_UNSUPPORTED, "box workspace exceeds LDS (force field, real-space table, k range)");
+  BBox h;
+  std::memset(&h, 0, sizeof(h));
+  h.bc = b->bc;
+  h.volume = b->volume;
+  h.fw = b->d_fw; h.fw_type = b->d_fw_type; h.n_fw = b->n_fw;
+  h.guest_cap = b->guest_cap;
+  h.g = b->d_g; h.g_type = b->d_g_type; h.g_molid = b->d_g_molid; h.g_comp = b->d_g_comp;
+  h.ff64 = b->d_ff64;
+  h.coul = b->d_coul; h.coul_n = b->coul_n;
+  h.kstride = 1 + std::max(b->bc.kmax[0], std::max(b->bc.kmax[1], b->bc.kmax[2]));
+  h.kact = b->d_kact; h.ktemp_act = b->d_ktemp_act; h.n_active = b->bc.use_charges ? b->n_active : 0;
+  h.sk_fw = b->d_sk_fw; h.sk[0] = b->d_sk_ads; h.sk[1] = b->d_sk_stage;
+  if (b->use_cells) {
+    h.ctab = b->d_ctab;
+    h.cell_cap = b->cell_cap;
+    for (int d = 0; d < 3; ++d) h.cell_n[d] = b->cell_n[d];
+  }
+  h.n_comp = ncomp;
+  for (int c = 1; c < ncomp; ++c) {
+    const graspa_species& sp = p->species[c - 1];
+    const Component& cc = b->comps[c];
+    if (cc.molsize > kBMaxMol) return fail(GRASPA_ERR_ARG, "molecules of at most 8 atoms");
+    if ((cc.molsize - 1) * p->n_trial_orient > kBMaxSites)
+      return fail(GRASPA_ERR_ARG, "n_trial_orient * (molsize - 1) must be <= 64");
+    BSpecies& d = h.sp[c];
+    d.p_trans = sp.p_translation; d.p_rot = sp.p_rotation; d.p_widom = sp.p_widom; d.p_swap = sp.p_swap;
+    d.mol_fraction = sp.mol_fraction; d.fugacity = sp.fugacity_coeff; d.ideal_rosen = sp.ideal_rosen;
+    d.excl = sp.excl_intra + sp.excl_self;
+    d.base = cc.base; d.molsize = cc.molsize; d.capacity = cc.capacity;
+    double qabs = 0.0;
+    for (int a = 0; a < cc.molsize; ++a) qabs += std::fabs(cc.q[a]);
+    d.charged = (b->bc.use_charges && qabs > 1e-6) ? 1 : 0;   // hasPartialCharge (read_data.cpp:2160-2164)
+    for (int t = 0; t < 16; ++t) d.pseudo[t] = (int)sp.pseudo_per_mol[t];
+    if (sp.p_swap > 1e-10) h.valid[h.n_valid++] = c;
+  }
+  if (h.n_valid == 0) return fail(GRASPA_ERR_ARG, "no species with a move probability");
+  h.K = p->n_trial_pos; h.O = p->n_trial_orient; h.bias = p->vdw_real_bias;
+  h.use_max_step = p->use_max_step; h.max_step = p->max_step_per_cycle;
+  h.beta = p->beta; h.pressure = p->pressure;
+  h.seed = in.seed; h.stream = in.stream;
+  h.has_tail = p->has_tail;
+  h.trace_cap = bt->trace_cap;
+  h.cyc_cap = bt->cyc_cap;
+  h.trace = bt->d_trace;
+  h.cyc = bt->d_cyc ? bt->d_cyc + (size_t)i * bt->cyc_cap : nullptr;
+  h.tail = bt->d_tail + (size_t)i * kMaxTypes * kMaxTypes;
+  h.use_tail = bt->d_use_tail + (size_t)i * kMaxTypes * kMaxTypes;
+  if (p->has_tail) {
+    if (hip_check(hipMemcpy(bt->d_tail + (size_t)i * kMaxTypes * kMaxTypes, p->tail, (size_t)T * T * sizeof(double),
+                            hipMemcpyHostToDevice), "tail upload"))
+      return GRASPA_ERR_HIP;
+    if (hip_check(hipMemcpy(bt->d_use_tail + (size_t)i * kMaxTypes * kMaxTypes, p->use_tail,
+                            (size_t)T * T * sizeof(int), hipMemcpyHostToDevice), "tail upload"))
+      return GRASPA_ERR_HIP;
+  }
+  BState st;
+  std::memset(&st, 0, sizeof(st));
+  st.total_mol = 1;
+  for (int c = 1; c < ncomp; ++c) {
+    st.n_mol[c] = b->comps[c].n_mol;
+    st.total_mol += b->comps[c].n_mol;
+    for (int d = 0; d < 3; ++d) {
+      st.max_trans[c][d] = p->species[c - 1].max_translation[d];
+      st.max_rot[c][d] = p->species[c - 1].max_rotation[d];
+    }
+  }
+  for (int t = 0; t < 16; ++t) st.n_pseudo[t] = p->n_pseudo[t];
+  hipSetDevice(b->device);
+  if (hip_check(hipStreamSynchronize(b->stream), "box stream")) return GRASPA_ERR_HIP;
+  // both S(k) buffers start equal, so entries a move never writes (inactive
+  // k-vectors) agree whichever buffer is current
+  if (b->bc.use_charges && b->nvec > 0 &&
+      hip_check(hipMemcpy(b->d_sk_stage, b->d_sk_ads, b->nvec * sizeof(double2), hipMemcpyDeviceToDevice), "sk copy"))
+    return GRASPA_ERR_HIP;
+  bt->hb[i] = h;
+  bt->hs[i] = st;
+  bt->boxes[i] = b;
+  bt->lds = std::max(bt->lds, need);
+  bt->cubic = 1;
+  for (int k = 0; k < bt->n; ++k)
+    if (bt->boxes[k]) bt->cubic &= bt->hb[k].bc.cubic ? 1 : 0;
+  return GRASPA_OK;
+}
+
 template <bool MIXED, bool CUBIC>
 int launch_batch(graspa_batch* bt, int max_moves)
 {
@@ -953,180 +1158,120 @@ int launch_batch(graspa_batch* bt, int max_moves)
   return GRASPA_OK;
 }
 
+int launch_once(graspa_batch* bt, int max_moves)
+{
+  if (bt->precision == GRASPA_MIXED)
+    return bt->cubic ? launch_batch<true, true>(bt, max_moves) : launch_batch<true, false>(bt, max_moves);
+  return bt->cubic ? launch_batch<false, true>(bt, max_moves) : launch_batch<false, false>(bt, max_moves);
+}
+
 }  // namespace
 
 extern "C" {
 
 int graspa_batch_create(const graspa_batch_box* boxes, int n_boxes, int trace_cap, int cycle_cap, graspa_batch** out)
 {
-  if (!boxes || n_boxes < 1 || !out || trace_cap < 0 || cycle_cap < 0) return fail(GRASPA_ERR_ARG, "bad arguments");
+  if (n_boxes < 1 || !out || trace_cap < 0 || cycle_cap < 0) return fail(GRASPA_ERR_ARG, "bad arguments");
   *out = nullptr;
   graspa_batch* bt = new graspa_batch();
   bt->n = n_boxes;
-  bt->device = boxes[0].box ? boxes[0].box->device : 0;
-  bt->precision = boxes[0].box ? boxes[0].box->precision : GRASPA_FP64;
   bt->trace_cap = trace_cap;
   bt->cyc_cap = cycle_cap;
-  bt->hb.resize(n_boxes);
-  bt->hs.resize(n_boxes);
-  bt->tails.resize(n_boxes);
-  bt->use_tails.resize(n_boxes);
-  int max_t = 1, max_coul = 0, max_ks = 1;
-  size_t tail_total = 0;
+  bt->boxes.assign(n_boxes, nullptr);
+  bt->hb.assign(n_boxes, BBox{});
+  bt->hs.assign(n_boxes, BState{});
   for (int i = 0; i < n_boxes; ++i) {
-    const graspa_batch_box& in = boxes[i];
-    graspa_box* b = in.box;
-    const graspa_mc_params* p = in.params;
-    if (!b || !p) { free_batch(bt); return fail(GRASPA_ERR_ARG, "null box or params"); }
-    if (b->device != bt->device || b->precision != bt->precision) {
-      free_batch(bt);
-      return fail(GRASPA_ERR_ARG, "all boxes of a batch must share the device and the precision form");
-    }
-    const int ncomp = (int)b->comps.size();
-    if (p->n_species != ncomp - 1 || ncomp < 2) {
-      free_batch(bt);
-      return fail(GRASPA_ERR_ARG, "n_species must match the box's adsorbate components");
-    }
-    if (p->n_trial_pos < 1 || p->n_trial_pos > kBMaxTrial || p->n_trial_orient < 1 || p->n_trial_orient > kBMaxTrial) {
-      free_batch(bt);
-      return fail(GRASPA_ERR_ARG, "trial counts must be in 1..32");
-    }
-    if (b->bc.ntypes > kMaxTypes) { free_batch(bt); return fail(GRASPA_ERR_ARG, "too many atom types"); }
-    if (b->bc.use_charges && b->nvec <= 0) {
-      free_batch(bt);
-      return fail(GRASPA_ERR_STATE, "charged box without Ewald tables");
-    }
-    if (p->has_tail && (!p->tail || !p->use_tail)) {
-      free_batch(bt);
-      return fail(GRASPA_ERR_ARG, "tail tables required when has_tail");
-    }
-    hipSetDevice(b->device);
-    hipStreamSynchronize(b->stream);
-    BBox& h = bt->hb[i];
-    std::memset(&h, 0, sizeof(h));
-    h.bc = b->bc;
-    h.volume = b->volume;
-    h.fw = b->d_fw; h.fw_type = b->d_fw_type; h.n_fw = b->n_fw;
-    h.guest_cap = b->guest_cap;
-    h.g = b->d_g; h.g_type = b->d_g_type; h.g_molid = b->d_g_molid; h.g_comp = b->d_g_comp;
-    h.ff64 = b->d_ff64;
-    h.coul = b->d_coul; h.coul_n = b->coul_n;
-    h.kstride = 1 + std::max(b->bc.kmax[0], std::max(b->bc.kmax[1], b->bc.kmax[2]));
-    h.kact = b->d_kact; h.ktemp_act = b->d_ktemp_act; h.n_active = b->bc.use_charges ? b->n_active : 0;
-    h.sk_fw = b->d_sk_fw; h.sk[0] = b->d_sk_ads; h.sk[1] = b->d_sk_stage;
-    h.n_comp = ncomp;
-    h.n_valid = 0;
-    for (int c = 1; c < ncomp; ++c) {
-      const graspa_species& s = p->species[c - 1];
-      const Component& cc = b->comps[c];
-      if (cc.molsize > kBMaxMol) { free_batch(bt); return fail(GRASPA_ERR_ARG, "molecules of at most 8 atoms"); }
-      if ((cc.molsize - 1) * p->n_trial_orient > kBMaxSites) {
-        free_batch(bt);
-        return fail(GRASPA_ERR_ARG, "n_trial_orient * (molsize - 1) must be <= 64");
-      }
-      BSpecies& d = h.sp[c];
-      d.p_trans = s.p_translation; d.p_rot = s.p_rotation; d.p_widom = s.p_widom; d.p_swap = s.p_swap;
-      d.mol_fraction = s.mol_fraction; d.fugacity = s.fugacity_coeff; d.ideal_rosen = s.ideal_rosen;
-      d.excl = s.excl_intra + s.excl_self;
-      d.base = cc.base; d.molsize = cc.molsize; d.capacity = cc.capacity;
-      double qabs = 0.0;
-      for (int a = 0; a < cc.molsize; ++a) qabs += std::fabs(cc.q[a]);
-      d.charged = (b->bc.use_charges && qabs > 1e-6) ? 1 : 0;   // hasPartialCharge (read_data.cpp:2160-2164)
-      for (int t = 0; t < 16; ++t) d.pseudo[t] = (int)s.pseudo_per_mol[t];
-      if (s.p_swap > 1e-10) h.valid[h.n_valid++] = c;
-    }
-    if (h.n_valid == 0) { free_batch(bt); return fail(GRASPA_ERR_ARG, "no species with a move probability"); }
-    h.K = p->n_trial_pos; h.O = p->n_trial_orient; h.bias = p->vdw_real_bias;
-    h.use_max_step = p->use_max_step; h.max_step = p->max_step_per_cycle;
-    h.beta = p->beta; h.pressure = p->pressure;
-    h.seed = in.seed; h.stream = in.stream;
-    h.has_tail = p->has_tail;
-    h.trace_cap = trace_cap;
-    h.cyc_cap = cycle_cap;
-    const int T = b->bc.ntypes;
-    if (p->has_tail) {
-      bt->tails[i].assign(p->tail, p->tail + T * T);
-      bt->use_tails[i].assign(p->use_tail, p->use_tail + T * T);
-      tail_total += (size_t)T * T;
-    }
-    max_t = std::max(max_t, T);
-    max_coul = std::max(max_coul, b->coul_n);
-    max_ks = std::max(max_ks, h.kstride);
-
-    BState& s = bt->hs[i];
-    std::memset(&s, 0, sizeof(s));
-    s.total_mol = 1;
-    for (int c = 1; c < ncomp; ++c) {
-      s.n_mol[c] = b->comps[c].n_mol;
-      s.total_mol += b->comps[c].n_mol;
-      for (int d = 0; d < 3; ++d) {
-        s.max_trans[c][d] = p->species[c - 1].max_translation[d];
-        s.max_rot[c][d] = p->species[c - 1].max_rotation[d];
-      }
-    }
-    for (int t = 0; t < 16; ++t) s.n_pseudo[t] = p->n_pseudo[t];
+    bt->hs[i].error = 0;
+    bt->hs[i].target = 0;   // empty slots never run
   }
+  // device and precision of the batch: those of the first box given
+  bool have = false;
+  for (int i = 0; boxes && i < n_boxes && !have; ++i)
+    if (boxes[i].box) {
+      bt->device = boxes[i].box->device;
+      bt->precision = boxes[i].box->precision;
+      have = true;
+    }
   hipSetDevice(bt->device);
   int rc = hip_check(hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking), "batch stream");
-  // LDS: force field, real-space table (fast form), e^{ik.r} tables of 2 x 8 sites
-  const size_t nff4 = ((size_t)4 * max_t * max_t + 3) & ~(size_t)3;
-  const size_t nct = bt->precision == GRASPA_MIXED ? (((size_t)max_coul + 3) & ~(size_t)3) : 0;
-  bt->lds = (nff4 + nct) * sizeof(double) + (size_t)2 * kBMaxMol * 3 * max_ks * sizeof(double2);
-  int max_lds = 65536;
   {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, bt->device) == hipSuccess) {
-      max_lds = (int)prop.sharedMemPerBlock;
+      bt->max_lds = (int)prop.sharedMemPerBlock;
       int v = 0;
-      if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, bt->device) == hipSuccess && v > max_lds)
-        max_lds = v;
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, bt->device) == hipSuccess &&
+          v > bt->max_lds)
+        bt->max_lds = v;
     }
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, bt->device) == hipSuccess && khz > 0)
       bt->tick_hz = 1000.0 * khz;
   }
-  const size_t static_lds = sizeof(BState) + sizeof(BMove);
-  if (!rc && bt->lds + static_lds > (size_t)max_lds) rc = fail(GRASPA_ERR_UNSUPPORTED, "batch workspace exceeds LDS");
-  for (int i = 0; i < n_boxes && !rc; ++i) bt->boxes.push_back(boxes[i].box);
-  if (!rc && tail_total) {
-    rc = hip_check(hipMalloc(&bt->d_tail, tail_total * sizeof(double)), "hipMalloc batch");
-    rc = rc ? rc : hip_check(hipMalloc(&bt->d_use_tail, tail_total * sizeof(int)), "hipMalloc batch");
-    size_t off = 0;
-    for (int i = 0; i < n_boxes && !rc; ++i) {
-      if (bt->tails[i].empty()) continue;
-      const size_t m = bt->tails[i].size();
-      rc = hip_check(hipMemcpy(bt->d_tail + off, bt->tails[i].data(), m * sizeof(double), hipMemcpyHostToDevice),
-                     "tail upload");
-      rc = rc ? rc : hip_check(hipMemcpy(bt->d_use_tail + off, bt->use_tails[i].data(), m * sizeof(int),
-                                         hipMemcpyHostToDevice), "tail upload");
-      bt->hb[i].tail = bt->d_tail + off;
-      bt->hb[i].use_tail = bt->d_use_tail + off;
-      off += m;
-    }
-  }
-  if (!rc && trace_cap > 0) {
+  const size_t nt = (size_t)n_boxes * kMaxTypes * kMaxTypes;
+  rc = rc ? rc : hip_check(hipMalloc(&bt->d_tail, nt * sizeof(double)), "hipMalloc batch");
+  rc = rc ? rc : hip_check(hipMalloc(&bt->d_use_tail, nt * sizeof(int)), "hipMalloc batch");
+  if (!rc && trace_cap > 0)
     rc = hip_check(hipMalloc(&bt->d_trace, (size_t)n_boxes * trace_cap * sizeof(graspa_trace)), "hipMalloc batch");
-    for (int i = 0; i < n_boxes; ++i) bt->hb[i].trace = bt->d_trace;
-  }
-  if (!rc && cycle_cap > 0) {
+  if (!rc && cycle_cap > 0)
     rc = hip_check(hipMalloc(&bt->d_cyc, (size_t)n_boxes * cycle_cap * sizeof(float)), "hipMalloc batch");
-    for (int i = 0; i < n_boxes; ++i) bt->hb[i].cyc = bt->d_cyc + (size_t)i * cycle_cap;
-  }
   rc = rc ? rc : hip_check(hipMalloc(&bt->d_box, n_boxes * sizeof(BBox)), "hipMalloc batch");
   rc = rc ? rc : hip_check(hipMalloc(&bt->d_state, n_boxes * sizeof(BState)), "hipMalloc batch");
-  // both S(k) buffers start equal, so entries a move never writes (inactive
-  // k-vectors) agree whichever buffer is current
-  for (int i = 0; i < n_boxes && !rc; ++i) {
-    graspa_box* b = boxes[i].box;
-    if (b->bc.use_charges && b->nvec > 0)
-      rc = hip_check(hipMemcpyAsync(b->d_sk_stage, b->d_sk_ads, b->nvec * sizeof(double2), hipMemcpyDeviceToDevice,
-                                    bt->stream), "sk copy");
-  }
+  for (int i = 0; boxes && i < n_boxes && !rc; ++i)
+    if (boxes[i].box) rc = fill_slot(bt, i, boxes[i]);
   rc = rc ? rc : upload_batch(bt);
   if (rc) { free_batch(bt); return rc; }
-  bt->cubic = 1;
-  for (int i = 0; i < n_boxes; ++i) bt->cubic &= bt->hb[i].bc.cubic ? 1 : 0;
   *out = bt;
+  return GRASPA_OK;
+}
+
+int graspa_batch_set_box(graspa_batch* bt, int slot, const graspa_batch_box* box)
+{
+  if (!bt || slot < 0 || slot >= bt->n) return fail(GRASPA_ERR_ARG, "bad slot");
+  hipSetDevice(bt->device);
+  if (int rc = download_states(bt)) return rc;
+  if (!box || !box->box) {
+    bt->boxes[slot] = nullptr;
+    bt->hb[slot] = BBox{};
+    bt->hs[slot] = BState{};
+  } else {
+    const bool first = std::none_of(bt->boxes.begin(), bt->boxes.end(), [](graspa_box* x) { return x != nullptr; });
+    if (first) {
+      bt->device = box->box->device;
+      bt->precision = box->box->precision;
+    }
+    if (int rc = fill_slot(bt, slot, *box)) return rc;
+  }
+  return upload_batch(bt);
+}
+
+int graspa_batch_set_cycles(graspa_batch* bt, int slot, int n_cycles, int production)
+{
+  if (!bt || slot < 0 || slot >= bt->n || n_cycles < 0) return fail(GRASPA_ERR_ARG, "bad arguments");
+  if (!bt->boxes[slot]) return fail(GRASPA_ERR_STATE, "empty slot");
+  hipSetDevice(bt->device);
+  if (int rc = download_states(bt)) return rc;
+  BState& s = bt->hs[slot];
+  s.target = s.cycle + n_cycles;
+  s.production = production ? 1 : 0;
+  s.trace_n = 0;
+  s.cyc_first = s.cycle;
+  return hip_check(hipMemcpy(&bt->d_state[slot], &s, sizeof(BState), hipMemcpyHostToDevice), "batch state upload");
+}
+
+int graspa_batch_launch(graspa_batch* bt, int moves_per_launch, int* n_running)
+{
+  if (!bt) return fail(GRASPA_ERR_ARG, "null batch");
+  hipSetDevice(bt->device);
+  if (int rc = launch_once(bt, moves_per_launch > 0 ? moves_per_launch : 4096)) return rc;
+  if (int rc = download_states(bt)) return rc;
+  int run = 0;
+  for (int i = 0; i < bt->n; ++i) {
+    const BState& s = bt->hs[i];
+    if (!bt->boxes[i]) continue;
+    if (s.error) return fail(s.error, "a box of the batch stopped on an error (molecule capacity)");
+    if (s.cycle < s.target) ++run;
+  }
+  if (n_running) *n_running = run;
   return GRASPA_OK;
 }
 
@@ -1140,6 +1285,7 @@ int graspa_batch_run(graspa_batch* bt, int n_cycles, int production, int moves_p
   int64_t before = 0;
   for (int i = 0; i < bt->n; ++i) {
     BState& s = bt->hs[i];
+    if (!bt->boxes[i]) continue;
     if (s.error) return fail(s.error, "a box of the batch stopped on an error (molecule capacity)");
     s.target = s.cycle + n_cycles;
     s.production = production ? 1 : 0;
@@ -1153,17 +1299,14 @@ int graspa_batch_run(graspa_batch* bt, int n_cycles, int production, int moves_p
   const int mpl = moves_per_launch > 0 ? moves_per_launch : 4096;
   int64_t last = before;
   for (;;) {
-    int rc;
-    if (bt->precision == GRASPA_MIXED)
-      rc = bt->cubic ? launch_batch<true, true>(bt, mpl) : launch_batch<true, false>(bt, mpl);
-    else
-      rc = bt->cubic ? launch_batch<false, true>(bt, mpl) : launch_batch<false, false>(bt, mpl);
+    int rc = launch_once(bt, mpl);
     if (rc) return rc;
     if ((rc = download_states(bt))) return rc;
     bool done = true;
     int err = 0;
     for (int i = 0; i < bt->n; ++i) {
       const BState& s = bt->hs[i];
+      if (!bt->boxes[i]) continue;
       if (s.error) err = s.error;
       else if (s.cycle < s.target) done = false;
     }
@@ -1184,7 +1327,7 @@ int graspa_batch_box_stats(graspa_batch* bt, int i, graspa_energy* running, int*
                            double* widom_sum_w, double* widom_n, double* load_sum, double* load_sum2,
                            int64_t* n_load_cycles, int64_t* moves, int64_t* cycles)
 {
-  if (!bt || i < 0 || i >= bt->n) return fail(GRASPA_ERR_ARG, "bad box index");
+  if (!bt || i < 0 || i >= bt->n || !bt->boxes[i]) return fail(GRASPA_ERR_ARG, "bad box index or empty slot");
   hipSetDevice(bt->device);
   if (int rc = download_states(bt)) return rc;
   const BState& s = bt->hs[i];
@@ -1235,6 +1378,20 @@ int graspa_batch_trace(graspa_batch* bt, int i, graspa_trace* out, int max_n)
   return n;
 }
 
+int graspa_batch_progress(graspa_batch* bt, int64_t* moves, int64_t* cycles, int64_t* targets)
+{
+  if (!bt) return fail(GRASPA_ERR_ARG, "null batch");
+  hipSetDevice(bt->device);
+  if (int rc = download_states(bt)) return rc;
+  for (int i = 0; i < bt->n; ++i) {
+    const bool on = bt->boxes[i] != nullptr;
+    if (moves) moves[i] = on ? bt->hs[i].move : 0;
+    if (cycles) cycles[i] = on ? bt->hs[i].cycle : 0;
+    if (targets) targets[i] = on ? bt->hs[i].target : 0;
+  }
+  return GRASPA_OK;
+}
+
 int graspa_batch_kernel_time(graspa_batch* bt, double* total_ms, int64_t* launches)
 {
   if (!bt) return fail(GRASPA_ERR_ARG, "null batch");
@@ -1251,6 +1408,7 @@ int graspa_batch_sync(graspa_batch* bt)
   bool flipped = false;
   for (int i = 0; i < bt->n; ++i) {
     graspa_box* b = bt->boxes[i];
+    if (!b) continue;
     BState& s = bt->hs[i];
     const int cap = b->guest_cap;
     std::vector<double> g(4 * (size_t)std::max(cap, 1));

@@ -218,7 +218,7 @@ static uint32_t spread3(uint32_t v)
 // screen scans all atoms.
 int graspa::build_screen_cells(graspa_box* b)
 {
-  constexpr int kCellCapMax = 4;
+  constexpr int kCellCapMax = 16;
   hipFree(b->d_fws); hipFree(b->d_fws_type); hipFree(b->d_ctab);
   b->d_fws = nullptr; b->d_fws_type = nullptr; b->d_ctab = nullptr;
   b->use_cells = 0;

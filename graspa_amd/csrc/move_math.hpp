@@ -92,4 +92,34 @@ __device__ inline double coul_table(const double2* __restrict__ ct, const int ni
   return fma(g, d, c01.x);
 }
 
+// Cell fractions of a position as 32-bit fixed point (the screen's and the
+// cell grid's coordinates).
+__device__ inline uint4 to_fixed4(const BoxConst& bc, const double p[3])
+{
+  double s[3];
+  s[0] = bc.inv[0] * p[0] + bc.inv[3] * p[1] + bc.inv[6] * p[2];
+  s[1] = bc.inv[1] * p[0] + bc.inv[4] * p[1] + bc.inv[7] * p[2];
+  s[2] = bc.inv[2] * p[0] + bc.inv[5] * p[1] + bc.inv[8] * p[2];
+  uint32_t f[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double fr = s[d] - floor(s[d]);
+    f[d] = (uint32_t)(int64_t)(fr * 4294967296.0);
+  }
+  return make_uint4(f[0], f[1], f[2], 0u);
+}
+
+// Squared-distance threshold of the overlap screen for one (atom type, site
+// type) pair: E_LJ(r^2) = 4eps (x^2 - x) - shift with x = (sigma^2/r^2)^3
+// exceeds OverlapCriteria exactly when x > (1 + sqrt(1 + 4c))/2,
+// c = (OverlapCriteria + shift)/(4 eps) > 0, i.e. r^2 < sigma^2 / x^(1/3); the
+// reference also flags r^2 < 0.01; both only inside the VDW cutoff
+// (overlap_r2).  The FP32 value is lowered by 1e-5 relative so that a screen
+// hit is a certain overlap in both precision forms; pairs near the threshold
+// are left to the exact evaluation.
+__device__ inline float screen_threshold(const double* f, double overlap, double cut2)
+{
+  return (float)(overlap_r2(f[0], f[1], f[2], overlap, cut2) * (1.0 - 1.0e-5));
+}
+
 }  // namespace graspa

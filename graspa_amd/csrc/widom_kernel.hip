@@ -184,34 +184,6 @@ __device__ inline void orient_pos(const WidomArgs& A, uint64_t idx, int o, int a
   p[2] = fb[2] + vec[2];
 }
 
-__device__ inline uint4 to_fixed4(const BoxConst& bc, const double p[3])
-{
-  double s[3];
-  s[0] = bc.inv[0] * p[0] + bc.inv[3] * p[1] + bc.inv[6] * p[2];
-  s[1] = bc.inv[1] * p[0] + bc.inv[4] * p[1] + bc.inv[7] * p[2];
-  s[2] = bc.inv[2] * p[0] + bc.inv[5] * p[1] + bc.inv[8] * p[2];
-  uint32_t f[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double fr = s[d] - floor(s[d]);
-    f[d] = (uint32_t)(int64_t)(fr * 4294967296.0);
-  }
-  return make_uint4(f[0], f[1], f[2], 0u);
-}
-
-// Squared-distance threshold of the overlap screen for one (atom type, site
-// type) pair: E_LJ(r^2) = 4eps (x^2 - x) - shift with x = (sigma^2/r^2)^3
-// exceeds OverlapCriteria exactly when x > (1 + sqrt(1 + 4c))/2,
-// c = (OverlapCriteria + shift)/(4 eps) > 0, i.e. r^2 < sigma^2 / x^(1/3); the
-// reference also flags r^2 < 0.01; both only inside the VDW cutoff
-// (overlap_r2).  The FP32 value is lowered by 1e-5 relative so that a screen
-// hit is a certain overlap in both precision forms; pairs near the threshold
-// are left to the exact evaluation.
-__device__ inline float screen_threshold(const double* f, double overlap, double cut2)
-{
-  return (float)(overlap_r2(f[0], f[1], f[2], overlap, cut2) * (1.0 - 1.0e-5));
-}
-
 template <bool CUBIC>
 __device__ inline bool screen_hit(const BoxConst& bc, const float* __restrict__ thr, const int T, const int site,
                                   const uint4 at, const bool valid, const uint4 P)
@@ -1003,7 +975,7 @@ template <bool CUBIC>
 static int launch_screen_stage(graspa_box* b, WidomArgs& A, size_t base_lds, size_t tab_bytes, int stage)
 {
   const size_t waves = kInsWaves * sizeof(WaveLds);
-  if (b->use_cells && base_lds + align16(tab_bytes) + waves <= (size_t)b->max_lds) {
+  if (A.use_cells && base_lds + align16(tab_bytes) + waves <= (size_t)b->max_lds) {
     A.lds_wave = (int)(base_lds + align16(tab_bytes));
     const int rc = launch_screens<CUBIC, true>(b, A, base_lds + align16(tab_bytes) + waves, stage);
     if (rc != GRASPA_ERR_UNSUPPORTED) return rc;
@@ -1032,7 +1004,8 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   WidomArgs A{};
   A.bc = b->bc;
   A.fw = b->d_fws; A.fw_type = b->d_fws_type; A.fw_fixT = b->d_fw_fixT; A.n_fw = b->n_fw;
-  A.use_cells = b->use_cells;
+  // the screen's lane mapping covers 27 cells x at most 4 slots (two 64-lane groups)
+  A.use_cells = (b->use_cells && b->cell_cap <= 4) ? 1 : 0;
   for (int d = 0; d < 3; ++d) A.cell_n[d] = b->cell_n[d];
   A.cell_cap = b->cell_cap;
   A.ctab = b->d_ctab;
@@ -1082,7 +1055,7 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   const size_t scr_base = align16((size_t)M * T * sizeof(float));
   A.lds_tab = (int)scr_base;
   const size_t tab_bytes =
-      b->use_cells ? (size_t)b->cell_n[0] * b->cell_n[1] * b->cell_n[2] * b->cell_cap * sizeof(uint4) : 0;
+      A.use_cells ? (size_t)b->cell_n[0] * b->cell_n[1] * b->cell_n[2] * b->cell_cap * sizeof(uint4) : 0;
   // pair-energy kernel: FF rows | Coulomb table (fast form) | framework (optional)
   A.lds_ffl = 0;
   size_t off = align16((size_t)M * T * sizeof(double4));

@@ -140,6 +140,10 @@ _SIGS = {
     "graspa_batch_trace": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "graspa_batch_kernel_time": (C.c_int, [C.c_void_p, _f64p, _i64p]),
     "graspa_batch_sync": (C.c_int, [C.c_void_p]),
+    "graspa_batch_set_box": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "graspa_batch_set_cycles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "graspa_batch_launch": (C.c_int, [C.c_void_p, C.c_int, _i32p]),
+    "graspa_batch_progress": (C.c_int, [C.c_void_p, _i64p, _i64p, _i64p]),
 }
 
 
@@ -488,23 +492,60 @@ class Batch:
     """Device-resident GCMC of many boxes (graspa_batch_*): box i draws its
     moves from the Philox stream (seed_i, stream_i, move)."""
 
-    def __init__(self, boxes, seeds, streams=None, trace_cap=0, cycle_cap=0, params=None):
-        self.boxes = list(boxes)
-        n = len(self.boxes)
-        streams = list(range(n)) if streams is None else list(streams)
-        self.params = list(params) if params is not None else [make_mc_params(b.system, b) for b in self.boxes]
+    def __init__(self, boxes, seeds, streams=None, trace_cap=0, cycle_cap=0, params=None, n_slots=None):
+        boxes = list(boxes)
+        n = max(len(boxes), n_slots or 0)
+        streams = list(range(len(boxes))) if streams is None else list(streams)
+        self.boxes = boxes + [None] * (n - len(boxes))
+        self.params = (list(params) if params is not None else [make_mc_params(b.system, b) for b in boxes])
+        self.params += [None] * (n - len(self.params))
         arr = (BatchBox * n)()
-        for i, b in enumerate(self.boxes):
+        for i, b in enumerate(boxes):
             arr[i].box = b.h
             arr[i].params = C.pointer(self.params[i])
             arr[i].seed = int(seeds[i])
             arr[i].stream = int(streams[i])
         self._arr = arr
         self.trace_cap = trace_cap
+        self.n = n
         h = C.c_void_p()
         _check(lib().graspa_batch_create(C.cast(arr, C.c_void_p), n, trace_cap, cycle_cap, C.byref(h)),
                "graspa_batch_create")
         self.h = h
+
+    def set_box(self, slot, box, params=None, seed=0, stream=0):
+        """Puts `box` into `slot` (None empties it)."""
+        if box is None:
+            _check(lib().graspa_batch_set_box(self.h, slot, None), "batch_set_box")
+            self.boxes[slot], self.params[slot] = None, None
+            return
+        params = params if params is not None else make_mc_params(box.system, box)
+        bb = BatchBox()
+        bb.box, bb.params, bb.seed, bb.stream = box.h, C.pointer(params), int(seed), int(stream)
+        _check(lib().graspa_batch_set_box(self.h, slot, C.byref(bb)), "batch_set_box")
+        self.boxes[slot], self.params[slot] = box, params
+
+    def set_cycles(self, slot, n_cycles, production=True):
+        _check(lib().graspa_batch_set_cycles(self.h, slot, n_cycles, int(production)), "batch_set_cycles")
+
+    def launch(self, moves_per_launch=0):
+        """One launch; returns the number of boxes still short of their cycles."""
+        r = C.c_int()
+        _check(lib().graspa_batch_launch(self.h, moves_per_launch, C.byref(r)), "batch_launch")
+        return r.value
+
+    def progress(self):
+        m, c, t = (np.zeros(self.n, dtype=np.int64) for _ in range(3))
+        _check(lib().graspa_batch_progress(self.h, _p(m, _i64p), _p(c, _i64p), _p(t, _i64p)), "batch_progress")
+        self._last_progress = (m, c, t)
+        return m, c, t
+
+    def total_moves(self):
+        return int(self.progress()[0].sum())
+
+    def slot_done(self, slot):
+        m, c, t = getattr(self, "_last_progress", None) or self.progress()
+        return self.boxes[slot] is not None and c[slot] >= t[slot]
 
     def run(self, n_cycles, production=True, moves_per_launch=0):
         m = C.c_int64()

@@ -308,11 +308,20 @@ typedef struct {
   int      _pad;
 } graspa_batch_box;
 
-/* trace_cap: move records kept per box and run (0 = none); cycle_cap:
+/* n_boxes slots; boxes[i].box == NULL (or boxes == NULL) leaves slot i empty.
+ * trace_cap: move records kept per box and run (0 = none); cycle_cap:
  * per-cycle latencies kept per box (ring, 0 = none). */
 int  graspa_batch_create(const graspa_batch_box* boxes, int n_boxes, int trace_cap, int cycle_cap,
                          graspa_batch** out);
 void graspa_batch_destroy(graspa_batch* batch);
+/* Continuous batching: puts a box into a slot (NULL empties it; the slot's
+ * statistics start from zero), gives a slot n_cycles more cycles, and runs
+ * one launch of at most moves_per_launch moves per box (*n_running = boxes
+ * still short of their cycles).  A slot can be refilled while the others
+ * keep their state. */
+int  graspa_batch_set_box(graspa_batch* batch, int slot, const graspa_batch_box* box);
+int  graspa_batch_set_cycles(graspa_batch* batch, int slot, int n_cycles, int production);
+int  graspa_batch_launch(graspa_batch* batch, int moves_per_launch, int* n_running);
 /* Advances every box by n_cycles cycles (production != 0 records Widom and
  * loading statistics); launches of at most moves_per_launch moves per box
  * (<= 0: a default) until all boxes are done.  *moves = moves made. */
@@ -331,6 +340,9 @@ int  graspa_batch_cycle_times(graspa_batch* batch, int i, double* seconds, int m
 /* Move records of box i from the last graspa_batch_run (graspa_trace layout,
  * cycle and step counted from the batch's creation); returns the number. */
 int  graspa_batch_trace(graspa_batch* batch, int i, graspa_trace* out, int max_n);
+/* Moves and cycles made and cycle targets of every slot ([n_boxes] each, 0 for
+ * empty slots); any pointer may be NULL. */
+int  graspa_batch_progress(graspa_batch* batch, int64_t* moves, int64_t* cycles, int64_t* targets);
 /* Summed kernel time (HIP events) and launches of the batch since creation. */
 int  graspa_batch_kernel_time(graspa_batch* batch, double* total_ms, int64_t* launches);
 /* Copies the device state of every box back to its graspa_box. */

@@ -27,6 +27,14 @@ def oracle_lib():
     return binding.lib()
 
 
+@pytest.fixture(scope="session")
+def engine_mod():
+    """The engine's Python binding with the HIP library loaded (no fallback)."""
+    from graspa_amd import engine
+    engine.lib()
+    return engine
+
+
 def load_golden_system(name):
     from graspa_amd import deck
     return deck.load_system(os.path.join(GOLDEN, name + ".npz"))

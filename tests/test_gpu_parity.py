@@ -22,13 +22,6 @@ def _close(a, b, rtol=FP64_RTOL, atol=1e-9):
     return np.all(np.abs(a - b) <= atol + rtol * np.abs(b))
 
 
-@pytest.fixture(scope="module")
-def engine_mod():
-    from graspa_amd import engine
-    engine.lib()
-    return engine
-
-
 def _oracle(s):
     from oracle.binding import OracleSystem
     return OracleSystem(s)
