@@ -208,7 +208,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=15000)
     ap.add_argument("--quick", action="store_true", help="skip the secondary measurements")
     ap.add_argument("--batch-boxes", type=int, default=256)
-    ap.add_argument("--batch-cycles", type=int, default=10)
+    ap.add_argument("--batch-cycles", type=int, default=20)
+    ap.add_argument("--batch-warm-cycles", type=int, default=100)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run 3 plain device-to-device copies of BYTES (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -281,7 +282,8 @@ def main():
     w = r["sum_w"] / max(1, r["n"])
     line["config"]["henry_mol_per_kg_pa"] = w / (GAS_CONSTANT * s1.temperature * rho)
     if args.batch_boxes > 0:
-        line["gcmc_batched"] = run_batch_gcmc(args.batch_boxes, args.batch_cycles, engine.GRASPA_MIXED)
+        line["gcmc_batched"] = run_batch_gcmc(args.batch_boxes, args.batch_cycles, engine.GRASPA_MIXED,
+                                              warm_cycles=args.batch_warm_cycles)
     if not args.quick:
         r64 = run_widom(s1, engine.GRASPA_FP64, args.insertions, max(1, args.steps // 5), 1, timing=False)
         line["fp64_widom_insertions_per_s"] = args.insertions * max(1, args.steps // 5) / r64["seconds"]

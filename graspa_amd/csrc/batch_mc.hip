@@ -850,7 +850,7 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
     thr[i] = screen_threshold(B.ff64 + 4 * (at * T + st), B.bc.overlap, B.bc.cut_vdw2);
   }
   if (MIXED)
-    for (int i = tid; i < B.coul_n; i += kBT) dyn[nff4 + i] = B.coul[i];
+    for (int i = tid; i < B.coul_n; i += kBT) dyn[nff4 + nthr + i] = B.coul[i];
   __syncthreads();
   const int nd = kBDrawTrial + 3 * B.K + 3 * B.O;
   for (int mv = 0; mv < max_moves; ++mv) {
