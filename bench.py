@@ -186,6 +186,48 @@ def run_batch_gcmc(n_boxes, cycles, precision, warm_cycles=1, capacity=2048):
     return out
 
 
+def run_batch_single(system, cycles, precision, warm_cycles=0, capacity=1024, seed=11):
+    """One box through the device-resident driver (batch of one)."""
+    import numpy as np
+    from graspa_amd import engine
+    bx = engine.Box(system, precision=precision, capacity=capacity)
+    bx.init_ewald()
+    bt = engine.Batch([bx], [seed], [0], cycle_cap=max(cycles, 1))
+    if warm_cycles:
+        bt.run(warm_cycles, production=False)
+    t0 = time.perf_counter()
+    moves = bt.run(cycles, production=True)
+    t1 = time.perf_counter()
+    lat = bt.cycle_times(0, cycles)
+    st = bt.stats(0)
+    out = dict(moves=int(moves), seconds=t1 - t0, moves_per_s=moves / (t1 - t0),
+               cycle_p50_ms=float(np.percentile(lat, 50) * 1e3), cycle_p99_ms=float(np.percentile(lat, 99) * 1e3),
+               n_molecules=[int(x) for x in st.n_mol])
+    bt.close()
+    bx.close()
+    return out
+
+
+def run_served(n_jobs, rate, slots, precision, cycles_lo=50, cycles_hi=5000, seed=4000):
+    """Served stream: Poisson arrivals of config-3 style jobs, continuous
+    batching; per-cycle latency over every job's cycles and aggregate moves/s."""
+    import numpy as np
+    from graspa_amd import serve
+    jobs = serve.poisson_jobs(n_jobs, rate, seed=seed, cycles_lo=cycles_lo, cycles_hi=cycles_hi)
+    srv = serve.Server(n_slots=slots, precision=precision, moves_per_launch=256, cycle_cap=cycles_hi)
+    done = srv.run(jobs)
+    srv.close()
+    lat = np.concatenate([j.cycle_seconds for j in done])
+    moves = sum(j.stats.moves for j in done)
+    wait = np.array([j.admitted - j.arrival for j in done])
+    return dict(jobs=n_jobs, arrival_rate_per_s=rate, slots=slots, cycles_range=[cycles_lo, cycles_hi],
+                total_cycles=int(sum(j.cycles for j in done)), moves=int(moves), seconds=srv.elapsed,
+                moves_per_s=moves / srv.elapsed, launches=srv.launches,
+                cycle_p50_ms=float(np.percentile(lat, 50) * 1e3), cycle_p99_ms=float(np.percentile(lat, 99) * 1e3),
+                admission_wait_p50_ms=float(np.percentile(wait, 50) * 1e3),
+                admission_wait_p99_ms=float(np.percentile(wait, 99) * 1e3))
+
+
 def cpu_baseline(system, n_insert):
     from oracle import binding as ob
     o = ob.OracleSystem(system)
@@ -299,6 +341,11 @@ def main():
         s0 = synth.config0(cycles=args.gcmc_cycles)
         g = run_gcmc(s0, args.gcmc_cycles)
         line["gcmc_fp64_single_box"] = g
+        line["gcmc_device_driver_config0_fp64"] = run_batch_single(synth.config0(cycles=args.gcmc_cycles),
+                                                                   args.gcmc_cycles, engine.GRASPA_FP64)
+        line["gcmc_config2_butane_mixed"] = run_batch_single(synth.config2(), 200, engine.GRASPA_MIXED,
+                                                             warm_cycles=100)
+        line["served_stream"] = run_served(48, 20.0, 64, engine.GRASPA_MIXED)
         if "gcmc_batched" in line:
             line["gcmc_batched"]["over_single_box_fp64"] = line["gcmc_batched"]["moves_per_s"] / g["moves_per_s"]
         line["cpu_baseline"] = cpu_baseline(s1, args.cpu_sample)

@@ -1,5 +1,5 @@
 // batch_mc.hip -- device-resident GCMC of many independent boxes in one
-// persistent launch on gfx950: one workgroup (16 waves) owns one box and runs
+// persistent launch on gfx950: one workgroup (8 waves) owns one box and runs
 // its Monte Carlo steps with no host round trip.
 //
 // One step, as RunMoves (axpy.cu:102-298) does it:
@@ -21,7 +21,7 @@
 //
 // Work inside a step: the pair energies of up to 64 probe sites (trial sites,
 // or the old and new sites of a moved molecule) against every framework and
-// guest atom run over all 1024 lanes (lanes over atoms, up to 8 sites per pass
+// guest atom run over all 512 lanes (lanes over atoms, up to 8 sites per pass
 // held in registers, FP64 sums reduced by a transposing butterfly across the
 // wave, then in wave order across the workgroup, so every sum is
 // deterministic); the Ewald change runs one lane per active k-vector with
@@ -46,12 +46,12 @@
 
 namespace graspa {
 
-constexpr int kBT = 1024;              // threads of a box workgroup
+constexpr int kBT = 512;               // threads of a box workgroup
 constexpr int kBW = kBT / 64;          // waves
 constexpr int kBMaxTrial = 32;         // K and O
 constexpr int kBMaxMol = 8;            // atoms per molecule
 constexpr int kBMaxSites = 64;         // probe sites of one pair pass
-constexpr int kBGroup = 4;             // sites per pass over the atoms
+constexpr int kBGroup = 8;             // sites per pass over the atoms
 constexpr int kBDrawTrial = 16;        // first trial draw
 constexpr int kBMaxDraws = kBDrawTrial + 6 * kBMaxTrial;
 
@@ -64,6 +64,8 @@ struct BSpecies {
   double mol_fraction, fugacity, ideal_rosen, excl;   // excl = ExclusionIntra + ExclusionAtom
   int base, molsize, capacity, charged;
   int pseudo[16];
+  double q[kBMaxMol];               // charges and types of the species' atoms (rigid: the same in
+  int type[kBMaxMol];               // every molecule)
 };
 
 // Read-only description of one box.
@@ -86,8 +88,8 @@ struct BBox {
   const double* ktemp_act;
   int n_active;
   int has_tail;
-  const double2* sk_fw;
-  double2* sk[2];                   // adsorbate S(k) double buffer
+  const double2* skfw;              // framework S(k) of the active k-vectors
+  double2* skc[2];                  // adsorbate S(k) of the active k-vectors, double buffer
   const uint4* ctab;                // overlap-screen cells: cap slots of {fx, fy, fz, type}
   int cell_n[3];
   int cell_cap;                     // 0: no cell grid
@@ -142,6 +144,7 @@ struct BMove {
   double part[2][kBW][2 * kBGroup];
   double ud[kBMaxDraws];
   double lw[kBMaxTrial], shf[kBMaxTrial], ew[kBMaxTrial], t4[kBMaxTrial][4];
+  double tmpl[kMaxComps][kBMaxMol][3];   // slot 0 of each species, read at the start of a move
   double molp[kBMaxMol][3], molq[kBMaxMol];
   int molt[kBMaxMol];
   double ewp[kBW][2];
@@ -226,29 +229,48 @@ __device__ inline void pair_site(const BoxConst& bc, const double* __restrict__ 
 // Energies of the sites L.idx[0 .. n_sites) into L.e[site] and their overlap
 // flags into L.ovl (bit site), against the framework (host-guest) and every
 // guest atom except the index range [ex_lo, ex_hi) (the molecule itself).
+// The sites form G groups of at most kBGroup; the 8 waves are dealt out to
+// the groups (kBW / G waves each), so all groups run in one pass over the
+// atoms with every lane busy, and one reduction finishes the pass.
 template <bool MIXED, bool CUBIC>
 __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double* __restrict__ ffl,
-                          const double2* __restrict__ ct, const int nint, const int n_sites, const int ex_lo,
-                          const int ex_hi)
+                          const double2* __restrict__ ct, const int nint, const double* __restrict__ fwl,
+                          const int n_sites, const int ex_lo, const int ex_hi)
 {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const BoxConst& bc = B.bc;
   const int T = bc.ntypes;
-  const int ngrp = (n_sites + kBGroup - 1) / kBGroup;
-  const int per = (n_sites + ngrp - 1) / ngrp;
+  const int ngrp = (n_sites + kBGroup - 1) / kBGroup;    // <= kBW
+  const int per = (n_sites + ngrp - 1) / ngrp;           // <= kBGroup
+  const int wpg = kBW / ngrp;                            // waves per group
+  const int grp = wave / wpg, r = wave - grp * wpg;
+  const bool active = grp < ngrp;
+  const int g0 = grp * per;
+  const int cnt = active ? max(0, min(per, n_sites - g0)) : 0;
+  const int first = r * 64 + lane, stride = wpg * 64;
   const int nf = B.n_fw, cap = B.guest_cap;
-  unsigned long long ovl = 0ull;
-  for (int g0 = 0; g0 < n_sites; g0 += per) {
-    const int cnt = min(per, n_sites - g0);
-    uint32_t ovb = 0;
+  uint32_t ovb = 0;
 #pragma unroll 1
-    for (int part = 0; part < 2; ++part) {
-      double av[kBGroup], ar[kBGroup];
+  for (int part = 0; part < 2; ++part) {
+    double av[kBGroup], ar[kBGroup];
 #pragma unroll
-      for (int j = 0; j < kBGroup; ++j) { av[j] = 0.0; ar[j] = 0.0; }
-      if (part == 0) {
+    for (int j = 0; j < kBGroup; ++j) { av[j] = 0.0; ar[j] = 0.0; }
+    if (cnt > 0) {
+      if (part == 0 && fwl) {
+        // framework staged in LDS for the launch: SoA x, y, z, q, then types
+        const int* __restrict__ ftl = reinterpret_cast<const int*>(fwl + 4 * nf);
 #pragma unroll 1
-        for (int i = tid; i < nf; i += kBT) {
+        for (int i = first; i < nf; i += stride) {
+          const double ax = fwl[i], ay = fwl[nf + i], az = fwl[2 * nf + i], aq = fwl[3 * nf + i];
+          const int at = ftl[i];
+#pragma unroll
+          for (int j = 0; j < kBGroup; ++j)
+            if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j],
+                                                 ar[j], ovb, 1u << j);
+        }
+      } else if (part == 0) {
+#pragma unroll 1
+        for (int i = first; i < nf; i += stride) {
           const double ax = B.fw[i], ay = B.fw[nf + i], az = B.fw[2 * nf + i], aq = B.fw[3 * nf + i];
           const int at = B.fw_type[i];
 #pragma unroll
@@ -260,7 +282,7 @@ __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double
         for (int c = 1; c < B.n_comp; ++c) {
           const int lo = B.sp[c].base, hi = lo + S.n_mol[c] * B.sp[c].molsize;
 #pragma unroll 1
-          for (int i = lo + tid; i < hi; i += kBT) {
+          for (int i = lo + first; i < hi; i += stride) {
             if (i >= ex_lo && i < ex_hi) continue;
             const double ax = B.g[i], ay = B.g[cap + i], az = B.g[2 * cap + i], aq = B.g[3 * cap + i];
             const int at = B.g_type[i];
@@ -271,22 +293,23 @@ __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double
           }
         }
       }
-      wave_sum_group(av, ar, L.part[part][wave]);
     }
-#pragma unroll
-    for (int j = 0; j < kBGroup; ++j)
-      if (j < cnt && ((ovb >> j) & 1u)) ovl |= 1ull << L.idx[g0 + j];
-    __syncthreads();
-    if (tid < 4 * cnt) {
-      const int j = tid >> 2, t = tid & 3, part = t >> 1, k = t & 1;
-      double s = 0.0;
-      for (int w = 0; w < kBW; ++w) s += L.part[part][w][2 * j + k];
-      L.e[L.idx[g0 + j]][t] = s;
-    }
-    __syncthreads();
+    wave_sum_group(av, ar, L.part[part][wave]);
   }
+  unsigned long long ovl = 0ull;
+#pragma unroll
+  for (int j = 0; j < kBGroup; ++j)
+    if (j < cnt && ((ovb >> j) & 1u)) ovl |= 1ull << L.idx[g0 + j];
   ovl = wave_or64(ovl);
   if (lane == 0 && ovl) atomicOr(&L.ovl, ovl);
+  __syncthreads();
+  if (tid < 4 * n_sites) {
+    const int slot = tid >> 2, t = tid & 3, part = t >> 1, k = t & 1;
+    const int g = slot / per, j = slot - g * per;
+    double s = 0.0;
+    for (int w = g * wpg; w < (g + 1) * wpg; ++w) s += L.part[part][w][2 * j + k];
+    L.e[L.idx[slot]][t] = s;
+  }
   __syncthreads();
 }
 
@@ -350,12 +373,14 @@ __device__ void ewald_pass(const BBox& B, const BState& S, BMove& L, double2* __
     axis_table(B.bc, L.site[site].x, L.site[site].y, L.site[site].z, axis, eik + (site * 3 + axis) * ks);
   }
   __syncthreads();
-  const double2* __restrict__ cur = B.sk[S.sk_cur];
-  double2* __restrict__ stg = B.sk[S.sk_cur ^ 1];
+  const double2* __restrict__ cur = B.skc[S.sk_cur];
+  double2* __restrict__ stg = B.skc[S.sk_cur ^ 1];
   double a0 = 0.0, a1 = 0.0;
   for (int j = tid; j < B.n_active; j += kBT) {
     const int4 kv = B.kact[j];
     const double t = B.ktemp_act[j];
+    const double2 ov = cur[j];
+    const double2 cv = B.skfw[j];
     double2 co = make_double2(0.0, 0.0), cn = make_double2(0.0, 0.0);
     for (int i = 0; i < ns; ++i) {
       const double2* e = eik + i * 3 * ks;
@@ -368,12 +393,10 @@ __device__ void ewald_pass(const BBox& B, const BState& S, BMove& L, double2* __
       if (i < n_old) { co.x += q * p.x; co.y += q * p.y; }
       else { cn.x += q * p.x; cn.y += q * p.y; }
     }
-    const double2 ov = cur[kv.w];
     const double2 nv = make_double2(ov.x + cn.x - co.x, ov.y + cn.y - co.y);
     a0 += t * (nv.x * nv.x + nv.y * nv.y) - t * (ov.x * ov.x + ov.y * ov.y);
-    const double2 cv = B.sk_fw[kv.w];
     a1 += t * (cv.x * (cn.x - co.x) + cv.y * (cn.y - co.y));
-    stg[kv.w] = nv;
+    stg[j] = nv;
   }
   a0 = wave_sum(a0);
   a1 = wave_sum(a1);
@@ -401,61 +424,59 @@ __device__ int select_wave0(const BBox& B, BMove& L, const int n, const uint32_t
 {
   const int lane = threadIdx.x & 63;
   const bool alive = lane < n && !((ovm >> lane) & 1u);
-  double lw = -INFINITY;
+  double lw = -INFINITY, er = 0.0;
+  if (lane < n) er = L.t4[lane][1] + L.t4[lane][3];
   if (alive) {
     double tot = L.t4[lane][0] + L.t4[lane][2];
-    if (B.bias) tot += L.t4[lane][1] + L.t4[lane][3];
+    if (B.bias) tot += er;
     lw = -B.beta * tot;
   }
   const double mx = wave_max(lw);
-  if (lane < n) {
-    L.shf[lane] = alive ? exp(lw - mx) : -1.0;
-    L.ew[lane] = alive ? exp(lw) : -1.0;
+  const double shf = alive ? exp(lw - mx) : -1.0;   // shifted factors; -1 marks an overlap
+  const double ew = alive ? exp(lw) : 0.0;
+  // ordered sums over the survivors (uniform values through lane reads)
+  int nk = 0, first = -1;
+  double ssum = 0.0, rb = 0.0;
+  for (int t = 0; t < n; ++t) {
+    const double sv = readlane_d(shf, t);
+    if (sv >= 0.0) {
+      if (first < 0) first = t;
+      ssum += sv;
+      rb += readlane_d(ew, t);
+      ++nk;
+    }
   }
-  wave_sync();
   int sel = -1;
-  if (lane == 0) {
-    int nk = 0, first = -1;
-    double ssum = 0.0, rb = 0.0;
-    for (int t = 0; t < n; ++t)
-      if (L.shf[t] >= 0.0) {
-        if (first < 0) first = t;
-        ssum += L.shf[t];
-        rb += L.ew[t];
-        ++nk;
-      }
-    bool fail = false;
-    if (!del) {
-      if (nk == 0) {
-        fail = true;
-      } else {
-        const double ws = u * ssum;
-        int t = first;
-        double cumw = L.shf[t];
-        int left = nk - 1;
-        while (cumw < ws && left > 0) {
-          ++t;
-          while (L.shf[t] < 0.0) ++t;
-          cumw += L.shf[t];
-          --left;
-        }
-        sel = t;
-        if (rb < 1e-150) fail = true;
-      }
+  bool fail = false;
+  if (!del) {
+    if (nk == 0) {
+      fail = true;
     } else {
-      sel = first;
-      if (chain && nk == 0) fail = true;
+      const double ws = u * ssum;
+      int t = first;
+      double cumw = readlane_d(shf, t);
+      int left = nk - 1;
+      while (cumw < ws && left > 0) {
+        ++t;
+        while (readlane_d(shf, t) < 0.0) ++t;
+        cumw += readlane_d(shf, t);
+        --left;
+      }
+      sel = t;
+      if (rb < 1e-150) fail = true;
     }
-    if (!fail) {
-      double w = rb / (double)n;
-      if (sel >= 0 && !B.bias) w *= exp(-B.beta * (L.t4[sel][1] + L.t4[sel][3]));
-      w *= *W;
-      if (w <= 1e-150) fail = true;
-      *W = w;
-    }
-    if (fail) sel = -1;
+  } else {
+    sel = first;
+    if (chain && nk == 0) fail = true;
   }
-  return sel;
+  if (!fail) {
+    double w = rb / (double)n;
+    if (sel >= 0 && !B.bias) w *= exp(-B.beta * readlane_d(er, sel));
+    w *= *W;
+    if (w <= 1e-150) fail = true;
+    *W = w;
+  }
+  return fail ? -1 : sel;
 }
 
 // TailCorrectionDifference (TailCorrection_Energy_Functions.h:36-82)
@@ -491,15 +512,16 @@ enum { kHGV = 1, kGGV = 2, kHGR = 4, kGGR = 5, kHGE = 7, kGGE = 8, kTail = 9 };
 // ---------------------------------------------------------------------------
 template <bool MIXED, bool CUBIC>
 __device__ void single_body(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr, const double2* ct,
-                            int nint, double2* eik)
+                            int nint, const double* fwl, double2* eik)
 {
   const int tid = threadIdx.x;
   const int M = L.M, start = L.start, cap = B.guest_cap, comp = L.comp;
   if (tid < M) {
     const int j = start + tid;
     BSite s;
-    s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j]; s.q = B.g[3 * cap + j];
-    s.type = B.g_type[j]; s._p = 0;
+    s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j];
+    s.q = B.sp[comp].q[tid];
+    s.type = B.sp[comp].type[tid]; s._p = 0;
     L.site[tid] = s;
   }
   __syncthreads();
@@ -532,7 +554,7 @@ __device__ void single_body(const BBox& B, BState& S, BMove& L, const double* ff
   if (tid < 2 * M) L.idx[tid] = (unsigned char)tid;
   if (tid == 0) L.ovl = 0ull;
   __syncthreads();
-  pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, 2 * M, start, start + M);
+  pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, 2 * M, start, start + M);
   if (tid == 0) {
     const unsigned long long newmask = ((1ull << M) - 1ull) << M;
     L.overlap = (L.ovl & newmask) ? 1 : 0;
@@ -592,14 +614,13 @@ __device__ inline void list_survivors(BMove& L, const int n, const int g)
 
 template <bool MIXED, bool CUBIC>
 __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr, const double2* ct,
-                     int nint, double2* eik)
+                     int nint, const double* fwl, double2* eik)
 {
   const int tid = threadIdx.x;
   const int M = L.M, comp = L.comp, cap = B.guest_cap;
   const int K = B.K, O = B.O, chain = M - 1;
   const bool del = (L.type == kMvDel);
   const BSpecies& sp = B.sp[comp];
-  const int tstart = del ? L.start : sp.base;   // types and charges of the molecule's atoms
   const int ex_lo = del ? L.start : 0, ex_hi = del ? L.start + M : 0;
   // first-bead trials (get_random_trial_position, mc_widom.h:122-213)
   if (tid < K) {
@@ -611,8 +632,8 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
       s.y = B.bc.cell[4] * L.ud[kBDrawTrial + 3 * tid + 1];
       s.z = B.bc.cell[8] * L.ud[kBDrawTrial + 3 * tid + 2];
     }
-    s.q = B.g[3 * cap + tstart];
-    s.type = B.g_type[tstart];
+    s.q = sp.q[0];
+    s.type = sp.type[0];
     s._p = 0;
     L.site[tid] = s;
   }
@@ -621,7 +642,7 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
   else if (tid == 0) L.smask = 0ull;
   if (tid == 0) list_survivors(L, K, 1);
   __syncthreads();
-  if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, L.n_eval, ex_lo, ex_hi);
+  if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, L.n_eval, ex_lo, ex_hi);
   if (tid < K) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) L.t4[tid][t] = L.e[tid][t];
@@ -655,14 +676,15 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
         const int j = L.start + 1 + a;
         s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j];
       } else {
-        const int j0 = sp.base, j = sp.base + 1 + a;
-        double v[3] = {B.g[j] - B.g[j0], B.g[cap + j] - B.g[cap + j0], B.g[2 * cap + j] - B.g[2 * cap + j0]};
+        const double* t0 = L.tmpl[comp][0];
+        const double* t1 = L.tmpl[comp][1 + a];
+        double v[3] = {t1[0] - t0[0], t1[1] - t0[1], t1[2] - t0[2]};
         const int d = kBDrawTrial + 3 * K + 3 * o;
         quat_rotate(v, L.ud[d], L.ud[d + 1], L.ud[d + 2]);
         s.x = L.molp[0][0] + v[0]; s.y = L.molp[0][1] + v[1]; s.z = L.molp[0][2] + v[2];
       }
-      s.q = B.g[3 * cap + tstart + 1 + a];
-      s.type = B.g_type[tstart + 1 + a];
+      s.q = sp.q[1 + a];
+      s.type = sp.type[1 + a];
       s._p = 0;
       L.site[t] = s;
     }
@@ -671,7 +693,7 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
     else if (tid == 0) L.smask = 0ull;
     if (tid == 0) list_survivors(L, O * chain, chain);
     __syncthreads();
-    if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, L.n_eval, ex_lo, ex_hi);
+    if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, L.n_eval, ex_lo, ex_hi);
     if (tid < O) {
       double e4[4] = {0.0, 0.0, 0.0, 0.0};
       for (int a = 0; a < chain; ++a)
@@ -711,7 +733,7 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
       BSite s;
       if (del) {
         const int j = L.start + tid;
-        s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j]; s.q = B.g[3 * cap + j]; s.type = B.g_type[j];
+        s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j]; s.q = sp.q[tid]; s.type = sp.type[tid];
       } else {
         s.x = L.molp[tid][0]; s.y = L.molp[tid][1]; s.z = L.molp[tid][2]; s.q = L.molq[tid]; s.type = L.molt[tid];
       }
@@ -817,7 +839,7 @@ __device__ inline void update_max(double* v, int64_t& acc, int64_t& tot, double 
 
 template <bool MIXED, bool CUBIC>
 __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ boxes, BState* __restrict__ states,
-                                                       const int max_moves)
+                                                       const int max_moves, const int dyn_bytes)
 {
   extern __shared__ double4 dyn4[];
   __shared__ BState S;
@@ -844,6 +866,16 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
   const int nct = MIXED ? ((B.coul_n + 3) & ~3) : 0;
   double2* eik = reinterpret_cast<double2*>(dyn + nff4 + nthr + nct);
   const int nint = B.coul_n / kCoulTabStride;
+  // the framework goes to LDS for the launch when it fits after the tables
+  const int fw_off = nff4 + nthr + nct + 2 * (2 * kBMaxMol * 3 * B.kstride);
+  const int nf = B.n_fw;
+  const bool fw_in_lds = (size_t)fw_off * 8 + (size_t)nf * 36 <= (size_t)dyn_bytes;
+  double* fwl = fw_in_lds ? dyn + fw_off : nullptr;
+  if (fw_in_lds) {
+    for (int i = tid; i < 4 * nf; i += kBT) fwl[i] = B.fw[i];
+    int* ftl = reinterpret_cast<int*>(fwl + 4 * nf);
+    for (int i = tid; i < nf; i += kBT) ftl[i] = B.fw_type[i];
+  }
   for (int i = tid; i < nff; i += kBT) ffl[i] = B.ff64[i];
   for (int i = tid; i < T * T; i += kBT) {
     const int st = i / T, at = i - st * T;
@@ -861,6 +893,11 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
       philox4x32_10(c, (uint32_t)B.seed, (uint32_t)(B.seed >> 32));
       L.ud[2 * tid] = ((double)((((uint64_t)c[0] << 32) | c[1]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
       L.ud[2 * tid + 1] = ((double)((((uint64_t)c[2] << 32) | c[3]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    }
+    if (tid >= 256 && tid < 256 + (kMaxComps - 1) * kBMaxMol * 3) {
+      // slot 0 of every species (the orientation template, mc_widom.h:256)
+      const int t = tid - 256, c = 1 + t / (kBMaxMol * 3), a = (t / 3) % kBMaxMol, d = t % 3;
+      if (c < B.n_comp && a < B.sp[c].molsize) L.tmpl[c][a][d] = B.g[d * B.guest_cap + B.sp[c].base + a];
     }
     if (tid == 0 && S.step == 0) {
       // Determine_Number_Of_Steps (axpy.cu:580-591)
@@ -908,8 +945,8 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
     }
     __syncthreads();
     const int type = L.type;
-    if (type == kMvTrans || type == kMvRot) single_body<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, eik);
-    else if (type != kMvNone) cbmc<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, eik);
+    if (type == kMvTrans || type == kMvRot) single_body<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
+    else if (type != kMvNone) cbmc<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
     __syncthreads();
     if (tid == 0) {
       const int comp = L.comp;
@@ -972,6 +1009,7 @@ struct graspa_batch {
   std::vector<BState> hs;
   BBox* d_box = nullptr;
   BState* d_state = nullptr;
+  std::vector<double2*> skc;        // per slot: framework S(k), adsorbate S(k) x 2 (active k-vectors)
   double* d_tail = nullptr;         // kMaxTypes^2 per slot
   int* d_use_tail = nullptr;
   graspa_trace* d_trace = nullptr;
@@ -1013,6 +1051,7 @@ void free_batch(graspa_batch* bt)
   if (bt->stream) hipStreamSynchronize(bt->stream);
   hipFree(bt->d_box); hipFree(bt->d_state); hipFree(bt->d_tail); hipFree(bt->d_use_tail);
   hipFree(bt->d_trace); hipFree(bt->d_cyc);
+  for (double2* p : bt->skc) hipFree(p);
   if (bt->stream) hipStreamDestroy(bt->stream);
   delete bt;
 }
@@ -1061,7 +1100,30 @@ int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
   h.coul = b->d_coul; h.coul_n = b->coul_n;
   h.kstride = 1 + std::max(b->bc.kmax[0], std::max(b->bc.kmax[1], b->bc.kmax[2]));
   h.kact = b->d_kact; h.ktemp_act = b->d_ktemp_act; h.n_active = b->bc.use_charges ? b->n_active : 0;
-  h.sk_fw = b->d_sk_fw; h.sk[0] = b->d_sk_ads; h.sk[1] = b->d_sk_stage;
+  hipFree(bt->skc[i]);
+  bt->skc[i] = nullptr;
+  if (h.n_active > 0) {
+    // structure factors of the active k-vectors in the batch's own compact
+    // arrays (one load per k-vector, no index gather); written back by sync
+    const int na = h.n_active, nv = b->nvec;
+    std::vector<int4> ka(na);
+    std::vector<double2> fw(nv), ads(nv), c(3 * (size_t)na);
+    if (hip_check(hipMemcpy(ka.data(), b->d_kact, na * sizeof(int4), hipMemcpyDeviceToHost), "k copy") ||
+        hip_check(hipMemcpy(fw.data(), b->d_sk_fw, nv * sizeof(double2), hipMemcpyDeviceToHost), "sk copy") ||
+        hip_check(hipMemcpy(ads.data(), b->d_sk_ads, nv * sizeof(double2), hipMemcpyDeviceToHost), "sk copy"))
+      return GRASPA_ERR_HIP;
+    for (int j = 0; j < na; ++j) {
+      c[j] = fw[ka[j].w];
+      c[na + j] = ads[ka[j].w];
+      c[2 * (size_t)na + j] = ads[ka[j].w];
+    }
+    if (hip_check(hipMalloc(&bt->skc[i], 3 * (size_t)na * sizeof(double2)), "hipMalloc batch")) return GRASPA_ERR_HIP;
+    if (hip_check(hipMemcpy(bt->skc[i], c.data(), 3 * (size_t)na * sizeof(double2), hipMemcpyHostToDevice), "sk copy"))
+      return GRASPA_ERR_HIP;
+    h.skfw = bt->skc[i];
+    h.skc[0] = bt->skc[i] + na;
+    h.skc[1] = bt->skc[i] + 2 * (size_t)na;
+  }
   if (b->use_cells) {
     h.ctab = b->d_ctab;
     h.cell_cap = b->cell_cap;
@@ -1082,6 +1144,10 @@ int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
     double qabs = 0.0;
     for (int a = 0; a < cc.molsize; ++a) qabs += std::fabs(cc.q[a]);
     d.charged = (b->bc.use_charges && qabs > 1e-6) ? 1 : 0;   // hasPartialCharge (read_data.cpp:2160-2164)
+    for (int a = 0; a < cc.molsize; ++a) {
+      d.q[a] = cc.q[a];
+      d.type[a] = cc.type[a];
+    }
     for (int t = 0; t < 16; ++t) d.pseudo[t] = (int)sp.pseudo_per_mol[t];
     if (sp.p_swap > 1e-10) h.valid[h.n_valid++] = c;
   }
@@ -1138,12 +1204,20 @@ template <bool MIXED, bool CUBIC>
 int launch_batch(graspa_batch* bt, int max_moves)
 {
   auto kern = batch_mc_kernel<MIXED, CUBIC>;
-  (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bt->lds);
+  // all the LDS a workgroup can have (one box workgroup fills a CU anyway):
+  // what the tables leave holds the framework when it fits
+  hipFuncAttributes fa;
+  size_t stat = 32768;
+  if (hipFuncGetAttributes(&fa, (const void*)kern) == hipSuccess) stat = fa.sharedSizeBytes;
+  const size_t avail = bt->max_lds > (int)stat ? (((size_t)bt->max_lds - stat) & ~(size_t)15) : 0;
+  if (avail < bt->lds) return fail(GRASPA_ERR_UNSUPPORTED, "batch workspace exceeds LDS");
+  const size_t lds = avail;
+  (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   hipEventRecord(e0, bt->stream);
-  hipLaunchKernelGGL(kern, dim3(bt->n), dim3(kBT), bt->lds, bt->stream, bt->d_box, bt->d_state, max_moves);
+  hipLaunchKernelGGL(kern, dim3(bt->n), dim3(kBT), lds, bt->stream, bt->d_box, bt->d_state, max_moves, (int)lds);
   hipError_t le = hipGetLastError();
   hipEventRecord(e1, bt->stream);
   hipError_t se = hipEventSynchronize(e1);
@@ -1178,6 +1252,7 @@ int graspa_batch_create(const graspa_batch_box* boxes, int n_boxes, int trace_ca
   bt->trace_cap = trace_cap;
   bt->cyc_cap = cycle_cap;
   bt->boxes.assign(n_boxes, nullptr);
+  bt->skc.assign(n_boxes, nullptr);
   bt->hb.assign(n_boxes, BBox{});
   bt->hs.assign(n_boxes, BState{});
   for (int i = 0; i < n_boxes; ++i) {
@@ -1233,6 +1308,8 @@ int graspa_batch_set_box(graspa_batch* bt, int slot, const graspa_batch_box* box
     bt->boxes[slot] = nullptr;
     bt->hb[slot] = BBox{};
     bt->hs[slot] = BState{};
+    hipFree(bt->skc[slot]);
+    bt->skc[slot] = nullptr;
   } else {
     const bool first = std::none_of(bt->boxes.begin(), bt->boxes.end(), [](graspa_box* x) { return x != nullptr; });
     if (first) {
@@ -1405,7 +1482,6 @@ int graspa_batch_sync(graspa_batch* bt)
   if (!bt) return fail(GRASPA_ERR_ARG, "null batch");
   hipSetDevice(bt->device);
   if (int rc = download_states(bt)) return rc;
-  bool flipped = false;
   for (int i = 0; i < bt->n; ++i) {
     graspa_box* b = bt->boxes[i];
     if (!b) continue;
@@ -1437,17 +1513,24 @@ int graspa_batch_sync(graspa_batch* bt)
       // refreshes the fixed-point copy the screening kernels read
       if (int rc = upload_guest_atoms(b, (int)c, 0, na)) return rc;
     }
-    if (s.sk_cur) {
-      std::swap(b->d_sk_ads, b->d_sk_stage);
-      bt->hb[i].sk[0] = b->d_sk_ads;
-      bt->hb[i].sk[1] = b->d_sk_stage;
-      s.sk_cur = 0;
-      flipped = true;
+    const BBox& h = bt->hb[i];
+    if (h.n_active > 0) {
+      // the current adsorbate S(k) of the active k-vectors back into the box's arrays
+      const int na = h.n_active, nv = b->nvec;
+      std::vector<int4> ka(na);
+      std::vector<double2> ads(nv), c(na);
+      if (hip_check(hipMemcpy(ka.data(), b->d_kact, na * sizeof(int4), hipMemcpyDeviceToHost), "sync") ||
+          hip_check(hipMemcpy(ads.data(), b->d_sk_ads, nv * sizeof(double2), hipMemcpyDeviceToHost), "sync") ||
+          hip_check(hipMemcpy(c.data(), h.skc[s.sk_cur], na * sizeof(double2), hipMemcpyDeviceToHost), "sync"))
+        return GRASPA_ERR_HIP;
+      for (int j = 0; j < na; ++j) ads[ka[j].w] = c[j];
+      if (hip_check(hipMemcpy(b->d_sk_ads, ads.data(), nv * sizeof(double2), hipMemcpyHostToDevice), "sync") ||
+          hip_check(hipMemcpy(b->d_sk_stage, ads.data(), nv * sizeof(double2), hipMemcpyHostToDevice), "sync"))
+        return GRASPA_ERR_HIP;
     }
     b->sk_staged = false;
     if (hip_check(hipStreamSynchronize(b->stream), "sync")) return GRASPA_ERR_HIP;
   }
-  if (flipped) return upload_batch(bt);
   return GRASPA_OK;
 }
 
