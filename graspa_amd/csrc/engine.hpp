@@ -146,7 +146,8 @@ struct graspa_box {
   void* d_wlist1 = nullptr;
   void* d_wlist2 = nullptr;
   void* d_wstate = nullptr;
-  void* d_wcounts = nullptr;    // sub-list lengths of the work lists
+  void* d_wcounts = nullptr;    // sub-list lengths of work list 1, live count, length of list 2
+  void* d_wlive = nullptr;      // insertions with a first-bead trial left after the screen
   size_t wp_n = 0, wp_e1 = 0, wp_e2 = 0;
   graspa_widom_result* d_wres = nullptr;
   size_t wres_cap = 0;

@@ -511,7 +511,7 @@ void graspa_box_destroy(graspa_box* b)
   hipFree(b->d_wacc); hipFree(b->d_wres);
   hipFree(b->d_wrec_w); hipFree(b->d_wrec_e); hipFree(b->d_wrec_f); hipFree(b->d_wpart);
   hipFree(b->d_wmask1); hipFree(b->d_wmask2); hipFree(b->d_we1); hipFree(b->d_we2);
-  hipFree(b->d_wlist1); hipFree(b->d_wlist2); hipFree(b->d_wstate); hipFree(b->d_wcounts);
+  hipFree(b->d_wlist1); hipFree(b->d_wlist2); hipFree(b->d_wstate); hipFree(b->d_wcounts); hipFree(b->d_wlive);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
 }

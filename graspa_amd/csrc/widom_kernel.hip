@@ -4,17 +4,20 @@
 // A batch runs as a short pipeline of small kernels on the box's stream,
 // linked by compact work lists in HBM, so that each kernel keeps few registers
 // live and runs at high occupancy:
-//   A  (one wave per insertion)  K first-bead trial positions from the
-//      insertion's Philox stream; overlap screen; the surviving trial sites
-//      are appended to list 1;
+//   A  (one lane per first-bead trial)  the trial's position from the
+//      insertion's Philox stream and its overlap screen; an insertion with no
+//      trial left is finished here (W = 0, most insertions in a dense
+//      framework), the others go to a list of live insertions and their
+//      surviving trial sites to list 1;
 //   B  (one wave per trial site) full pair energies (LJ + real-space Coulomb,
 //      FP64 accumulation) of every site in the list, with the force-field rows,
 //      the real-space table and the framework staged in LDS; exact overlap
 //      flags;
-//   C  (one wave per insertion)  first-bead selection; O orientations of the
-//      template about the selected bead; screen; survivors to list 2;
+//   C  (one wave per live insertion)  first-bead selection; O orientations
+//      of the template about the selected bead, one lane per orientation
+//      site, and their screen; survivors to list 2;
 //   D  = B on list 2;
-//   E  (one wave per insertion)  orientation selection, Fourier-space energy
+//   E  (one wave per live insertion)  orientation selection, Fourier-space energy
 //      of the selected molecule against the resident structure factors, W and
 //      the per-insertion record.
 // Two small kernels then sum the records in index order (deterministic sums).
@@ -48,13 +51,6 @@ constexpr int kRedPer = 8;       // records per thread in the first reduction pa
 constexpr int kInsWaves = 4;     // waves per workgroup of the per-insertion kernels
 constexpr int kEvalWaves = 16;   // waves per workgroup of the pair-energy kernel
 
-// Per-wave LDS workspace of the screening kernels.
-constexpr int kMaxDraws = 3 * kMaxTrials + 1 + 3 * kMaxTrials + 1;
-struct WaveLds {
-  double pos[kMaxPts][3];        // trial sites, FP64
-  uint4 pfix[kMaxPts];           // same, fixed point
-  double ud[kMaxDraws + 2];      // this insertion's uniforms
-};
 // Work lists are appended per workgroup of the screening kernels: workgroup g
 // of a grid of G owns insertions [g*chunk, (g+1)*chunk) and the sub-list at
 // g*chunk*cap_per_insertion; its length goes to counts[g].
@@ -125,7 +121,9 @@ struct WidomArgs {
   double2* e2;              // [n_insert * O * chain] per chain site
   TrialPt* list2;
   unsigned* counts1;        // [groups] sub-list lengths of list 1
-  unsigned* counts2;        // [groups] of list 2
+  unsigned* n2;             // length of list 2 (one list, appended with a global counter)
+  int* live;                // insertions with a first-bead trial left after the screen
+  unsigned* live_n;         // their number
   int groups, chunk;        // screening grid: workgroups and insertions per workgroup
   // outputs
   graspa_widom_result* res;   // may be null
@@ -134,7 +132,7 @@ struct WidomArgs {
   int* rec_f;                 // [n_insert] bit 0 first bead placed, bit 1 complete
   unsigned long long* stat;   // screened pairs, evaluated pairs
   // LDS layout (bytes from the start of the dynamic area)
-  int lds_thr, lds_tab, lds_wave;       // screening kernels
+  int lds_thr, lds_tab;                 // screening kernels
   int lds_ffl, lds_coul, lds_fw;        // pair-energy kernel
 };
 
@@ -143,20 +141,20 @@ __device__ inline double draw(const WidomArgs& A, uint64_t idx, uint32_t d)
   return philox_uniform(A.seed, A.stream, idx, d);
 }
 
-// Uniforms d0 .. d0+nd-1 of insertion idx into ud[0 .. nd), one Philox block
-// (two uniforms, philox_uniform's layout) per lane.
-__device__ inline void fill_draws(const WidomArgs& A, uint64_t idx, int d0, int nd, double* ud)
+// Uniforms d0, d0 + 1, d0 + 2 of insertion idx (philox_uniform's layout: draw d
+// is half d & 1 of Philox block d >> 1; the three draws span two blocks).
+__device__ inline void draw3(const WidomArgs& A, uint64_t idx, int d0, double u[3])
 {
-  const int lane = threadIdx.x & 63;
-  const int b0 = d0 >> 1, b1 = (d0 + nd - 1) >> 1;
-  for (int b = b0 + lane; b <= b1; b += 64) {
-    uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)b, A.stream};
-    philox4x32_10(c, (uint32_t)A.seed, (uint32_t)(A.seed >> 32));
-    const int d = 2 * b;
-    if (d >= d0 && d < d0 + nd)
-      ud[d - d0] = ((double)((((uint64_t)c[0] << 32) | c[1]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
-    if (d + 1 >= d0 && d + 1 < d0 + nd)
-      ud[d + 1 - d0] = ((double)((((uint64_t)c[2] << 32) | c[3]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  uint32_t c0[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)(d0 >> 1), A.stream};
+  uint32_t c1[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)((d0 >> 1) + 1), A.stream};
+  philox4x32_10(c0, (uint32_t)A.seed, (uint32_t)(A.seed >> 32));
+  philox4x32_10(c1, (uint32_t)A.seed, (uint32_t)(A.seed >> 32));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int d = d0 + k;
+    const uint32_t* c = ((d >> 1) == (d0 >> 1)) ? c0 : c1;
+    const uint32_t a = (d & 1) ? c[2] : c[0], b = (d & 1) ? c[3] : c[1];
+    u[k] = ((double)((((uint64_t)a << 32) | b) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
   }
 }
 
@@ -184,124 +182,40 @@ __device__ inline void orient_pos(const WidomArgs& A, uint64_t idx, int o, int a
   p[2] = fb[2] + vec[2];
 }
 
+// Overlap screen of one trial site per lane: FP32 squared distances from the
+// fixed-point site P to the framework atoms of the 27 grid cells around it
+// (the cells are at least as wide as the largest overlap distance, so they
+// hold every framework atom that can overlap the site) against the site's
+// thresholds th[atom type]; stops at the first hit.  Guests are not screened:
+// their overlaps are found by the exact evaluation.
 template <bool CUBIC>
-__device__ inline bool screen_hit(const BoxConst& bc, const float* __restrict__ thr, const int T, const int site,
-                                  const uint4 at, const bool valid, const uint4 P)
+__device__ inline bool screen_lane(const WidomArgs& A, const uint4* __restrict__ tab, const float* __restrict__ th,
+                                   const uint4 P, unsigned long long& st)
 {
-  const float th = valid ? thr[site * T + (int)at.w] : -1.0f;
-  const float rr = r2_fixed32<CUBIC>(at.x, at.y, at.z, P.x, P.y, P.z, bc.h32);
-  return rr < th;
-}
-
-// Overlap screen of n_trials trials x n_sites sites (trial site p = a *
-// n_trials + t in W.pfix, molecule site site0 + a).  With the cell grid, a
-// trial site is tested against the atoms of the 27 cells around its own cell,
-// which hold every atom that can overlap it; otherwise against all atoms.
-// Guests are always scanned.  Returns the mask of trials with a certain
-// overlap.
-template <bool CUBIC>
-__device__ inline uint32_t screen(const WidomArgs& A, const uint4* __restrict__ tab, const float* __restrict__ thr,
-                                  const WaveLds& W, const int n_trials, const int n_sites, const int site0,
-                                  unsigned long long& stat)
-{
-  const BoxConst& bc = A.bc;
-  const int T = bc.ntypes;
-  const int lane = threadIdx.x & 63;
-  const uint32_t all = (n_trials >= 32) ? ~0u : ((1u << n_trials) - 1u);
-  uint32_t alive = all;
-  if (A.use_cells) {
-    // this lane's (neighbour cell, slot) in each of the two lane groups
-    const int C = A.cell_cap, nslot = 27 * C;
-    int off[2][3], slot[2];
-    bool lv[2];
-#pragma unroll
-    for (int g = 0; g < 2; ++g) {
-      const int l = g * 64 + lane;
-      lv[g] = l < nslot;
-      const int nb = lv[g] ? l / C : 0;
-      slot[g] = lv[g] ? l - nb * C : 0;
-      off[g][0] = nb % 3 - 1;
-      off[g][1] = (nb / 3) % 3 - 1;
-      off[g][2] = nb / 9 - 1;
-    }
-    const int ngroups = nslot > 64 ? 2 : 1;
-    const int n0 = A.cell_n[0], n1 = A.cell_n[1], n2 = A.cell_n[2];
-    // four trial sites per step, so that their LDS gathers are in flight together
-    constexpr int kStep = 4;
-    for (int a = 0; a < n_sites; ++a) {
-      for (int t0 = 0; t0 < n_trials; t0 += kStep) {
-        if (!((alive >> t0) & 0xFu)) continue;
-        uint4 P[kStep];
-        int cell[kStep][3];
-#pragma unroll
-        for (int g = 0; g < kStep; ++g) {
-          P[g] = W.pfix[a * n_trials + min(t0 + g, n_trials - 1)];
-          cell[g][0] = __builtin_amdgcn_readfirstlane((int)(((uint64_t)P[g].x * (uint64_t)n0) >> 32));
-          cell[g][1] = __builtin_amdgcn_readfirstlane((int)(((uint64_t)P[g].y * (uint64_t)n1) >> 32));
-          cell[g][2] = __builtin_amdgcn_readfirstlane((int)(((uint64_t)P[g].z * (uint64_t)n2) >> 32));
-        }
-        for (int gg = 0; gg < ngroups; ++gg) {
-          uint4 at[kStep];
-#pragma unroll
-          for (int g = 0; g < kStep; ++g) {
-            int x = cell[g][0] + off[gg][0], y = cell[g][1] + off[gg][1], z = cell[g][2] + off[gg][2];
-            x = x < 0 ? x + n0 : (x >= n0 ? x - n0 : x);
-            y = y < 0 ? y + n1 : (y >= n1 ? y - n1 : y);
-            z = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
-            at[g] = tab[__mul24(x + __mul24(n0, y + __mul24(n1, z)), C) + slot[gg]];
-          }
-#pragma unroll
-          for (int g = 0; g < kStep; ++g) {
-            const int t = t0 + g;
-            const bool act = t < n_trials && ((alive >> t) & 1u);
-            const bool valid = lv[gg] && at[g].w != ~0u;
-            if (act) stat += (unsigned long long)__popcll(__ballot(valid));
-            if (__any(act && screen_hit<CUBIC>(bc, thr, T, site0 + a, at[g], valid, P[g]))) alive &= ~(1u << t);
-          }
-        }
-      }
-    }
-  } else {
-    for (int c0 = 0; c0 < A.n_fw && alive; c0 += 64) {
-      const int j = c0 + lane;
-      const bool valid = j < A.n_fw;
-      const uint4 f = valid ? A.fw_fixT[j] : make_uint4(0u, 0u, 0u, 0u);
-      const unsigned nval = (unsigned)min(64, A.n_fw - c0);
-      for (int a = 0; a < n_sites; ++a) {
-        for (int t = 0; t < n_trials; ++t) {
-          if (!((alive >> t) & 1u)) continue;
-          stat += nval;
-          if (__any(screen_hit<CUBIC>(bc, thr, T, site0 + a, f, valid, W.pfix[a * n_trials + t])))
-            alive &= ~(1u << t);
+  const int C = A.cell_cap, n0 = A.cell_n[0], n1 = A.cell_n[1], n2 = A.cell_n[2];
+  const int cx = (int)(((uint64_t)P.x * (uint64_t)n0) >> 32);
+  const int cy = (int)(((uint64_t)P.y * (uint64_t)n1) >> 32);
+  const int cz = (int)(((uint64_t)P.z * (uint64_t)n2) >> 32);
+  for (int dz = -1; dz <= 1; ++dz) {
+    int z = cz + dz;
+    z = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
+    for (int dy = -1; dy <= 1; ++dy) {
+      int y = cy + dy;
+      y = y < 0 ? y + n1 : (y >= n1 ? y - n1 : y);
+      for (int dx = -1; dx <= 1; ++dx) {
+        int x = cx + dx;
+        x = x < 0 ? x + n0 : (x >= n0 ? x - n0 : x);
+        const int cb = __mul24(x + __mul24(n0, y + __mul24(n1, z)), C);
+        for (int k = 0; k < C; ++k) {
+          const uint4 at = tab[cb + k];
+          if (at.w == ~0u) break;   // slots fill from the front
+          ++st;
+          if (r2_fixed32<CUBIC>(at.x, at.y, at.z, P.x, P.y, P.z, A.bc.h32) < th[at.w]) return true;
         }
       }
     }
   }
-  // guest molecules
-  for (int c = 1; c < A.gv.n_comp && alive; ++c) {
-    const int n = A.gv.seg[c].n_atoms, base = A.gv.seg[c].base;
-    for (int c0 = 0; c0 < n && alive; c0 += 64) {
-      const int jj = c0 + lane;
-      bool valid = jj < n;
-      uint4 f = make_uint4(0u, 0u, 0u, 0u);
-      if (valid) {
-        const int j = base + jj;
-        if (c == A.comp && A.g_molid[j] == A.new_molid) valid = false;
-        f = A.g_fix[j];
-        f.w = (uint32_t)A.g_type[j];
-      }
-      const unsigned nval = (unsigned)min(64, n - c0);
-      for (int a = 0; a < n_sites; ++a) {
-        for (int t = 0; t < n_trials; ++t) {
-          if (!((alive >> t) & 1u)) continue;
-          stat += nval;
-          if (__any(screen_hit<CUBIC>(bc, thr, T, site0 + a, f, valid, W.pfix[a * n_trials + t])))
-            alive &= ~(1u << t);
-        }
-      }
-    }
-  }
-  return all & ~alive;
+  return false;
 }
 
 // SelectTrialPosition / CBMC_FirstBead_Finish / Widom_Move_Chain_PARTIAL
@@ -383,32 +297,25 @@ __device__ inline const uint4* stage_screen(const WidomArgs& A, char* dyn, float
 // Appends this wave's surviving trials (bits of `alive`, n_sites sites each)
 // to its workgroup's sub-list (LDS counter wg_cnt); the sites of trial t are
 // W.pos[a * n_trials + t].
-__device__ inline void append_points(const WaveLds& W, TrialPt* sub, unsigned* wg_cnt, int ins, uint32_t alive,
-                                     int n_trials, int n_sites, int site0, int slot0)
+// Records of an insertion whose construction failed (W = 0).
+__device__ inline void failed_record(const WidomArgs& A, int ins, int fb_placed)
 {
-  const int lane = threadIdx.x & 63;
-  const int n_alive = __popc(alive);
-  if (n_alive == 0) return;
-  int base = 0;
-  if (lane == 0) base = (int)atomicAdd(wg_cnt, (unsigned)(n_alive * n_sites));
-  base = __builtin_amdgcn_readfirstlane(base);
-  if (lane < n_trials && ((alive >> lane) & 1u)) {
-    const int rank = __popc(alive & ((1u << lane) - 1u));
-    for (int a = 0; a < n_sites; ++a) {
-      const int p = a * n_trials + lane;
-      TrialPt q;
-      q.x = W.pos[p][0]; q.y = W.pos[p][1]; q.z = W.pos[p][2];
-      q.site = site0 + a;
-      q.slot = (slot0 + lane) * n_sites + a;
-      q.ins = ins;
-      q.bit = lane;
-      sub[base + rank * n_sites + a] = q;
-    }
+  A.rec_w[ins] = 0.0;
+  A.rec_e[ins] = 0.0;
+  A.rec_f[ins] = fb_placed ? 1 : 0;
+  if (A.res) {
+    graspa_widom_result r;
+    r.rosen = 0.0; r.energy = 0.0; r.vdw = 0.0; r.real = 0.0; r.ewald = 0.0;
+    r.first_bead = -1; r.orientation = -1; r.ok = 0; r._pad = 0;
+    A.res[ins] = r;
   }
 }
 
-// A: first-bead trial positions and their overlap screen.  Workgroup g owns
-// insertions [g*chunk, (g+1)*chunk).
+// A: first-bead trials, one lane per trial (64 / K insertions per wave step):
+// the trial's three draws, its position, the overlap screen.  Per insertion
+// the screened trials go to mask1; an insertion with no trial left is
+// finished here (W = 0); the others go to the live list and their surviving
+// trial sites to the workgroup's sub-list of list 1.
 template <bool CUBIC, bool TAB_LDS>
 __global__ __launch_bounds__(kInsWaves * 64) void widom_first_bead_kernel(const WidomArgs A)
 {
@@ -419,29 +326,55 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_first_bead_kernel(const 
   const uint4* tab = stage_screen<TAB_LDS>(A, dyn, thr);
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  WaveLds& W = *reinterpret_cast<WaveLds*>(dyn + A.lds_wave + (size_t)wv * sizeof(WaveLds));
-  const int K = A.K;
+  const int K = A.K, ipw = 64 / K;
+  const int lins = lane / K, trial = lane - lins * K;
   const int i0 = blockIdx.x * A.chunk, i1 = min(A.n_insert, i0 + A.chunk);
   TrialPt* sub = A.list1 + (size_t)i0 * K;
-  const uint32_t allK = (K >= 32) ? ~0u : ((1u << K) - 1u);
+  const unsigned long long allK = (1ull << K) - 1ull;
+  const unsigned long long lt = (1ull << lane) - 1ull;
   unsigned long long st = 0;
-  for (int ins = i0 + wv; ins < i1; ins += kInsWaves) {
-    const uint64_t idx = (uint64_t)(A.first_index + ins);
-    fill_draws(A, idx, 0, 3 * K, W.ud);
-    wave_sync();
-    if (lane < K) {
-      double p[3];
-      p[0] = A.bc.cell[0] * W.ud[3 * lane];
-      p[1] = A.bc.cell[4] * W.ud[3 * lane + 1];
-      p[2] = A.bc.cell[8] * W.ud[3 * lane + 2];
-      W.pos[lane][0] = p[0]; W.pos[lane][1] = p[1]; W.pos[lane][2] = p[2];
-      W.pfix[lane] = to_fixed4(A.bc, p);
+  for (int base = i0 + wv * ipw; base < i1; base += kInsWaves * ipw) {
+    const int ins = base + lins;
+    const bool act = lins < ipw && ins < i1;
+    double p[3] = {0.0, 0.0, 0.0};
+    bool hit = false;
+    if (act) {
+      double u[3];
+      draw3(A, (uint64_t)(A.first_index + ins), 3 * trial, u);
+      p[0] = A.bc.cell[0] * u[0];
+      p[1] = A.bc.cell[4] * u[1];
+      p[2] = A.bc.cell[8] * u[2];
+      if (A.use_cells) hit = screen_lane<CUBIC>(A, tab, thr, to_fixed4(A.bc, p), st);
     }
-    wave_sync();
-    const uint32_t dead = screen<CUBIC>(A, tab, thr, W, K, 1, 0, st);
-    if (lane == 0) A.mask1[ins] = dead;
-    append_points(W, sub, &wg_cnt, ins, allK & ~dead, K, 1, 0, ins * K);
-    wave_sync();
+    const unsigned long long hits = __ballot(act && hit);
+    const unsigned long long alive = __ballot(act && !hit);
+    const bool lead = act && trial == 0;
+    const bool live = lead && ((alive >> (lins * K)) & allK) != 0ull;
+    if (lead) {
+      A.mask1[ins] = (uint32_t)((hits >> (lins * K)) & allK);
+      if (!live) failed_record(A, ins, 0);
+    }
+    const unsigned long long lm = __ballot(live);
+    if (lm) {
+      unsigned b = 0;
+      if (lane == 0) b = atomicAdd(A.live_n, (unsigned)__popcll(lm));
+      b = (unsigned)__builtin_amdgcn_readfirstlane((int)b);
+      if (live) A.live[b + __popcll(lm & lt)] = ins;
+    }
+    if (alive) {
+      unsigned b = 0;
+      if (lane == 0) b = atomicAdd(&wg_cnt, (unsigned)__popcll(alive));
+      b = (unsigned)__builtin_amdgcn_readfirstlane((int)b);
+      if (act && !hit) {
+        TrialPt q;
+        q.x = p[0]; q.y = p[1]; q.z = p[2];
+        q.site = 0;
+        q.slot = ins * K + trial;
+        q.ins = ins;
+        q.bit = trial;
+        sub[b + __popcll(alive & lt)] = q;
+      }
+    }
   }
   if (lane == 0) atomicAdd(&A.stat[0], st);
   __syncthreads();
@@ -454,13 +387,14 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_first_bead_kernel(const 
 // The framework (x, y, z, q and type) is staged in LDS when it fits.
 template <bool MIXED, bool CUBIC, bool FW_LDS>
 __global__ __launch_bounds__(kEvalWaves * 64) void widom_eval_kernel(const WidomArgs A, const TrialPt* list,
-                                                                   const unsigned* counts, int per_ins,
-                                                                   double2* energy, uint32_t* mask)
+                                                                   const unsigned* counts, int groups, int chunk,
+                                                                   int per_ins, double2* energy, uint32_t* mask)
 {
   extern __shared__ __align__(16) char dyn[];
-  // exclusive prefix sums of the screening workgroups' sub-list lengths
+  // exclusive prefix sums of the sub-list lengths (groups sub-lists of
+  // chunk * per_ins entries each)
   __shared__ unsigned pre[kMaxScreenGroups + 1];
-  const int G = A.groups;
+  const int G = groups;
   for (int i = threadIdx.x; i < G; i += blockDim.x) pre[i + 1] = counts[i];
   if (threadIdx.x == 0) pre[0] = 0u;
   __syncthreads();
@@ -502,7 +436,7 @@ __global__ __launch_bounds__(kEvalWaves * 64) void widom_eval_kernel(const Widom
       const int mid = (lo + hi + 1) >> 1;
       if ((int)pre[mid] <= pi) lo = mid; else hi = mid - 1;
     }
-    const TrialPt P = list[(size_t)lo * A.chunk * per_ins + (pi - (int)pre[lo])];
+    const TrialPt P = list[(size_t)lo * chunk * per_ins + (pi - (int)pre[lo])];
     const double px = P.x, py = P.y, pz = P.z;
     const double sq = A.site_q[P.site];
     const bool coul = bc.use_charges && sq != 0.0;
@@ -632,26 +566,29 @@ __global__ __launch_bounds__(kEvalWaves * 64) void widom_eval_kernel(const Widom
 
 // C: first-bead selection; orientations about the selected bead and their
 // overlap screen.  Workgroup g owns insertions [g*chunk, (g+1)*chunk).
-template <bool CUBIC, bool TAB_LDS>
+// C: one wave per live insertion: first-bead selection from the evaluated
+// trials, the O orientations of the template about the selected bead (one
+// lane per orientation site), their overlap screen, survivors appended to
+// list 2 (one list, global counter).
+template <bool CUBIC>
 __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const WidomArgs A)
 {
   extern __shared__ __align__(16) char dyn[];
-  __shared__ unsigned wg_cnt;
-  if (threadIdx.x == 0) wg_cnt = 0u;
-  float* thr;
-  const uint4* tab = stage_screen<TAB_LDS>(A, dyn, thr);
+  const int T = A.bc.ntypes;
+  float* thr = reinterpret_cast<float*>(dyn);
+  for (int i = threadIdx.x; i < A.M * T; i += blockDim.x) {
+    const int a = i / T, ty = i % T;
+    thr[i] = screen_threshold(A.ff64 + 4 * (ty * T + A.site_type[a]), A.bc.overlap, A.bc.cut_vdw2);
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  WaveLds& W = *reinterpret_cast<WaveLds*>(dyn + A.lds_wave + (size_t)wv * sizeof(WaveLds));
   const int K = A.K, O = A.O, chain = A.chain;
-  const int i0 = blockIdx.x * A.chunk, i1 = min(A.n_insert, i0 + A.chunk);
-  TrialPt* sub = A.list2 + (size_t)i0 * O * chain;
-  const uint32_t allO = (O >= 32) ? ~0u : ((1u << O) - 1u);
+  const unsigned n_live = *A.live_n;
+  const unsigned long long lt = (1ull << lane) - 1ull;
   unsigned long long st = 0;
-  for (int ins = i0 + wv; ins < i1; ins += kInsWaves) {
+  for (unsigned li = blockIdx.x * kInsWaves + wv; li < n_live; li += gridDim.x * kInsWaves) {
+    const int ins = A.live[li];
     const uint64_t idx = (uint64_t)(A.first_index + ins);
-    // draws 0 .. 3K+3O: first-bead trials, first-bead selection, orientations
-    fill_draws(A, idx, 0, 3 * K + 1 + 3 * O, W.ud);
     const uint32_t dead = A.mask1[ins];
     double ev = 0.0, er = 0.0;
     if (lane < K && !((dead >> lane) & 1u)) {
@@ -659,10 +596,9 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
       ev = e.x;
       er = e.y;
     }
-    wave_sync();
     int sel;
     double Wfb;
-    wave_select(ev, er, dead, K, W.ud[3 * K], A.beta, A.bias, sel, Wfb);
+    wave_select(ev, er, dead, K, draw(A, idx, 3 * K), A.beta, A.bias, sel, Wfb);
     const bool fb_ok = sel >= 0 && Wfb > 1e-150;
     InsState S;
     S.sel = fb_ok ? sel : -1;
@@ -672,38 +608,54 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
     S.er = fb_ok ? readlane_d(er, sel) : 0.0;
     S.fb[0] = S.fb[1] = S.fb[2] = 0.0;
     if (fb_ok) {
-      S.fb[0] = A.bc.cell[0] * W.ud[3 * sel];
-      S.fb[1] = A.bc.cell[4] * W.ud[3 * sel + 1];
-      S.fb[2] = A.bc.cell[8] * W.ud[3 * sel + 2];
+      double u[3];
+      draw3(A, idx, 3 * sel, u);
+      S.fb[0] = A.bc.cell[0] * u[0];
+      S.fb[1] = A.bc.cell[4] * u[1];
+      S.fb[2] = A.bc.cell[8] * u[2];
     }
     if (lane == 0) {
       A.state[ins] = S;
       A.mask2[ins] = 0u;
     }
     if (fb_ok && chain > 0) {
-      if (lane < O) {
-        const double u = W.ud[3 * K + 1 + 3 * lane];
-        const double v = W.ud[3 * K + 2 + 3 * lane];
-        const double w = W.ud[3 * K + 3 + 3 * lane];
-        for (int a = 0; a < chain; ++a) {
-          double vec[3] = {A.tvec[a][0], A.tvec[a][1], A.tvec[a][2]};
-          quat_rotate(vec, u, v, w);
-          const double p[3] = {S.fb[0] + vec[0], S.fb[1] + vec[1], S.fb[2] + vec[2]};
-          const int s = a * O + lane;
-          W.pos[s][0] = p[0]; W.pos[s][1] = p[1]; W.pos[s][2] = p[2];
-          W.pfix[s] = to_fixed4(A.bc, p);
+      const int o = lane / chain, a = lane - o * chain;
+      const bool act = lane < O * chain;
+      double p[3] = {0.0, 0.0, 0.0};
+      bool hit = false;
+      if (act) {
+        double u[3];
+        draw3(A, idx, 3 * K + 1 + 3 * o, u);
+        double vec[3] = {A.tvec[a][0], A.tvec[a][1], A.tvec[a][2]};
+        quat_rotate(vec, u[0], u[1], u[2]);
+        p[0] = S.fb[0] + vec[0]; p[1] = S.fb[1] + vec[1]; p[2] = S.fb[2] + vec[2];
+        if (A.use_cells) hit = screen_lane<CUBIC>(A, A.ctab, thr + (1 + a) * T, to_fixed4(A.bc, p), st);
+      }
+      const unsigned long long hits = __ballot(act && hit);
+      const unsigned long long am = (1ull << chain) - 1ull;
+      uint32_t dead2 = 0u;
+      for (int oo = 0; oo < O; ++oo)
+        if ((hits >> (oo * chain)) & am) dead2 |= 1u << oo;
+      if (lane == 0) A.mask2[ins] = dead2;
+      const bool alive = act && !((dead2 >> o) & 1u);
+      const unsigned long long al = __ballot(alive);
+      if (al) {
+        unsigned b = 0;
+        if (lane == 0) b = atomicAdd(A.n2, (unsigned)__popcll(al));
+        b = (unsigned)__builtin_amdgcn_readfirstlane((int)b);
+        if (alive) {
+          TrialPt q;
+          q.x = p[0]; q.y = p[1]; q.z = p[2];
+          q.site = 1 + a;
+          q.slot = (ins * O + o) * chain + a;
+          q.ins = ins;
+          q.bit = o;
+          A.list2[b + __popcll(al & lt)] = q;
         }
       }
-      wave_sync();
-      const uint32_t dead2 = screen<CUBIC>(A, tab, thr, W, O, chain, 1, st);
-      if (lane == 0) A.mask2[ins] = dead2;
-      append_points(W, sub, &wg_cnt, ins, allO & ~dead2, O, chain, 1, ins * O);
     }
-    wave_sync();
   }
   if (lane == 0) atomicAdd(&A.stat[0], st);
-  __syncthreads();
-  if (threadIdx.x == 0) A.counts2[blockIdx.x] = wg_cnt;
 }
 
 // E: orientation selection, Fourier-space energy of the selected molecule
@@ -718,7 +670,9 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_finish_kernel(const Wido
   const size_t wave_bytes = (size_t)M * 3 * KS * sizeof(double2) + kMaxMol * 3 * sizeof(double);
   double2* eik = reinterpret_cast<double2*>(dyn + (size_t)wv * wave_bytes);
   double (*mol)[3] = reinterpret_cast<double (*)[3]>(dyn + (size_t)wv * wave_bytes + (size_t)M * 3 * KS * sizeof(double2));
-  for (int ins = blockIdx.x * kInsWaves + wv; ins < A.n_insert; ins += gridDim.x * kInsWaves) {
+  const unsigned n_live = *A.live_n;
+  for (unsigned li = blockIdx.x * kInsWaves + wv; li < n_live; li += gridDim.x * kInsWaves) {
+    const int ins = A.live[li];
     const uint64_t idx = (uint64_t)(A.first_index + ins);
     const InsState S = A.state[ins];
     bool ok = S.ok != 0;
@@ -826,11 +780,13 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_finish_kernel(const Wido
   }
 }
 
-__global__ void widom_reset_kernel(unsigned long long* stat)
+__global__ void widom_reset_kernel(unsigned long long* stat, unsigned* live_n, unsigned* n2)
 {
   if (threadIdx.x == 0) {
     stat[0] = 0ull;
     stat[1] = 0ull;
+    *live_n = 0u;
+    *n2 = 0u;
   }
 }
 
@@ -922,19 +878,20 @@ static int ensure_pipeline(graspa_box* b, int n_insert, int K, int OC)
   const size_t cn = std::max(n, b->wp_n), c1 = std::max(need_e1, b->wp_e1), c2 = std::max(need_e2, b->wp_e2);
   hipStreamSynchronize(b->stream);
   hipFree(b->d_wmask1); hipFree(b->d_wmask2); hipFree(b->d_we1); hipFree(b->d_we2);
-  hipFree(b->d_wlist1); hipFree(b->d_wlist2); hipFree(b->d_wstate);
+  hipFree(b->d_wlist1); hipFree(b->d_wlist2); hipFree(b->d_wstate); hipFree(b->d_wlive);
   b->d_wmask1 = b->d_wmask2 = nullptr;
-  b->d_we1 = b->d_we2 = b->d_wlist1 = b->d_wlist2 = b->d_wstate = nullptr;
+  b->d_we1 = b->d_we2 = b->d_wlist1 = b->d_wlist2 = b->d_wstate = b->d_wlive = nullptr;
   b->wp_n = b->wp_e1 = b->wp_e2 = 0;
   int rc = hip_check(hipMalloc(&b->d_wmask1, cn * sizeof(uint32_t)), "hipMalloc widom pipeline");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_wmask2, cn * sizeof(uint32_t)), "hipMalloc widom pipeline");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_wstate, cn * sizeof(InsState)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_wlive, cn * sizeof(int)), "hipMalloc widom pipeline");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_we1, c1 * sizeof(double2)), "hipMalloc widom pipeline");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_wlist1, c1 * sizeof(TrialPt)), "hipMalloc widom pipeline");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_we2, c2 * sizeof(double2)), "hipMalloc widom pipeline");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_wlist2, c2 * sizeof(TrialPt)), "hipMalloc widom pipeline");
   if (!rc && !b->d_wcounts)
-    rc = hip_check(hipMalloc(&b->d_wcounts, 2 * kMaxScreenGroups * sizeof(unsigned)), "hipMalloc widom pipeline");
+    rc = hip_check(hipMalloc(&b->d_wcounts, (kMaxScreenGroups + 2) * sizeof(unsigned)), "hipMalloc widom pipeline");
   if (!rc) {
     b->wp_n = cn;
     b->wp_e1 = c1;
@@ -944,8 +901,8 @@ static int ensure_pipeline(graspa_box* b, int n_insert, int K, int OC)
 }
 
 template <bool MIXED, bool CUBIC>
-static int launch_eval(graspa_box* b, const WidomArgs& A, const TrialPt* list, const unsigned* counts, int per_ins,
-                       double2* energy, uint32_t* mask)
+static int launch_eval(graspa_box* b, const WidomArgs& A, const TrialPt* list, const unsigned* counts, int groups,
+                       int chunk, int per_ins, double2* energy, uint32_t* mask)
 {
   const size_t fw_bytes = (size_t)A.n_fw * (sizeof(double4) + sizeof(int));
   for (int pass = 0; pass < 2; ++pass) {
@@ -956,32 +913,23 @@ static int launch_eval(graspa_box* b, const WidomArgs& A, const TrialPt* list, c
     if (occ < 1) continue;
     timing_begin(b, "widom");
     hipLaunchKernelGGL(kern, dim3((unsigned)(occ * b->n_cu)), dim3(kEvalWaves * 64), lds, b->stream, A, list, counts,
-                       per_ins, energy, mask);
+                       groups, chunk, per_ins, energy, mask);
     timing_end(b, "widom");
     return hip_check(hipGetLastError(), "widom_eval_kernel launch");
   }
   return fail(GRASPA_ERR_UNSUPPORTED, "widom: no launch shape fits the LDS (pair energies)");
 }
 
-template <bool CUBIC, bool TAB_LDS>
-static int launch_screens(graspa_box* b, const WidomArgs& A, size_t lds, int stage)
-{
-  if (stage == 0)
-    return launch_ins(b, widom_first_bead_kernel<CUBIC, TAB_LDS>, A, lds, "widom_first_bead_kernel", A.groups);
-  return launch_ins(b, widom_chain_kernel<CUBIC, TAB_LDS>, A, lds, "widom_chain_kernel", A.groups);
-}
-
+// First-bead screen: the cell table in LDS when it fits, else read from L2.
 template <bool CUBIC>
-static int launch_screen_stage(graspa_box* b, WidomArgs& A, size_t base_lds, size_t tab_bytes, int stage)
+static int launch_first_bead(graspa_box* b, const WidomArgs& A, size_t base_lds, size_t tab_bytes)
 {
-  const size_t waves = kInsWaves * sizeof(WaveLds);
-  if (A.use_cells && base_lds + align16(tab_bytes) + waves <= (size_t)b->max_lds) {
-    A.lds_wave = (int)(base_lds + align16(tab_bytes));
-    const int rc = launch_screens<CUBIC, true>(b, A, base_lds + align16(tab_bytes) + waves, stage);
+  if (A.use_cells && base_lds + tab_bytes <= (size_t)b->max_lds) {
+    const int rc = launch_ins(b, widom_first_bead_kernel<CUBIC, true>, A, base_lds + tab_bytes,
+                              "widom_first_bead_kernel", A.groups);
     if (rc != GRASPA_ERR_UNSUPPORTED) return rc;
   }
-  A.lds_wave = (int)base_lds;
-  return launch_screens<CUBIC, false>(b, A, base_lds + waves, stage);
+  return launch_ins(b, widom_first_bead_kernel<CUBIC, false>, A, base_lds, "widom_first_bead_kernel", A.groups);
 }
 
 int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
@@ -1004,8 +952,7 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   WidomArgs A{};
   A.bc = b->bc;
   A.fw = b->d_fws; A.fw_type = b->d_fws_type; A.fw_fixT = b->d_fw_fixT; A.n_fw = b->n_fw;
-  // the screen's lane mapping covers 27 cells x at most 4 slots (two 64-lane groups)
-  A.use_cells = (b->use_cells && b->cell_cap <= 4) ? 1 : 0;
+  A.use_cells = b->use_cells;
   for (int d = 0; d < 3; ++d) A.cell_n[d] = b->cell_n[d];
   A.cell_cap = b->cell_cap;
   A.ctab = b->d_ctab;
@@ -1042,7 +989,9 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   A.list2 = static_cast<TrialPt*>(b->d_wlist2);
   A.state = static_cast<InsState*>(b->d_wstate);
   A.counts1 = static_cast<unsigned*>(b->d_wcounts);
-  A.counts2 = A.counts1 + kMaxScreenGroups;
+  A.live_n = A.counts1 + kMaxScreenGroups;
+  A.n2 = A.live_n + 1;
+  A.live = static_cast<int*>(b->d_wlive);
   // screening grid: up to kMaxScreenGroups workgroups, each owning a contiguous range of insertions
   A.groups = (int)std::min<long long>(kMaxScreenGroups, std::max<long long>(1, ((long long)n_insert + 31) / 32));
   A.chunk = (n_insert + A.groups - 1) / A.groups;
@@ -1064,24 +1013,23 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   A.lds_fw = (int)off;
   const bool mixed = b->precision == GRASPA_MIXED, cubic = A.bc.cubic != 0;
 
-  hipLaunchKernelGGL(widom_reset_kernel, dim3(1), dim3(64), 0, b->stream, A.stat);
+  hipLaunchKernelGGL(widom_reset_kernel, dim3(1), dim3(64), 0, b->stream, A.stat, A.live_n, A.n2);
   if (hip_check(hipGetLastError(), "widom_reset launch")) return GRASPA_ERR_HIP;
-  int rc = cubic ? launch_screen_stage<true>(b, A, scr_base, tab_bytes, 0)
-                 : launch_screen_stage<false>(b, A, scr_base, tab_bytes, 0);
+  int rc = cubic ? launch_first_bead<true>(b, A, scr_base, tab_bytes) : launch_first_bead<false>(b, A, scr_base, tab_bytes);
   if (rc) return rc;
-  auto eval = [&](const TrialPt* list, const unsigned* counts, int per_ins, double2* energy, uint32_t* mask) {
-    if (mixed) return cubic ? launch_eval<true, true>(b, A, list, counts, per_ins, energy, mask)
-                            : launch_eval<true, false>(b, A, list, counts, per_ins, energy, mask);
-    return cubic ? launch_eval<false, true>(b, A, list, counts, per_ins, energy, mask)
-                 : launch_eval<false, false>(b, A, list, counts, per_ins, energy, mask);
+  auto eval = [&](const TrialPt* list, const unsigned* counts, int groups, int chunk, int per_ins, double2* energy,
+                  uint32_t* mask) {
+    if (mixed) return cubic ? launch_eval<true, true>(b, A, list, counts, groups, chunk, per_ins, energy, mask)
+                            : launch_eval<true, false>(b, A, list, counts, groups, chunk, per_ins, energy, mask);
+    return cubic ? launch_eval<false, true>(b, A, list, counts, groups, chunk, per_ins, energy, mask)
+                 : launch_eval<false, false>(b, A, list, counts, groups, chunk, per_ins, energy, mask);
   };
-  rc = eval(A.list1, A.counts1, K, A.e1, A.mask1);
+  rc = eval(A.list1, A.counts1, A.groups, A.chunk, K, A.e1, A.mask1);
   if (rc) return rc;
-  rc = cubic ? launch_screen_stage<true>(b, A, scr_base, tab_bytes, 1)
-             : launch_screen_stage<false>(b, A, scr_base, tab_bytes, 1);
+  rc = launch_ins(b, cubic ? widom_chain_kernel<true> : widom_chain_kernel<false>, A, scr_base, "widom_chain_kernel");
   if (rc) return rc;
   if (chain > 0) {
-    rc = eval(A.list2, A.counts2, O * chain, A.e2, A.mask2);
+    rc = eval(A.list2, A.n2, 1, 0, O * chain, A.e2, A.mask2);
     if (rc) return rc;
   }
   const size_t fin_lds = (size_t)kInsWaves * ((size_t)M * 3 * A.kstride * sizeof(double2) + kMaxMol * 3 * sizeof(double));
