@@ -58,9 +58,9 @@ def reference_flops(n_sys, K, O, chain, M, n_active_k, n_total, n_fb, n_ok, char
 
 
 def load_traffic(kernel_prefix):
-    """HBM bytes per launch of the Widom kernel from the committed rocprofv3
+    """HBM bytes per launch of the Widom trial-evaluation kernel from the committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy)."""
-    p = os.path.join(ROOT, "profiles", "widom_kernel_mixed_config1.json")
+    p = os.path.join(ROOT, "profiles", "widom_eval_kernel_config1_100k.json")
     try:
         d = json.load(open(p))
     except (OSError, ValueError):

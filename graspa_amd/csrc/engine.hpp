@@ -99,6 +99,10 @@ struct graspa_box {
   int cell_n[3] = {0, 0, 0};
   int cell_cap = 0;
   int use_cells = 0;
+  // maps of the framework's certain-overlap volume per probe-site type
+  // (dead_map.cpp), built on first use
+  std::vector<uint32_t*> d_dmap;
+  int dmap_nv[3] = {0, 0, 0};
   // real-space Coulomb table of the fast form (coulomb_table.cpp)
   double* d_coul = nullptr;
   int coul_n = 0;               // doubles
@@ -146,8 +150,9 @@ struct graspa_box {
   void* d_wlist1 = nullptr;
   void* d_wlist2 = nullptr;
   void* d_wstate = nullptr;
-  void* d_wcounts = nullptr;    // sub-list lengths of work list 1, live count, length of list 2
+  void* d_wcounts = nullptr;    // sub-list lengths of work lists 1 and 2 and of the live lists
   void* d_wlive = nullptr;      // insertions with a first-bead trial left after the screen
+  void* d_wstat = nullptr;      // per-workgroup work counts of the pipeline
   size_t wp_n = 0, wp_e1 = 0, wp_e2 = 0;
   graspa_widom_result* d_wres = nullptr;
   size_t wres_cap = 0;
@@ -200,6 +205,15 @@ __host__ __device__ inline double overlap_r2(double a4e, double s2, double shift
 
 // screen cell grid of the framework (engine_api.hip)
 int build_screen_cells(graspa_box* b);
+
+// Certain-overlap voxel maps of the framework (dead_map.cpp): voxels of about
+// kDeadVoxel A; bit (x + nv0 (y + nv1 z)) set when every point of the voxel
+// overlaps a framework atom for a probe site of the map's type.
+constexpr double kDeadVoxel = 0.15;
+void dead_map_dims(const graspa_box* b, int nv[3]);
+void build_dead_map(const graspa_box* b, int site_type, std::vector<uint32_t>& bits, int nv[3]);
+int ensure_dead_map(graspa_box* b, int site_type, const uint32_t** out);
+void free_dead_maps(graspa_box* b);
 
 // Real-space Coulomb table g(r^2) = erfc(alpha r)/r of the fast pair form
 // (coulomb_table.cpp): kCoulTabM intervals per octave of r^2 from

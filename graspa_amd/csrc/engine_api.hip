@@ -506,12 +506,13 @@ void graspa_box_destroy(graspa_box* b)
   hipFree(b->d_g); hipFree(b->d_g_type); hipFree(b->d_g_molid); hipFree(b->d_g_comp); hipFree(b->d_g_fix);
   hipFree(b->d_ktemp); hipFree(b->d_sk_fw); hipFree(b->d_sk_ads); hipFree(b->d_sk_stage);
   hipFree(b->d_kact); hipFree(b->d_ktemp_act); hipFree(b->d_coul);
+  free_dead_maps(b);
   hipFree(b->d_part);
   if (b->h_part) hipHostFree(b->h_part);
   hipFree(b->d_wacc); hipFree(b->d_wres);
   hipFree(b->d_wrec_w); hipFree(b->d_wrec_e); hipFree(b->d_wrec_f); hipFree(b->d_wpart);
   hipFree(b->d_wmask1); hipFree(b->d_wmask2); hipFree(b->d_we1); hipFree(b->d_we2);
-  hipFree(b->d_wlist1); hipFree(b->d_wlist2); hipFree(b->d_wstate); hipFree(b->d_wcounts); hipFree(b->d_wlive);
+  hipFree(b->d_wlist1); hipFree(b->d_wlist2); hipFree(b->d_wstate); hipFree(b->d_wcounts); hipFree(b->d_wlive); hipFree(b->d_wstat);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
 }
@@ -526,6 +527,7 @@ int graspa_box_set_framework(graspa_box* b, int n, const double* xyz, const doub
   hipStreamSynchronize(b->stream);
   hipFree(b->d_fw); hipFree(b->d_fw_type); hipFree(b->d_fw_fix); hipFree(b->d_fw_fixT);
   b->d_fw = nullptr; b->d_fw_type = nullptr; b->d_fw_fix = nullptr; b->d_fw_fixT = nullptr;
+  free_dead_maps(b);
   b->n_fw = n;
   b->fw_pos.assign(xyz, xyz + 3 * (size_t)n);
   b->fw_q.assign(q, q + n);

@@ -48,13 +48,15 @@ constexpr int kMaxPts = 64;      // trial sites of one screen: K, or O x (M-1)
 constexpr int kSums = 6;         // W, W^2, W dE, n, n first bead placed, n complete
 constexpr int kRedBlock = 256;
 constexpr int kRedPer = 8;       // records per thread in the first reduction pass
-constexpr int kInsWaves = 4;     // waves per workgroup of the per-insertion kernels
+constexpr int kInsWaves = 16;    // waves per workgroup of the per-insertion kernels (one live insertion each)
 constexpr int kEvalWaves = 16;   // waves per workgroup of the pair-energy kernel
+constexpr int kFbWaves = 16;     // waves per workgroup of the first-bead kernel
 
 // Work lists are appended per workgroup of the screening kernels: workgroup g
 // of a grid of G owns insertions [g*chunk, (g+1)*chunk) and the sub-list at
 // g*chunk*cap_per_insertion; its length goes to counts[g].
 constexpr int kMaxScreenGroups = 1024;
+constexpr int kMaxEvalGroups = 4096;   // workgroups of one pair-energy launch (work-count slots)
 
 // State of one insertion between the kernels.
 struct InsState {
@@ -121,16 +123,20 @@ struct WidomArgs {
   double2* e2;              // [n_insert * O * chain] per chain site
   TrialPt* list2;
   unsigned* counts1;        // [groups] sub-list lengths of list 1
-  unsigned* n2;             // length of list 2 (one list, appended with a global counter)
-  int* live;                // insertions with a first-bead trial left after the screen
-  unsigned* live_n;         // their number
+  unsigned* counts2;        // [groups] sub-list lengths of list 2
+  unsigned* live_counts;    // [groups] lengths of the live sub-lists
+  const uint32_t* dmap[kMaxMol];   // certain-overlap voxel map of each molecule site's type (or null)
+  int dnv[3];               // its voxels per cell axis
+  int* live;                // insertions with a first-bead trial left after the screen, sub-list
+                            // of workgroup g at g * chunk
+  unsigned long long* stat_part;   // per-workgroup work counts (screened, evaluated pairs)
   int groups, chunk;        // screening grid: workgroups and insertions per workgroup
   // outputs
   graspa_widom_result* res;   // may be null
   double* rec_w;              // [n_insert] W
   double* rec_e;              // [n_insert] dE
   int* rec_f;                 // [n_insert] bit 0 first bead placed, bit 1 complete
-  unsigned long long* stat;   // screened pairs, evaluated pairs
+  unsigned long long* stat;   // screened pairs, evaluated pairs (sums of stat_part)
   // LDS layout (bytes from the start of the dynamic area)
   int lds_thr, lds_tab;                 // screening kernels
   int lds_ffl, lds_coul, lds_fw;        // pair-energy kernel
@@ -196,26 +202,46 @@ __device__ inline bool screen_lane(const WidomArgs& A, const uint4* __restrict__
   const int cx = (int)(((uint64_t)P.x * (uint64_t)n0) >> 32);
   const int cy = (int)(((uint64_t)P.y * (uint64_t)n1) >> 32);
   const int cz = (int)(((uint64_t)P.z * (uint64_t)n2) >> 32);
+  const int xm = cx == 0 ? n0 - 1 : cx - 1, xp = cx == n0 - 1 ? 0 : cx + 1;
   for (int dz = -1; dz <= 1; ++dz) {
     int z = cz + dz;
     z = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
     for (int dy = -1; dy <= 1; ++dy) {
       int y = cy + dy;
       y = y < 0 ? y + n1 : (y >= n1 ? y - n1 : y);
-      for (int dx = -1; dx <= 1; ++dx) {
-        int x = cx + dx;
-        x = x < 0 ? x + n0 : (x >= n0 ? x - n0 : x);
-        const int cb = __mul24(x + __mul24(n0, y + __mul24(n1, z)), C);
-        for (int k = 0; k < C; ++k) {
-          const uint4 at = tab[cb + k];
-          if (at.w == ~0u) break;   // slots fill from the front
-          ++st;
-          if (r2_fixed32<CUBIC>(at.x, at.y, at.z, P.x, P.y, P.z, A.bc.h32) < th[at.w]) return true;
+      const int row = __mul24(n0, y + __mul24(n1, z));
+      const int cb[3] = {__mul24(row + xm, C), __mul24(row + cx, C), __mul24(row + xp, C)};
+      // a row of three cells per step, up to four slots each, loaded together
+      // (slots fill from the front, empty ones have type ~0u)
+      for (int k0 = 0; k0 < C; k0 += 4) {
+        uint4 at[12];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) at[4 * c + j] = (k0 + j < C) ? tab[cb[c] + k0 + j] : make_uint4(0u, 0u, 0u, ~0u);
+        bool hit = false;
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+          if (at[q].w != ~0u) {
+            ++st;
+            hit = hit || (r2_fixed32<CUBIC>(at[q].x, at[q].y, at[q].z, P.x, P.y, P.z, A.bc.h32) < th[at[q].w]);
+          }
         }
+        if (hit) return true;
       }
     }
   }
   return false;
+}
+
+// Certain overlap from the voxel map: the voxel of the fixed-point site P.
+__device__ inline bool dead_voxel(const uint32_t* __restrict__ dm, const int* nv, const uint4 P)
+{
+  const uint32_t x = (uint32_t)(((uint64_t)P.x * (uint64_t)nv[0]) >> 32);
+  const uint32_t y = (uint32_t)(((uint64_t)P.y * (uint64_t)nv[1]) >> 32);
+  const uint32_t z = (uint32_t)(((uint64_t)P.z * (uint64_t)nv[2]) >> 32);
+  const uint32_t lin = x + (uint32_t)nv[0] * (y + (uint32_t)nv[1] * z);
+  return (dm[lin >> 5] >> (lin & 31u)) & 1u;
 }
 
 // SelectTrialPosition / CBMC_FirstBead_Finish / Widom_Move_Chain_PARTIAL
@@ -312,40 +338,79 @@ __device__ inline void failed_record(const WidomArgs& A, int ins, int fb_placed)
 }
 
 // A: first-bead trials, one lane per trial (64 / K insertions per wave step):
-// the trial's three draws, its position, the overlap screen.  Per insertion
-// the screened trials go to mask1; an insertion with no trial left is
-// finished here (W = 0); the others go to the live list and their surviving
-// trial sites to the workgroup's sub-list of list 1.
+// the trial's three draws and its position; the voxel map decides most trials
+// with one bit, the others are queued in LDS and screened against the cell
+// grid by the whole workgroup, one trial per lane.  Per insertion the
+// screened trials go to mask1; an insertion with no trial left is finished
+// here (W = 0, most insertions in a dense framework), the others go to the
+// live list and their surviving trial sites to the workgroup's sub-list of
+// list 1.
 template <bool CUBIC, bool TAB_LDS>
-__global__ __launch_bounds__(kInsWaves * 64) void widom_first_bead_kernel(const WidomArgs A)
+__global__ __launch_bounds__(kFbWaves * 64) void widom_first_bead_kernel(const WidomArgs A)
 {
   extern __shared__ __align__(16) char dyn[];
-  __shared__ unsigned wg_cnt;
-  if (threadIdx.x == 0) wg_cnt = 0u;
+  __shared__ unsigned wg_cnt, q_n[2], live_cnt;
+  __shared__ unsigned long long wg_st;
+  __shared__ uint4 q_p[kFbWaves * 64];             // undecided trial sites (fixed point)
+  __shared__ unsigned short q_t[kFbWaves * 64];    // their thread
+  __shared__ unsigned char q_hit[kFbWaves * 64];   // screen result by thread
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid == 0) {
+    wg_cnt = 0u;
+    q_n[0] = 0u;
+    q_n[1] = 0u;
+    live_cnt = 0u;
+    wg_st = 0ull;
+  }
   float* thr;
   const uint4* tab = stage_screen<TAB_LDS>(A, dyn, thr);
   __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int K = A.K, ipw = 64 / K;
   const int lins = lane / K, trial = lane - lins * K;
   const int i0 = blockIdx.x * A.chunk, i1 = min(A.n_insert, i0 + A.chunk);
+  const int per_step = kFbWaves * ipw;
+  const int nsteps = i1 > i0 ? (i1 - i0 + per_step - 1) / per_step : 0;
   TrialPt* sub = A.list1 + (size_t)i0 * K;
   const unsigned long long allK = (1ull << K) - 1ull;
   const unsigned long long lt = (1ull << lane) - 1ull;
+  const uint32_t* dm = A.dmap[0];
   unsigned long long st = 0;
-  for (int base = i0 + wv * ipw; base < i1; base += kInsWaves * ipw) {
-    const int ins = base + lins;
+  for (int step = 0; step < nsteps; ++step) {
+    const int ins = i0 + step * per_step + wv * ipw + lins;
     const bool act = lins < ipw && ins < i1;
     double p[3] = {0.0, 0.0, 0.0};
-    bool hit = false;
+    uint4 P = make_uint4(0u, 0u, 0u, 0u);
+    bool hit = false, undecided = false;
     if (act) {
       double u[3];
       draw3(A, (uint64_t)(A.first_index + ins), 3 * trial, u);
       p[0] = A.bc.cell[0] * u[0];
       p[1] = A.bc.cell[4] * u[1];
       p[2] = A.bc.cell[8] * u[2];
-      if (A.use_cells) hit = screen_lane<CUBIC>(A, tab, thr, to_fixed4(A.bc, p), st);
+      P = to_fixed4(A.bc, p);
+      if (dm) hit = dead_voxel(dm, A.dnv, P);
+      undecided = !hit && A.use_cells;
     }
+    // queue the undecided trials of the workgroup
+    unsigned* qn = &q_n[step & 1];
+    const unsigned long long um = __ballot(undecided);
+    if (um) {
+      unsigned b = 0;
+      if (lane == 0) b = atomicAdd(qn, (unsigned)__popcll(um));
+      b = (unsigned)__builtin_amdgcn_readfirstlane((int)b);
+      if (undecided) {
+        const unsigned r = b + (unsigned)__popcll(um & lt);
+        q_p[r] = P;
+        q_t[r] = (unsigned short)tid;
+      }
+    }
+    __syncthreads();
+    if (tid == 0) q_n[(step + 1) & 1] = 0u;
+    const unsigned nq = *qn;
+    for (unsigned q = tid; q < nq; q += kFbWaves * 64)
+      q_hit[q_t[q]] = screen_lane<CUBIC>(A, tab, thr, q_p[q], st) ? 1 : 0;
+    __syncthreads();
+    if (undecided) hit = q_hit[tid] != 0;
     const unsigned long long hits = __ballot(act && hit);
     const unsigned long long alive = __ballot(act && !hit);
     const bool lead = act && trial == 0;
@@ -357,9 +422,9 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_first_bead_kernel(const 
     const unsigned long long lm = __ballot(live);
     if (lm) {
       unsigned b = 0;
-      if (lane == 0) b = atomicAdd(A.live_n, (unsigned)__popcll(lm));
+      if (lane == 0) b = atomicAdd(&live_cnt, (unsigned)__popcll(lm));
       b = (unsigned)__builtin_amdgcn_readfirstlane((int)b);
-      if (live) A.live[b + __popcll(lm & lt)] = ins;
+      if (live) A.live[i0 + b + __popcll(lm & lt)] = ins;
     }
     if (alive) {
       unsigned b = 0;
@@ -376,9 +441,13 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_first_bead_kernel(const 
       }
     }
   }
-  if (lane == 0) atomicAdd(&A.stat[0], st);
+  if (lane == 0) atomicAdd(&wg_st, st);
   __syncthreads();
-  if (threadIdx.x == 0) A.counts1[blockIdx.x] = wg_cnt;
+  if (tid == 0) {
+    A.counts1[blockIdx.x] = wg_cnt;
+    A.live_counts[blockIdx.x] = live_cnt;
+    A.stat_part[blockIdx.x] = wg_st;
+  }
 }
 
 // B / D: full energies of the trial sites of a work list (VDW_Coulomb.cu:
@@ -388,9 +457,12 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_first_bead_kernel(const 
 template <bool MIXED, bool CUBIC, bool FW_LDS>
 __global__ __launch_bounds__(kEvalWaves * 64) void widom_eval_kernel(const WidomArgs A, const TrialPt* list,
                                                                    const unsigned* counts, int groups, int chunk,
-                                                                   int per_ins, double2* energy, uint32_t* mask)
+                                                                   int per_ins, double2* energy, uint32_t* mask,
+                                                                   unsigned long long* stat_slot)
 {
   extern __shared__ __align__(16) char dyn[];
+  __shared__ unsigned long long wg_eval;
+  if (threadIdx.x == 0) wg_eval = 0ull;
   // exclusive prefix sums of the sub-list lengths (groups sub-lists of
   // chunk * per_ins entries each)
   __shared__ unsigned pre[kMaxScreenGroups + 1];
@@ -561,33 +633,44 @@ __global__ __launch_bounds__(kEvalWaves * 64) void widom_eval_kernel(const Widom
       if (ov) atomicOr(mask + P.ins, 1u << P.bit);
     }
   }
-  if (lane == 0 && n_eval) atomicAdd(&A.stat[1], n_eval);
+  if (lane == 0 && n_eval) atomicAdd(&wg_eval, n_eval);
+  __syncthreads();
+  if (threadIdx.x == 0) stat_slot[blockIdx.x] = wg_eval;
 }
 
 // C: first-bead selection; orientations about the selected bead and their
 // overlap screen.  Workgroup g owns insertions [g*chunk, (g+1)*chunk).
-// C: one wave per live insertion: first-bead selection from the evaluated
-// trials, the O orientations of the template about the selected bead (one
-// lane per orientation site), their overlap screen, survivors appended to
-// list 2 (one list, global counter).
+// C: one wave per live insertion (workgroup g takes the live sub-list of
+// first-bead workgroup g): first-bead selection from the evaluated trials,
+// the O orientations of the template about the selected bead (one lane per
+// orientation site), their overlap screen, survivors appended to the
+// workgroup's sub-list of list 2.
 template <bool CUBIC>
 __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const WidomArgs A)
 {
   extern __shared__ __align__(16) char dyn[];
+  __shared__ unsigned wg_cnt;
+  __shared__ unsigned long long wg_st;
   const int T = A.bc.ntypes;
   float* thr = reinterpret_cast<float*>(dyn);
   for (int i = threadIdx.x; i < A.M * T; i += blockDim.x) {
     const int a = i / T, ty = i % T;
     thr[i] = screen_threshold(A.ff64 + 4 * (ty * T + A.site_type[a]), A.bc.overlap, A.bc.cut_vdw2);
   }
+  if (threadIdx.x == 0) {
+    wg_cnt = 0u;
+    wg_st = 0ull;
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int K = A.K, O = A.O, chain = A.chain;
-  const unsigned n_live = *A.live_n;
+  const int i0 = blockIdx.x * A.chunk;
+  const unsigned n_live = A.live_counts[blockIdx.x];
+  TrialPt* sub = A.list2 + (size_t)i0 * O * chain;
   const unsigned long long lt = (1ull << lane) - 1ull;
   unsigned long long st = 0;
-  for (unsigned li = blockIdx.x * kInsWaves + wv; li < n_live; li += gridDim.x * kInsWaves) {
-    const int ins = A.live[li];
+  for (unsigned li = wv; li < n_live; li += kInsWaves) {
+    const int ins = A.live[i0 + li];
     const uint64_t idx = (uint64_t)(A.first_index + ins);
     const uint32_t dead = A.mask1[ins];
     double ev = 0.0, er = 0.0;
@@ -629,7 +712,9 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
         double vec[3] = {A.tvec[a][0], A.tvec[a][1], A.tvec[a][2]};
         quat_rotate(vec, u[0], u[1], u[2]);
         p[0] = S.fb[0] + vec[0]; p[1] = S.fb[1] + vec[1]; p[2] = S.fb[2] + vec[2];
-        if (A.use_cells) hit = screen_lane<CUBIC>(A, A.ctab, thr + (1 + a) * T, to_fixed4(A.bc, p), st);
+        const uint4 P = to_fixed4(A.bc, p);
+        if (A.dmap[1 + a]) hit = dead_voxel(A.dmap[1 + a], A.dnv, P);
+        if (!hit && A.use_cells) hit = screen_lane<CUBIC>(A, A.ctab, thr + (1 + a) * T, P, st);
       }
       const unsigned long long hits = __ballot(act && hit);
       const unsigned long long am = (1ull << chain) - 1ull;
@@ -641,7 +726,7 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
       const unsigned long long al = __ballot(alive);
       if (al) {
         unsigned b = 0;
-        if (lane == 0) b = atomicAdd(A.n2, (unsigned)__popcll(al));
+        if (lane == 0) b = atomicAdd(&wg_cnt, (unsigned)__popcll(al));
         b = (unsigned)__builtin_amdgcn_readfirstlane((int)b);
         if (alive) {
           TrialPt q;
@@ -650,12 +735,17 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
           q.slot = (ins * O + o) * chain + a;
           q.ins = ins;
           q.bit = o;
-          A.list2[b + __popcll(al & lt)] = q;
+          sub[b + __popcll(al & lt)] = q;
         }
       }
     }
   }
-  if (lane == 0) atomicAdd(&A.stat[0], st);
+  if (lane == 0) atomicAdd(&wg_st, st);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    A.counts2[blockIdx.x] = wg_cnt;
+    A.stat_part[kMaxScreenGroups + blockIdx.x] = wg_st;
+  }
 }
 
 // E: orientation selection, Fourier-space energy of the selected molecule
@@ -670,9 +760,9 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_finish_kernel(const Wido
   const size_t wave_bytes = (size_t)M * 3 * KS * sizeof(double2) + kMaxMol * 3 * sizeof(double);
   double2* eik = reinterpret_cast<double2*>(dyn + (size_t)wv * wave_bytes);
   double (*mol)[3] = reinterpret_cast<double (*)[3]>(dyn + (size_t)wv * wave_bytes + (size_t)M * 3 * KS * sizeof(double2));
-  const unsigned n_live = *A.live_n;
-  for (unsigned li = blockIdx.x * kInsWaves + wv; li < n_live; li += gridDim.x * kInsWaves) {
-    const int ins = A.live[li];
+  const unsigned n_live = A.live_counts[blockIdx.x];
+  for (unsigned li = wv; li < n_live; li += kInsWaves) {
+    const int ins = A.live[blockIdx.x * A.chunk + li];
     const uint64_t idx = (uint64_t)(A.first_index + ins);
     const InsState S = A.state[ins];
     bool ok = S.ok != 0;
@@ -780,13 +870,26 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_finish_kernel(const Wido
   }
 }
 
-__global__ void widom_reset_kernel(unsigned long long* stat, unsigned* live_n, unsigned* n2)
+// Work counts: per-workgroup partials summed in a fixed order: screened pairs
+// of the first-bead and chain workgroups ([0, n_screen) and
+// [kMaxScreenGroups, kMaxScreenGroups + n_screen)), then evaluated pairs of
+// the two pair-energy launches (kMaxEvalGroups slots each).
+__global__ void widom_stat_kernel(const unsigned long long* part, int n_screen, int n_eval1, int n_eval2,
+                                  unsigned long long* stat)
 {
+  __shared__ unsigned long long s[2][64];
+  unsigned long long a = 0, e = 0;
+  for (int i = threadIdx.x; i < n_screen; i += 64) a += part[i] + part[kMaxScreenGroups + i];
+  for (int i = threadIdx.x; i < n_eval1; i += 64) e += part[2 * kMaxScreenGroups + i];
+  for (int i = threadIdx.x; i < n_eval2; i += 64) e += part[2 * kMaxScreenGroups + kMaxEvalGroups + i];
+  s[0][threadIdx.x] = a;
+  s[1][threadIdx.x] = e;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    stat[0] = 0ull;
-    stat[1] = 0ull;
-    *live_n = 0u;
-    *n2 = 0u;
+    unsigned long long x = 0, y = 0;
+    for (int i = 0; i < 64; ++i) { x += s[0][i]; y += s[1][i]; }
+    stat[0] = x;
+    stat[1] = y;
   }
 }
 
@@ -886,12 +989,15 @@ static int ensure_pipeline(graspa_box* b, int n_insert, int K, int OC)
   rc = rc ? rc : hip_check(hipMalloc(&b->d_wmask2, cn * sizeof(uint32_t)), "hipMalloc widom pipeline");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_wstate, cn * sizeof(InsState)), "hipMalloc widom pipeline");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_wlive, cn * sizeof(int)), "hipMalloc widom pipeline");
+  if (!rc && !b->d_wstat)
+    rc = hip_check(hipMalloc(&b->d_wstat, (2 * kMaxScreenGroups + 2 * kMaxEvalGroups) * sizeof(unsigned long long)),
+                   "hipMalloc widom pipeline");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_we1, c1 * sizeof(double2)), "hipMalloc widom pipeline");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_wlist1, c1 * sizeof(TrialPt)), "hipMalloc widom pipeline");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_we2, c2 * sizeof(double2)), "hipMalloc widom pipeline");
   rc = rc ? rc : hip_check(hipMalloc(&b->d_wlist2, c2 * sizeof(TrialPt)), "hipMalloc widom pipeline");
   if (!rc && !b->d_wcounts)
-    rc = hip_check(hipMalloc(&b->d_wcounts, (kMaxScreenGroups + 2) * sizeof(unsigned)), "hipMalloc widom pipeline");
+    rc = hip_check(hipMalloc(&b->d_wcounts, 3 * kMaxScreenGroups * sizeof(unsigned)), "hipMalloc widom pipeline");
   if (!rc) {
     b->wp_n = cn;
     b->wp_e1 = c1;
@@ -902,7 +1008,8 @@ static int ensure_pipeline(graspa_box* b, int n_insert, int K, int OC)
 
 template <bool MIXED, bool CUBIC>
 static int launch_eval(graspa_box* b, const WidomArgs& A, const TrialPt* list, const unsigned* counts, int groups,
-                       int chunk, int per_ins, double2* energy, uint32_t* mask)
+                       int chunk, int per_ins, double2* energy, uint32_t* mask, unsigned long long* stat_slot,
+                       int* grid_out)
 {
   const size_t fw_bytes = (size_t)A.n_fw * (sizeof(double4) + sizeof(int));
   for (int pass = 0; pass < 2; ++pass) {
@@ -911,9 +1018,11 @@ static int launch_eval(graspa_box* b, const WidomArgs& A, const TrialPt* list, c
     auto kern = fw_lds ? widom_eval_kernel<MIXED, CUBIC, true> : widom_eval_kernel<MIXED, CUBIC, false>;
     const int occ = occupancy(b, kern, kEvalWaves * 64, lds);
     if (occ < 1) continue;
+    const int grid = std::min(kMaxEvalGroups, occ * b->n_cu);
+    *grid_out = grid;
     timing_begin(b, "widom");
-    hipLaunchKernelGGL(kern, dim3((unsigned)(occ * b->n_cu)), dim3(kEvalWaves * 64), lds, b->stream, A, list, counts,
-                       groups, chunk, per_ins, energy, mask);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kEvalWaves * 64), lds, b->stream, A, list, counts,
+                       groups, chunk, per_ins, energy, mask, stat_slot);
     timing_end(b, "widom");
     return hip_check(hipGetLastError(), "widom_eval_kernel launch");
   }
@@ -924,12 +1033,12 @@ static int launch_eval(graspa_box* b, const WidomArgs& A, const TrialPt* list, c
 template <bool CUBIC>
 static int launch_first_bead(graspa_box* b, const WidomArgs& A, size_t base_lds, size_t tab_bytes)
 {
-  if (A.use_cells && base_lds + tab_bytes <= (size_t)b->max_lds) {
-    const int rc = launch_ins(b, widom_first_bead_kernel<CUBIC, true>, A, base_lds + tab_bytes,
-                              "widom_first_bead_kernel", A.groups);
-    if (rc != GRASPA_ERR_UNSUPPORTED) return rc;
-  }
-  return launch_ins(b, widom_first_bead_kernel<CUBIC, false>, A, base_lds, "widom_first_bead_kernel", A.groups);
+  const bool tab_lds = A.use_cells && base_lds + tab_bytes + 20 * 1024 <= (size_t)b->max_lds;
+  auto kern = tab_lds ? widom_first_bead_kernel<CUBIC, true> : widom_first_bead_kernel<CUBIC, false>;
+  const size_t lds = tab_lds ? base_lds + tab_bytes : base_lds;
+  if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(kern, dim3((unsigned)A.groups), dim3(kFbWaves * 64), lds, b->stream, A);
+  return hip_check(hipGetLastError(), "widom_first_bead_kernel launch");
 }
 
 int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
@@ -948,7 +1057,14 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   }
   if (ensure_widom_records(b, n_insert)) return GRASPA_ERR_HIP;
   const int chain = c.molsize - 1;
-  if (ensure_pipeline(b, n_insert, K, O * chain)) return GRASPA_ERR_HIP;
+  // screening grid: up to kMaxScreenGroups workgroups, each owning a contiguous range of insertions
+  // (four wave steps of kFbWaves x 64 / K insertions each); the work lists and the
+  // live list have a sub-list of chunk insertions per workgroup
+  const long long per_group = 4LL * kFbWaves * (64 / K);
+  const int groups = (int)std::min<long long>(kMaxScreenGroups,
+                                              std::max<long long>(1, ((long long)n_insert + per_group - 1) / per_group));
+  const int chunk = (n_insert + groups - 1) / groups;
+  if (ensure_pipeline(b, groups * chunk, K, O * chain)) return GRASPA_ERR_HIP;
   WidomArgs A{};
   A.bc = b->bc;
   A.fw = b->d_fws; A.fw_type = b->d_fws_type; A.fw_fixT = b->d_fw_fixT; A.n_fw = b->n_fw;
@@ -989,12 +1105,15 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   A.list2 = static_cast<TrialPt*>(b->d_wlist2);
   A.state = static_cast<InsState*>(b->d_wstate);
   A.counts1 = static_cast<unsigned*>(b->d_wcounts);
-  A.live_n = A.counts1 + kMaxScreenGroups;
-  A.n2 = A.live_n + 1;
+  A.counts2 = A.counts1 + kMaxScreenGroups;
+  A.live_counts = A.counts2 + kMaxScreenGroups;
   A.live = static_cast<int*>(b->d_wlive);
-  // screening grid: up to kMaxScreenGroups workgroups, each owning a contiguous range of insertions
-  A.groups = (int)std::min<long long>(kMaxScreenGroups, std::max<long long>(1, ((long long)n_insert + 31) / 32));
-  A.chunk = (n_insert + A.groups - 1) / A.groups;
+  A.stat_part = static_cast<unsigned long long*>(b->d_wstat);
+  for (int a = 0; a < c.molsize; ++a)
+    if (int rc = ensure_dead_map(b, c.type[a], &A.dmap[a])) return rc;
+  for (int d = 0; d < 3; ++d) A.dnv[d] = b->dmap_nv[d];
+  A.groups = groups;
+  A.chunk = chunk;
   A.res = d_res;
   A.rec_w = b->d_wrec_w; A.rec_e = b->d_wrec_e; A.rec_f = b->d_wrec_f;
   A.stat = reinterpret_cast<unsigned long long*>(b->d_wacc + 8);
@@ -1013,27 +1132,28 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   A.lds_fw = (int)off;
   const bool mixed = b->precision == GRASPA_MIXED, cubic = A.bc.cubic != 0;
 
-  hipLaunchKernelGGL(widom_reset_kernel, dim3(1), dim3(64), 0, b->stream, A.stat, A.live_n, A.n2);
-  if (hip_check(hipGetLastError(), "widom_reset launch")) return GRASPA_ERR_HIP;
   int rc = cubic ? launch_first_bead<true>(b, A, scr_base, tab_bytes) : launch_first_bead<false>(b, A, scr_base, tab_bytes);
   if (rc) return rc;
-  auto eval = [&](const TrialPt* list, const unsigned* counts, int groups, int chunk, int per_ins, double2* energy,
-                  uint32_t* mask) {
-    if (mixed) return cubic ? launch_eval<true, true>(b, A, list, counts, groups, chunk, per_ins, energy, mask)
-                            : launch_eval<true, false>(b, A, list, counts, groups, chunk, per_ins, energy, mask);
-    return cubic ? launch_eval<false, true>(b, A, list, counts, groups, chunk, per_ins, energy, mask)
-                 : launch_eval<false, false>(b, A, list, counts, groups, chunk, per_ins, energy, mask);
+  int egrid[2] = {0, 0};
+  auto eval = [&](const TrialPt* list, const unsigned* counts, int per_ins, double2* energy, uint32_t* mask, int e) {
+    unsigned long long* slot = A.stat_part + 2 * kMaxScreenGroups + (size_t)e * kMaxEvalGroups;
+    int* g = &egrid[e];
+    if (mixed) return cubic ? launch_eval<true, true>(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, g)
+                            : launch_eval<true, false>(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, g);
+    return cubic ? launch_eval<false, true>(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, g)
+                 : launch_eval<false, false>(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, g);
   };
-  rc = eval(A.list1, A.counts1, A.groups, A.chunk, K, A.e1, A.mask1);
+  rc = eval(A.list1, A.counts1, K, A.e1, A.mask1, 0);
   if (rc) return rc;
-  rc = launch_ins(b, cubic ? widom_chain_kernel<true> : widom_chain_kernel<false>, A, scr_base, "widom_chain_kernel");
+  rc = launch_ins(b, cubic ? widom_chain_kernel<true> : widom_chain_kernel<false>, A, scr_base, "widom_chain_kernel",
+                  A.groups);
   if (rc) return rc;
   if (chain > 0) {
-    rc = eval(A.list2, A.n2, 1, 0, O * chain, A.e2, A.mask2);
+    rc = eval(A.list2, A.counts2, O * chain, A.e2, A.mask2, 1);
     if (rc) return rc;
   }
   const size_t fin_lds = (size_t)kInsWaves * ((size_t)M * 3 * A.kstride * sizeof(double2) + kMaxMol * 3 * sizeof(double));
-  rc = launch_ins(b, widom_finish_kernel, A, fin_lds, "widom_finish_kernel");
+  rc = launch_ins(b, widom_finish_kernel, A, fin_lds, "widom_finish_kernel", A.groups);
   if (rc) return rc;
   const int nblk = (n_insert + kRedBlock * kRedPer - 1) / (kRedBlock * kRedPer);
   hipLaunchKernelGGL(widom_reduce_kernel, dim3(nblk), dim3(kRedBlock), 0, b->stream, A.rec_w, A.rec_e, A.rec_f,
@@ -1041,6 +1161,9 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   if (hip_check(hipGetLastError(), "widom_reduce launch")) return GRASPA_ERR_HIP;
   hipLaunchKernelGGL(widom_final_kernel, dim3(1), dim3(kRedBlock), 0, b->stream, b->d_wpart, nblk, b->d_wacc);
   if (hip_check(hipGetLastError(), "widom_final launch")) return GRASPA_ERR_HIP;
+  hipLaunchKernelGGL(widom_stat_kernel, dim3(1), dim3(64), 0, b->stream, A.stat_part, A.groups, egrid[0], egrid[1],
+                     A.stat);
+  if (hip_check(hipGetLastError(), "widom_stat launch")) return GRASPA_ERR_HIP;
   if (summary) {
     double acc[12];
     if (hip_check(hipMemcpyAsync(acc, b->d_wacc, 12 * sizeof(double), hipMemcpyDeviceToHost, b->stream),

@@ -131,6 +131,8 @@ _SIGS = {
     "graspa_box_sync": (C.c_int, [C.c_void_p]),
     "graspa_glibc_rand_sequence": (C.c_int, [C.c_uint, C.c_int, _i32p]),
     "graspa_coulomb_table": (C.c_int, [C.c_double, C.c_double, _f64p, C.c_int]),
+    "graspa_dead_map": (C.c_int, [C.POINTER(BoxParams), C.c_int, _f64p, _i32p, C.c_int, C.POINTER(C.c_uint32),
+                                  C.c_int, _i32p]),
     "graspa_batch_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "graspa_batch_destroy": (None, [C.c_void_p]),
     "graspa_batch_run": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _i64p]),
@@ -170,6 +172,52 @@ def _check(rc, what):
 
 def _p(a, t=_f64p):
     return a.ctypes.data_as(t)
+
+
+def box_params(system, precision=GRASPA_FP64, device=0, keep=None):
+    """graspa_box_params of a System; `keep` (a list) holds the arrays the
+    struct points into."""
+    s = system
+    T = s.ntypes
+    eps = np.ascontiguousarray(s.eps, dtype=np.float64).reshape(T * T)
+    sig = np.ascontiguousarray(s.sig, dtype=np.float64).reshape(T * T)
+    sh = np.ascontiguousarray(s.shift, dtype=np.float64).reshape(T * T)
+    if keep is not None:
+        keep += [eps, sig, sh]
+    p = BoxParams()
+    for i in range(9):
+        p.cell[i] = float(s.cell[i])
+    p.ntypes = T
+    p.epsilon, p.sigma, p.shift = _p(eps), _p(sig), _p(sh)
+    p.cutoff_vdw, p.cutoff_coulomb, p.overlap_criteria = s.cut_vdw, s.cut_coul, s.overlap
+    p.use_charges = 0 if s.no_charges else 1
+    p.alpha = s.alpha
+    for i in range(3):
+        p.kmax[i] = int(s.kmax[i])
+    p.reciprocal_cutoff = s.recip_cut
+    p.lammps_ewald = int(s.lammps_ewald)
+    p.prefactor = s.prefactor
+    p.precision = precision
+    p.device = device
+    return p
+
+
+def dead_map(system, site_type):
+    """The certain-overlap voxel map of the system's framework for probe sites
+    of site_type (host computation): (bits as a bool array [nz, ny, nx], nv)."""
+    keep = []
+    p = box_params(system, keep=keep)
+    xyz = np.ascontiguousarray(system.fw_pos, dtype=np.float64).reshape(-1)
+    ty = np.ascontiguousarray(system.fw_type, dtype=np.int32)
+    nv = np.zeros(3, dtype=np.int32)
+    n = lib().graspa_dead_map(C.byref(p), len(ty), _p(xyz), _p(ty, _i32p), site_type, None, 0, _p(nv, _i32p))
+    if n < 0:
+        raise EngineError("graspa_dead_map failed")
+    w = np.zeros(n, dtype=np.uint32)
+    lib().graspa_dead_map(C.byref(p), len(ty), _p(xyz), _p(ty, _i32p), site_type,
+                          w.ctypes.data_as(C.POINTER(C.c_uint32)), n, _p(nv, _i32p))
+    bits = np.unpackbits(w.view(np.uint8), bitorder="little")[: int(nv[0]) * int(nv[1]) * int(nv[2])]
+    return bits.reshape(int(nv[2]), int(nv[1]), int(nv[0])).astype(bool), tuple(int(x) for x in nv)
 
 
 class Box:

@@ -373,6 +373,16 @@ int  graspa_glibc_rand_sequence(unsigned seed, int n, int* out);
  * written, the number needed when cap is too small, or -1 for bad arguments. */
 int  graspa_coulomb_table(double alpha, double cut2, double* out, int cap);
 
+/* Host helper with no counterpart in the reference: the map of the
+ * framework's certain-overlap volume for probe sites of type site_type that
+ * the overlap screens use (voxels of about 0.15 A, nv[3] per cell axis; bit
+ * x + nv0 (y + nv1 z) set when every point of the voxel has a framework pair
+ * above OverlapCriteria or closer than 0.1 A).  Uses params' cell, types,
+ * force field, VDW cutoff and OverlapCriteria.  Returns the number of 32-bit
+ * words written, the number needed when cap is too small, or -1. */
+int  graspa_dead_map(const graspa_box_params* params, int n_fw, const double* xyz, const int* type, int site_type,
+                     uint32_t* bits, int cap, int* nv);
+
 #ifdef __cplusplus
 }
 #endif
