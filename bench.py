@@ -346,6 +346,14 @@ def main():
         line["gcmc_config2_butane_mixed"] = run_batch_single(synth.config2(), 200, engine.GRASPA_MIXED,
                                                              warm_cycles=100)
         line["served_stream"] = run_served(48, 20.0, 64, engine.GRASPA_MIXED)
+        # the reference's form of the same Widom workload: one insertion at a
+        # time, host-driven, plain FP64 (one insertion per cycle, as the
+        # Henry-coefficient decks run it)
+        wp = run_gcmc(synth.config1(), 2000)
+        line["widom_plain_single_box_fp64"] = dict(insertions=wp["moves"], seconds=wp["seconds"],
+                                                   insertions_per_s=wp["moves_per_s"],
+                                                   cycle_p50_ms=wp["cycle_p50_ms"], cycle_p99_ms=wp["cycle_p99_ms"])
+        line["widom_batched_over_plain_fp64"] = value / wp["moves_per_s"]
         if "gcmc_batched" in line:
             line["gcmc_batched"]["over_single_box_fp64"] = line["gcmc_batched"]["moves_per_s"] / g["moves_per_s"]
         line["cpu_baseline"] = cpu_baseline(s1, args.cpu_sample)

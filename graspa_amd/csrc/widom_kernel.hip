@@ -48,7 +48,8 @@ constexpr int kMaxPts = 64;      // trial sites of one screen: K, or O x (M-1)
 constexpr int kSums = 6;         // W, W^2, W dE, n, n first bead placed, n complete
 constexpr int kRedBlock = 256;
 constexpr int kRedPer = 8;       // records per thread in the first reduction pass
-constexpr int kInsWaves = 16;    // waves per workgroup of the per-insertion kernels (one live insertion each)
+constexpr int kInsWaves = 16;    // waves per workgroup of the chain kernel (one live insertion each)
+constexpr int kFinWaves = 8;     // waves per workgroup of the finish kernel (256 VGPRs per lane available)
 constexpr int kEvalWaves = 16;   // waves per workgroup of the pair-energy kernel
 constexpr int kFbWaves = 16;     // waves per workgroup of the first-bead kernel
 
@@ -56,6 +57,7 @@ constexpr int kFbWaves = 16;     // waves per workgroup of the first-bead kernel
 // of a grid of G owns insertions [g*chunk, (g+1)*chunk) and the sub-list at
 // g*chunk*cap_per_insertion; its length goes to counts[g].
 constexpr int kMaxScreenGroups = 1024;
+constexpr int kChainScreenMinAtoms = 4096;   // frameworks from this size screen orientation sites
 constexpr int kMaxEvalGroups = 4096;   // workgroups of one pair-energy launch (work-count slots)
 
 // State of one insertion between the kernels.
@@ -82,6 +84,7 @@ struct WidomArgs {
   const uint4* fw_fixT;    // fixed-point x,y,z + type (scan without a cell grid)
   int n_fw;
   int use_cells;
+  int screen_chain;             // cell screen of the orientation sites (large frameworks only)
   int cell_n[3];
   int cell_cap;
   const uint4* ctab;       // [cells * cell_cap] fixed-point x,y,z + type, ~0u = empty
@@ -714,7 +717,7 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
         p[0] = S.fb[0] + vec[0]; p[1] = S.fb[1] + vec[1]; p[2] = S.fb[2] + vec[2];
         const uint4 P = to_fixed4(A.bc, p);
         if (A.dmap[1 + a]) hit = dead_voxel(A.dmap[1 + a], A.dnv, P);
-        if (!hit && A.use_cells) hit = screen_lane<CUBIC>(A, A.ctab, thr + (1 + a) * T, P, st);
+        if (!hit && A.screen_chain) hit = screen_lane<CUBIC>(A, A.ctab, thr + (1 + a) * T, P, st);
       }
       const unsigned long long hits = __ballot(act && hit);
       const unsigned long long am = (1ull << chain) - 1ull;
@@ -751,7 +754,7 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
 // E: orientation selection, Fourier-space energy of the selected molecule
 // (Fourier_Ewald_Diff, Ewald_Energy_Functions.h:280-397, with Oldsize = 0)
 // and the record: W = W_fb W_ch exp(-beta (dE_ewald + dE_tail)).
-__global__ __launch_bounds__(kInsWaves * 64) void widom_finish_kernel(const WidomArgs A)
+__global__ __launch_bounds__(kFinWaves * 64) void widom_finish_kernel(const WidomArgs A)
 {
   extern __shared__ __align__(16) char dyn[];
   const BoxConst& bc = A.bc;
@@ -761,14 +764,15 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_finish_kernel(const Wido
   double2* eik = reinterpret_cast<double2*>(dyn + (size_t)wv * wave_bytes);
   double (*mol)[3] = reinterpret_cast<double (*)[3]>(dyn + (size_t)wv * wave_bytes + (size_t)M * 3 * KS * sizeof(double2));
   const unsigned n_live = A.live_counts[blockIdx.x];
-  for (unsigned li = wv; li < n_live; li += kInsWaves) {
+  for (unsigned li = wv; li < n_live; li += kFinWaves) {
     const int ins = A.live[blockIdx.x * A.chunk + li];
     const uint64_t idx = (uint64_t)(A.first_index + ins);
     const InsState S = A.state[ins];
     bool ok = S.ok != 0;
     double Wtot = S.wfb, e_v = S.ev, e_r = S.er;
     int orient = 0;
-    if (lane < 3) mol[0][lane] = S.fb[lane];
+    // selects, not an index into the loaded struct (which would place it in scratch)
+    if (lane < 3) mol[0][lane] = lane == 0 ? S.fb[0] : (lane == 1 ? S.fb[1] : S.fb[2]);
     if (ok && chain > 0) {
       const uint32_t dead2 = A.mask2[ins];
       double cv = 0.0, cr = 0.0;
@@ -962,13 +966,14 @@ static int occupancy(graspa_box* b, Kern kern, int threads, size_t lds)
 
 // Launches a per-insertion kernel on a grid of resident workgroups.
 template <typename Kern>
-static int launch_ins(graspa_box* b, Kern kern, const WidomArgs& A, size_t lds, const char* what, int fixed_grid = 0)
+static int launch_ins(graspa_box* b, Kern kern, const WidomArgs& A, size_t lds, const char* what, int waves,
+                      int fixed_grid = 0)
 {
-  const int occ = occupancy(b, kern, kInsWaves * 64, lds);
+  const int occ = occupancy(b, kern, waves * 64, lds);
   if (occ < 1) return fail(GRASPA_ERR_UNSUPPORTED, std::string("widom: no launch shape fits the LDS (") + what + ")");
-  const long long need = ((long long)A.n_insert + kInsWaves - 1) / kInsWaves;
+  const long long need = ((long long)A.n_insert + waves - 1) / waves;
   const long long grid = fixed_grid > 0 ? fixed_grid : std::max(1LL, std::min(need, (long long)occ * b->n_cu));
-  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kInsWaves * 64), lds, b->stream, A);
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(waves * 64), lds, b->stream, A);
   return hip_check(hipGetLastError(), what);
 }
 
@@ -1060,15 +1065,26 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   // screening grid: up to kMaxScreenGroups workgroups, each owning a contiguous range of insertions
   // (four wave steps of kFbWaves x 64 / K insertions each); the work lists and the
   // live list have a sub-list of chunk insertions per workgroup
+  // The first-bead, chain and finish kernels hold one workgroup per CU, so
+  // the group count is rounded down to a multiple of the CU count: a few
+  // groups beyond it would take a second round of the whole kernel.
   const long long per_group = 4LL * kFbWaves * (64 / K);
-  const int groups = (int)std::min<long long>(kMaxScreenGroups,
-                                              std::max<long long>(1, ((long long)n_insert + per_group - 1) / per_group));
+  long long raw = std::max<long long>(1, ((long long)n_insert + per_group - 1) / per_group);
+  const long long ncu = std::max(1, b->n_cu);
+  if (raw > ncu) raw = raw / ncu * ncu;
+  const long long cap = kMaxScreenGroups >= ncu ? kMaxScreenGroups / ncu * ncu : kMaxScreenGroups;
+  const int groups = (int)std::min<long long>(raw, cap);
   const int chunk = (n_insert + groups - 1) / groups;
   if (ensure_pipeline(b, groups * chunk, K, O * chain)) return GRASPA_ERR_HIP;
   WidomArgs A{};
   A.bc = b->bc;
   A.fw = b->d_fws; A.fw_type = b->d_fws_type; A.fw_fixT = b->d_fw_fixT; A.n_fw = b->n_fw;
   A.use_cells = b->use_cells;
+  // Orientation sites: the cell screen costs about three times what it saves
+  // in the pair kernel at 1,536 framework atoms (35 us of screening against
+  // 12 us of pair energies per 100,000 insertions); the pair kernel's cost
+  // grows with the framework and the screen's does not.
+  A.screen_chain = b->use_cells && b->n_fw >= kChainScreenMinAtoms;
   for (int d = 0; d < 3; ++d) A.cell_n[d] = b->cell_n[d];
   A.cell_cap = b->cell_cap;
   A.ctab = b->d_ctab;
@@ -1146,14 +1162,14 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   rc = eval(A.list1, A.counts1, K, A.e1, A.mask1, 0);
   if (rc) return rc;
   rc = launch_ins(b, cubic ? widom_chain_kernel<true> : widom_chain_kernel<false>, A, scr_base, "widom_chain_kernel",
-                  A.groups);
+                  kInsWaves, A.groups);
   if (rc) return rc;
   if (chain > 0) {
     rc = eval(A.list2, A.counts2, O * chain, A.e2, A.mask2, 1);
     if (rc) return rc;
   }
-  const size_t fin_lds = (size_t)kInsWaves * ((size_t)M * 3 * A.kstride * sizeof(double2) + kMaxMol * 3 * sizeof(double));
-  rc = launch_ins(b, widom_finish_kernel, A, fin_lds, "widom_finish_kernel", A.groups);
+  const size_t fin_lds = (size_t)kFinWaves * ((size_t)M * 3 * A.kstride * sizeof(double2) + kMaxMol * 3 * sizeof(double));
+  rc = launch_ins(b, widom_finish_kernel, A, fin_lds, "widom_finish_kernel", kFinWaves, A.groups);
   if (rc) return rc;
   const int nblk = (n_insert + kRedBlock * kRedPer - 1) / (kRedBlock * kRedPer);
   hipLaunchKernelGGL(widom_reduce_kernel, dim3(nblk), dim3(kRedBlock), 0, b->stream, A.rec_w, A.rec_e, A.rec_f,
