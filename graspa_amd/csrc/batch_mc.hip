@@ -106,6 +106,7 @@ struct BBox {
   uint32_t stream, _pad2;
   graspa_trace* trace;
   float* cyc;                       // per-cycle ticks (ring of cyc_cap)
+  graspa_block_stats* blk;          // block statistics of the production run
 };
 
 // Mutable state of one box; held in LDS during a launch.
@@ -126,8 +127,12 @@ struct BState {
   uint64_t cycle_t0;
   int64_t trace_n;
   int64_t cyc_first;                // first cycle of the current run
+  int64_t prod_first, blk_size;     // first cycle and block length of the production run
 };
 static_assert(sizeof(BState) % 8 == 0, "BState is copied as 64-bit words");
+static_assert(kMaxComps == GRASPA_MAX_COMPS, "block statistics are sized for kMaxComps components");
+static_assert(kBT >= 10 + 3 * (kMaxComps - 1) + (kMaxComps - 1) * (kMaxComps - 1) + 1,
+              "one thread per block sum of a cycle");
 static_assert(sizeof(BBox) % 8 == 0, "BBox is copied as 64-bit words");
 
 struct BSite {
@@ -232,6 +237,48 @@ __device__ inline void pair_site(const BoxConst& bc, const double* __restrict__ 
 // The sites form G groups of at most kBGroup; the 8 waves are dealt out to
 // the groups (kBW / G waves each), so all groups run in one pass over the
 // atoms with every lane busy, and one reduction finishes the pass.
+// Block of the current cycle in the production run (Gather_Averages_Types,
+// print_statistics.cuh:646-655: cycle / block size; the remainder goes to the
+// last block).
+__device__ inline int block_of(const BState& S)
+{
+  const int64_t b = (S.cycle - S.prod_first) / S.blk_size;
+  return (int)(b < GRASPA_NBLOCK - 1 ? b : GRASPA_NBLOCK - 1);
+}
+
+// Block sums of one production cycle (axpy.cu:420-453): the running energy
+// change per term, and per adsorbate N, N^2, N times the running change
+// without its host-host terms, N_c N_d, and the cycle count; one thread per
+// sum, each sum a single read-modify-write.
+__device__ void block_sums(const BBox& B, const BState& S, const int bk, const int tid)
+{
+  graspa_block_stats& st = *B.blk;
+  const int na = B.n_comp - 1;
+  int t = tid;
+  if (t < 10) { st.e[bk][t] += S.running[t]; return; }
+  t -= 10;
+  if (t < 3 * na) {
+    const int c = 1 + t / 3, k = t % 3;
+    const double nm = (double)S.n_mol[c];
+    if (k == 0) st.n[bk][c] += nm;
+    else if (k == 1) st.n2[bk][c] += nm * nm;
+    else {
+      const double ads = S.running[1] + S.running[2] + S.running[4] + S.running[5] + S.running[7] +
+                         S.running[8] + S.running[9];
+      st.en[bk][c] += ads * nm;
+    }
+    return;
+  }
+  t -= 3 * na;
+  if (t < na * na) {
+    const int c = 1 + t / na, d = 1 + t % na;
+    st.nn[bk][c][d] += (double)S.n_mol[c] * (double)S.n_mol[d];
+    return;
+  }
+  t -= na * na;
+  if (t == 0) st.cycles[bk] += 1;
+}
+
 template <bool MIXED, bool CUBIC>
 __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double* __restrict__ ffl,
                           const double2* __restrict__ ct, const int nint, const double* __restrict__ fwl,
@@ -769,7 +816,13 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
     }
     L.W = W;
     if (L.type == kMvWidom) {
-      if (S.production) { S.widom_w[comp] += W; S.widom_n[comp] += 1.0; }
+      if (S.production) {
+        S.widom_w[comp] += W;
+        S.widom_n[comp] += 1.0;
+        const int bk = block_of(S);
+        B.blk->widom_w[bk][comp] += W;
+        B.blk->widom_n[bk][comp] += 1.0;
+      }
     } else {
       double pacc = del ? pref * sp.ideal_rosen / W : pref * W / sp.ideal_rosen;
       if (!del && W <= 1e-150) pacc = 0.0;
@@ -845,6 +898,8 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
   __shared__ BState S;
   __shared__ BMove L;
   __shared__ BBox B;
+  __shared__ int cyc_block;          // block of a production cycle that just ended, else -1
+  if (threadIdx.x == 0) cyc_block = -1;
   double* dyn = reinterpret_cast<double*>(dyn4);
   const int tid = threadIdx.x;
   {
@@ -950,7 +1005,10 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
     __syncthreads();
     if (tid == 0) {
       const int comp = L.comp;
-      if (L.fail && L.type == kMvWidom && S.production) S.widom_n[comp] += 1.0;
+      if (L.fail && L.type == kMvWidom && S.production) {
+        S.widom_n[comp] += 1.0;
+        B.blk->widom_n[block_of(S)][comp] += 1.0;
+      }
       if (B.trace && S.trace_n < B.trace_cap) {
         graspa_trace& tr = B.trace[(size_t)blockIdx.x * B.trace_cap + S.trace_n];
         tr.cycle = (int)S.cycle; tr.step = S.step; tr.move = L.tmove; tr.comp = comp; tr.mol = L.mol;
@@ -973,6 +1031,7 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
             S.load_sum2[c] += nm * nm;
           }
           S.load_n++;
+          cyc_block = block_of(S);     // block sums by all threads after the barrier
         }
         if (S.cycle % 500 == 0) {
           for (int c = 1; c < B.n_comp; ++c) {
@@ -986,6 +1045,11 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
       }
     }
     __syncthreads();
+    if (cyc_block >= 0) {             // set by thread 0 before the barrier: the same for all threads
+      block_sums(B, S, cyc_block, tid);
+      __syncthreads();
+      if (tid == 0) cyc_block = -1;
+    }
   }
   {
     const uint64_t* src = reinterpret_cast<const uint64_t*>(&S);
@@ -1016,6 +1080,7 @@ struct graspa_batch {
   int trace_cap = 0;
   float* d_cyc = nullptr;
   int cyc_cap = 0;
+  graspa_block_stats* d_blk = nullptr;   // per slot
   size_t lds = 0;
   int max_lds = 65536;
   double tick_hz = 1.0e8;
@@ -1050,7 +1115,7 @@ void free_batch(graspa_batch* bt)
   hipSetDevice(bt->device);
   if (bt->stream) hipStreamSynchronize(bt->stream);
   hipFree(bt->d_box); hipFree(bt->d_state); hipFree(bt->d_tail); hipFree(bt->d_use_tail);
-  hipFree(bt->d_trace); hipFree(bt->d_cyc);
+  hipFree(bt->d_trace); hipFree(bt->d_cyc); hipFree(bt->d_blk);
   for (double2* p : bt->skc) hipFree(p);
   if (bt->stream) hipStreamDestroy(bt->stream);
   delete bt;
@@ -1161,6 +1226,7 @@ int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
   h.cyc_cap = bt->cyc_cap;
   h.trace = bt->d_trace;
   h.cyc = bt->d_cyc ? bt->d_cyc + (size_t)i * bt->cyc_cap : nullptr;
+  h.blk = bt->d_blk + i;
   h.tail = bt->d_tail + (size_t)i * kMaxTypes * kMaxTypes;
   h.use_tail = bt->d_use_tail + (size_t)i * kMaxTypes * kMaxTypes;
   if (p->has_tail) {
@@ -1183,6 +1249,8 @@ int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
     }
   }
   for (int t = 0; t < 16; ++t) st.n_pseudo[t] = p->n_pseudo[t];
+  st.blk_size = 1;
+  if (hip_check(hipMemset(bt->d_blk + i, 0, sizeof(graspa_block_stats)), "block statistics")) return GRASPA_ERR_HIP;
   hipSetDevice(b->device);
   if (hip_check(hipStreamSynchronize(b->stream), "box stream")) return GRASPA_ERR_HIP;
   // both S(k) buffers start equal, so entries a move never writes (inactive
@@ -1289,6 +1357,8 @@ int graspa_batch_create(const graspa_batch_box* boxes, int n_boxes, int trace_ca
     rc = hip_check(hipMalloc(&bt->d_trace, (size_t)n_boxes * trace_cap * sizeof(graspa_trace)), "hipMalloc batch");
   if (!rc && cycle_cap > 0)
     rc = hip_check(hipMalloc(&bt->d_cyc, (size_t)n_boxes * cycle_cap * sizeof(float)), "hipMalloc batch");
+  rc = rc ? rc : hip_check(hipMalloc(&bt->d_blk, n_boxes * sizeof(graspa_block_stats)), "hipMalloc batch");
+  rc = rc ? rc : hip_check(hipMemset(bt->d_blk, 0, n_boxes * sizeof(graspa_block_stats)), "block statistics");
   rc = rc ? rc : hip_check(hipMalloc(&bt->d_box, n_boxes * sizeof(BBox)), "hipMalloc batch");
   rc = rc ? rc : hip_check(hipMalloc(&bt->d_state, n_boxes * sizeof(BState)), "hipMalloc batch");
   for (int i = 0; boxes && i < n_boxes && !rc; ++i)
@@ -1321,6 +1391,16 @@ int graspa_batch_set_box(graspa_batch* bt, int slot, const graspa_batch_box* box
   return upload_batch(bt);
 }
 
+// A production run starts: blocks of n_cycles / GRASPA_NBLOCK cycles
+// (axpy.cu:500-502), statistics cleared.
+static int start_production(graspa_batch* bt, int slot, int n_cycles)
+{
+  BState& s = bt->hs[slot];
+  s.prod_first = s.cycle;
+  s.blk_size = std::max(1, n_cycles / GRASPA_NBLOCK);
+  return hip_check(hipMemset(bt->d_blk + slot, 0, sizeof(graspa_block_stats)), "block statistics");
+}
+
 int graspa_batch_set_cycles(graspa_batch* bt, int slot, int n_cycles, int production)
 {
   if (!bt || slot < 0 || slot >= bt->n || n_cycles < 0) return fail(GRASPA_ERR_ARG, "bad arguments");
@@ -1332,6 +1412,8 @@ int graspa_batch_set_cycles(graspa_batch* bt, int slot, int n_cycles, int produc
   s.production = production ? 1 : 0;
   s.trace_n = 0;
   s.cyc_first = s.cycle;
+  if (production)
+    if (int rc = start_production(bt, slot, n_cycles)) return rc;
   return hip_check(hipMemcpy(&bt->d_state[slot], &s, sizeof(BState), hipMemcpyHostToDevice), "batch state upload");
 }
 
@@ -1368,6 +1450,8 @@ int graspa_batch_run(graspa_batch* bt, int n_cycles, int production, int moves_p
     s.production = production ? 1 : 0;
     s.trace_n = 0;
     s.cyc_first = s.cycle;
+    if (production)
+      if (int rc = start_production(bt, i, n_cycles)) return rc;
     before += s.move;
   }
   if (int rc = hip_check(hipMemcpyAsync(bt->d_state, bt->hs.data(), bt->n * sizeof(BState), hipMemcpyHostToDevice,
@@ -1397,6 +1481,18 @@ int graspa_batch_run(graspa_batch* bt, int n_cycles, int production, int moves_p
   int64_t after = 0;
   for (int i = 0; i < bt->n; ++i) after += bt->hs[i].move;
   if (moves) *moves = after - before;
+  return GRASPA_OK;
+}
+
+int graspa_batch_block_stats(graspa_batch* bt, int i, graspa_block_stats* out)
+{
+  if (!bt || i < 0 || i >= bt->n || !out) return fail(GRASPA_ERR_ARG, "bad arguments");
+  if (!bt->boxes[i]) return fail(GRASPA_ERR_STATE, "empty slot");
+  hipSetDevice(bt->device);
+  if (int rc = download_states(bt)) return rc;
+  if (hip_check(hipMemcpy(out, bt->d_blk + i, sizeof(graspa_block_stats), hipMemcpyDeviceToHost), "block statistics"))
+    return GRASPA_ERR_HIP;
+  out->block_size = bt->hs[i].blk_size;
   return GRASPA_OK;
 }
 

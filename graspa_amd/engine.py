@@ -89,6 +89,25 @@ class Trace(C.Structure):
                 ("rosen", C.c_double), ("de", Energy)]
 
 
+NBLOCK, MAX_COMPS = 5, 8
+
+
+class BlockStats(C.Structure):
+    """graspa_block_stats: per-block sums of the current production run."""
+    _fields_ = [("block_size", C.c_int64), ("cycles", C.c_int64 * NBLOCK),
+                ("n", (C.c_double * MAX_COMPS) * NBLOCK), ("n2", (C.c_double * MAX_COMPS) * NBLOCK),
+                ("nn", ((C.c_double * MAX_COMPS) * MAX_COMPS) * NBLOCK), ("en", (C.c_double * MAX_COMPS) * NBLOCK),
+                ("e", (C.c_double * 10) * NBLOCK), ("widom_w", (C.c_double * MAX_COMPS) * NBLOCK),
+                ("widom_n", (C.c_double * MAX_COMPS) * NBLOCK)]
+
+    def arrays(self):
+        """numpy views: dict of arrays indexed [block][component ...]."""
+        out = {"block_size": int(self.block_size), "cycles": np.ctypeslib.as_array(self.cycles).copy()}
+        for k in ("n", "n2", "nn", "en", "e", "widom_w", "widom_n"):
+            out[k] = np.ctypeslib.as_array(getattr(self, k)).copy()
+        return out
+
+
 _lib = None
 
 # name -> (restype, argtypes)
@@ -142,6 +161,7 @@ _SIGS = {
     "graspa_batch_trace": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "graspa_batch_kernel_time": (C.c_int, [C.c_void_p, _f64p, _i64p]),
     "graspa_batch_sync": (C.c_int, [C.c_void_p]),
+    "graspa_batch_block_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(BlockStats)]),
     "graspa_batch_set_box": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "graspa_batch_set_cycles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "graspa_batch_launch": (C.c_int, [C.c_void_p, C.c_int, _i32p]),
@@ -611,6 +631,12 @@ class Batch:
         _check(lib().graspa_batch_box_stats(self.h, i, C.byref(e), _p(nm, _i32p), _p(cnt, _i64p), _p(ww), _p(wn),
                                             _p(ls), _p(ls2), C.byref(nl), C.byref(mv), C.byref(cy)), "batch_stats")
         return BatchStats(e, nm, cnt.reshape(ns, 9), ww, wn, ls, ls2, nl.value, mv.value, cy.value)
+
+    def block_stats(self, i):
+        """Per-block sums of slot i's current production run (graspa_batch_block_stats)."""
+        b = BlockStats()
+        _check(lib().graspa_batch_block_stats(self.h, i, C.byref(b)), "batch_block_stats")
+        return b.arrays()
 
     def cycle_times(self, i, n):
         out = np.zeros(n)

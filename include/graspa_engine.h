@@ -348,6 +348,31 @@ int  graspa_batch_kernel_time(graspa_batch* batch, double* total_ms, int64_t* la
 /* Copies the device state of every box back to its graspa_box. */
 int  graspa_batch_sync(graspa_batch* batch);
 
+/* Block statistics of the current production run of a box, the data behind
+ * the reference's block averages (axpy.cu:420-453, 176-185; Gather_Averages_*
+ * in print_statistics.cuh:412-456, 646-665): the run's cycles are split into
+ * GRASPA_NBLOCK blocks of block_size cycles (the last block also takes the
+ * remainder; block = min(cycle / block_size, GRASPA_NBLOCK - 1)).  Per block
+ * and component c (1..n_comp-1; 0 is the framework): sums over the block's
+ * cycles of N_c, N_c^2, N_c N_d, dE N_c (dE = running energy change without
+ * the host-host terms) and of the running energy change per term; Widom
+ * sums of W and counts.  A production set_cycles / run starts a new run and
+ * clears them. */
+#define GRASPA_NBLOCK    5
+#define GRASPA_MAX_COMPS 8
+typedef struct {
+  int64_t block_size;                                    /* cycles per block                   */
+  int64_t cycles[GRASPA_NBLOCK];                         /* cycles recorded per block          */
+  double  n[GRASPA_NBLOCK][GRASPA_MAX_COMPS];            /* sum N_c                            */
+  double  n2[GRASPA_NBLOCK][GRASPA_MAX_COMPS];           /* sum N_c^2                          */
+  double  nn[GRASPA_NBLOCK][GRASPA_MAX_COMPS][GRASPA_MAX_COMPS]; /* sum N_c N_d              */
+  double  en[GRASPA_NBLOCK][GRASPA_MAX_COMPS];           /* sum dE N_c                         */
+  double  e[GRASPA_NBLOCK][10];                          /* sum of the running graspa_energy   */
+  double  widom_w[GRASPA_NBLOCK][GRASPA_MAX_COMPS];      /* sum W of Widom moves               */
+  double  widom_n[GRASPA_NBLOCK][GRASPA_MAX_COMPS];      /* Widom moves                        */
+} graspa_block_stats;
+int  graspa_batch_block_stats(graspa_batch* batch, int i, graspa_block_stats* out);
+
 /* ------------------------------------------------------------------------ */
 /* Measurement                                                               */
 /* ------------------------------------------------------------------------ */
