@@ -100,7 +100,7 @@ struct BBox {
   BSpecies sp[kMaxComps];
   int K, O, bias, use_max_step;
   int max_step, trace_cap;
-  int cyc_cap, _pad;
+  int cyc_cap, n_block;             // n_block: blocks of a production run
   double beta, pressure;
   uint64_t seed;
   uint32_t stream, _pad2;
@@ -240,10 +240,10 @@ __device__ inline void pair_site(const BoxConst& bc, const double* __restrict__ 
 // Block of the current cycle in the production run (Gather_Averages_Types,
 // print_statistics.cuh:646-655: cycle / block size; the remainder goes to the
 // last block).
-__device__ inline int block_of(const BState& S)
+__device__ inline int block_of(const BState& S, const int n_block)
 {
   const int64_t b = (S.cycle - S.prod_first) / S.blk_size;
-  return (int)(b < GRASPA_NBLOCK - 1 ? b : GRASPA_NBLOCK - 1);
+  return (int)(b < n_block - 1 ? b : n_block - 1);
 }
 
 // Block sums of one production cycle (axpy.cu:420-453): the running energy
@@ -819,7 +819,7 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
       if (S.production) {
         S.widom_w[comp] += W;
         S.widom_n[comp] += 1.0;
-        const int bk = block_of(S);
+        const int bk = block_of(S, B.n_block);
         B.blk->widom_w[bk][comp] += W;
         B.blk->widom_n[bk][comp] += 1.0;
       }
@@ -1007,7 +1007,7 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
       const int comp = L.comp;
       if (L.fail && L.type == kMvWidom && S.production) {
         S.widom_n[comp] += 1.0;
-        B.blk->widom_n[block_of(S)][comp] += 1.0;
+        B.blk->widom_n[block_of(S, B.n_block)][comp] += 1.0;
       }
       if (B.trace && S.trace_n < B.trace_cap) {
         graspa_trace& tr = B.trace[(size_t)blockIdx.x * B.trace_cap + S.trace_n];
@@ -1031,7 +1031,7 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
             S.load_sum2[c] += nm * nm;
           }
           S.load_n++;
-          cyc_block = block_of(S);     // block sums by all threads after the barrier
+          cyc_block = block_of(S, B.n_block);     // block sums by all threads after the barrier
         }
         if (S.cycle % 500 == 0) {
           for (int c = 1; c < B.n_comp; ++c) {
@@ -1140,6 +1140,7 @@ int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
   graspa_box* b = in.box;
   const graspa_mc_params* p = in.params;
   if (!b || !p) return fail(GRASPA_ERR_ARG, "null box or params");
+  if (in.n_block < 0 || in.n_block > GRASPA_NBLOCK) return fail(GRASPA_ERR_ARG, "n_block must be in 0..16");
   if (b->device != bt->device || b->precision != bt->precision)
     return fail(GRASPA_ERR_ARG, "all boxes of a batch must share the device and the precision form");
   const int ncomp = (int)b->comps.size();
@@ -1224,6 +1225,7 @@ int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
   h.has_tail = p->has_tail;
   h.trace_cap = bt->trace_cap;
   h.cyc_cap = bt->cyc_cap;
+  h.n_block = in.n_block > 0 ? in.n_block : 5;
   h.trace = bt->d_trace;
   h.cyc = bt->d_cyc ? bt->d_cyc + (size_t)i * bt->cyc_cap : nullptr;
   h.blk = bt->d_blk + i;
@@ -1391,13 +1393,13 @@ int graspa_batch_set_box(graspa_batch* bt, int slot, const graspa_batch_box* box
   return upload_batch(bt);
 }
 
-// A production run starts: blocks of n_cycles / GRASPA_NBLOCK cycles
+// A production run starts: blocks of n_cycles / n_block cycles
 // (axpy.cu:500-502), statistics cleared.
 static int start_production(graspa_batch* bt, int slot, int n_cycles)
 {
   BState& s = bt->hs[slot];
   s.prod_first = s.cycle;
-  s.blk_size = std::max(1, n_cycles / GRASPA_NBLOCK);
+  s.blk_size = std::max(1, n_cycles / bt->hb[slot].n_block);
   return hip_check(hipMemset(bt->d_blk + slot, 0, sizeof(graspa_block_stats)), "block statistics");
 }
 
@@ -1493,6 +1495,7 @@ int graspa_batch_block_stats(graspa_batch* bt, int i, graspa_block_stats* out)
   if (hip_check(hipMemcpy(out, bt->d_blk + i, sizeof(graspa_block_stats), hipMemcpyDeviceToHost), "block statistics"))
     return GRASPA_ERR_HIP;
   out->block_size = bt->hs[i].blk_size;
+  out->n_block = bt->hb[i].n_block;
   return GRASPA_OK;
 }
 

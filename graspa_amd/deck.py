@@ -132,6 +132,10 @@ class System:
     n_trial_orient: int = 10
     vdw_real_bias: bool = True
     allocate_space: int = 10240
+    n_blocks: int = 5                     # NumberOfBlocks (block averages)
+    print_every: int = 0                  # PrintEvery (cycles between progress lines)
+    movies_every: int = 0                 # MoviesEvery (cycles between LAMMPS snapshots, 0 = none)
+    save_output: bool = False             # SaveOutputToFile
 
     # -- derived geometry (maths.cuh:28-55) --
     @property
@@ -496,6 +500,41 @@ def _pbc(d, cell, inv, cubic):
                      cell[2] * s[0] + cell[5] * s[1] + cell[8] * s[2]])
 
 
+# Keywords of simulation.input for methods this engine does not run
+# (read_data.cpp:233-709): a deck that switches one of them on is refused
+# rather than run without it.
+_UNSUPPORTED_NUMERIC = ("GibbsParticleXferProbability", "GibbsVolumeChangeProbability",
+                        "NPTVolumeChangeProbability")
+_UNSUPPORTED_YES = ("RunTMMC", "UseDNNforHostGuest", "BlockPockets", "TURN_OFF_CBMC_SWAP", "SeparateFrameworkComponents",
+                    "DifferentFrameworks")
+_UNSUPPORTED_COMPONENT = ("CBCFProbability", "RotationSpecialProbability", "GibbsSwapProbability")
+
+
+def _reject_unsupported(get, comp_blocks):
+    for key in _UNSUPPORTED_NUMERIC:
+        t = get(key)
+        if t and len(t) > 1 and float(t[1]) > 0.0:
+            raise NotImplementedError("%s %s: this engine runs GCMC / Widom in one box only" % (key, t[1]))
+    for key in _UNSUPPORTED_YES:
+        t = get(key)
+        if t and len(t) > 1 and t[1].lower() == "yes":
+            raise NotImplementedError("%s yes is not supported by this engine" % key)
+    t = get("NumberOfSimulations")
+    if t and len(t) > 1 and int(t[1]) > 1:
+        raise NotImplementedError("NumberOfSimulations %s: one box per deck (run several decks as a batch)" % t[1])
+    t = get("InputFileType")
+    if t and len(t) > 1 and t[1].lower() != "cif":
+        raise NotImplementedError("InputFileType %s: frameworks are read from P1 CIF files" % t[1])
+    t = get("NumberofFrameworkComponents")
+    if t and len(t) > 1 and int(t[1]) > 1:
+        raise NotImplementedError("NumberofFrameworkComponents %s: one framework component" % t[1])
+    for blk in comp_blocks:
+        for key in _UNSUPPORTED_COMPONENT:
+            if key in blk and float(blk[key]) > 0.0:
+                raise NotImplementedError("%s %s (component %s) is not supported by this engine"
+                                          % (key, blk[key], blk["MoleculeName"]))
+
+
 def read_deck(dirpath: str) -> System:
     """Read a gRASPA deck directory into a :class:`System`."""
     sim = os.path.join(dirpath, "simulation.input")
@@ -581,6 +620,14 @@ def read_deck(dirpath: str) -> System:
         t = get(key)
         if t:
             setattr(s, attr, conv(t[1]))
+    for key, attr in (("NumberOfBlocks", "n_blocks"), ("PrintEvery", "print_every"),
+                      ("MoviesEvery", "movies_every")):
+        t = get(key)
+        if t:
+            setattr(s, attr, int(t[1]))
+    t = get("SaveOutputToFile")
+    s.save_output = bool(t and t[1].lower() == "yes")
+    _reject_unsupported(get, comp_blocks)
     t = get("UseMaxStep")
     s.use_max_step = bool(t and t[1].lower() == "yes")
     t = get("CBMCBiasingMethod")
@@ -614,7 +661,7 @@ def read_deck(dirpath: str) -> System:
 _SCALARS = ("cut_vdw", "cut_coul", "overlap", "no_charges", "ewald_precision", "alpha", "recip_cut", "lammps_ewald",
             "prefactor", "fw_mass", "temperature", "pressure_pa", "seed", "n_init", "n_equil", "n_prod",
             "use_max_step", "max_step_per_cycle", "n_trial_pos", "n_trial_orient", "vdw_real_bias",
-            "allocate_space")
+            "allocate_space", "n_blocks", "print_every", "movies_every", "save_output")
 _COMP_SCALARS = ("molsize", "rigid", "ideal_rosen", "fugacity_coeff", "mol_fraction", "p_translation", "p_rotation",
                  "p_widom", "p_reinsertion", "p_identity", "p_swap", "create_molecules")
 
@@ -646,8 +693,8 @@ def load_system(path: str) -> System:
                pseudo_mass=z["pseudo_mass"], pseudo_charge=z["pseudo_charge"], eps=z["eps"], sig=z["sig"],
                shift=z["shift"], tail=z["tail"], use_tail=z["use_tail"])
     for k in _SCALARS:
-        v = z[k].item()
-        setattr(s, k, type(getattr(s, k))(v))
+        if k in z.files:        # files written before a field existed keep its default
+            setattr(s, k, type(getattr(s, k))(z[k].item()))
     s.kmax = tuple(int(v) for v in z["kmax"])
     s.unit_cells = tuple(int(v) for v in z["unit_cells"])
     s.fw_pos, s.fw_charge, s.fw_type = z["fw_pos"].reshape(-1, 3), z["fw_charge"], z["fw_type"].astype(np.int32)

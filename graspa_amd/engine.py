@@ -80,7 +80,7 @@ class MCParams(C.Structure):
 
 class BatchBox(C.Structure):
     _fields_ = [("box", C.c_void_p), ("params", C.POINTER(MCParams)), ("seed", C.c_uint64), ("stream", C.c_uint32),
-                ("_pad", C.c_int)]
+                ("n_block", C.c_int)]
 
 
 class Trace(C.Structure):
@@ -89,12 +89,12 @@ class Trace(C.Structure):
                 ("rosen", C.c_double), ("de", Energy)]
 
 
-NBLOCK, MAX_COMPS = 5, 8
+NBLOCK, MAX_COMPS = 16, 8
 
 
 class BlockStats(C.Structure):
     """graspa_block_stats: per-block sums of the current production run."""
-    _fields_ = [("block_size", C.c_int64), ("cycles", C.c_int64 * NBLOCK),
+    _fields_ = [("n_block", C.c_int64), ("block_size", C.c_int64), ("cycles", C.c_int64 * NBLOCK),
                 ("n", (C.c_double * MAX_COMPS) * NBLOCK), ("n2", (C.c_double * MAX_COMPS) * NBLOCK),
                 ("nn", ((C.c_double * MAX_COMPS) * MAX_COMPS) * NBLOCK), ("en", (C.c_double * MAX_COMPS) * NBLOCK),
                 ("e", (C.c_double * 10) * NBLOCK), ("widom_w", (C.c_double * MAX_COMPS) * NBLOCK),
@@ -102,9 +102,11 @@ class BlockStats(C.Structure):
 
     def arrays(self):
         """numpy views: dict of arrays indexed [block][component ...]."""
-        out = {"block_size": int(self.block_size), "cycles": np.ctypeslib.as_array(self.cycles).copy()}
+        nb = int(self.n_block)
+        out = {"n_block": nb, "block_size": int(self.block_size),
+               "cycles": np.ctypeslib.as_array(self.cycles).copy()[:nb]}
         for k in ("n", "n2", "nn", "en", "e", "widom_w", "widom_n"):
-            out[k] = np.ctypeslib.as_array(getattr(self, k)).copy()
+            out[k] = np.ctypeslib.as_array(getattr(self, k)).copy()[:nb]
         return out
 
 
@@ -573,6 +575,7 @@ class Batch:
             arr[i].params = C.pointer(self.params[i])
             arr[i].seed = int(seeds[i])
             arr[i].stream = int(streams[i])
+            arr[i].n_block = int(getattr(boxes[i].system, "n_blocks", 5))
         self._arr = arr
         self.trace_cap = trace_cap
         self.n = n
@@ -590,6 +593,7 @@ class Batch:
         params = params if params is not None else make_mc_params(box.system, box)
         bb = BatchBox()
         bb.box, bb.params, bb.seed, bb.stream = box.h, C.pointer(params), int(seed), int(stream)
+        bb.n_block = int(getattr(box.system, "n_blocks", 5))
         _check(lib().graspa_batch_set_box(self.h, slot, C.byref(bb)), "batch_set_box")
         self.boxes[slot], self.params[slot] = box, params
 

@@ -32,7 +32,6 @@ from .deck import AVOGADRO, BOLTZMANN, ENERGY_TO_KELVIN, GAS_CONSTANT
 
 TERMS = ("hh_vdw", "hg_vdw", "gg_vdw", "hh_real", "hg_real", "gg_real", "hh_ewald", "hg_ewald", "gg_ewald", "tail")
 ENERGY_TO_KJ = 0.01          # internal energy unit (10 J/mol) in kJ/mol
-NBLOCK = engine.NBLOCK
 
 
 def _zero():
@@ -184,7 +183,7 @@ def _print_values(out, sums, sq, nb):
     2-sigma error bar within the block, then the mean and 2-sigma spread of
     the block averages."""
     avgs = []
-    for b in range(NBLOCK):
+    for b in range(len(nb)):
         a = sums[b] / nb[b]
         s2 = sq[b] / nb[b]
         err = 2.0 * math.sqrt(s2 - a * a) if s2 - a * a > 0 else 0.0
@@ -206,8 +205,9 @@ def heat_of_adsorption(rec: RunRecord):
     nb = _block_lengths(blk)
     createmol = _total(rec.drift.initial) - rec.drift.initial["hh_vdw"] - rec.drift.initial["hh_real"] - \
         rec.drift.initial["hh_ewald"]
-    q = np.zeros((ns, NBLOCK))
-    for b in range(NBLOCK):
+    nbk = len(nb)
+    q = np.zeros((ns, nbk))
+    for b in range(nbk):
         e = np.asarray(blk["e"][b])
         avg_e = (e.sum() - e[0] - e[3] - e[6]) / nb[b]
         n = np.array([blk["n"][b][c + 1] for c in range(ns)]) / nb[b]
@@ -287,14 +287,15 @@ def write_output(rec: RunRecord, path_or_file):
     blk = rec.blocks
     if blk is not None and int(np.sum(blk["cycles"])) > 0:
         nb = _block_lengths(blk)
+        nblk = len(nb)
         # production averages of the energy (print_statistics.cuh:439-454)
         avg = _zero()
         sq = _zero()
-        for b in range(NBLOCK):
+        for b in range(nblk):
             for k, key in enumerate(TERMS):
                 a = blk["e"][b][k] / nb[b]
-                avg[key] += a / NBLOCK
-                sq[key] += a * a / NBLOCK
+                avg[key] += a / nblk
+                sq[key] += a * a / nblk
         err = {k: 2.0 * math.sqrt(max(sq[k] - avg[k] * avg[k], 0.0)) for k in TERMS}
         out.append("======================== PRODUCTION PHASE AVERAGE ENERGIES (Simulation 0) =========================")
         _energy_block(out, "PRODUCTION PHASE AVERAGE ENERGY", _add(avg, d.initial))
@@ -304,13 +305,13 @@ def write_output(rec: RunRecord, path_or_file):
         # Rosenbluth summary (print_statistics.cuh:112-200)
         for ci, name in enumerate(rec.names):
             c = ci + 1
-            wn = np.array([blk["widom_n"][b][c] for b in range(NBLOCK)])
+            wn = np.array([blk["widom_n"][b][c] for b in range(nblk)])
             if wn.sum() <= 0:
                 continue
             out.append("=====================Rosenbluth Summary For Component [%d] (%s)=====================" % (c, name))
-            out.append("There are %d blocks" % NBLOCK)
+            out.append("There are %d blocks" % nblk)
             tot = {k: [0.0, 0.0] for k in ("wr", "mu", "henry", "fug")}
-            for b in range(NBLOCK):
+            for b in range(nblk):
                 out.append("=====BLOCK %d=====" % b)
                 out.append("Widom Performed: %.1f" % wn[b])
                 if wn[b] > 0:
@@ -328,8 +329,8 @@ def write_output(rec: RunRecord, path_or_file):
                         tot[k][0] += v
                         tot[k][1] += v * v
             def avg_err(k):
-                a = tot[k][0] / NBLOCK
-                v = tot[k][1] / NBLOCK - a * a
+                a = tot[k][0] / nblk
+                v = tot[k][1] / nblk - a * a
                 return a, 2.0 * math.sqrt(v) if v > 0 else 0.0
             out.append("=========================AVERAGE========================")
             out.append("Averaged Rosenbluth Weight: %.10f +/- %.10f" % avg_err("wr"))
@@ -341,7 +342,7 @@ def write_output(rec: RunRecord, path_or_file):
         q = heat_of_adsorption(rec)
         for ci, name in enumerate(rec.names):
             out.append("COMPONENT [%d] (%s)" % (ci + 1, name))
-            for b in range(NBLOCK):
+            for b in range(nblk):
                 out.append("BLOCK [%d], Blocksize: %d, Average: %.5f" % (b, int(nb[b]), q[ci, b]))
             with np.errstate(invalid="ignore"):
                 m = float(np.mean(q[ci]))
@@ -355,9 +356,9 @@ def write_output(rec: RunRecord, path_or_file):
         mw = [float(np.sum(c.masses)) for c in s.components]
 
         def comp_sums(c):
-            return ([blk["n"][b][c] for b in range(NBLOCK)], [blk["n2"][b][c] for b in range(NBLOCK)])
+            return ([blk["n"][b][c] for b in range(nblk)], [blk["n2"][b][c] for b in range(nblk)])
 
-        one = ([nb[b] for b in range(NBLOCK)], [nb[b] for b in range(NBLOCK)])   # framework: N = 1
+        one = ([nb[b] for b in range(nblk)], [nb[b] for b in range(nblk)])   # framework: N = 1
         out.append("=================== BLOCK AVERAGES (LOADING: # MOLECULES)=============")
         out.append("COMPONENT [0] (%s)" % rec.framework_name)
         _print_values(out, one[0], one[1], nb)
@@ -389,7 +390,7 @@ def write_output(rec: RunRecord, path_or_file):
             out.append("----------------------------------------------------------")
         out.append("=" * 62)
         out.append("=====================BLOCK AVERAGES (VOLUME Å^3)================")
-        _print_values(out, [s.volume * nb[b] for b in range(NBLOCK)], [s.volume ** 2 * nb[b] for b in range(NBLOCK)], nb)
+        _print_values(out, [s.volume * nb[b] for b in range(nblk)], [s.volume ** 2 * nb[b] for b in range(nblk)], nb)
         out.append("----------------------------------------------------------")
         out.append("=" * 62)
     text = "\n".join(out) + "\n"

@@ -305,7 +305,7 @@ typedef struct {
                                       (seed and pool_size are not used)                          */
   uint64_t seed;                   /* Philox key of this box's moves                            */
   uint32_t stream;                 /* Philox stream of this box                                 */
-  int      _pad;
+  int      n_block;                /* blocks of a production run, 1..GRASPA_NBLOCK (0: 5)        */
 } graspa_batch_box;
 
 /* n_boxes slots; boxes[i].box == NULL (or boxes == NULL) leaves slot i empty.
@@ -351,16 +351,18 @@ int  graspa_batch_sync(graspa_batch* batch);
 /* Block statistics of the current production run of a box, the data behind
  * the reference's block averages (axpy.cu:420-453, 176-185; Gather_Averages_*
  * in print_statistics.cuh:412-456, 646-665): the run's cycles are split into
- * GRASPA_NBLOCK blocks of block_size cycles (the last block also takes the
- * remainder; block = min(cycle / block_size, GRASPA_NBLOCK - 1)).  Per block
+ * n_block blocks of block_size cycles (n_block from graspa_batch_box, 5 by
+ * default as the reference's NumberOfBlocks; the last block also takes the
+ * remainder; block = min(cycle / block_size, n_block - 1)).  Per block
  * and component c (1..n_comp-1; 0 is the framework): sums over the block's
  * cycles of N_c, N_c^2, N_c N_d, dE N_c (dE = running energy change without
  * the host-host terms) and of the running energy change per term; Widom
  * sums of W and counts.  A production set_cycles / run starts a new run and
  * clears them. */
-#define GRASPA_NBLOCK    5
+#define GRASPA_NBLOCK    16  /* most blocks per run */
 #define GRASPA_MAX_COMPS 8
 typedef struct {
+  int64_t n_block;                                       /* blocks of this run                 */
   int64_t block_size;                                    /* cycles per block                   */
   int64_t cycles[GRASPA_NBLOCK];                         /* cycles recorded per block          */
   double  n[GRASPA_NBLOCK][GRASPA_MAX_COMPS];            /* sum N_c                            */

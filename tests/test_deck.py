@@ -1,6 +1,7 @@
 """Deck reader: synthetic decks written in gRASPA's formats read back to the
 same system; the reference's unit conversions."""
 import numpy as np
+import pytest
 
 from graspa_amd import deck, synth
 
@@ -42,3 +43,38 @@ def test_npz_round_trip(tmp_path):
     deck.save_system(p, s)
     r = deck.load_system(p)
     assert np.array_equal(r.fw_pos, s.fw_pos) and r.kmax == s.kmax and r.alpha == s.alpha
+
+
+def _with_lines(tmp_path, globals_=(), component=()):
+    from graspa_amd import synth
+    s = synth.config0()
+    synth.write_deck(s, str(tmp_path), cycles=10)
+    p = tmp_path / "simulation.input"
+    lines = p.read_text().splitlines()
+    k = next(i for i, ln in enumerate(lines) if ln.lstrip().startswith("Component ") and "MoleculeName" in ln)
+    lines = lines[:k] + list(globals_) + lines[k:k + 1] + list(component) + lines[k + 1:]
+    p.write_text("\n".join(lines) + "\n")
+
+
+def test_output_keywords(tmp_path):
+    _with_lines(tmp_path, ["NumberOfBlocks 8", "PrintEvery 50", "MoviesEvery 100", "SaveOutputToFile yes"])
+    r = deck.read_deck(str(tmp_path))
+    assert (r.n_blocks, r.print_every, r.movies_every, r.save_output) == (8, 50, 100, True)
+
+
+@pytest.mark.parametrize("glob,comp", [(["GibbsVolumeChangeProbability 0.1"], []),
+                                       (["NPTVolumeChangeProbability 1.0"], []),
+                                       (["RunTMMC yes"], []),
+                                       (["NumberOfSimulations 2"], []),
+                                       ([], ["CBCFProbability 1.0"]),
+                                       ([], ["RotationSpecialProbability 0.5"])])
+def test_unsupported_methods_are_refused(tmp_path, glob, comp):
+    _with_lines(tmp_path, glob, comp)
+    with pytest.raises(NotImplementedError):
+        deck.read_deck(str(tmp_path))
+
+
+def test_switched_off_methods_are_accepted(tmp_path):
+    _with_lines(tmp_path, ["GibbsVolumeChangeProbability 0.0", "RunTMMC no", "NumberOfSimulations 1"],
+                ["CBCFProbability 0.0"])
+    deck.read_deck(str(tmp_path))

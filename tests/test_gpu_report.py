@@ -37,7 +37,7 @@ def test_block_stats_and_drift(engine_mod, prec):
     bt.run(n_prod, production=True)
     blk = bt.block_stats(0)
     st = bt.stats(0)
-    assert blk["block_size"] == n_prod // 5
+    assert blk["n_block"] == 5 and blk["block_size"] == n_prod // 5
     assert list(blk["cycles"]) == [2, 2, 2, 2, 4]        # the last block takes the remainder
     # loadings and Widom sums agree with the production sums of the box
     np.testing.assert_allclose(blk["n"].sum(0)[1:3], st.load_sum, rtol=0, atol=0)
@@ -55,7 +55,7 @@ def test_block_stats_and_drift(engine_mod, prec):
     cyc = np.array([t.cycle for t in tr])
     start = run_end - de.sum(0)
     first = cyc.min()
-    e_blocks = np.zeros((5, 10))
+    e_blocks = np.zeros((blk["n_block"], 10))
     for c in range(first, first + n_prod):
         e_c = start + de[cyc <= c].sum(0)
         e_blocks[min((c - first) // blk["block_size"], 4)] += e_c

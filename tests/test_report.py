@@ -52,7 +52,7 @@ def _blocks_from_series(N, E, ns=1, block=None):
     N[cycle][species] and E[cycle][term]."""
     n_cyc = len(N)
     bs = block or max(1, n_cyc // 5)
-    blk = {"block_size": bs, "cycles": np.zeros(5, np.int64), "n": np.zeros((5, 8)), "n2": np.zeros((5, 8)),
+    blk = {"n_block": 5, "block_size": bs, "cycles": np.zeros(5, np.int64), "n": np.zeros((5, 8)), "n2": np.zeros((5, 8)),
            "nn": np.zeros((5, 8, 8)), "en": np.zeros((5, 8)), "e": np.zeros((5, 10)),
            "widom_w": np.zeros((5, 8)), "widom_n": np.zeros((5, 8))}
     for i in range(n_cyc):
