@@ -218,11 +218,15 @@ void free_dead_maps(graspa_box* b);
 // Real-space Coulomb table g(r^2) = erfc(alpha r)/r of the fast pair form
 // (coulomb_table.cpp): kCoulTabM intervals per octave of r^2 from
 // 2^kCoulTabELo, a degree-kCoulTabDeg polynomial in r^2 - r_i^2 per interval.
+// 16 intervals per octave with degree 7 keep the table at 15 KB for a 12 A
+// cutoff (64 intervals with degree 5 took 46 KB at the same accuracy), so
+// the pair kernels' LDS leaves room for a second workgroup per CU.
 constexpr int kCoulTabELo = -7;
-constexpr int kCoulTabMBits = 6;
+constexpr int kCoulTabMBits = 4;
 constexpr int kCoulTabM = 1 << kCoulTabMBits;
-constexpr int kCoulTabDeg = 5;
-constexpr int kCoulTabStride = 6;    // doubles per interval
+constexpr int kCoulTabDeg = 7;
+constexpr int kCoulTabStride = 8;    // doubles per interval (kCoulTabDeg + 1)
+static_assert(kCoulTabStride == kCoulTabDeg + 1 && kCoulTabStride % 2 == 0, "table rows are read as double2");
 int coulomb_table_octaves(double cut2);
 void build_coulomb_table(double alpha, double cut2, std::vector<double>& tab);
 

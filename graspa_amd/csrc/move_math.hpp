@@ -71,10 +71,10 @@ __device__ inline void axis_table(const BoxConst& bc, double px, double py, doub
   for (int k = 2; k <= km; ++k) out[k] = cmul2(out[k - 1], e1);
 }
 
-// erfc(alpha r)/r from the per-box table (coulomb_table.cpp): the interval is
-// read from the exponent and top mantissa bits of r^2 as FP32, the value is a
-// degree-5 polynomial in r^2 - s_i.  Clamped indices only occur for r^2 below
-// 2^-7, i.e. pairs the overlap rule already rejects.
+// erfc(alpha r)/r from the per-box table (coulomb_table.cpp): the interval
+// is read from the exponent and top mantissa bits of r^2 as FP32, the value is
+// a degree-kCoulTabDeg polynomial in r^2 - s_i (Horner).  Clamped indices only
+// occur for r^2 below 2^-7, i.e. pairs the overlap rule already rejects.
 __device__ inline double coul_table(const double2* __restrict__ ct, const int nint, const double rr)
 {
   constexpr int kShift = 23 - kCoulTabMBits;
@@ -83,13 +83,18 @@ __device__ inline double coul_table(const double2* __restrict__ ct, const int ni
   idx = min(max(idx, 0), nint - 1);
   const double si = (double)__uint_as_float(bits & ~((1u << kShift) - 1u));
   const double d = rr - si;
-  const double2* c = ct + 3 * idx;
-  const double2 c01 = c[0], c23 = c[1], c45 = c[2];
-  double g = fma(c45.y, d, c45.x);
-  g = fma(g, d, c23.y);
-  g = fma(g, d, c23.x);
-  g = fma(g, d, c01.y);
-  return fma(g, d, c01.x);
+  const double2* c = ct + (kCoulTabStride / 2) * idx;
+  double2 cc[kCoulTabStride / 2];
+#pragma unroll
+  for (int k = 0; k < kCoulTabStride / 2; ++k) cc[k] = c[k];
+  double g = cc[kCoulTabStride / 2 - 1].y;
+  g = fma(g, d, cc[kCoulTabStride / 2 - 1].x);
+#pragma unroll
+  for (int k = kCoulTabStride / 2 - 2; k >= 0; --k) {
+    g = fma(g, d, cc[k].y);
+    g = fma(g, d, cc[k].x);
+  }
+  return g;
 }
 
 // Cell fractions of a position as 32-bit fixed point (the screen's and the

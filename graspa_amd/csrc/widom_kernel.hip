@@ -50,7 +50,7 @@ constexpr int kRedBlock = 256;
 constexpr int kRedPer = 8;       // records per thread in the first reduction pass
 constexpr int kInsWaves = 16;    // waves per workgroup of the chain kernel (one live insertion each)
 constexpr int kFinWaves = 8;     // waves per workgroup of the finish kernel (256 VGPRs per lane available)
-constexpr int kEvalWaves = 16;   // waves per workgroup of the pair-energy kernel
+constexpr int kEvalWaves = 12;   // waves per workgroup of the pair-energy kernel (two workgroups per CU)
 constexpr int kFbWaves = 16;     // waves per workgroup of the first-bead kernel
 
 // Work lists are appended per workgroup of the screening kernels: workgroup g
@@ -474,7 +474,7 @@ __global__ __launch_bounds__(kEvalWaves * 64) void widom_eval_kernel(const Widom
   if (threadIdx.x == 0) pre[0] = 0u;
   __syncthreads();
   for (int h = 1; h < G; h <<= 1) {
-    unsigned add[kMaxScreenGroups / (kEvalWaves * 64)];
+    unsigned add[(kMaxScreenGroups + kEvalWaves * 64 - 1) / (kEvalWaves * 64)];
     for (int k = 0, i = threadIdx.x; i < G; ++k, i += blockDim.x) add[k] = (i >= h) ? pre[i + 1 - h] : 0u;
     __syncthreads();
     for (int k = 0, i = threadIdx.x; i < G; ++k, i += blockDim.x) pre[i + 1] += add[k];

@@ -81,24 +81,28 @@ def _coulomb_table(alpha, cut2):
     from graspa_amd import engine
     L = engine.lib()
     n = L.graspa_coulomb_table(alpha, cut2, None, 0)
-    assert n > 0 and n % 6 == 0
+    assert n > 0 and n % (DEG + 1) == 0
     buf = (C.c_double * n)()
     assert L.graspa_coulomb_table(alpha, cut2, buf, n) == n
     return np.frombuffer(buf, dtype=np.float64).copy()
 
 
+MBITS, DEG = 4, 7          # intervals per octave = 2**MBITS; polynomial degree
+
+
 def _coul_table_eval(tab, rr):
-    """erfc(alpha r)/r the way the fast Widom kernel reads the table: interval
-    from the exponent and top 6 mantissa bits of r^2 as FP32, degree-5
+    """erfc(alpha r)/r the way the fast kernels read the table: interval from
+    the exponent and top MBITS mantissa bits of r^2 as FP32, degree-DEG
     polynomial in r^2 - s_i."""
+    shift = 23 - MBITS
     bits = rr.astype(np.float32).view(np.uint32)
-    idx = (bits >> 17).astype(np.int64) - ((127 - 7) << 6)
-    idx = np.clip(idx, 0, tab.size // 6 - 1)
-    si = (bits & np.uint32(0xFFFE0000)).view(np.float32).astype(np.float64)
+    idx = (bits >> shift).astype(np.int64) - ((127 - 7) << MBITS)
+    idx = np.clip(idx, 0, tab.size // (DEG + 1) - 1)
+    si = (bits & np.uint32((0xFFFFFFFF << shift) & 0xFFFFFFFF)).view(np.float32).astype(np.float64)
     d = rr - si
-    c = tab.reshape(-1, 6)[idx]
-    g = c[:, 5]
-    for k in (4, 3, 2, 1, 0):
+    c = tab.reshape(-1, DEG + 1)[idx]
+    g = c[:, DEG]
+    for k in range(DEG - 1, -1, -1):
         g = g * d + c[:, k]
     return g
 
@@ -126,7 +130,7 @@ def test_coulomb_table_pins_a_wrong_interval():
     """Reading the neighbouring interval's polynomial breaks the bound."""
     tab = _coulomb_table(0.265058, 144.0)
     rr = np.geomspace(0.5, 100.0, 1001)
-    shifted = np.roll(tab.reshape(-1, 6), 1, axis=0).ravel()
+    shifted = np.roll(tab.reshape(-1, DEG + 1), 1, axis=0).ravel()
     r = np.sqrt(rr)
     assert np.abs(_coul_table_eval(shifted, rr) - erfc(0.265058 * r) / r).max() > 1e-6
 
