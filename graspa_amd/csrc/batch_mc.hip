@@ -423,27 +423,46 @@ __device__ void ewald_pass(const BBox& B, const BState& S, BMove& L, double2* __
   const double2* __restrict__ cur = B.skc[S.sk_cur];
   double2* __restrict__ stg = B.skc[S.sk_cur ^ 1];
   double a0 = 0.0, a1 = 0.0;
-  for (int j = tid; j < B.n_active; j += kBT) {
-    const int4 kv = B.kact[j];
-    const double t = B.ktemp_act[j];
-    const double2 ov = cur[j];
-    const double2 cv = B.skfw[j];
-    double2 co = make_double2(0.0, 0.0), cn = make_double2(0.0, 0.0);
-    for (int i = 0; i < ns; ++i) {
-      const double2* e = eik + i * 3 * ks;
-      double2 ey = e[ks + abs(kv.y)];
-      if (kv.y < 0) ey.y = -ey.y;
-      double2 ez = e[2 * ks + abs(kv.z)];
-      if (kv.z < 0) ez.y = -ez.y;
-      const double2 p = cmul2(cmul2(e[kv.x], ey), ez);
-      const double q = L.site[i].q;
-      if (i < n_old) { co.x += q * p.x; co.y += q * p.y; }
-      else { cn.x += q * p.x; cn.y += q * p.y; }
+  // kEwU k-vectors per thread per step, their loads issued together (one
+  // workgroup per box leaves few waves to hide memory latency); a thread
+  // still visits its k-vectors in ascending order, so the sums are the same
+  constexpr int kEwU = 4;
+  const int na = B.n_active;
+  for (int j0 = tid; j0 < na; j0 += kBT * kEwU) {
+    int4 kv[kEwU];
+    double tk[kEwU];
+    double2 ov[kEwU], cv[kEwU];
+#pragma unroll
+    for (int u = 0; u < kEwU; ++u) {
+      const int j = j0 + u * kBT;
+      if (j < na) {
+        kv[u] = B.kact[j];
+        tk[u] = B.ktemp_act[j];
+        ov[u] = cur[j];
+        cv[u] = B.skfw[j];
+      }
     }
-    const double2 nv = make_double2(ov.x + cn.x - co.x, ov.y + cn.y - co.y);
-    a0 += t * (nv.x * nv.x + nv.y * nv.y) - t * (ov.x * ov.x + ov.y * ov.y);
-    a1 += t * (cv.x * (cn.x - co.x) + cv.y * (cn.y - co.y));
-    stg[j] = nv;
+#pragma unroll
+    for (int u = 0; u < kEwU; ++u) {
+      const int j = j0 + u * kBT;
+      if (j >= na) break;
+      double2 co = make_double2(0.0, 0.0), cn = make_double2(0.0, 0.0);
+      for (int i = 0; i < ns; ++i) {
+        const double2* e = eik + i * 3 * ks;
+        double2 ey = e[ks + abs(kv[u].y)];
+        if (kv[u].y < 0) ey.y = -ey.y;
+        double2 ez = e[2 * ks + abs(kv[u].z)];
+        if (kv[u].z < 0) ez.y = -ez.y;
+        const double2 p = cmul2(cmul2(e[kv[u].x], ey), ez);
+        const double q = L.site[i].q;
+        if (i < n_old) { co.x += q * p.x; co.y += q * p.y; }
+        else { cn.x += q * p.x; cn.y += q * p.y; }
+      }
+      const double2 nv = make_double2(ov[u].x + cn.x - co.x, ov[u].y + cn.y - co.y);
+      a0 += tk[u] * (nv.x * nv.x + nv.y * nv.y) - tk[u] * (ov[u].x * ov[u].x + ov[u].y * ov[u].y);
+      a1 += tk[u] * (cv[u].x * (cn.x - co.x) + cv[u].y * (cn.y - co.y));
+      stg[j] = nv;
+    }
   }
   a0 = wave_sum(a0);
   a1 = wave_sum(a1);
