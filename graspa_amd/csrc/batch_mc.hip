@@ -53,14 +53,16 @@ constexpr int kBMaxMol = 8;            // atoms per molecule
 constexpr int kBMaxSites = 64;         // probe sites of one pair pass
 constexpr int kBGroup = 8;             // sites per pass over the atoms
 constexpr int kBDrawTrial = 16;        // first trial draw
-constexpr int kBMaxDraws = kBDrawTrial + 6 * kBMaxTrial;
+constexpr int kBMaxDraws = kBDrawTrial + 9 * kBMaxTrial;   // K trials, O orientations, O retrace orientations
 
 enum { kMvTrans = GRASPA_MOVE_TRANSLATION, kMvRot = GRASPA_MOVE_ROTATION, kMvIns = GRASPA_MOVE_INSERTION,
-       kMvDel = GRASPA_MOVE_DELETION, kMvWidom = GRASPA_MOVE_WIDOM, kMvNone = GRASPA_MOVE_NONE };
+       kMvDel = GRASPA_MOVE_DELETION, kMvWidom = GRASPA_MOVE_WIDOM, kMvNone = GRASPA_MOVE_NONE,
+       kMvReins = GRASPA_MOVE_REINSERTION };
 
 // Per-species set-up of one box (the Components fields RunMoves reads).
 struct BSpecies {
   double p_trans, p_rot, p_widom, p_swap;     // cumulative
+  double p_reins;                             // cumulative, between p_widom and p_swap
   double mol_fraction, fugacity, ideal_rosen, excl;   // excl = ExclusionIntra + ExclusionAtom
   int base, molsize, capacity, charged;
   int pseudo[16];
@@ -118,7 +120,7 @@ struct BState {
   int64_t total_mol;
   int64_t n_pseudo[16];
   double max_trans[kMaxComps][3], max_rot[kMaxComps][3];
-  int64_t cnt[kMaxComps][9];
+  int64_t cnt[kMaxComps][GRASPA_N_COUNTERS];
   int64_t upd_tot[kMaxComps][2], upd_acc[kMaxComps][2];
   double widom_w[kMaxComps], widom_n[kMaxComps];
   double running[10];
@@ -155,6 +157,7 @@ struct BMove {
   double ewp[kBW][2];
   double en[10];                    // graspa_energy of the move
   double W, pacc, same, cross;
+  double Wo, stored, eo[4];          // reinsertion: retrace weight, other first-bead weights, old energies
   unsigned long long ovl, smask;
   int type, tmove, comp, mol, M, start, charged, n_eval;
   int fail, accept, overlap;
@@ -486,7 +489,7 @@ __device__ void ewald_pass(const BBox& B, const BState& S, BMove& L, double2* __
 // is LJ only.  Returns the selected trial (-1 on failure) and multiplies *W.
 // ---------------------------------------------------------------------------
 __device__ int select_wave0(const BBox& B, BMove& L, const int n, const uint32_t ovm, const double u, const bool del,
-                            const bool chain, double* W)
+                            const bool chain, double* W, double* others = nullptr)
 {
   const int lane = threadIdx.x & 63;
   const bool alive = lane < n && !((ovm >> lane) & 1u);
@@ -536,6 +539,9 @@ __device__ int select_wave0(const BBox& B, BMove& L, const int n, const uint32_t
     if (chain && nk == 0) fail = true;
   }
   if (!fail) {
+    // the weights of the survivors other than the selected one (StoredR of a
+    // reinsertion, CBMC_FirstBead_Finish mc_widom.h:365)
+    if (others && sel >= 0) *others = rb - readlane_d(ew, sel);
     double w = rb / (double)n;
     if (sel >= 0 && !B.bias) w *= exp(-B.beta * readlane_d(er, sel));
     w *= *W;
@@ -896,6 +902,232 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
   }
 }
 
+// Pair energies of the sites L.site[0 .. n) into L.t4 per trial (chain
+// sites of a trial summed) and the trials' overlap bits: the overlap screen,
+// then the pair pass over the survivors; the molecule [ex_lo, ex_hi) is
+// excluded.
+template <bool MIXED, bool CUBIC>
+__device__ uint32_t trial_energies(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr,
+                                   const double2* ct, int nint, const double* fwl, const int n_trials, const int chain,
+                                   const int ex_lo, const int ex_hi)
+{
+  const int tid = threadIdx.x;
+  const int ns = n_trials * chain;
+  if (B.cell_cap > 0) screen_pass<CUBIC>(B, L, thr, 0, ns);
+  else if (tid == 0) L.smask = 0ull;
+  if (tid == 0) list_survivors(L, ns, chain);
+  __syncthreads();
+  if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, L.n_eval, ex_lo, ex_hi);
+  if (tid < n_trials) {
+    double e4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int a = 0; a < chain; ++a)
+      for (int t = 0; t < 4; ++t) e4[t] += L.e[tid * chain + a][t];
+    for (int t = 0; t < 4; ++t) L.t4[tid][t] = e4[t];
+  }
+  uint32_t ov = 0;
+  const unsigned long long amask = (chain >= 64) ? ~0ull : ((1ull << chain) - 1ull);
+  for (int o = 0; o < n_trials; ++o)
+    if ((L.ovl >> (o * chain)) & amask) ov |= 1u << o;
+  __syncthreads();
+  return ov;
+}
+
+// Retrace weight of trials 0 .. n on thread 0: the ordered sum of the
+// survivors' Boltzmann factors plus `others`, divided by `div`; the first
+// survivor (SelectedTrial 0 of the retrace, mc_widom.h:350-356, 588-604)
+// gives the energies added to L.eo and the real-space correction when the
+// bias is LJ only.
+__device__ double retrace_weight(const BBox& B, BMove& L, const int n, const uint32_t ov, const double others,
+                                 const int div)
+{
+  double rb = others;
+  int first = -1;
+  for (int t = 0; t < n; ++t) {
+    if ((ov >> t) & 1u) continue;
+    double tot = L.t4[t][0] + L.t4[t][2];
+    if (B.bias) tot += L.t4[t][1] + L.t4[t][3];
+    rb += exp(-B.beta * tot);
+    if (first < 0) first = t;
+  }
+  double w = rb / (double)div;
+  if (first >= 0) {
+    if (!B.bias) w *= exp(-B.beta * (L.t4[first][1] + L.t4[first][3]));
+    for (int t = 0; t < 4; ++t) L.eo[t] += L.t4[first][t];
+  }
+  return w;
+}
+
+// CBMC reinsertion (ReinsertionMove::Run, move_struct.h:170-407): a new
+// configuration grown as an insertion (K random first beads, O orientations
+// of the template about the selected one), the old one retraced (its own
+// first bead, whose weight joins the growth's other first-bead weights, and
+// O orientations with trial 0 its own), energies without the molecule
+// itself; W_new times the Ewald factor of moving the molecule; Pacc =
+// W_new / W_old.  Draws: trial k at kBDrawTrial + 3k, orientation o at
+// kBDrawTrial + 3K + 3o, retrace orientation o at kBDrawTrial + 3K + 3O + 3o.
+template <bool MIXED, bool CUBIC>
+__device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr,
+                            const double2* ct, int nint, const double* fwl, double2* eik)
+{
+  const int tid = threadIdx.x;
+  const int M = L.M, comp = L.comp, cap = B.guest_cap;
+  const int K = B.K, O = B.O, chain = M - 1;
+  const BSpecies& sp = B.sp[comp];
+  const int ex_lo = L.start, ex_hi = L.start + M;
+  // growth: first bead
+  if (tid < K) {
+    BSite s;
+    s.x = B.bc.cell[0] * L.ud[kBDrawTrial + 3 * tid];
+    s.y = B.bc.cell[4] * L.ud[kBDrawTrial + 3 * tid + 1];
+    s.z = B.bc.cell[8] * L.ud[kBDrawTrial + 3 * tid + 2];
+    s.q = sp.q[0];
+    s.type = sp.type[0];
+    s._p = 0;
+    L.site[tid] = s;
+  }
+  __syncthreads();
+  uint32_t ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, K, 1, ex_lo, ex_hi);
+  if (tid < 64) {
+    double W = 1.0, others = 0.0;
+    const int sel = select_wave0(B, L, K, ov, L.ud[5], false, false, &W, &others);
+    if (tid == 0) {
+      if (sel < 0) {
+        L.fail = 1;
+      } else {
+        L.W = W;
+        L.stored = others;
+        L.en[kHGV] = L.t4[sel][0]; L.en[kHGR] = L.t4[sel][1];
+        L.en[kGGV] = L.t4[sel][2]; L.en[kGGR] = L.t4[sel][3];
+        L.molp[0][0] = L.site[sel].x; L.molp[0][1] = L.site[sel].y; L.molp[0][2] = L.site[sel].z;
+      }
+    }
+  }
+  __syncthreads();
+  if (L.fail) return;
+  // growth: orientations about the selected first bead
+  if (chain > 0) {
+    for (int t = tid; t < O * chain; t += kBT) {
+      const int o = t / chain, a = t - o * chain;
+      const double* t0 = L.tmpl[comp][0];
+      const double* t1 = L.tmpl[comp][1 + a];
+      double v[3] = {t1[0] - t0[0], t1[1] - t0[1], t1[2] - t0[2]};
+      const int d = kBDrawTrial + 3 * K + 3 * o;
+      quat_rotate(v, L.ud[d], L.ud[d + 1], L.ud[d + 2]);
+      BSite s;
+      s.x = L.molp[0][0] + v[0]; s.y = L.molp[0][1] + v[1]; s.z = L.molp[0][2] + v[2];
+      s.q = sp.q[1 + a];
+      s.type = sp.type[1 + a];
+      s._p = 0;
+      L.site[t] = s;
+    }
+    __syncthreads();
+    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, O, chain, ex_lo, ex_hi);
+    if (tid < 64) {
+      double W = L.W;
+      const int sel = select_wave0(B, L, O, ov, L.ud[6], false, true, &W);
+      if (tid == 0) {
+        if (sel < 0) {
+          L.fail = 1;
+        } else {
+          L.W = W;
+          L.en[kHGV] += L.t4[sel][0]; L.en[kHGR] += L.t4[sel][1];
+          L.en[kGGV] += L.t4[sel][2]; L.en[kGGR] += L.t4[sel][3];
+          for (int a = 0; a < chain; ++a) {
+            const BSite& s = L.site[sel * chain + a];
+            L.molp[1 + a][0] = s.x; L.molp[1 + a][1] = s.y; L.molp[1 + a][2] = s.z;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (L.fail) return;
+  }
+  // retrace: the molecule's own first bead
+  if (tid == 0) {
+    BSite s;
+    s.x = B.g[L.start]; s.y = B.g[cap + L.start]; s.z = B.g[2 * cap + L.start];
+    s.q = sp.q[0];
+    s.type = sp.type[0];
+    s._p = 0;
+    L.site[0] = s;
+    for (int t = 0; t < 4; ++t) L.eo[t] = 0.0;
+  }
+  __syncthreads();
+  ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, 1, 1, ex_lo, ex_hi);
+  if (tid == 0) L.Wo = retrace_weight(B, L, 1, ov, L.stored, K);   // (own weight + StoredR) / K
+  __syncthreads();
+  // retrace: orientations about the old first bead, trial 0 its own
+  if (chain > 0) {
+    for (int t = tid; t < O * chain; t += kBT) {
+      const int o = t / chain, a = t - o * chain;
+      BSite s;
+      if (o == 0) {
+        const int j = L.start + 1 + a;
+        s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j];
+      } else {
+        const double* t0 = L.tmpl[comp][0];
+        const double* t1 = L.tmpl[comp][1 + a];
+        double v[3] = {t1[0] - t0[0], t1[1] - t0[1], t1[2] - t0[2]};
+        const int d = kBDrawTrial + 3 * K + 3 * O + 3 * o;
+        quat_rotate(v, L.ud[d], L.ud[d + 1], L.ud[d + 2]);
+        s.x = B.g[L.start] + v[0]; s.y = B.g[cap + L.start] + v[1]; s.z = B.g[2 * cap + L.start] + v[2];
+      }
+      s.q = sp.q[1 + a];
+      s.type = sp.type[1 + a];
+      s._p = 0;
+      L.site[t] = s;
+    }
+    __syncthreads();
+    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, O, chain, ex_lo, ex_hi);
+    if (tid == 0) L.Wo *= retrace_weight(B, L, O, ov, 0.0, O);
+    __syncthreads();
+  }
+  // Ewald change of moving the molecule: old sites, then new ones
+  if (L.charged) {
+    if (tid < 2 * M) {
+      const int i = tid < M ? tid : tid - M;
+      BSite s;
+      if (tid < M) {
+        const int j = L.start + i;
+        s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j];
+      } else {
+        s.x = L.molp[i][0]; s.y = L.molp[i][1]; s.z = L.molp[i][2];
+      }
+      s.q = sp.q[i];
+      s.type = sp.type[i];
+      s._p = 0;
+      L.site[tid] = s;
+    }
+    __syncthreads();
+    ewald_pass(B, S, L, eik, M, M);
+  }
+  if (tid == 0) {
+    L.en[kHGV] -= L.eo[0]; L.en[kHGR] -= L.eo[1];
+    L.en[kGGV] -= L.eo[2]; L.en[kGGR] -= L.eo[3];
+    double W = L.W;
+    if (L.charged) {
+      L.en[kGGE] = L.same;
+      L.en[kHGE] = 2.0 * L.cross;
+      W *= exp(-B.beta * (L.same + 2.0 * L.cross));
+    }
+    L.W = W;
+    const double pacc = W / L.Wo;
+    L.pacc = pacc;
+    const int acc = (W > 1e-150 && L.ud[4] < pacc) ? 1 : 0;
+    L.accept = acc;
+    if (acc) {
+      S.cnt[comp][10]++;
+      add_energy(S.running, L.en);
+      if (L.charged) S.sk_cur ^= 1;
+    }
+  }
+  __syncthreads();
+  if (L.accept && tid < M) {
+    const int j = L.start + tid;
+    B.g[j] = L.molp[tid][0]; B.g[cap + j] = L.molp[tid][1]; B.g[2 * cap + j] = L.molp[tid][2];
+  }
+}
+
 // Update_Max_Translation / Update_Max_Rotation (mc_utilities.h:612-665)
 __device__ inline void update_max(double* v, int64_t& acc, int64_t& tot, double hi)
 {
@@ -958,7 +1190,7 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
   if (MIXED)
     for (int i = tid; i < B.coul_n; i += kBT) dyn[nff4 + nthr + i] = B.coul[i];
   __syncthreads();
-  const int nd = kBDrawTrial + 3 * B.K + 3 * B.O;
+  const int nd = kBDrawTrial + 3 * B.K + 6 * B.O;
   for (int mv = 0; mv < max_moves; ++mv) {
     if (S.cycle >= S.target || S.error) break;
     // this move's uniforms, one Philox block (two uniforms) per thread
@@ -995,6 +1227,7 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
       if (r < sp.p_trans) { tm = kMvTrans; work = n > 0; }
       else if (r < sp.p_rot) { tm = kMvRot; work = n > 0; }
       else if (r < sp.p_widom) { tm = kMvWidom; work = 1; }
+      else if (r < sp.p_reins) { tm = kMvReins; work = n > 0; }
       else if (r < sp.p_swap) {
         if (u[3] < 0.5) { tm = kMvIns; work = 1; }
         else { tm = kMvDel; work = n > 0; }
@@ -1015,11 +1248,13 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
         else if (tm == kMvIns) S.cnt[comp][4]++;
         else if (tm == kMvDel) S.cnt[comp][6]++;
         else if (tm == kMvWidom) S.cnt[comp][8]++;
+        else if (tm == kMvReins) S.cnt[comp][9]++;
       }
     }
     __syncthreads();
     const int type = L.type;
     if (type == kMvTrans || type == kMvRot) single_body<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
+    else if (type == kMvReins) reinsertion<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
     else if (type != kMvNone) cbmc<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
     __syncthreads();
     if (tid == 0) {
@@ -1223,6 +1458,7 @@ int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
       return fail(GRASPA_ERR_ARG, "n_trial_orient * (molsize - 1) must be <= 64");
     BSpecies& d = h.sp[c];
     d.p_trans = sp.p_translation; d.p_rot = sp.p_rotation; d.p_widom = sp.p_widom; d.p_swap = sp.p_swap;
+    d.p_reins = sp.p_reinsertion;
     d.mol_fraction = sp.mol_fraction; d.fugacity = sp.fugacity_coeff; d.ideal_rosen = sp.ideal_rosen;
     d.excl = sp.excl_intra + sp.excl_self;
     d.base = cc.base; d.molsize = cc.molsize; d.capacity = cc.capacity;
@@ -1531,7 +1767,7 @@ int graspa_batch_box_stats(graspa_batch* bt, int i, graspa_energy* running, int*
   for (int c = 1; c < ncomp; ++c) {
     if (n_mol) n_mol[c - 1] = s.n_mol[c];
     if (counters)
-      for (int k = 0; k < 9; ++k) counters[9 * (c - 1) + k] = s.cnt[c][k];
+      for (int k = 0; k < GRASPA_N_COUNTERS; ++k) counters[GRASPA_N_COUNTERS * (c - 1) + k] = s.cnt[c][k];
     if (widom_sum_w) widom_sum_w[c - 1] = s.widom_w[c];
     if (widom_n) widom_n[c - 1] = s.widom_n[c];
     if (load_sum) load_sum[c - 1] = s.load_sum[c];

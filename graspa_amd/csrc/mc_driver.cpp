@@ -67,7 +67,8 @@ struct graspa_mc {
   size_t pool_off = 0;
   int64_t total_molecules = 1;
   int64_t n_pseudo[16] = {0};
-  int64_t cnt[8][9] = {{0}};  // trans_tot, trans_acc, rot_tot, rot_acc, ins_tot, ins_acc, del_tot, del_acc, widom
+  // trans_tot, trans_acc, rot_tot, rot_acc, ins_tot, ins_acc, del_tot, del_acc, widom, reins_tot, reins_acc
+  int64_t cnt[8][GRASPA_N_COUNTERS] = {{0}};
   int64_t upd_tot[8][2] = {{0}}, upd_acc[8][2] = {{0}};  // since the last max-step update
   double widom_w[8] = {0}, widom_n[8] = {0};
   graspa_energy running{};
@@ -466,6 +467,164 @@ int cbmc_move(graspa_mc* mc, int comp, int mol, int move, graspa_trace& tr, int 
   return GRASPA_OK;
 }
 
+// CBMC reinsertion: ReinsertionMove::Run (move_struct.h:170-407).  Growth of
+// a new configuration as an insertion (K random first beads, O orientations
+// of the slot-0 template about the selected bead, energies without the
+// molecule itself); retrace of the old one with one first-bead trial (its own
+// position, whose weight is added to the growth's other first-bead weights,
+// CBMC_FirstBead_Finish mc_widom.h:365-366) and O orientations (trial 0 = its
+// own); W_new times the Ewald factor of moving the molecule; Pacc = W_new /
+// W_old.  Pool draws: K triples, selection, O triples, selection, one triple
+// (retrace first bead, unused), O triples, Metropolis.
+int reinsertion_move(graspa_mc* mc, int comp, int mol, graspa_trace& tr)
+{
+  graspa_box* b = mc->box;
+  Component& c = b->comps[comp];
+  const int M = c.molsize, chain = M - 1;
+  const int K = mc->p.n_trial_pos, O = mc->p.n_trial_orient;
+  const int start = mol * M;
+  const int64_t molid = c.molid[start];
+  const double beta = mc->p.beta;
+  const int bias = mc->p.vdw_real_bias;
+  mc->cnt[comp][9]++;
+  tr.accepted = 0; tr.overlap = 0; tr.pacc = 0.0; tr.rosen = 0.0; tr.de = graspa_energy{};
+  std::vector<double> sp3(3 * (size_t)std::max(K, O * std::max(chain, 1)));
+  std::vector<double> e4(4 * (size_t)std::max(K, O));
+  std::vector<uint8_t> ov(std::max(K, O));
+  std::vector<double> oldp(3 * M), newp(3 * M), q(M);
+  std::vector<int> ct(std::max(chain, 1));
+  std::vector<double> cq(std::max(chain, 1));
+  for (int i = 0; i < M; ++i) {
+    for (int d = 0; d < 3; ++d) oldp[3 * i + d] = c.pos[3 * (start + i) + d];
+    q[i] = c.q[start + i];
+  }
+  for (int a = 0; a < chain; ++a) {
+    ct[a] = c.type[start + 1 + a];
+    cq[a] = c.q[start + 1 + a];
+  }
+  const int t0 = c.type[start];
+  const double q0 = c.q[start];
+  Trials ts;
+  double en_new[4] = {0, 0, 0, 0}, en_old[4] = {0, 0, 0, 0};
+  // growth of the new configuration
+  mc->pool_check((size_t)K);
+  for (int k = 0; k < K; ++k) {
+    const double* R = &mc->pool[3 * (mc->pool_off + k)];
+    for (int d = 0; d < 3; ++d) sp3[3 * k + d] = b->bc.cell[4 * d] * R[d];
+  }
+  mc->pool_off += (size_t)K;
+  if (int rc = graspa_trial_energies(b, K, 1, sp3.data(), &t0, &q0, comp, molid, -1, -1, e4.data(), ov.data()))
+    return rc;
+  survivors(e4, ov, K, beta, bias, ts);
+  if (ts.keep.empty()) return GRASPA_OK;
+  const int sel = select_trial(ts.logw, mc->uniform());
+  const double rb = sum_exp(ts);
+  if (rb < 1e-150) return GRASPA_OK;
+  const double stored_r = rb - std::exp(ts.logw[sel]);
+  double Wn = rb / (double)K;
+  if (!bias) Wn *= std::exp(-beta * (ts.e[sel][1] + ts.e[sel][3]));
+  if (Wn <= 1e-150) return GRASPA_OK;
+  for (int k = 0; k < 4; ++k) en_new[k] = ts.e[sel][k];
+  for (int d = 0; d < 3; ++d) newp[d] = sp3[3 * ts.keep[sel] + d];
+  if (chain > 0) {
+    mc->pool_check((size_t)O);
+    for (int o = 0; o < O; ++o) {
+      const double* R = &mc->pool[3 * (mc->pool_off + o)];
+      for (int a = 0; a < chain; ++a) {
+        double v[3] = {c.pos[3 * (1 + a)] - c.pos[0], c.pos[3 * (1 + a) + 1] - c.pos[1],
+                       c.pos[3 * (1 + a) + 2] - c.pos[2]};
+        rotate_quaternion(v, R[0], R[1], R[2]);
+        for (int d = 0; d < 3; ++d) sp3[3 * (o * chain + a) + d] = newp[d] + v[d];
+      }
+    }
+    mc->pool_off += (size_t)O;
+    if (int rc = graspa_trial_energies(b, O, chain, sp3.data(), ct.data(), cq.data(), comp, molid, -1, -1, e4.data(),
+                                       ov.data()))
+      return rc;
+    survivors(e4, ov, O, beta, bias, ts);
+    if (ts.keep.empty()) return GRASPA_OK;
+    const int osel = select_trial(ts.logw, mc->uniform());
+    const double rc_sum = sum_exp(ts);
+    if (rc_sum < 1e-150) return GRASPA_OK;
+    double Wc = rc_sum / (double)O;
+    if (!bias) Wc *= std::exp(-beta * (ts.e[osel][1] + ts.e[osel][3]));
+    Wn *= Wc;
+    if (Wn <= 1e-150) return GRASPA_OK;
+    for (int k = 0; k < 4; ++k) en_new[k] += ts.e[osel][k];
+    const int ot = ts.keep[osel];
+    for (int a = 0; a < chain; ++a)
+      for (int d = 0; d < 3; ++d) newp[3 * (1 + a) + d] = sp3[3 * (ot * chain + a) + d];
+  }
+  // retrace of the old configuration
+  mc->pool_check(1);
+  mc->pool_off += 1;
+  if (int rc = graspa_trial_energies(b, 1, 1, oldp.data(), &t0, &q0, comp, molid, -1, -1, e4.data(), ov.data()))
+    return rc;
+  survivors(e4, ov, 1, beta, bias, ts);
+  double Wo = (sum_exp(ts) + stored_r) / (double)K;
+  if (!ts.keep.empty()) {
+    if (!bias) Wo *= std::exp(-beta * (ts.e[0][1] + ts.e[0][3]));
+    for (int k = 0; k < 4; ++k) en_old[k] = ts.e[0][k];
+  }
+  if (chain > 0) {
+    mc->pool_check((size_t)O);
+    for (int o = 0; o < O; ++o) {
+      const double* R = &mc->pool[3 * (mc->pool_off + o)];
+      for (int a = 0; a < chain; ++a) {
+        const int s = o * chain + a;
+        if (o == 0) {
+          for (int d = 0; d < 3; ++d) sp3[3 * s + d] = oldp[3 * (1 + a) + d];
+        } else {
+          double v[3] = {c.pos[3 * (1 + a)] - c.pos[0], c.pos[3 * (1 + a) + 1] - c.pos[1],
+                         c.pos[3 * (1 + a) + 2] - c.pos[2]};
+          rotate_quaternion(v, R[0], R[1], R[2]);
+          for (int d = 0; d < 3; ++d) sp3[3 * s + d] = oldp[d] + v[d];
+        }
+      }
+    }
+    mc->pool_off += (size_t)O;
+    if (int rc = graspa_trial_energies(b, O, chain, sp3.data(), ct.data(), cq.data(), comp, molid, -1, -1, e4.data(),
+                                       ov.data()))
+      return rc;
+    survivors(e4, ov, O, beta, bias, ts);
+    double Wc = sum_exp(ts) / (double)O;
+    if (!ts.keep.empty()) {    // SelectedTrial = 0 of the survivors (mc_widom.h:588-604)
+      if (!bias) Wc *= std::exp(-beta * (ts.e[0][1] + ts.e[0][3]));
+      for (int k = 0; k < 4; ++k) en_old[k] += ts.e[0][k];
+    }
+    Wo *= Wc;
+  }
+  graspa_energy de{};
+  de.hg_vdw = en_new[0] - en_old[0];
+  de.hg_real = en_new[1] - en_old[1];
+  de.gg_vdw = en_new[2] - en_old[2];
+  de.gg_real = en_new[3] - en_old[3];
+  double qabs = 0.0;
+  for (int i = 0; i < M; ++i) qabs += std::fabs(q[i]);
+  const bool charged = b->bc.use_charges && qabs > 1e-6;
+  if (charged) {
+    double same = 0.0, cross = 0.0;
+    if (int rc = graspa_ewald_delta(b, M, oldp.data(), q.data(), M, newp.data(), q.data(), &same, &cross)) return rc;
+    de.gg_ewald = same;
+    de.hg_ewald = 2.0 * cross;
+    Wn *= std::exp(-beta * (same + 2.0 * cross));
+  }
+  // ReinsertionMove::Acceptance (move_struct.h:323-358)
+  const double u = mc->uniform();
+  const double pacc = Wn / Wo;
+  tr.rosen = Wn;
+  tr.pacc = pacc;
+  if (Wn <= 1e-150 || !(u < pacc)) return GRASPA_OK;
+  if (int rc = graspa_commit_move(b, comp, mol, newp.data())) return rc;
+  if (charged)
+    if (int rc = graspa_ewald_accept(b)) return rc;
+  mc->cnt[comp][10]++;
+  tr.accepted = 1;
+  tr.de = de;
+  eadd(mc->running, de);
+  return GRASPA_OK;
+}
+
 // Update_Max_Translation / Update_Max_Rotation (mc_utilities.h:612-665)
 void update_max(double v[3], int64_t& acc, int64_t& tot, double hi)
 {
@@ -561,6 +720,9 @@ int graspa_mc_cycles(graspa_mc* mc, int first_cycle, int n_cycles, int productio
       } else if (r < sp.p_widom) {
         tr.move = GRASPA_MOVE_WIDOM;
         rc = cbmc_move(mc, comp, mol, GRASPA_MOVE_WIDOM, tr, production);
+      } else if (r < sp.p_reinsertion) {
+        tr.move = GRASPA_MOVE_REINSERTION;
+        if (b->comps[comp].n_mol > 0) rc = reinsertion_move(mc, comp, mol, tr);
       } else if (r < sp.p_swap) {
         if (mc->uniform() < 0.5) {
           tr.move = GRASPA_MOVE_INSERTION;
@@ -598,7 +760,7 @@ int graspa_mc_stats(graspa_mc* mc, graspa_energy* running, int* n_mol, int64_t* 
   for (int c = 1; c < ncomp; ++c) {
     if (n_mol) n_mol[c - 1] = mc->box->comps[c].n_mol;
     if (counters)
-      for (int k = 0; k < 9; ++k) counters[9 * (c - 1) + k] = mc->cnt[c][k];
+      for (int k = 0; k < GRASPA_N_COUNTERS; ++k) counters[GRASPA_N_COUNTERS * (c - 1) + k] = mc->cnt[c][k];
     if (widom_sum_w) widom_sum_w[c - 1] = mc->widom_w[c];
     if (widom_n) widom_n[c - 1] = mc->widom_n[c];
   }

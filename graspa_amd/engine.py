@@ -67,7 +67,7 @@ class Species(C.Structure):
                 ("p_swap", C.c_double), ("mol_fraction", C.c_double), ("fugacity_coeff", C.c_double),
                 ("ideal_rosen", C.c_double), ("excl_intra", C.c_double), ("excl_self", C.c_double),
                 ("max_translation", C.c_double * 3), ("max_rotation", C.c_double * 3),
-                ("pseudo_per_mol", C.c_int64 * 16)]
+                ("pseudo_per_mol", C.c_int64 * 16), ("p_reinsertion", C.c_double)]
 
 
 class MCParams(C.Structure):
@@ -90,6 +90,8 @@ class Trace(C.Structure):
 
 
 NBLOCK, MAX_COMPS = 16, 8
+N_COUNTERS = 11          # move counters per species (GRASPA_N_COUNTERS)
+MOVE_REINSERTION = 6
 
 
 class BlockStats(C.Structure):
@@ -470,11 +472,12 @@ def make_mc_params(system, box, pool_size=333334):
     p.n_species = len(s.components)
     for i, c in enumerate(s.components):
         cp = c.cumulative_probabilities()
-        if cp["reinsertion"] != cp["widom"] or cp["identity"] != cp["reinsertion"]:
-            raise EngineError("reinsertion / identity-swap moves are not in the parity driver")
+        if cp["identity"] != cp["reinsertion"]:
+            raise EngineError("identity-swap moves are not supported")
         sp = p.species[i]
         sp.p_translation, sp.p_rotation, sp.p_widom, sp.p_swap = (cp["translation"], cp["rotation"], cp["widom"],
                                                                   cp["swap"])
+        sp.p_reinsertion = cp["reinsertion"]
         sp.mol_fraction, sp.fugacity_coeff, sp.ideal_rosen = c.mol_fraction, c.fugacity_coeff, c.ideal_rosen
         sp.excl_intra, sp.excl_self = exclusion_energy(s, i + 1)
         for d in range(3):
@@ -526,11 +529,11 @@ class MonteCarlo:
         ns = len(self.box.system.components)
         e = Energy()
         nm = np.zeros(ns, dtype=np.int32)
-        cnt = np.zeros(9 * ns, dtype=np.int64)
+        cnt = np.zeros(N_COUNTERS * ns, dtype=np.int64)
         ww = np.zeros(ns)
         wn = np.zeros(ns)
         _check(lib().graspa_mc_stats(self.h, C.byref(e), _p(nm, _i32p), _p(cnt, _i64p), _p(ww), _p(wn)), "mc_stats")
-        return e, nm, cnt.reshape(ns, 9), ww, wn
+        return e, nm, cnt.reshape(ns, N_COUNTERS), ww, wn
 
     def close(self):
         if self.h:
@@ -629,12 +632,12 @@ class Batch:
         ns = len(self.boxes[i].system.components)
         e = Energy()
         nm = np.zeros(ns, dtype=np.int32)
-        cnt = np.zeros(9 * ns, dtype=np.int64)
+        cnt = np.zeros(N_COUNTERS * ns, dtype=np.int64)
         ww, wn, ls, ls2 = np.zeros(ns), np.zeros(ns), np.zeros(ns), np.zeros(ns)
         nl, mv, cy = C.c_int64(), C.c_int64(), C.c_int64()
         _check(lib().graspa_batch_box_stats(self.h, i, C.byref(e), _p(nm, _i32p), _p(cnt, _i64p), _p(ww), _p(wn),
                                             _p(ls), _p(ls2), C.byref(nl), C.byref(mv), C.byref(cy)), "batch_stats")
-        return BatchStats(e, nm, cnt.reshape(ns, 9), ww, wn, ls, ls2, nl.value, mv.value, cy.value)
+        return BatchStats(e, nm, cnt.reshape(ns, N_COUNTERS), ww, wn, ls, ls2, nl.value, mv.value, cy.value)
 
     def block_stats(self, i):
         """Per-block sums of slot i's current production run (graspa_batch_block_stats)."""

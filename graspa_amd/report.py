@@ -149,7 +149,7 @@ class RunRecord:
     names: List[str]                        # species names (adsorbates)
     framework_name: str
     precision: int
-    counters: np.ndarray                    # (n_species, 9): graspa_mc_stats layout
+    counters: np.ndarray                    # (n_species, 11): graspa_mc_stats layout
     n_mol: List[int]
     drift: DriftReport
     blocks: Optional[dict] = None           # graspa_batch_block_stats arrays
@@ -268,6 +268,9 @@ def write_output(rec: RunRecord, path_or_file):
         out.append("Insertion Accepted:    %d" % c[5])
         out.append("Deletion Performed:    %d" % c[6])
         out.append("Deletion Accepted:     %d" % c[7])
+        if len(c) > 10:
+            out.append("Reinsertion Performed: %d" % c[9])
+            out.append("Reinsertion Accepted:  %d" % c[10])
         out.append("====================================================")
         if c[8]:
             out.append("Widom Performed: %d" % c[8])

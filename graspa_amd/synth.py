@@ -64,7 +64,13 @@ def _butane():
                 mass=[15.035, 14.027], atoms=[(0, pts[0]), (1, pts[1]), (1, pts[2]), (0, pts[3])])
 
 
-SPECIES = {"CO2": _co2, "N2": _n2, "butane": _butane}
+def _methane():
+    # united-atom methane (TraPPE: eps/k 148 K, sigma 3.73 A), uncharged, one site
+    return dict(name="methane", names=["CH4"], eps=[148.0], sig=[3.73], charge=[0.0], mass=[16.04246],
+                atoms=[(0, (0.0, 0.0, 0.0))])
+
+
+SPECIES = {"CO2": _co2, "N2": _n2, "butane": _butane, "methane": _methane}
 
 
 def build(n_fw=1536, L=24.0, species=("CO2",), mol_fraction=None, seed=0, T=300.0, P=1e5, cutoff=12.0,
@@ -122,6 +128,7 @@ def build(n_fw=1536, L=24.0, species=("CO2",), mol_fraction=None, seed=0, T=300.
         c.p_translation = pr.get("translation", 0.0)
         c.p_rotation = pr.get("rotation", 0.0)
         c.p_widom = pr.get("widom", 0.0)
+        c.p_reinsertion = pr.get("reinsertion", 0.0)
         c.p_swap = pr.get("swap", 0.0)
     return s
 

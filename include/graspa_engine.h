@@ -225,6 +225,7 @@ typedef struct {
   double excl_intra, excl_self;                        /* rigid exclusion (ewald_preparation.h:351-366) */
   double max_translation[3], max_rotation[3];
   int64_t pseudo_per_mol[16];                          /* pseudo-atom counts of one molecule            */
+  double p_reinsertion;            /* cumulative, between p_widom and p_swap (a value <= p_widom: none)  */
 } graspa_species;
 
 typedef struct {
@@ -256,6 +257,10 @@ typedef struct {
 #define GRASPA_MOVE_DELETION    3
 #define GRASPA_MOVE_WIDOM       4
 #define GRASPA_MOVE_NONE        5
+#define GRASPA_MOVE_REINSERTION 6   /* ReinsertionMove (move_struct.h:170-407) */
+/* Move counters per species: trans_tot, trans_acc, rot_tot, rot_acc,
+ * ins_tot, ins_acc, del_tot, del_acc, widom_tot, reins_tot, reins_acc. */
+#define GRASPA_N_COUNTERS      11
 
 /* RandomNumber::Setup (data_struct.h:1287-1346) plus the move bookkeeping of
  * Initialize (main.cpp:58-343).  Uses the engine's restatement of glibc rand(). */
@@ -267,8 +272,7 @@ void graspa_mc_destroy(graspa_mc* mc);
 int  graspa_mc_cycles(graspa_mc* mc, int first_cycle, int n_cycles, int production, graspa_trace* trace,
                       int max_trace, int64_t* n_steps);
 /* Running energy change (SystemComponents.deltaE), molecule counts, move
- * acceptance counters [species][{trans_tot, trans_acc, rot_tot, rot_acc,
- * ins_tot, ins_acc, del_tot, del_acc, widom_tot}], Widom sums. */
+ * counters [species][GRASPA_N_COUNTERS], Widom sums. */
 int  graspa_mc_stats(graspa_mc* mc, graspa_energy* running, int* n_mol, int64_t* counters, double* widom_sum_w,
                      double* widom_n);
 /* Per-step wall-clock latency (seconds) of the last graspa_mc_cycles call,

@@ -1043,6 +1043,171 @@ fail:
   if (move == OR_WIDOM && mc->production) { mc->widom_rosen_n[comp] += 1.0; }
 }
 
+/* CBMC reinsertion: ReinsertionMove::Run (move_struct.h:170-407).  A new
+ * configuration of molecule `mol` is grown as an insertion (K random first
+ * beads, O orientations of the template about the selected bead, energies
+ * excluding the molecule itself); the old one is retraced with one first-bead
+ * trial (its own position, whose weight is added to the new growth's other
+ * first-bead weights, StoredR, CBMC_FirstBead_Finish mc_widom.h:365-366) and
+ * O orientations (trial 0 = its own).  W_new is multiplied by the Ewald
+ * change of moving the molecule; Pacc = W_new / W_old.  Draws in the pool
+ * order: K triples, selection, O triples, selection, 1 triple (retrace first
+ * bead, unused), O triples (retrace), Metropolis uniform. */
+static void reinsertion_move(mc_ctx* x, int comp, int mol, or_trace* tr)
+{
+  or_atoms* A = &x->comps[comp];
+  or_mc* mc = x->mc;
+  const or_box* b = x->b;
+  const int M = A->molsize, chain = M - 1;
+  const int K = mc->n_trial_pos, O = mc->n_trial_orient;
+  const int start = mol * M;
+  const int64_t molid = A->molid[start];
+  const double beta = mc->beta;
+  mc->reins_total[comp]++;
+  static double sp[3 * 1024], sq[1024], s4[4 * 1024];
+  static int st[1024];
+  static uint8_t sov[1024];
+  static trial_set ts;
+  or_energy en_new, en_old, de;
+  memset(&en_new, 0, sizeof(en_new));
+  memset(&en_old, 0, sizeof(en_old));
+  memset(&de, 0, sizeof(de));
+  tr->accepted = 0; tr->overlap = 0; tr->pacc = 0.0; tr->rosen = 0.0;
+  memset(&tr->de, 0, sizeof(tr->de));
+
+  /* ---- growth of the new configuration (REINSERTION_INSERTION) ---- */
+  if (!x->philox) pool_check((size_t)K);
+  for (int k = 0; k < K; k++) {
+    double Rp[3];
+    const double* R = Rp;
+    if (x->philox) for (int d = 0; d < 3; d++) Rp[d] = mc_draw(x, D_TRIAL + 3 * k + d);
+    else R = &g_pool[3 * (g_pool_offset + k)];
+    sp[3 * k] = b->cell[0] * R[0]; sp[3 * k + 1] = b->cell[4] * R[1]; sp[3 * k + 2] = b->cell[8] * R[2];
+    st[k] = A->type[start]; sq[k] = A->charge[start];
+  }
+  if (!x->philox) g_pool_offset += (size_t)K;
+  or_site_energies(b, x->comps, x->ncomp, K, sp, st, sq, comp, molid, -1, -1, s4, sov);
+  sum_trials(s4, sov, K, 1, beta, mc->vdw_real_bias, &ts);
+  if (ts.n_keep == 0) return;
+  int sel = select_trial(ts.rosen, ts.n_keep, x->philox ? mc_draw(x, D_SEL_FB) : or_rand_uniform());
+  double rb = sum_exp(&ts);
+  if (rb < 1e-150) return;
+  const double stored_r = rb - exp(ts.rosen[sel]);
+  double Wn = rb / (double)K;
+  if (!mc->vdw_real_bias) Wn *= exp(-beta * (ts.e4[sel][1] + ts.e4[sel][3]));
+  if (Wn <= 1e-150) return;
+  const int fbt = ts.keep[sel];
+  en_new.hg_vdw = ts.e4[sel][0]; en_new.hg_real = ts.e4[sel][1];
+  en_new.gg_vdw = ts.e4[sel][2]; en_new.gg_real = ts.e4[sel][3];
+  double new_pos[3 * 64], old_pos[3 * 64], q[64];
+  for (int i = 0; i < M; i++) {
+    for (int d = 0; d < 3; d++) old_pos[3 * i + d] = A->pos[3 * (start + i) + d];
+    q[i] = A->charge[start + i];
+  }
+  for (int d = 0; d < 3; d++) new_pos[d] = sp[3 * fbt + d];
+  if (chain > 0) {
+    if (!x->philox) pool_check((size_t)O);
+    for (int t = 0; t < O; t++) {
+      double Rp[3];
+      const double* R = Rp;
+      if (x->philox) for (int d = 0; d < 3; d++) Rp[d] = mc_draw(x, D_TRIAL + 3 * K + 3 * t + d);
+      else R = &g_pool[3 * (g_pool_offset + t)];
+      for (int a = 0; a < chain; a++) {
+        const int s = t * chain + a;
+        double vec[3] = {A->pos[3 * (1 + a)] - A->pos[0], A->pos[3 * (1 + a) + 1] - A->pos[1],
+                         A->pos[3 * (1 + a) + 2] - A->pos[2]};
+        rotate_quaternion(vec, R[0], R[1], R[2]);
+        for (int d = 0; d < 3; d++) sp[3 * s + d] = new_pos[d] + vec[d];
+        st[s] = A->type[start + 1 + a]; sq[s] = A->charge[start + 1 + a];
+      }
+    }
+    if (!x->philox) g_pool_offset += (size_t)O;
+    or_site_energies(b, x->comps, x->ncomp, O * chain, sp, st, sq, comp, molid, -1, -1, s4, sov);
+    sum_trials(s4, sov, O, chain, beta, mc->vdw_real_bias, &ts);
+    if (ts.n_keep == 0) return;
+    const int osel = select_trial(ts.rosen, ts.n_keep, x->philox ? mc_draw(x, D_SEL_OR) : or_rand_uniform());
+    const double rc = sum_exp(&ts);
+    if (rc < 1e-150) return;
+    double Wc = rc / (double)O;
+    if (!mc->vdw_real_bias) Wc *= exp(-beta * (ts.e4[osel][1] + ts.e4[osel][3]));
+    Wn *= Wc;
+    if (Wn <= 1e-150) return;
+    const int ot = ts.keep[osel];
+    en_new.hg_vdw += ts.e4[osel][0]; en_new.hg_real += ts.e4[osel][1];
+    en_new.gg_vdw += ts.e4[osel][2]; en_new.gg_real += ts.e4[osel][3];
+    for (int a = 0; a < chain; a++)
+      for (int d = 0; d < 3; d++) new_pos[3 * (1 + a) + d] = sp[3 * (ot * chain + a) + d];
+  }
+
+  /* ---- retrace of the old configuration (REINSERTION_RETRACE) ---- */
+  if (!x->philox) { pool_check(1); g_pool_offset += 1; }      /* the retrace first bead takes one triple */
+  for (int d = 0; d < 3; d++) sp[d] = old_pos[d];
+  st[0] = A->type[start]; sq[0] = A->charge[start];
+  or_site_energies(b, x->comps, x->ncomp, 1, sp, st, sq, comp, molid, -1, -1, s4, sov);
+  sum_trials(s4, sov, 1, 1, beta, mc->vdw_real_bias, &ts);
+  double Wo = (sum_exp(&ts) + stored_r) / (double)K;
+  if (ts.n_keep > 0) {
+    if (!mc->vdw_real_bias) Wo *= exp(-beta * (ts.e4[0][1] + ts.e4[0][3]));
+    en_old.hg_vdw = ts.e4[0][0]; en_old.hg_real = ts.e4[0][1];
+    en_old.gg_vdw = ts.e4[0][2]; en_old.gg_real = ts.e4[0][3];
+  }
+  if (chain > 0) {
+    if (!x->philox) pool_check((size_t)O);
+    for (int t = 0; t < O; t++) {
+      double Rp[3];
+      const double* R = Rp;
+      if (x->philox) for (int d = 0; d < 3; d++) Rp[d] = mc_draw(x, D_TRIAL + 3 * K + 3 * O + 3 * t + d);
+      else R = &g_pool[3 * (g_pool_offset + t)];
+      for (int a = 0; a < chain; a++) {
+        const int s = t * chain + a;
+        if (t == 0) {
+          for (int d = 0; d < 3; d++) sp[3 * s + d] = old_pos[3 * (1 + a) + d];
+        } else {
+          double vec[3] = {A->pos[3 * (1 + a)] - A->pos[0], A->pos[3 * (1 + a) + 1] - A->pos[1],
+                           A->pos[3 * (1 + a) + 2] - A->pos[2]};
+          rotate_quaternion(vec, R[0], R[1], R[2]);
+          for (int d = 0; d < 3; d++) sp[3 * s + d] = old_pos[d] + vec[d];
+        }
+        st[s] = A->type[start + 1 + a]; sq[s] = A->charge[start + 1 + a];
+      }
+    }
+    if (!x->philox) g_pool_offset += (size_t)O;
+    or_site_energies(b, x->comps, x->ncomp, O * chain, sp, st, sq, comp, molid, -1, -1, s4, sov);
+    sum_trials(s4, sov, O, chain, beta, mc->vdw_real_bias, &ts);
+    double Wc = sum_exp(&ts) / (double)O;
+    if (ts.n_keep > 0) {           /* SelectedTrial = 0 of the survivors (mc_widom.h:588-604) */
+      if (!mc->vdw_real_bias) Wc *= exp(-beta * (ts.e4[0][1] + ts.e4[0][3]));
+      en_old.hg_vdw += ts.e4[0][0]; en_old.hg_real += ts.e4[0][1];
+      en_old.gg_vdw += ts.e4[0][2]; en_old.gg_real += ts.e4[0][3];
+    }
+    Wo *= Wc;
+  }
+  de.hg_vdw = en_new.hg_vdw - en_old.hg_vdw; de.hg_real = en_new.hg_real - en_old.hg_real;
+  de.gg_vdw = en_new.gg_vdw - en_old.gg_vdw; de.gg_real = en_new.gg_real - en_old.gg_real;
+
+  /* ---- Ewald change of moving the molecule (Calculate_AdjustRosenbluth) ---- */
+  if (comp_has_charge(x, comp)) {
+    double same, cross;
+    or_ewald_delta(b, x->sk_ads, x->sk_fw, M, old_pos, q, M, new_pos, q, &same, &cross, x->sk_tmp);
+    de.gg_ewald = same;
+    de.hg_ewald = 2.0 * cross;
+    Wn *= exp(-beta * (same + 2.0 * cross));
+  }
+  /* ---- acceptance (ReinsertionMove::Acceptance) ---- */
+  const double u = x->philox ? mc_draw(x, D_ACC) : or_rand_uniform();
+  const double pacc = Wn / Wo;
+  tr->rosen = Wn;
+  tr->pacc = pacc;
+  if (Wn <= 1e-150 || !(u < pacc)) return;
+  tr->accepted = 1;
+  for (int i = 0; i < M; i++)
+    for (int d = 0; d < 3; d++) A->pos[3 * (start + i) + d] = new_pos[3 * i + d];
+  if (comp_has_charge(x, comp)) memcpy(x->sk_ads, x->sk_tmp, sizeof(double) * 2 * or_nvec(b));
+  mc->reins_acc[comp]++;
+  tr->de = de;
+  energy_add(&mc->running, &de);
+}
+
 /* Update_Max_Translation / Update_Max_Rotation (mc_utilities.h:612-665) */
 static void update_max(double v[3], int64_t* acc, int64_t* tot, double hi)
 {
@@ -1088,6 +1253,9 @@ int or_mc_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
       } else if (r < mc->p_widom[comp]) {
         tr.move = OR_WIDOM;
         cbmc_move(&x, comp, mol, OR_WIDOM, &tr);
+      } else if (r < mc->p_reinsertion[comp]) {
+        tr.move = OR_REINSERTION;
+        if (comps[comp].n_mol > 0) reinsertion_move(&x, comp, mol, &tr);
       } else if (r < mc->p_swap[comp]) {
         if (or_rand_uniform() < 0.5) {
           tr.move = OR_INSERTION;
@@ -1156,6 +1324,9 @@ int or_batch_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
       } else if (r < mc->p_widom[comp]) {
         tr.move = OR_WIDOM;
         cbmc_move(&x, comp, mol, OR_WIDOM, &tr);
+      } else if (r < mc->p_reinsertion[comp]) {
+        tr.move = OR_REINSERTION;
+        if (comps[comp].n_mol > 0) reinsertion_move(&x, comp, mol, &tr);
       } else if (r < mc->p_swap[comp]) {
         if (mc_draw(&x, D_COIN) < 0.5) {
           tr.move = OR_INSERTION;

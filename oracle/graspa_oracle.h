@@ -122,7 +122,7 @@ double or_philox_uniform(uint64_t seed, uint64_t stream, uint64_t index, uint32_
 
 /* ---- Monte Carlo driver in reference draw order (parity mode) ---- */
 enum { OR_TRANSLATION = 0, OR_ROTATION = 1, OR_INSERTION = 2, OR_DELETION = 3, OR_WIDOM = 4,
-       OR_NOMOVE = 5 };
+       OR_NOMOVE = 5, OR_REINSERTION = 6 };
 
 typedef struct {
   int     cycle, step, move, comp, mol, accepted, overlap, _pad;
@@ -154,6 +154,10 @@ typedef struct {
   int64_t ins_total[8], ins_acc[8], del_total[8], del_acc[8], widom_total[8];
   double  widom_rosen_sum[8], widom_rosen_n[8];
   or_energy running;
+  /* reinsertion (ReinsertionMove, move_struct.h:170-407): cumulative
+   * probability between p_widom and p_swap, and its counters */
+  double  p_reinsertion[8];
+  int64_t reins_total[8], reins_acc[8];
 } or_mc;
 
 /* Allocates the device-style random pool and seeds it (data_struct.h:1287-1346).
@@ -172,7 +176,8 @@ int  or_mc_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
  * the box draws uniform d from or_philox_uniform(seed, stream, t, d) --
  * d0 component, d1 molecule, d2 move, d3 insertion/deletion coin, d4
  * Metropolis, d5 first-bead selection, d6 orientation selection, d8..d10
- * translation / rotation, d16+3k.. trial k, d16+3K+3o.. orientation o. */
+ * translation / rotation, d16+3k.. trial k, d16+3K+3o.. orientation o,
+ * d16+3K+3O+3o.. retrace orientation o of a reinsertion. */
 int  or_batch_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
                      double* sk_ads, double* sk_fw, uint64_t seed, uint64_t stream, int64_t* move_index,
                      int first_cycle, int n_cycles, or_trace* trace, int max_trace);

@@ -21,7 +21,7 @@ MIXED_DE_TOL = 1e-3      # internal units (10 J/mol): 1e-5 kJ/mol
 def _mixture(seed=11, n_fw=200, L=24.0, P=5e6):
     from graspa_amd import synth
     return synth.build(n_fw=n_fw, L=L, species=("CO2", "N2"), mol_fraction=[0.5, 0.5], seed=seed, T=300.0, P=P,
-                       cycles=10, probs=dict(translation=1.0, rotation=1.0, swap=2.0, widom=0.5))
+                       cycles=10, probs=dict(translation=1.0, rotation=1.0, swap=2.0, widom=0.5, reinsertion=1.0))
 
 
 def _oracle_run(system, seed, stream, cycles, n_max=20000):
@@ -75,6 +75,7 @@ def test_batch_fp64_reproduces_oracle(engine_mod, moves_per_launch):
         assert abs(a.pacc - b.pacc) <= 1e-10 * abs(a.pacc) + 1e-300, (i, a.pacc, b.pacc)
         assert abs(a.rosen - b.rosen) <= 1e-10 * abs(a.rosen) + 1e-300, (i, a.rosen, b.rosen)
     assert n_acc[2] > 3 and n_acc[0] + n_acc[1] > 3
+    assert n_acc.get(6, 0) >= 2              # reinsertions accepted
     st = bt.stats(0)
     assert st.moves == len(otr) and st.cycles == cycles
     for ci in (1, 2):

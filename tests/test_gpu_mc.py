@@ -64,3 +64,34 @@ def test_gcmc_fp64_reproduces_oracle_sequence():
     na = len(q)
     assert np.allclose(pos, opos[:na], atol=1e-12)
     assert np.array_equal(m, om[:na])
+
+
+def test_reinsertion_fp64_reproduces_oracle_sequence():
+    """CO2/N2 mixture with translation, rotation, Widom, reinsertion and swap
+    moves in the reference's random-number order."""
+    from graspa_amd import synth
+    s = synth.build(n_fw=200, L=24.0, species=("CO2", "N2"), mol_fraction=[0.5, 0.5], seed=11, T=300.0, P=5e6,
+                    cycles=12, probs=dict(translation=1.0, rotation=1.0, swap=2.0, widom=0.5, reinsertion=1.0))
+    r = _run_pair(s, 12)
+    assert r["n_o"] == r["n_g"]
+    otr, gtr = r["otr"], r["gtr"]
+    n_reins = 0
+    for i in range(r["n_o"]):
+        a, b = otr[i], gtr[i]
+        assert (a.cycle, a.step, a.move, a.comp, a.mol) == (b.cycle, b.step, b.move, b.comp, b.mol), i
+        assert a.accepted == b.accepted and a.overlap == b.overlap, (i, a.move)
+        if a.move == 6:
+            n_reins += a.accepted
+            assert abs(a.rosen - b.rosen) <= 1e-10 * abs(a.rosen) + 1e-300, (i, a.rosen, b.rosen)
+        for k in ("hg_vdw", "hg_real", "gg_vdw", "gg_real", "hg_ewald", "gg_ewald"):
+            x, y = getattr(a.de, k), getattr(b.de, k)
+            assert abs(x - y) <= 1e-9 + 1e-10 * abs(x), (i, k, x, y)
+        if a.pacc > 0 and a.pacc < 1e300:
+            assert abs(a.pacc - b.pacc) <= 1e-9 * a.pacc, (i, a.pacc, b.pacc)
+    assert n_reins >= 2
+    assert abs(r["e"].total() - r["omc"].running.total()) <= 1e-8 * max(1.0, abs(r["omc"].running.total()))
+    for ci in (1, 2):
+        pos, q, t, m = r["box"].get_molecules(ci)
+        opos, oq, ot, om = r["o"].comp_arrays(ci)
+        na = len(q)
+        assert np.allclose(pos, opos[:na], atol=1e-12)

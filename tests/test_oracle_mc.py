@@ -5,8 +5,8 @@ equal the change of the total energy recomputed from scratch (pair sums over
 all atoms, Fourier sums from structure factors rebuilt from the atoms, minus
 the rigid exclusion of every molecule), and the running adsorbate structure
 factor must equal the rebuilt one.  A dropped or mis-signed term in any move's
-energy change (translation, rotation, insertion, deletion, the Ewald
-same/cross split, the exclusion correction) or a commit that stores the wrong
+energy change (translation, rotation, insertion, deletion, reinsertion, the
+Ewald same/cross split, the exclusion correction) or a commit that stores the wrong
 molecule breaks one of these equalities.  Both drivers are checked: the
 reference's random-number order (or_mc_cycles) and the batched form with
 counter-based streams (or_batch_cycles)."""
@@ -16,10 +16,11 @@ import numpy as np
 import pytest
 
 
-def _system(P=5e6, n_fw=200):
+def _system(P=5e6, n_fw=200, reinsertion=1.0):
     from graspa_amd import synth
     s = synth.build(n_fw=n_fw, L=24.0, species=("CO2", "N2"), mol_fraction=[0.5, 0.5], seed=11, T=300.0, P=P,
-                    cycles=10, probs=dict(translation=1.0, rotation=1.0, swap=2.0, widom=0.5))
+                    cycles=10, probs=dict(translation=1.0, rotation=1.0, swap=2.0, widom=0.5,
+                                          reinsertion=reinsertion))
     return s
 
 
@@ -66,11 +67,12 @@ def test_running_energy_equals_recomputed_totals(oracle_lib, form):
     assert n > 0
     moves = [tr[i].move for i in range(min(n, n_max))]
     acc = [tr[i].accepted for i in range(min(n, n_max))]
-    # every move type ran and some were accepted
-    for m in (0, 1, 2, 3, 4):
+    # every move type ran and some were accepted (6: reinsertion)
+    for m in (0, 1, 2, 3, 4, 6):
         assert m in moves, m
     assert sum(a for a, m in zip(acc, moves) if m == 2) > 3
     assert sum(a for a, m in zip(acc, moves) if m in (0, 1)) > 3
+    assert sum(a for a, m in zip(acc, moves) if m == 6) >= 2
     after, ska_rebuilt = _totals(o)
     _check_drift(o, mc, before, after, ska, ska_rebuilt)
 
@@ -93,3 +95,60 @@ def test_batched_form_is_deterministic_and_stream_dependent(oracle_lib):
     a, b, c = run(5, 0), run(5, 0), run(5, 1)
     assert a == b
     assert a != c
+
+
+def test_reinsertion_samples_the_boltzmann_distribution(oracle_lib):
+    """One methane molecule among 150 framework atoms, reinsertion moves only:
+    the oracle's chain must sample exp(-beta U(r)), so its mean energy equals
+    <U> = int U e^{-beta U} / int e^{-beta U} over the box, estimated here by
+    plain Monte Carlo integration over 400,000 uniform points.  Accepting the
+    grown configuration without the retrace weight (or with the retrace
+    missing the other first-bead weights) samples the CBMC growth
+    distribution instead, which is shifted towards low energies."""
+    from graspa_amd import synth
+    from oracle import binding as ob
+    s = synth.build(n_fw=150, L=25.0, species=("methane",), seed=5, T=200.0, P=1e3, charges=False, K=10,
+                    probs=dict(reinsertion=1.0))
+    T = s.ntypes
+    fw = np.asarray(s.fw_pos)
+    t_fw = np.asarray(s.fw_type)
+    site_t = int(s.components[0].types[0])
+    eps = np.asarray(s.eps)[t_fw, site_t]
+    sig2 = np.asarray(s.sig)[t_fw, site_t] ** 2
+    L = s.cell[0]
+
+    def energy(p):
+        d = fw[None, :, :] - p[:, None, :]
+        d -= L * np.round(d / L)
+        rr = (d * d).sum(-1)
+        x = (sig2[None, :] / rr) ** 3
+        e = 4.0 * eps[None, :] * (x * x - x)
+        return np.where(rr < s.cut_vdw ** 2, e, 0.0).sum(1)
+
+    rng = np.random.default_rng(0)
+    pts = rng.uniform(0, L, (400000, 3))
+    U = np.concatenate([energy(pts[i:i + 20000]) for i in range(0, len(pts), 20000)])
+    w = np.exp(-s.beta * (U - U.min()))
+    u_ref = float((U * w).sum() / w.sum())
+    start = pts[np.argmin(U)]
+    c = s.components[0]
+    c.init_pos = start[None, :].copy()
+    c.init_molid = np.zeros(1, dtype=np.int64)
+    c.init_charge = np.zeros(1)
+    o = ob.OracleSystem(s, capacity=4)
+    mc = ob.make_mc(s, o, production=1)
+    _, skf, ska, _, _, _ = o.ewald_total()
+    e0 = o.total_vdw_real().hg_vdw
+    assert abs(e0 - energy(start[None, :])[0]) < 1e-9 * max(1.0, abs(e0))
+    assert ob.lib().or_mc_setup(C.byref(mc)) == 0
+    n_cyc = 1500
+    tr = (ob.OrTrace * (20 * n_cyc))()
+    n = ob.lib().or_mc_cycles(C.byref(mc), o.box, o.comps, o.ncomp, ob._p(ska), ob._p(skf), 0, n_cyc, tr, 20 * n_cyc)
+    assert n == 20 * n_cyc
+    de = np.array([tr[i].de.hg_vdw for i in range(n)])
+    e = e0 + np.cumsum(de)
+    acc = np.mean([tr[i].accepted for i in range(n)])
+    assert 0.02 < acc < 0.98
+    blocks = e.reshape(20, -1).mean(1)
+    err = blocks.std(ddof=1) / np.sqrt(len(blocks))
+    assert abs(e.mean() - u_ref) < 4.0 * err + 0.01 * abs(u_ref), (e.mean(), u_ref, err)
