@@ -637,7 +637,7 @@ int graspa_box_init_ewald(graspa_box* b, graspa_energy* e)
 {
   GRASPA_CHECK_BOX(b);
   hipSetDevice(b->device);
-  const int nb_max = (b->nvec + kBlock - 1) / kBlock;
+  const int nb_max = (b->nvec + 3) / 4;        // one partial per workgroup of four k-vectors
   if (int rc = ensure_part(b, 3 * (size_t)nb_max + 8)) return rc;
   int nb = 0;
   if (int rc = launch_ewald_build(b, b->d_part, &nb)) return rc;

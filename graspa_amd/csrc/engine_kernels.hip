@@ -218,6 +218,12 @@ __global__ __launch_bounds__(kBlock) void atom_axis_tables(const BoxConst bc, co
   }
 }
 
+// Structure factors of the framework and of the adsorbates and the Fourier
+// energies (Ewald_Total, ewald_preparation.h:5-259): a workgroup per
+// kSkK consecutive k-vectors, its threads over the atoms (coalesced axis-table
+// reads), wave sums then a fixed-order sum over the waves.
+constexpr int kSkK = 4;
+
 __global__ __launch_bounds__(kBlock) void sk_build_kernel(const BoxConst bc, const double2* __restrict__ tab_fw,
                                                           const double* __restrict__ fw, int n_fw,
                                                           const double2* __restrict__ tab_g,
@@ -225,48 +231,73 @@ __global__ __launch_bounds__(kBlock) void sk_build_kernel(const BoxConst bc, con
                                                           const double* __restrict__ ktemp, double2* __restrict__ sk_fw,
                                                           double2* __restrict__ sk_ads, double* __restrict__ out)
 {
-  __shared__ double red[16];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double red[kBlock / 64][4 * kSkK];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int kstride = 1 + max(bc.kmax[0], max(bc.kmax[1], bc.kmax[2]));
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  if (k < bc.nvec) {
-    const double t = ktemp[k];
-    double2 sf = make_double2(0.0, 0.0), sa = make_double2(0.0, 0.0);
-    if (t >= 0.0) {
-      const int kzm = bc.kmax[2], kym = bc.kmax[1];
-      const int kz = k % (2 * kzm + 1) - kzm;
-      const int kxy = k / (2 * kzm + 1);
-      const int kx = kxy / (2 * kym + 1);
-      const int ky = kxy % (2 * kym + 1) - kym;
-      for (int part = 0; part < gv.n_comp; ++part) {
-        const int n = (part == 0) ? n_fw : gv.seg[part].n_atoms;
-        const int base = (part == 0) ? 0 : gv.seg[part].base;
-        const double2* tab = (part == 0) ? tab_fw : tab_g;
-        const int tn = (part == 0) ? n_fw : guest_cap;
-        const double* q = (part == 0) ? fw + 3 * n_fw : g + 3 * guest_cap;
-        double2 s = make_double2(0.0, 0.0);
-        for (int jj = 0; jj < n; ++jj) {
-          const int j = base + jj;
-          double2 ey = tab[((size_t)1 * kstride + abs(ky)) * tn + j];
-          if (ky < 0) ey.y = -ey.y;
-          const double2 exy = cmul2(tab[((size_t)0 * kstride + kx) * tn + j], ey);
-          double2 ez = tab[((size_t)2 * kstride + abs(kz)) * tn + j];
-          if (kz < 0) ez.y = -ez.y;
-          const double2 p = cmul2(exy, ez);
-          s.x += q[j] * p.x;
-          s.y += q[j] * p.y;
-        }
-        if (part == 0) sf = s; else { sa.x += s.x; sa.y += s.y; }
-      }
-      acc[0] = t * (sa.x * sa.x + sa.y * sa.y);           // guest-guest
-      acc[1] = t * (sf.x * sf.x + sf.y * sf.y);           // host-host
-      acc[2] = t * (sf.x * sa.x + sf.y * sa.y) * 2.0;     // host-guest
-    }
-    sk_fw[k] = sf;
-    sk_ads[k] = sa;
+  const int kzm = bc.kmax[2], kym = bc.kmax[1];
+  const int k0 = blockIdx.x * kSkK;
+  int kx[kSkK], ky[kSkK], kz[kSkK];
+  bool on[kSkK];
+#pragma unroll
+  for (int u = 0; u < kSkK; ++u) {
+    const int k = k0 + u;
+    on[u] = k < bc.nvec && ktemp[k] >= 0.0;
+    kz[u] = k % (2 * kzm + 1) - kzm;
+    const int kxy = k / (2 * kzm + 1);
+    kx[u] = kxy / (2 * kym + 1);
+    ky[u] = kxy % (2 * kym + 1) - kym;
   }
-  block_sum4(acc, red);
+  // v[4u + 0..1]: framework S(k_u), v[4u + 2..3]: adsorbate S(k_u)
+  double v[4 * kSkK];
+#pragma unroll
+  for (int i = 0; i < 4 * kSkK; ++i) v[i] = 0.0;
+  for (int part = 0; part < gv.n_comp; ++part) {
+    const int n = (part == 0) ? n_fw : gv.seg[part].n_atoms;
+    const int base = (part == 0) ? 0 : gv.seg[part].base;
+    const double2* tab = (part == 0) ? tab_fw : tab_g;
+    const int tn = (part == 0) ? n_fw : guest_cap;
+    const double* q = (part == 0) ? fw + 3 * n_fw : g + 3 * guest_cap;
+    const int off = (part == 0) ? 0 : 2;
+    for (int jj = threadIdx.x; jj < n; jj += blockDim.x) {
+      const int j = base + jj;
+      const double qj = q[j];
+#pragma unroll
+      for (int u = 0; u < kSkK; ++u) {
+        if (!on[u]) continue;
+        double2 ey = tab[((size_t)1 * kstride + abs(ky[u])) * tn + j];
+        if (ky[u] < 0) ey.y = -ey.y;
+        const double2 exy = cmul2(tab[((size_t)0 * kstride + kx[u]) * tn + j], ey);
+        double2 ez = tab[((size_t)2 * kstride + abs(kz[u])) * tn + j];
+        if (kz[u] < 0) ez.y = -ez.y;
+        const double2 p = cmul2(exy, ez);
+        v[4 * u + off] += qj * p.x;
+        v[4 * u + off + 1] += qj * p.y;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4 * kSkK; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0)
+    for (int i = 0; i < 4 * kSkK; ++i) red[wave][i] = v[i];
+  __syncthreads();
   if (threadIdx.x == 0) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int u = 0; u < kSkK; ++u) {
+      const int k = k0 + u;
+      if (k >= bc.nvec) break;
+      double s[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+        for (int i = 0; i < 4; ++i) s[i] += red[w][4 * u + i];
+      const double2 sf = make_double2(s[0], s[1]), sa = make_double2(s[2], s[3]);
+      sk_fw[k] = sf;
+      sk_ads[k] = sa;
+      if (on[u]) {
+        const double t = ktemp[k];
+        acc[0] += t * (sa.x * sa.x + sa.y * sa.y);           // guest-guest
+        acc[1] += t * (sf.x * sf.x + sf.y * sf.y);           // host-host
+        acc[2] += t * (sf.x * sa.x + sf.y * sa.y) * 2.0;     // host-guest
+      }
+    }
     out[3 * blockIdx.x] = acc[0];
     out[3 * blockIdx.x + 1] = acc[1];
     out[3 * blockIdx.x + 2] = acc[2];
@@ -292,7 +323,7 @@ int launch_ewald_build(graspa_box* b, double* d_out, int* n_blocks)
   if (b->guest_cap > 0)
     hipLaunchKernelGGL(atom_axis_tables, dim3((b->guest_cap + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream,
                        b->bc, b->d_g, b->guest_cap, b->guest_cap, kstride, tab_g);
-  const int nb = (b->nvec + kBlock - 1) / kBlock;
+  const int nb = (b->nvec + kSkK - 1) / kSkK;
   *n_blocks = nb;
   hipLaunchKernelGGL(sk_build_kernel, dim3(nb), dim3(kBlock), 0, b->stream, b->bc, tab_fw, b->d_fw, b->n_fw, tab_g,
                      b->d_g, b->guest_cap, gv, b->d_ktemp, b->d_sk_fw, b->d_sk_ads, d_out);
