@@ -304,9 +304,12 @@ def main():
                    "first_bead_placed_fraction": r["n_first_bead"] / max(1, r["n"]),
                    "complete_fraction": r["n_ok"] / max(1, r["n"]),
                    "mean_rosenbluth_weight": r["sum_w"] / max(1, r["n"])},
-        "roofline": {"bound": "mfma", "roof": "fp32 vector (VALU) peak; the pair kernel uses no MFMA",
-                     "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": (achieved_tflops / FP32_PEAK_TFLOPS) if achieved_tflops else None,
+        "roofline": {"bound": "mfma",
+                     "roof": "fp64 vector (VALU) peak: the pair terms compute in FP64 (the kernel uses no MFMA; "
+                             "FP64 FMA issues at the unpacked FP32 rate on gfx950)",
+                     "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": (achieved_tflops / FP64_PEAK_TFLOPS) if achieved_tflops else None,
+                     "frac_of_fp32_peak": (achieved_tflops / FP32_PEAK_TFLOPS) if achieved_tflops else None,
                      "traffic": traffic,
                      "traffic_source": prof.get("source") if prof else None,
                      "kernel": "widom_eval_kernel<mixed> (pair energies of the surviving trial sites)",
