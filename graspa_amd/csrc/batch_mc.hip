@@ -57,12 +57,15 @@ constexpr int kBMaxDraws = kBDrawTrial + 9 * kBMaxTrial;   // K trials, O orient
 
 enum { kMvTrans = GRASPA_MOVE_TRANSLATION, kMvRot = GRASPA_MOVE_ROTATION, kMvIns = GRASPA_MOVE_INSERTION,
        kMvDel = GRASPA_MOVE_DELETION, kMvWidom = GRASPA_MOVE_WIDOM, kMvNone = GRASPA_MOVE_NONE,
-       kMvReins = GRASPA_MOVE_REINSERTION };
+       kMvReins = GRASPA_MOVE_REINSERTION, kMvIdent = GRASPA_MOVE_IDENTITY_SWAP };
+// draws of an identity swap: old species, new species, old molecule
+enum { kBDrawIdOld = 11, kBDrawIdNew = 12, kBDrawIdMol = 13 };
 
 // Per-species set-up of one box (the Components fields RunMoves reads).
 struct BSpecies {
   double p_trans, p_rot, p_widom, p_swap;     // cumulative
   double p_reins;                             // cumulative, between p_widom and p_swap
+  double p_ident;                             // cumulative, between p_reins and p_swap
   double mol_fraction, fugacity, ideal_rosen, excl;   // excl = ExclusionIntra + ExclusionAtom
   int base, molsize, capacity, charged;
   int pseudo[16];
@@ -161,6 +164,7 @@ struct BMove {
   unsigned long long ovl, smask;
   int type, tmove, comp, mol, M, start, charged, n_eval;
   int fail, accept, overlap;
+  int nc, qo, qn, _pad;             // identity swap: new species, old / new species charged
 };
 
 // ---------------------------------------------------------------------------
@@ -560,6 +564,27 @@ __device__ double tail_delta(const BBox& B, const BState& S, const int comp, con
     const int64_t di = (int64_t)B.sp[comp].pseudo[i] * sign;
     for (int j = i; j < T && j < 16; ++j) {
       const int64_t dj = (int64_t)B.sp[comp].pseudo[j] * sign;
+      const int ij = i * T + j;
+      if (!B.use_tail[ij]) continue;
+      const int64_t dn = S.n_pseudo[i] * dj + S.n_pseudo[j] * di + di * dj;
+      double de = B.tail[ij] * (double)dn;
+      if (i != j) de *= 2.0;
+      e += de;
+    }
+  }
+  return e / B.volume;
+}
+
+// TailCorrectionIdentitySwap (TailCorrection_Energy_Functions.h:85-108): the
+// pseudo-atom change is one molecule of species nc in, one of oc out
+__device__ double tail_identity(const BBox& B, const BState& S, const int nc, const int oc)
+{
+  const int T = B.bc.ntypes;
+  double e = 0.0;
+  for (int i = 0; i < T && i < 16; ++i) {
+    const int64_t di = (int64_t)B.sp[nc].pseudo[i] - B.sp[oc].pseudo[i];
+    for (int j = i; j < T && j < 16; ++j) {
+      const int64_t dj = (int64_t)B.sp[nc].pseudo[j] - B.sp[oc].pseudo[j];
       const int ij = i * T + j;
       if (!B.use_tail[ij]) continue;
       const int64_t dn = S.n_pseudo[i] * dj + S.n_pseudo[j] * di + di * dj;
@@ -1128,6 +1153,220 @@ __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ff
   }
 }
 
+// Identity swap (IdentitySwapMove, mc_swap_moves.h:199-426): the OLD
+// molecule L.mol of species L.comp is replaced by one of species L.nc grown on
+// its first bead (one trial whose weight is not divided by K, then O
+// orientations of the NEW template about it), energies without the OLD
+// molecule; the OLD molecule is retraced with its own first bead and O
+// orientations (trial 0 its own).  W_new carries the Ewald change of the OLD
+// sites out and the NEW sites in and the tail change; Pacc = pref_ins(NEW)
+// pref_del(OLD) (W_new / W_id,NEW) / (W_old / W_id,OLD).  On acceptance the
+// OLD molecule is deleted (the last one into its hole) and the NEW one
+// appended, or, for the same species, moved in place.  Draws: orientation o
+// at kBDrawTrial + 3K + 3o, retrace orientation o at kBDrawTrial + 3K + 3O + 3o.
+template <bool MIXED, bool CUBIC>
+__device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr,
+                              const double2* ct, int nint, const double* fwl, double2* eik)
+{
+  const int tid = threadIdx.x;
+  const int oc = L.comp, nc = L.nc, cap = B.guest_cap;
+  const BSpecies& so = B.sp[oc];
+  const BSpecies& sn = B.sp[nc];
+  const int Mo = so.molsize, Mn = sn.molsize, K = B.K, O = B.O;
+  const int ex_lo = L.start, ex_hi = L.start + Mo;
+  // NEW first bead at the OLD molecule's first bead
+  if (tid == 0) {
+    BSite s;
+    s.x = B.g[L.start]; s.y = B.g[cap + L.start]; s.z = B.g[2 * cap + L.start];
+    s.q = sn.q[0];
+    s.type = sn.type[0];
+    s._p = 0;
+    L.site[0] = s;
+    for (int t = 0; t < 4; ++t) L.eo[t] = 0.0;
+  }
+  __syncthreads();
+  uint32_t ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, 1, 1, ex_lo, ex_hi);
+  if (tid < 64) {
+    double W = 1.0;
+    const int sel = select_wave0(B, L, 1, ov, 0.5, false, false, &W);
+    if (tid == 0) {
+      if (sel < 0) {
+        L.fail = 1;
+      } else {
+        L.W = W;
+        L.en[kHGV] = L.t4[0][0]; L.en[kHGR] = L.t4[0][1];
+        L.en[kGGV] = L.t4[0][2]; L.en[kGGR] = L.t4[0][3];
+        L.molp[0][0] = L.site[0].x; L.molp[0][1] = L.site[0].y; L.molp[0][2] = L.site[0].z;
+      }
+    }
+  }
+  __syncthreads();
+  if (L.fail) return;
+  // NEW chain: O orientations of the NEW template
+  if (Mn > 1) {
+    const int chain = Mn - 1;
+    for (int t = tid; t < O * chain; t += kBT) {
+      const int o = t / chain, a = t - o * chain;
+      const double* t0 = L.tmpl[nc][0];
+      const double* t1 = L.tmpl[nc][1 + a];
+      double v[3] = {t1[0] - t0[0], t1[1] - t0[1], t1[2] - t0[2]};
+      const int d = kBDrawTrial + 3 * K + 3 * o;
+      quat_rotate(v, L.ud[d], L.ud[d + 1], L.ud[d + 2]);
+      BSite s;
+      s.x = L.molp[0][0] + v[0]; s.y = L.molp[0][1] + v[1]; s.z = L.molp[0][2] + v[2];
+      s.q = sn.q[1 + a];
+      s.type = sn.type[1 + a];
+      s._p = 0;
+      L.site[t] = s;
+    }
+    __syncthreads();
+    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, O, chain, ex_lo, ex_hi);
+    if (tid < 64) {
+      double W = L.W;
+      const int sel = select_wave0(B, L, O, ov, L.ud[6], false, true, &W);
+      if (tid == 0) {
+        if (sel < 0) {
+          L.fail = 1;
+        } else {
+          L.W = W;
+          L.en[kHGV] += L.t4[sel][0]; L.en[kHGR] += L.t4[sel][1];
+          L.en[kGGV] += L.t4[sel][2]; L.en[kGGR] += L.t4[sel][3];
+          for (int a = 0; a < chain; ++a) {
+            const BSite& s = L.site[sel * chain + a];
+            L.molp[1 + a][0] = s.x; L.molp[1 + a][1] = s.y; L.molp[1 + a][2] = s.z;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (L.fail) return;
+  }
+  // retrace of the OLD molecule: its own first bead
+  if (tid == 0) {
+    BSite s;
+    s.x = B.g[L.start]; s.y = B.g[cap + L.start]; s.z = B.g[2 * cap + L.start];
+    s.q = so.q[0];
+    s.type = so.type[0];
+    s._p = 0;
+    L.site[0] = s;
+  }
+  __syncthreads();
+  ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, 1, 1, ex_lo, ex_hi);
+  if (tid == 0) L.Wo = retrace_weight(B, L, 1, ov, 0.0, 1);
+  __syncthreads();
+  if (Mo > 1) {
+    const int chain = Mo - 1;
+    for (int t = tid; t < O * chain; t += kBT) {
+      const int o = t / chain, a = t - o * chain;
+      BSite s;
+      if (o == 0) {
+        const int j = L.start + 1 + a;
+        s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j];
+      } else {
+        const double* t0 = L.tmpl[oc][0];
+        const double* t1 = L.tmpl[oc][1 + a];
+        double v[3] = {t1[0] - t0[0], t1[1] - t0[1], t1[2] - t0[2]};
+        const int d = kBDrawTrial + 3 * K + 3 * O + 3 * o;
+        quat_rotate(v, L.ud[d], L.ud[d + 1], L.ud[d + 2]);
+        s.x = B.g[L.start] + v[0]; s.y = B.g[cap + L.start] + v[1]; s.z = B.g[2 * cap + L.start] + v[2];
+      }
+      s.q = so.q[1 + a];
+      s.type = so.type[1 + a];
+      s._p = 0;
+      L.site[t] = s;
+    }
+    __syncthreads();
+    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, O, chain, ex_lo, ex_hi);
+    if (tid == 0) L.Wo *= retrace_weight(B, L, O, ov, 0.0, O);
+    __syncthreads();
+  }
+  // Ewald: the OLD sites out (charged species only), then the NEW sites in
+  const int n_old = L.qo ? Mo : 0, n_new = L.qn ? Mn : 0;
+  if (n_old + n_new > 0) {
+    if (tid < n_old + n_new) {
+      BSite s;
+      if (tid < n_old) {
+        const int j = L.start + tid;
+        s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j];
+        s.q = so.q[tid];
+        s.type = so.type[tid];
+      } else {
+        const int i = tid - n_old;
+        s.x = L.molp[i][0]; s.y = L.molp[i][1]; s.z = L.molp[i][2];
+        s.q = sn.q[i];
+        s.type = sn.type[i];
+      }
+      s._p = 0;
+      L.site[tid] = s;
+    }
+    __syncthreads();
+    ewald_pass(B, S, L, eik, n_old, n_new);
+  }
+  if (tid == 0) {
+    L.en[kHGV] -= L.eo[0]; L.en[kHGR] -= L.eo[1];
+    L.en[kGGV] -= L.eo[2]; L.en[kGGR] -= L.eo[3];
+    double W = L.W;
+    if (n_old + n_new > 0) {
+      double same = L.same;
+      if (L.qn) same -= sn.excl * 1.0;
+      if (L.qo) same -= so.excl * -1.0;
+      L.en[kGGE] = same;
+      L.en[kHGE] = 2.0 * L.cross;
+      W *= exp(-B.beta * (same + 2.0 * L.cross));
+    }
+    if (B.has_tail) L.en[kTail] = tail_identity(B, S, nc, oc);
+    W *= exp(-B.beta * L.en[kTail]);
+    L.W = W;
+    const double V = B.volume;
+    double pref = B.beta * sn.mol_fraction * B.pressure * sn.fugacity * V / (1.0 + (double)S.n_mol[nc]);
+    pref *= (double)S.n_mol[oc] / (B.beta * so.mol_fraction * B.pressure * so.fugacity * V);
+    const double pacc = pref * (W / sn.ideal_rosen) / (L.Wo / so.ideal_rosen);
+    L.pacc = pacc;
+    int acc = (L.ud[4] < pacc) ? 1 : 0;
+    if (acc && nc != oc && S.n_mol[nc] + 1 > sn.capacity) {
+      S.error = GRASPA_ERR_CAPACITY;
+      acc = 0;
+    }
+    L.accept = acc;
+    if (acc) {
+      S.cnt[oc][GRASPA_CNT_IDSW_ACC + nc - 1]++;
+      if (nc != oc)
+        for (int t = 0; t < 16; ++t) S.n_pseudo[t] += sn.pseudo[t] - so.pseudo[t];
+      add_energy(S.running, L.en);
+      if (n_old + n_new > 0) S.sk_cur ^= 1;
+    }
+  }
+  __syncthreads();
+  if (!L.accept) return;
+  if (nc != oc) {
+    // the last OLD molecule into the hole (its MolIDs stay), the NEW one appended
+    const int last = so.base + (S.n_mol[oc] - 1) * Mo;
+    if (L.start != last && tid < Mo) {
+      const int j = L.start + tid, k = last + tid;
+      B.g[j] = B.g[k]; B.g[cap + j] = B.g[cap + k]; B.g[2 * cap + j] = B.g[2 * cap + k];
+      B.g[3 * cap + j] = B.g[3 * cap + k];
+      B.g_type[j] = B.g_type[k];
+    }
+    const int nn = S.n_mol[nc];
+    if (tid >= 64 && tid < 64 + Mn) {
+      const int a = tid - 64, j = sn.base + nn * Mn + a;
+      B.g[j] = L.molp[a][0]; B.g[cap + j] = L.molp[a][1]; B.g[2 * cap + j] = L.molp[a][2];
+      B.g[3 * cap + j] = sn.q[a];
+      B.g_type[j] = sn.type[a];
+      B.g_molid[j] = nn;
+      B.g_comp[j] = nc;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      S.n_mol[oc] -= 1;
+      S.n_mol[nc] += 1;
+    }
+  } else if (tid < Mo) {
+    const int j = L.start + tid;
+    B.g[j] = L.molp[tid][0]; B.g[cap + j] = L.molp[tid][1]; B.g[2 * cap + j] = L.molp[tid][2];
+  }
+}
+
 // Update_Max_Translation / Update_Max_Rotation (mc_utilities.h:612-665)
 __device__ inline void update_max(double* v, int64_t& acc, int64_t& tot, double hi)
 {
@@ -1228,6 +1467,10 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
       else if (r < sp.p_rot) { tm = kMvRot; work = n > 0; }
       else if (r < sp.p_widom) { tm = kMvWidom; work = 1; }
       else if (r < sp.p_reins) { tm = kMvReins; work = n > 0; }
+      else if (r < sp.p_ident) {           // attempted while the box holds an adsorbate
+        tm = kMvIdent;
+        for (int c = 1; c < B.n_comp; ++c) work |= S.n_mol[c] > 0 ? 1 : 0;
+      }
       else if (r < sp.p_swap) {
         if (u[3] < 0.5) { tm = kMvIns; work = 1; }
         else { tm = kMvDel; work = n > 0; }
@@ -1242,7 +1485,30 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
       for (int i = 0; i < 10; ++i) L.en[i] = 0.0;
       L.W = 0.0; L.pacc = 0.0; L.same = 0.0; L.cross = 0.0;
       L.fail = 0; L.accept = 0; L.overlap = 0;
-      if (work) {
+      L.nc = 0; L.qo = 0; L.qn = 0;
+      if (work && tm == kMvIdent) {
+        // the species pair of IdentitySwapMove's rejection loop
+        // (mc_swap_moves.h:216-235): old uniform over the species holding
+        // molecules, new uniform over all; then the old molecule
+        int held[kMaxComps], nh = 0;
+        for (int c = 1; c < B.n_comp; ++c)
+          if (S.n_mol[c] > 0) held[nh++] = c;
+        int i = (int)(u[kBDrawIdOld] * (double)nh);
+        if (i >= nh) i = nh - 1;
+        const int oc = held[i];
+        int nc = 1 + (int)(u[kBDrawIdNew] * (double)(B.n_comp - 1));
+        if (nc >= B.n_comp) nc = B.n_comp - 1;
+        const BSpecies& so = B.sp[oc];
+        L.comp = oc;
+        L.nc = nc;
+        L.mol = (int)(u[kBDrawIdMol] * (double)S.n_mol[oc]);
+        L.M = so.molsize;
+        L.start = so.base + L.mol * so.molsize;
+        L.qo = so.charged;
+        L.qn = B.sp[nc].charged;
+        L.charged = L.qo | L.qn;
+        S.cnt[oc][GRASPA_CNT_IDSW_TOT + nc - 1]++;
+      } else if (work) {
         if (tm == kMvTrans) { S.cnt[comp][0]++; S.upd_tot[comp][0]++; }
         else if (tm == kMvRot) { S.cnt[comp][2]++; S.upd_tot[comp][1]++; }
         else if (tm == kMvIns) S.cnt[comp][4]++;
@@ -1255,6 +1521,7 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
     const int type = L.type;
     if (type == kMvTrans || type == kMvRot) single_body<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
     else if (type == kMvReins) reinsertion<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
+    else if (type == kMvIdent) identity_swap<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
     else if (type != kMvNone) cbmc<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
     __syncthreads();
     if (tid == 0) {
@@ -1459,6 +1726,7 @@ int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
     BSpecies& d = h.sp[c];
     d.p_trans = sp.p_translation; d.p_rot = sp.p_rotation; d.p_widom = sp.p_widom; d.p_swap = sp.p_swap;
     d.p_reins = sp.p_reinsertion;
+    d.p_ident = sp.p_identity;
     d.mol_fraction = sp.mol_fraction; d.fugacity = sp.fugacity_coeff; d.ideal_rosen = sp.ideal_rosen;
     d.excl = sp.excl_intra + sp.excl_self;
     d.base = cc.base; d.molsize = cc.molsize; d.capacity = cc.capacity;

@@ -67,7 +67,8 @@ struct graspa_mc {
   size_t pool_off = 0;
   int64_t total_molecules = 1;
   int64_t n_pseudo[16] = {0};
-  // trans_tot, trans_acc, rot_tot, rot_acc, ins_tot, ins_acc, del_tot, del_acc, widom, reins_tot, reins_acc
+  // trans_tot, trans_acc, rot_tot, rot_acc, ins_tot, ins_acc, del_tot, del_acc, widom, reins_tot, reins_acc,
+  // identity swaps to each species (attempted, accepted)
   int64_t cnt[8][GRASPA_N_COUNTERS] = {{0}};
   int64_t upd_tot[8][2] = {{0}}, upd_acc[8][2] = {{0}};  // since the last max-step update
   double widom_w[8] = {0}, widom_n[8] = {0};
@@ -625,6 +626,221 @@ int reinsertion_move(graspa_mc* mc, int comp, int mol, graspa_trace& tr)
   return GRASPA_OK;
 }
 
+// Identity swap: IdentitySwapMove (mc_swap_moves.h:199-426).  The species
+// pair is drawn two uniforms per try until the OLD species holds a molecule,
+// then the OLD molecule.  The NEW molecule is grown on the OLD molecule's
+// first bead (one trial whose weight is not divided by K,
+// copy_firstbead_to_new :178-181) with O orientations of the NEW template,
+// energies without the OLD molecule (ExcludeList); the OLD molecule is
+// retraced with its own first bead and O orientations (trial 0 its own).
+// W_new carries the Ewald change of the OLD sites out and the NEW sites in
+// (GPU_EwaldDifference_IdentitySwap, Ewald_Energy_Functions.h:582-632) and
+// the tail change (TailCorrectionIdentitySwap,
+// TailCorrection_Energy_Functions.h:85-108); Pacc = pref_ins(NEW)
+// pref_del(OLD) (W_new / W_id,NEW) / (W_old / W_id,OLD).  Pool draws: the
+// pair, the molecule, one triple (NEW first bead, unused), O triples and a
+// selection, one triple (OLD first bead, unused), O triples, Metropolis.
+int identity_swap_move(graspa_mc* mc, graspa_trace& tr)
+{
+  graspa_box* b = mc->box;
+  const int ncomp = (int)b->comps.size();
+  const int K = mc->p.n_trial_pos, O = mc->p.n_trial_orient;
+  const double beta = mc->p.beta;
+  const int bias = mc->p.vdw_real_bias;
+  tr.accepted = 0; tr.overlap = 0; tr.pacc = 0.0; tr.rosen = 0.0; tr.de = graspa_energy{};
+  if (mc->total_molecules - 1 == 0) return GRASPA_OK;      // no adsorbates: no draws
+  int oc = 0, nc = 0;
+  int64_t nold = 0;
+  while (oc == 0 || oc >= ncomp || nc == 0 || nc >= ncomp || nold == 0) {
+    oc = (int)(mc->uniform() * (double)(ncomp - 1)) + 1;
+    nc = (int)(mc->uniform() * (double)(ncomp - 1)) + 1;
+    nold = oc < ncomp ? b->comps[oc].n_mol : 0;
+  }
+  Component& co = b->comps[oc];
+  Component& cn = b->comps[nc];
+  const int oldmol = (int)(mc->uniform() * (double)co.n_mol);
+  const int Mo = co.molsize, Mn = cn.molsize, so = oldmol * Mo;
+  const int64_t nnew = cn.n_mol;
+  tr.comp = oc;
+  tr.mol = oldmol;
+  mc->cnt[oc][GRASPA_CNT_IDSW_TOT + nc - 1]++;
+  const int nmax = std::max(K, O * std::max(std::max(Mo, Mn), 1));
+  std::vector<double> sp3(3 * (size_t)nmax), e4(4 * (size_t)nmax);
+  std::vector<uint8_t> ov(nmax);
+  std::vector<double> oldp(3 * Mo), oldq(Mo), newp(3 * Mn), newq(Mn);
+  std::vector<int> oldt(Mo), newt(Mn);
+  for (int i = 0; i < Mo; ++i) {
+    for (int d = 0; d < 3; ++d) oldp[3 * i + d] = co.pos[3 * (so + i) + d];
+    oldq[i] = co.q[so + i];
+    oldt[i] = co.type[so + i];
+  }
+  for (int i = 0; i < Mn; ++i) {
+    newq[i] = cn.q[i];
+    newt[i] = cn.type[i];
+  }
+  Trials ts;
+  double en_new[4] = {0, 0, 0, 0}, en_old[4] = {0, 0, 0, 0};
+  // NEW first bead at the OLD molecule's first bead
+  mc->pool_check(1);
+  mc->pool_off += 1;
+  if (int rc = graspa_trial_energies(b, 1, 1, oldp.data(), &newt[0], &newq[0], nc, nnew, oc, oldmol, e4.data(),
+                                     ov.data()))
+    return rc;
+  survivors(e4, ov, 1, beta, bias, ts);
+  if (ts.keep.empty()) return GRASPA_OK;
+  double Wn = sum_exp(ts);
+  if (Wn < 1e-150) return GRASPA_OK;
+  if (!bias) Wn *= std::exp(-beta * (ts.e[0][1] + ts.e[0][3]));
+  if (Wn <= 1e-150) return GRASPA_OK;
+  for (int k = 0; k < 4; ++k) en_new[k] = ts.e[0][k];
+  for (int d = 0; d < 3; ++d) newp[d] = oldp[d];
+  // NEW chain: O orientations of the NEW template about it
+  if (Mn > 1) {
+    const int chain = Mn - 1;
+    mc->pool_check((size_t)O);
+    for (int o = 0; o < O; ++o) {
+      const double* R = &mc->pool[3 * (mc->pool_off + o)];
+      for (int a = 0; a < chain; ++a) {
+        double v[3] = {cn.pos[3 * (1 + a)] - cn.pos[0], cn.pos[3 * (1 + a) + 1] - cn.pos[1],
+                       cn.pos[3 * (1 + a) + 2] - cn.pos[2]};
+        rotate_quaternion(v, R[0], R[1], R[2]);
+        for (int d = 0; d < 3; ++d) sp3[3 * (o * chain + a) + d] = newp[d] + v[d];
+      }
+    }
+    mc->pool_off += (size_t)O;
+    if (int rc = graspa_trial_energies(b, O, chain, sp3.data(), &newt[1], &newq[1], nc, nnew, oc, oldmol, e4.data(),
+                                       ov.data()))
+      return rc;
+    survivors(e4, ov, O, beta, bias, ts);
+    if (ts.keep.empty()) return GRASPA_OK;
+    const int osel = select_trial(ts.logw, mc->uniform());
+    const double rc_sum = sum_exp(ts);
+    if (rc_sum < 1e-150) return GRASPA_OK;
+    double Wc = rc_sum / (double)O;
+    if (!bias) Wc *= std::exp(-beta * (ts.e[osel][1] + ts.e[osel][3]));
+    Wn *= Wc;
+    if (Wn <= 1e-150) return GRASPA_OK;
+    for (int k = 0; k < 4; ++k) en_new[k] += ts.e[osel][k];
+    const int ot = ts.keep[osel];
+    for (int a = 0; a < chain; ++a)
+      for (int d = 0; d < 3; ++d) newp[3 * (1 + a) + d] = sp3[3 * (ot * chain + a) + d];
+  }
+  // retrace of the OLD molecule: its own first bead
+  mc->pool_check(1);
+  mc->pool_off += 1;
+  if (int rc = graspa_trial_energies(b, 1, 1, oldp.data(), &oldt[0], &oldq[0], oc, oldmol, -1, -1, e4.data(),
+                                     ov.data()))
+    return rc;
+  survivors(e4, ov, 1, beta, bias, ts);
+  double Wo = sum_exp(ts);
+  if (!ts.keep.empty()) {
+    if (!bias) Wo *= std::exp(-beta * (ts.e[0][1] + ts.e[0][3]));
+    for (int k = 0; k < 4; ++k) en_old[k] = ts.e[0][k];
+  }
+  if (Mo > 1) {
+    const int chain = Mo - 1;
+    mc->pool_check((size_t)O);
+    for (int o = 0; o < O; ++o) {
+      const double* R = &mc->pool[3 * (mc->pool_off + o)];
+      for (int a = 0; a < chain; ++a) {
+        const int s = o * chain + a;
+        if (o == 0) {
+          for (int d = 0; d < 3; ++d) sp3[3 * s + d] = oldp[3 * (1 + a) + d];
+        } else {
+          double v[3] = {co.pos[3 * (1 + a)] - co.pos[0], co.pos[3 * (1 + a) + 1] - co.pos[1],
+                         co.pos[3 * (1 + a) + 2] - co.pos[2]};
+          rotate_quaternion(v, R[0], R[1], R[2]);
+          for (int d = 0; d < 3; ++d) sp3[3 * s + d] = oldp[d] + v[d];
+        }
+      }
+    }
+    mc->pool_off += (size_t)O;
+    if (int rc = graspa_trial_energies(b, O, chain, sp3.data(), &oldt[1], &oldq[1], oc, oldmol, -1, -1, e4.data(),
+                                       ov.data()))
+      return rc;
+    survivors(e4, ov, O, beta, bias, ts);
+    double Wc = sum_exp(ts) / (double)O;
+    if (!ts.keep.empty()) {
+      if (!bias) Wc *= std::exp(-beta * (ts.e[0][1] + ts.e[0][3]));
+      for (int k = 0; k < 4; ++k) en_old[k] += ts.e[0][k];
+    }
+    Wo *= Wc;
+  }
+  graspa_energy de{};
+  de.hg_vdw = en_new[0] - en_old[0];
+  de.hg_real = en_new[1] - en_old[1];
+  de.gg_vdw = en_new[2] - en_old[2];
+  de.gg_real = en_new[3] - en_old[3];
+  // Ewald: the OLD sites out, the NEW sites in, charged species only
+  auto has_q = [](const std::vector<double>& q) {
+    double a = 0.0;
+    for (double v : q) a += std::fabs(v);
+    return a > 1e-6;                 // hasPartialCharge (read_data.cpp:2160-2164)
+  };
+  const bool qo = b->bc.use_charges && has_q(oldq), qn = b->bc.use_charges && has_q(newq);
+  const bool ewald = qo || qn;
+  if (ewald) {
+    double same = 0.0, cross = 0.0;
+    if (int rc = graspa_ewald_delta(b, qo ? Mo : 0, oldp.data(), oldq.data(), qn ? Mn : 0, newp.data(), newq.data(),
+                                    &same, &cross))
+      return rc;
+    const graspa_species& sn = mc->p.species[nc - 1];
+    const graspa_species& so_ = mc->p.species[oc - 1];
+    if (qn) same -= (sn.excl_intra + sn.excl_self) * 1.0;
+    if (qo) same -= (so_.excl_intra + so_.excl_self) * -1.0;
+    de.gg_ewald = same;
+    de.hg_ewald = 2.0 * cross;
+    Wn *= std::exp(-beta * (same + 2.0 * cross));
+  }
+  // tail, with the pseudo-atom change NEW - OLD
+  if (mc->p.has_tail && !mc->tail.empty()) {
+    const int T = mc->ntypes;
+    const int64_t* pn = mc->p.species[nc - 1].pseudo_per_mol;
+    const int64_t* po = mc->p.species[oc - 1].pseudo_per_mol;
+    double e = 0.0;
+    for (int i = 0; i < T && i < 16; ++i) {
+      const int64_t di = pn[i] - po[i];
+      for (int j = i; j < T && j < 16; ++j) {
+        const int64_t dj = pn[j] - po[j];
+        const int ij = i * T + j;
+        if (!mc->use_tail[ij]) continue;
+        const int64_t dn = mc->n_pseudo[i] * dj + mc->n_pseudo[j] * di + di * dj;
+        double d = mc->tail[ij] * (double)dn;
+        if (i != j) d *= 2.0;
+        e += d;
+      }
+    }
+    de.tail = e / b->volume;
+  }
+  Wn *= std::exp(-beta * de.tail);
+  // GetPrefactor (mc_utilities.h:315-345) of the NEW insertion and the OLD deletion
+  const graspa_species& sn = mc->p.species[nc - 1];
+  const graspa_species& sq = mc->p.species[oc - 1];
+  const double V = b->volume, P = mc->p.pressure;
+  double pref = beta * sn.mol_fraction * P * sn.fugacity_coeff * V / (1.0 + (double)cn.n_mol);
+  pref *= (double)co.n_mol / (beta * sq.mol_fraction * P * sq.fugacity_coeff * V);
+  const double pacc = pref * (Wn / sn.ideal_rosen) / (Wo / sq.ideal_rosen);
+  tr.rosen = Wn;
+  tr.pacc = pacc;
+  const double u = mc->uniform();
+  if (!(u < pacc)) return GRASPA_OK;
+  if (nc != oc) {
+    if (cn.n_mol + 1 > cn.capacity) return fail(GRASPA_ERR_CAPACITY, "Need to allocate more space");
+    if (int rc = graspa_commit_delete(b, oc, oldmol)) return rc;
+    if (int rc = graspa_commit_insert(b, nc, newp.data(), newq.data(), newt.data(), nnew)) return rc;
+    for (int t = 0; t < 16; ++t) mc->n_pseudo[t] += sn.pseudo_per_mol[t] - sq.pseudo_per_mol[t];
+  } else {
+    if (int rc = graspa_commit_move(b, oc, oldmol, newp.data())) return rc;
+  }
+  if (ewald)
+    if (int rc = graspa_ewald_accept(b)) return rc;
+  mc->cnt[oc][GRASPA_CNT_IDSW_ACC + nc - 1]++;
+  tr.accepted = 1;
+  tr.de = de;
+  eadd(mc->running, de);
+  return GRASPA_OK;
+}
+
 // Update_Max_Translation / Update_Max_Rotation (mc_utilities.h:612-665)
 void update_max(double v[3], int64_t& acc, int64_t& tot, double hi)
 {
@@ -723,6 +939,9 @@ int graspa_mc_cycles(graspa_mc* mc, int first_cycle, int n_cycles, int productio
       } else if (r < sp.p_reinsertion) {
         tr.move = GRASPA_MOVE_REINSERTION;
         if (b->comps[comp].n_mol > 0) rc = reinsertion_move(mc, comp, mol, tr);
+      } else if (r < sp.p_identity) {
+        tr.move = GRASPA_MOVE_IDENTITY_SWAP;
+        rc = identity_swap_move(mc, tr);
       } else if (r < sp.p_swap) {
         if (mc->uniform() < 0.5) {
           tr.move = GRASPA_MOVE_INSERTION;

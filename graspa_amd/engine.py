@@ -67,7 +67,7 @@ class Species(C.Structure):
                 ("p_swap", C.c_double), ("mol_fraction", C.c_double), ("fugacity_coeff", C.c_double),
                 ("ideal_rosen", C.c_double), ("excl_intra", C.c_double), ("excl_self", C.c_double),
                 ("max_translation", C.c_double * 3), ("max_rotation", C.c_double * 3),
-                ("pseudo_per_mol", C.c_int64 * 16), ("p_reinsertion", C.c_double)]
+                ("pseudo_per_mol", C.c_int64 * 16), ("p_reinsertion", C.c_double), ("p_identity", C.c_double)]
 
 
 class MCParams(C.Structure):
@@ -90,8 +90,11 @@ class Trace(C.Structure):
 
 
 NBLOCK, MAX_COMPS = 16, 8
-N_COUNTERS = 11          # move counters per species (GRASPA_N_COUNTERS)
+N_COUNTERS = 25          # move counters per species (GRASPA_N_COUNTERS)
+CNT_IDSW_TOT = 11        # identity swaps to species s attempted at [CNT_IDSW_TOT + s], accepted at
+CNT_IDSW_ACC = 18        # [CNT_IDSW_ACC + s] (GRASPA_CNT_IDSW_TOT / _ACC)
 MOVE_REINSERTION = 6
+MOVE_IDENTITY_SWAP = 7
 
 
 class BlockStats(C.Structure):
@@ -472,12 +475,11 @@ def make_mc_params(system, box, pool_size=333334):
     p.n_species = len(s.components)
     for i, c in enumerate(s.components):
         cp = c.cumulative_probabilities()
-        if cp["identity"] != cp["reinsertion"]:
-            raise EngineError("identity-swap moves are not supported")
         sp = p.species[i]
         sp.p_translation, sp.p_rotation, sp.p_widom, sp.p_swap = (cp["translation"], cp["rotation"], cp["widom"],
                                                                   cp["swap"])
         sp.p_reinsertion = cp["reinsertion"]
+        sp.p_identity = cp["identity"]
         sp.mol_fraction, sp.fugacity_coeff, sp.ideal_rosen = c.mol_fraction, c.fugacity_coeff, c.ideal_rosen
         sp.excl_intra, sp.excl_self = exclusion_energy(s, i + 1)
         for d in range(3):

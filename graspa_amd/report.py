@@ -149,7 +149,7 @@ class RunRecord:
     names: List[str]                        # species names (adsorbates)
     framework_name: str
     precision: int
-    counters: np.ndarray                    # (n_species, 11): graspa_mc_stats layout
+    counters: np.ndarray                    # (n_species, GRASPA_N_COUNTERS): graspa_mc_stats layout
     n_mol: List[int]
     drift: DriftReport
     blocks: Optional[dict] = None           # graspa_batch_block_stats arrays
@@ -272,6 +272,14 @@ def write_output(rec: RunRecord, path_or_file):
             out.append("Reinsertion Performed: %d" % c[9])
             out.append("Reinsertion Accepted:  %d" % c[10])
         out.append("====================================================")
+        if len(c) > engine.CNT_IDSW_ACC and c[engine.CNT_IDSW_TOT:engine.CNT_IDSW_TOT + len(rec.names)].any():
+            # Print_IdentitySwap_Statistics (print_statistics.cuh:214-224)
+            out.append("=====================IDENTITY SWAP MOVES=====================")
+            for cj, other in enumerate(rec.names):
+                if cj != ci:
+                    out.append("Identity Swap Performed, FROM [%s (%d)] TO [%s (%d)]: %d (%d Accepted)" %
+                               (name, ci + 1, other, cj + 1, c[engine.CNT_IDSW_TOT + cj], c[engine.CNT_IDSW_ACC + cj]))
+            out.append("=============================================================")
         if c[8]:
             out.append("Widom Performed: %d" % c[8])
     out.append("=" * 96)

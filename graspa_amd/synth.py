@@ -70,7 +70,13 @@ def _methane():
                 atoms=[(0, (0.0, 0.0, 0.0))])
 
 
-SPECIES = {"CO2": _co2, "N2": _n2, "butane": _butane, "methane": _methane}
+def _argon():
+    # argon (eps/k 119.8 K, sigma 3.40 A), uncharged, one site
+    return dict(name="argon", names=["Ar"], eps=[119.8], sig=[3.40], charge=[0.0], mass=[39.948],
+                atoms=[(0, (0.0, 0.0, 0.0))])
+
+
+SPECIES = {"CO2": _co2, "N2": _n2, "butane": _butane, "methane": _methane, "argon": _argon}
 
 
 def build(n_fw=1536, L=24.0, species=("CO2",), mol_fraction=None, seed=0, T=300.0, P=1e5, cutoff=12.0,
@@ -129,6 +135,7 @@ def build(n_fw=1536, L=24.0, species=("CO2",), mol_fraction=None, seed=0, T=300.
         c.p_rotation = pr.get("rotation", 0.0)
         c.p_widom = pr.get("widom", 0.0)
         c.p_reinsertion = pr.get("reinsertion", 0.0)
+        c.p_identity = pr.get("identity", 0.0)
         c.p_swap = pr.get("swap", 0.0)
     return s
 

@@ -226,6 +226,7 @@ typedef struct {
   double max_translation[3], max_rotation[3];
   int64_t pseudo_per_mol[16];                          /* pseudo-atom counts of one molecule            */
   double p_reinsertion;            /* cumulative, between p_widom and p_swap (a value <= p_widom: none)  */
+  double p_identity;               /* cumulative, between p_reinsertion and p_swap (<= p_reinsertion: none) */
 } graspa_species;
 
 typedef struct {
@@ -258,9 +259,16 @@ typedef struct {
 #define GRASPA_MOVE_WIDOM       4
 #define GRASPA_MOVE_NONE        5
 #define GRASPA_MOVE_REINSERTION 6   /* ReinsertionMove (move_struct.h:170-407) */
+#define GRASPA_MOVE_IDENTITY_SWAP 7 /* IdentitySwapMove (mc_swap_moves.h:199-426); the trace's comp and
+                                       mol name the old molecule                                      */
 /* Move counters per species: trans_tot, trans_acc, rot_tot, rot_acc,
- * ins_tot, ins_acc, del_tot, del_acc, widom_tot, reins_tot, reins_acc. */
-#define GRASPA_N_COUNTERS      11
+ * ins_tot, ins_acc, del_tot, del_acc, widom_tot, reins_tot, reins_acc, then
+ * identity swaps from this species to species s (s = 0 .. 6, adsorbate
+ * component s + 1) attempted at [GRASPA_CNT_IDSW_TOT + s] and accepted at
+ * [GRASPA_CNT_IDSW_ACC + s] (IdentitySwap_Total_TO / _Acc_TO). */
+#define GRASPA_CNT_IDSW_TOT    11
+#define GRASPA_CNT_IDSW_ACC    18
+#define GRASPA_N_COUNTERS      25
 
 /* RandomNumber::Setup (data_struct.h:1287-1346) plus the move bookkeeping of
  * Initialize (main.cpp:58-343).  Uses the engine's restatement of glibc rand(). */
@@ -295,7 +303,10 @@ int  graspa_mc_cycle_times(graspa_mc* mc, double* seconds, int max_n);
  * d0 component (uniform over the species with SwapProbability > 0), d1
  * molecule, d2 move type, d3 insertion / deletion coin, d4 Metropolis, d5
  * first-bead selection, d6 orientation selection, d8..d10 translation or
- * rotation, d16+3k.. first-bead trial k, d16+3K+3o.. orientation o.
+ * rotation, d11 / d12 / d13 the old species (uniform over the species
+ * holding molecules), new species (uniform over all) and old molecule of an
+ * identity swap, d16+3k.. first-bead trial k, d16+3K+3o.. orientation o,
+ * d16+3K+3O+3o.. retrace orientation o (reinsertion, identity swap).
  *
  * Every box of a batch is on one device and has one precision form.  Between
  * graspa_batch_create and graspa_batch_destroy the boxes belong to the batch:
@@ -331,7 +342,7 @@ int  graspa_batch_launch(graspa_batch* batch, int moves_per_launch, int* n_runni
  * (<= 0: a default) until all boxes are done.  *moves = moves made. */
 int  graspa_batch_run(graspa_batch* batch, int n_cycles, int production, int moves_per_launch, int64_t* moves);
 /* State of box i: running energy change, molecule counts [n_species], move
- * counters [n_species][9] (graspa_mc_stats layout), Widom sums [n_species],
+ * counters [n_species][GRASPA_N_COUNTERS] (graspa_mc_stats layout), Widom sums [n_species],
  * per-cycle loading sums sum N and sum N^2 [n_species] over the production
  * cycles (*n_load_cycles), moves and cycles made since creation.  Any output
  * pointer may be NULL. */

@@ -71,7 +71,9 @@ class OrMC(C.Structure):
                 ("rot_acc", C.c_int64 * 8), ("ins_total", C.c_int64 * 8), ("ins_acc", C.c_int64 * 8),
                 ("del_total", C.c_int64 * 8), ("del_acc", C.c_int64 * 8), ("widom_total", C.c_int64 * 8),
                 ("widom_rosen_sum", C.c_double * 8), ("widom_rosen_n", C.c_double * 8), ("running", OrEnergy),
-                ("p_reinsertion", C.c_double * 8), ("reins_total", C.c_int64 * 8), ("reins_acc", C.c_int64 * 8)]
+                ("p_reinsertion", C.c_double * 8), ("reins_total", C.c_int64 * 8), ("reins_acc", C.c_int64 * 8),
+                ("p_identity", C.c_double * 8), ("idsw_total", (C.c_int64 * 8) * 8),
+                ("idsw_acc", (C.c_int64 * 8) * 8)]
 
 
 class OrWidomOut(C.Structure):
@@ -280,9 +282,8 @@ def make_mc(system, oracle_sys, production=1, pool_size=333334):
         mc.p_rotation[k] = cp["rotation"]
         mc.p_widom[k] = cp["widom"]
         mc.p_reinsertion[k] = cp["reinsertion"]
+        mc.p_identity[k] = cp["identity"]
         mc.p_swap[k] = cp["swap"]
-        if cp["identity"] != cp["reinsertion"]:
-            raise NotImplementedError("identity-swap moves are not in the parity driver")
         mc.mol_fraction[k] = c.mol_fraction
         mc.fugacity_coeff[k] = c.fugacity_coeff
         mc.ideal_rosen[k] = c.ideal_rosen

@@ -755,7 +755,7 @@ static double mc_draw(const mc_ctx* x, int d) { return or_philox_uniform(x->seed
 
 /* Draw layout of one move in the batched form (the engine's batch_mc.hip). */
 enum { D_COMP = 0, D_MOL = 1, D_MOVE = 2, D_COIN = 3, D_ACC = 4, D_SEL_FB = 5, D_SEL_OR = 6, D_VEC = 8,
-       D_TRIAL = 16 };
+       D_ID_OLD = 11, D_ID_NEW = 12, D_ID_MOL = 13, D_TRIAL = 16 };
 
 static int comp_has_charge(const mc_ctx* x, int c) { return x->mc->has_charge[c] && !x->b->no_charges; }
 
@@ -1208,6 +1208,233 @@ static void reinsertion_move(mc_ctx* x, int comp, int mol, or_trace* tr)
   energy_add(&mc->running, &de);
 }
 
+/* Identity swap: IdentitySwapMove (mc_swap_moves.h:199-426).  A molecule of
+ * the OLD species is replaced by one of the NEW species grown on its first
+ * bead: the NEW first bead sits at the OLD molecule's first bead (one trial,
+ * copy_firstbead_to_new :178-181, its weight not divided by K), the NEW chain
+ * takes O orientations of the NEW template (slot 0) about it, energies
+ * without the OLD molecule (ExcludeList); the OLD molecule is retraced with
+ * its own first bead (one trial) and O orientations (trial 0 its own).
+ * W_new carries the Ewald change of removing the OLD sites and adding the
+ * NEW ones (GPU_EwaldDifference_IdentitySwap, Ewald_Energy_Functions.h:582-632)
+ * and the tail change with d = pseudo(NEW) - pseudo(OLD)
+ * (TailCorrectionIdentitySwap, TailCorrection_Energy_Functions.h:85-108);
+ * Pacc = pref(NEW, insertion) pref(OLD, deletion) (W_new / W_id,NEW) /
+ * (W_old / W_id,OLD) with GetPrefactor (mc_utilities.h:315-345).  On
+ * acceptance the OLD molecule is deleted (the last one into its hole) and the
+ * NEW one appended, or, for the same species, moved in place.
+ * Draws in the pool order: the species pair (two uniforms per try until the
+ * OLD species holds a molecule), the OLD molecule, one triple (NEW first
+ * bead, unused), O triples and a selection (NEW chain), one triple (OLD first
+ * bead, unused), O triples (OLD chain), Metropolis. */
+static void identity_swap_move(mc_ctx* x, or_trace* tr)
+{
+  or_mc* mc = x->mc;
+  const or_box* b = x->b;
+  or_atoms* comps = x->comps;
+  const int ncomp = x->ncomp;
+  const int K = mc->n_trial_pos, O = mc->n_trial_orient;
+  const double beta = mc->beta;
+  static double sp[3 * 1024], sq[1024], s4[4 * 1024];
+  static int st[1024];
+  static uint8_t sov[1024];
+  static trial_set ts;
+  tr->accepted = 0; tr->overlap = 0; tr->pacc = 0.0; tr->rosen = 0.0;
+  memset(&tr->de, 0, sizeof(tr->de));
+  if (mc->total_molecules - 1 == 0) return;          /* no adsorbates: no draws */
+  int oc = 0, nc = 0;
+  int64_t nold = 0;
+  if (x->philox) {
+    int held[8], nh = 0;
+    for (int c = 1; c < ncomp && c < 8; c++)
+      if (comps[c].n_mol > 0) held[nh++] = c;
+    int i = (int)(mc_draw(x, D_ID_OLD) * (double)nh);
+    if (i >= nh) i = nh - 1;
+    oc = held[i];
+    nc = 1 + (int)(mc_draw(x, D_ID_NEW) * (double)(ncomp - 1));
+    if (nc >= ncomp) nc = ncomp - 1;
+  } else {
+    while (oc == 0 || oc >= ncomp || nc == 0 || nc >= ncomp || nold == 0) {
+      oc = (int)(or_rand_uniform() * (double)(ncomp - 1)) + 1;
+      nc = (int)(or_rand_uniform() * (double)(ncomp - 1)) + 1;
+      nold = oc < ncomp ? comps[oc].n_mol : 0;
+    }
+  }
+  or_atoms* AO = &comps[oc];
+  or_atoms* AN = &comps[nc];
+  const int oldmol = (int)((x->philox ? mc_draw(x, D_ID_MOL) : or_rand_uniform()) * (double)AO->n_mol);
+  const int Mo = AO->molsize, Mn = AN->molsize;
+  const int so = oldmol * Mo;
+  const int64_t nnew = AN->n_mol;
+  tr->comp = oc; tr->mol = oldmol;
+  mc->idsw_total[oc][nc]++;
+  or_energy de;
+  memset(&de, 0, sizeof(de));
+  double en_new[4] = {0, 0, 0, 0}, en_old[4] = {0, 0, 0, 0};
+  double new_pos[3 * 64], new_q[64], old_pos[3 * 64], old_q[64];
+  int new_t[64];
+  for (int i = 0; i < Mo; i++) {
+    for (int d = 0; d < 3; d++) old_pos[3 * i + d] = AO->pos[3 * (so + i) + d];
+    old_q[i] = AO->charge[so + i];
+  }
+  for (int i = 0; i < Mn; i++) { new_q[i] = AN->charge[i]; new_t[i] = AN->type[i]; }
+
+  /* ---- NEW first bead at the OLD molecule's first bead ---- */
+  if (!x->philox) { pool_check(1); g_pool_offset += 1; }
+  for (int d = 0; d < 3; d++) sp[d] = old_pos[d];
+  st[0] = new_t[0]; sq[0] = new_q[0];
+  or_site_energies(b, comps, ncomp, 1, sp, st, sq, nc, nnew, oc, oldmol, s4, sov);
+  sum_trials(s4, sov, 1, 1, beta, mc->vdw_real_bias, &ts);
+  if (ts.n_keep == 0) return;
+  double Wn = sum_exp(&ts);
+  if (Wn < 1e-150) return;
+  if (!mc->vdw_real_bias) Wn *= exp(-beta * (ts.e4[0][1] + ts.e4[0][3]));
+  if (Wn <= 1e-150) return;
+  for (int k = 0; k < 4; k++) en_new[k] = ts.e4[0][k];
+  for (int d = 0; d < 3; d++) new_pos[d] = old_pos[d];
+
+  /* ---- NEW chain: O orientations of the NEW template ---- */
+  if (Mn > 1) {
+    const int chain = Mn - 1;
+    if (!x->philox) pool_check((size_t)O);
+    for (int t = 0; t < O; t++) {
+      double Rp[3];
+      const double* R = Rp;
+      if (x->philox) for (int d = 0; d < 3; d++) Rp[d] = mc_draw(x, D_TRIAL + 3 * K + 3 * t + d);
+      else R = &g_pool[3 * (g_pool_offset + t)];
+      for (int a = 0; a < chain; a++) {
+        const int s = t * chain + a;
+        double vec[3] = {AN->pos[3 * (1 + a)] - AN->pos[0], AN->pos[3 * (1 + a) + 1] - AN->pos[1],
+                         AN->pos[3 * (1 + a) + 2] - AN->pos[2]};
+        rotate_quaternion(vec, R[0], R[1], R[2]);
+        for (int d = 0; d < 3; d++) sp[3 * s + d] = new_pos[d] + vec[d];
+        st[s] = new_t[1 + a]; sq[s] = new_q[1 + a];
+      }
+    }
+    if (!x->philox) g_pool_offset += (size_t)O;
+    or_site_energies(b, comps, ncomp, O * chain, sp, st, sq, nc, nnew, oc, oldmol, s4, sov);
+    sum_trials(s4, sov, O, chain, beta, mc->vdw_real_bias, &ts);
+    if (ts.n_keep == 0) return;
+    const int osel = select_trial(ts.rosen, ts.n_keep, x->philox ? mc_draw(x, D_SEL_OR) : or_rand_uniform());
+    const double rc = sum_exp(&ts);
+    if (rc < 1e-150) return;
+    double Wc = rc / (double)O;
+    if (!mc->vdw_real_bias) Wc *= exp(-beta * (ts.e4[osel][1] + ts.e4[osel][3]));
+    Wn *= Wc;
+    if (Wn <= 1e-150) return;
+    for (int k = 0; k < 4; k++) en_new[k] += ts.e4[osel][k];
+    const int ot = ts.keep[osel];
+    for (int a = 0; a < chain; a++)
+      for (int d = 0; d < 3; d++) new_pos[3 * (1 + a) + d] = sp[3 * (ot * chain + a) + d];
+  }
+
+  /* ---- retrace of the OLD molecule: its own first bead ---- */
+  if (!x->philox) { pool_check(1); g_pool_offset += 1; }
+  for (int d = 0; d < 3; d++) sp[d] = old_pos[d];
+  st[0] = AO->type[so]; sq[0] = old_q[0];
+  or_site_energies(b, comps, ncomp, 1, sp, st, sq, oc, oldmol, -1, -1, s4, sov);
+  sum_trials(s4, sov, 1, 1, beta, mc->vdw_real_bias, &ts);
+  double Wo = sum_exp(&ts);
+  if (ts.n_keep > 0) {
+    if (!mc->vdw_real_bias) Wo *= exp(-beta * (ts.e4[0][1] + ts.e4[0][3]));
+    for (int k = 0; k < 4; k++) en_old[k] = ts.e4[0][k];
+  }
+  /* ---- OLD chain: trial 0 its own orientation ---- */
+  if (Mo > 1) {
+    const int chain = Mo - 1;
+    if (!x->philox) pool_check((size_t)O);
+    for (int t = 0; t < O; t++) {
+      double Rp[3];
+      const double* R = Rp;
+      if (x->philox) for (int d = 0; d < 3; d++) Rp[d] = mc_draw(x, D_TRIAL + 3 * K + 3 * O + 3 * t + d);
+      else R = &g_pool[3 * (g_pool_offset + t)];
+      for (int a = 0; a < chain; a++) {
+        const int s = t * chain + a;
+        if (t == 0) {
+          for (int d = 0; d < 3; d++) sp[3 * s + d] = old_pos[3 * (1 + a) + d];
+        } else {
+          double vec[3] = {AO->pos[3 * (1 + a)] - AO->pos[0], AO->pos[3 * (1 + a) + 1] - AO->pos[1],
+                           AO->pos[3 * (1 + a) + 2] - AO->pos[2]};
+          rotate_quaternion(vec, R[0], R[1], R[2]);
+          for (int d = 0; d < 3; d++) sp[3 * s + d] = old_pos[d] + vec[d];
+        }
+        st[s] = AO->type[so + 1 + a]; sq[s] = old_q[1 + a];
+      }
+    }
+    if (!x->philox) g_pool_offset += (size_t)O;
+    or_site_energies(b, comps, ncomp, O * chain, sp, st, sq, oc, oldmol, -1, -1, s4, sov);
+    sum_trials(s4, sov, O, chain, beta, mc->vdw_real_bias, &ts);
+    double Wc = sum_exp(&ts) / (double)O;
+    if (ts.n_keep > 0) {
+      if (!mc->vdw_real_bias) Wc *= exp(-beta * (ts.e4[0][1] + ts.e4[0][3]));
+      for (int k = 0; k < 4; k++) en_old[k] += ts.e4[0][k];
+    }
+    Wo *= Wc;
+  }
+  de.hg_vdw = en_new[0] - en_old[0]; de.hg_real = en_new[1] - en_old[1];
+  de.gg_vdw = en_new[2] - en_old[2]; de.gg_real = en_new[3] - en_old[3];
+
+  /* ---- Ewald: OLD sites out, NEW sites in (charged species only) ---- */
+  const int qo = comp_has_charge(x, oc), qn = comp_has_charge(x, nc);
+  const int ewald = !b->no_charges && (qo || qn);
+  if (ewald) {
+    double same, cross;
+    or_ewald_delta(b, x->sk_ads, x->sk_fw, qo ? Mo : 0, old_pos, old_q, qn ? Mn : 0, new_pos, new_q, &same, &cross,
+                   x->sk_tmp);
+    if (qn) same -= (mc->excl_intra[nc] + mc->excl_self[nc]) * 1.0;
+    if (qo) same -= (mc->excl_intra[oc] + mc->excl_self[oc]) * -1.0;
+    de.gg_ewald = same;
+    de.hg_ewald = 2.0 * cross;
+    Wn *= exp(-beta * (same + 2.0 * cross));
+  }
+  /* ---- tail ---- */
+  if (mc->has_tail) {
+    int64_t d[16];
+    for (int t = 0; t < 16; t++) d[t] = mc->pseudo_per_mol[nc][t] - mc->pseudo_per_mol[oc][t];
+    de.tail = or_tail_delta(b, mc->n_pseudo, d, 1);
+  }
+  Wn *= exp(-beta * de.tail);
+
+  /* ---- acceptance ---- */
+  const double V = b->volume;
+  double pref = beta * mc->mol_fraction[nc] * mc->pressure * mc->fugacity_coeff[nc] * V / (1.0 + (double)AN->n_mol);
+  pref *= (double)AO->n_mol / (beta * mc->mol_fraction[oc] * mc->pressure * mc->fugacity_coeff[oc] * V);
+  const double pacc = pref * (Wn / mc->ideal_rosen[nc]) / (Wo / mc->ideal_rosen[oc]);
+  tr->rosen = Wn;
+  tr->pacc = pacc;
+  const double u = x->philox ? mc_draw(x, D_ACC) : or_rand_uniform();
+  if (!(u < pacc)) return;
+  if (nc != oc && AN->n_mol + 1 > AN->capacity) return;
+  tr->accepted = 1;
+  mc->idsw_acc[oc][nc]++;
+  if (nc != oc) {
+    const int last = (AO->n_mol - 1) * Mo;
+    if (so != last) {
+      for (int i = 0; i < Mo; i++) {
+        for (int d = 0; d < 3; d++) AO->pos[3 * (so + i) + d] = AO->pos[3 * (last + i) + d];
+        AO->charge[so + i] = AO->charge[last + i];
+        AO->type[so + i] = AO->type[last + i];
+      }
+    }
+    AO->n_mol--;
+    const int loc = (int)(nnew * Mn);
+    for (int i = 0; i < Mn; i++) {
+      for (int d = 0; d < 3; d++) AN->pos[3 * (loc + i) + d] = new_pos[3 * i + d];
+      AN->charge[loc + i] = new_q[i];
+      AN->type[loc + i] = new_t[i];
+      AN->molid[loc + i] = nnew;
+    }
+    AN->n_mol++;
+    for (int t = 0; t < b->ntypes && t < 16; t++) mc->n_pseudo[t] += mc->pseudo_per_mol[nc][t] - mc->pseudo_per_mol[oc][t];
+  } else {
+    for (int i = 0; i < Mo; i++)
+      for (int d = 0; d < 3; d++) AO->pos[3 * (so + i) + d] = new_pos[3 * i + d];
+  }
+  if (ewald) memcpy(x->sk_ads, x->sk_tmp, sizeof(double) * 2 * or_nvec(b));
+  tr->de = de;
+  energy_add(&mc->running, &de);
+}
+
 /* Update_Max_Translation / Update_Max_Rotation (mc_utilities.h:612-665) */
 static void update_max(double v[3], int64_t* acc, int64_t* tot, double hi)
 {
@@ -1256,6 +1483,9 @@ int or_mc_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
       } else if (r < mc->p_reinsertion[comp]) {
         tr.move = OR_REINSERTION;
         if (comps[comp].n_mol > 0) reinsertion_move(&x, comp, mol, &tr);
+      } else if (r < mc->p_identity[comp]) {
+        tr.move = OR_IDENTITY_SWAP;
+        identity_swap_move(&x, &tr);
       } else if (r < mc->p_swap[comp]) {
         if (or_rand_uniform() < 0.5) {
           tr.move = OR_INSERTION;
@@ -1327,6 +1557,9 @@ int or_batch_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
       } else if (r < mc->p_reinsertion[comp]) {
         tr.move = OR_REINSERTION;
         if (comps[comp].n_mol > 0) reinsertion_move(&x, comp, mol, &tr);
+      } else if (r < mc->p_identity[comp]) {
+        tr.move = OR_IDENTITY_SWAP;
+        identity_swap_move(&x, &tr);
       } else if (r < mc->p_swap[comp]) {
         if (mc_draw(&x, D_COIN) < 0.5) {
           tr.move = OR_INSERTION;

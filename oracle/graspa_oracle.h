@@ -122,7 +122,7 @@ double or_philox_uniform(uint64_t seed, uint64_t stream, uint64_t index, uint32_
 
 /* ---- Monte Carlo driver in reference draw order (parity mode) ---- */
 enum { OR_TRANSLATION = 0, OR_ROTATION = 1, OR_INSERTION = 2, OR_DELETION = 3, OR_WIDOM = 4,
-       OR_NOMOVE = 5, OR_REINSERTION = 6 };
+       OR_NOMOVE = 5, OR_REINSERTION = 6, OR_IDENTITY_SWAP = 7 };
 
 typedef struct {
   int     cycle, step, move, comp, mol, accepted, overlap, _pad;
@@ -158,6 +158,11 @@ typedef struct {
    * probability between p_widom and p_swap, and its counters */
   double  p_reinsertion[8];
   int64_t reins_total[8], reins_acc[8];
+  /* identity swap (IdentitySwapMove, mc_swap_moves.h:199-426): cumulative
+   * probability between p_reinsertion and p_swap; attempts and acceptances
+   * from component [old] to component [new] (IdentitySwap_Total_TO / _Acc_TO) */
+  double  p_identity[8];
+  int64_t idsw_total[8][8], idsw_acc[8][8];
 } or_mc;
 
 /* Allocates the device-style random pool and seeds it (data_struct.h:1287-1346).
@@ -177,7 +182,10 @@ int  or_mc_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
  * d0 component, d1 molecule, d2 move, d3 insertion/deletion coin, d4
  * Metropolis, d5 first-bead selection, d6 orientation selection, d8..d10
  * translation / rotation, d16+3k.. trial k, d16+3K+3o.. orientation o,
- * d16+3K+3O+3o.. retrace orientation o of a reinsertion. */
+ * d16+3K+3O+3o.. retrace orientation o of a reinsertion or identity swap;
+ * an identity swap takes its old species from d11 (uniform over the species
+ * holding molecules), its new species from d12 (uniform over all) and the
+ * old molecule from d13. */
 int  or_batch_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
                      double* sk_ads, double* sk_fw, uint64_t seed, uint64_t stream, int64_t* move_index,
                      int first_cycle, int n_cycles, or_trace* trace, int max_trace);

@@ -14,14 +14,24 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-ENERGY_KEYS = ("hg_vdw", "hg_real", "gg_vdw", "gg_real", "hg_ewald", "gg_ewald")
+ENERGY_KEYS = ("hg_vdw", "hg_real", "gg_vdw", "gg_real", "hg_ewald", "gg_ewald", "tail")
 MIXED_DE_TOL = 1e-3      # internal units (10 J/mol): 1e-5 kJ/mol
 
 
-def _mixture(seed=11, n_fw=200, L=24.0, P=5e6):
+def _mixture(seed=11, n_fw=200, L=24.0, P=5e6, tail=False):
     from graspa_amd import synth
-    return synth.build(n_fw=n_fw, L=L, species=("CO2", "N2"), mol_fraction=[0.5, 0.5], seed=seed, T=300.0, P=P,
-                       cycles=10, probs=dict(translation=1.0, rotation=1.0, swap=2.0, widom=0.5, reinsertion=1.0))
+    s = synth.build(n_fw=n_fw, L=L, species=("CO2", "N2"), mol_fraction=[0.5, 0.5], seed=seed, T=300.0, P=P,
+                    cycles=10, probs=dict(translation=1.0, rotation=1.0, swap=2.0, widom=0.5, reinsertion=1.0,
+                                          identity=1.0))
+    if tail:
+        # tail corrections on a random subset of type pairs
+        rng = np.random.default_rng(3)
+        T = s.ntypes
+        t = rng.uniform(-80.0, -5.0, (T, T))
+        s.tail = (t + t.T) / 2
+        u = (rng.uniform(size=(T, T)) < 0.6).astype(np.int32)
+        s.use_tail = np.maximum(u, u.T)
+    return s
 
 
 def _oracle_run(system, seed, stream, cycles, n_max=20000):
@@ -58,8 +68,9 @@ def _same_sequence(otr, gtr):
 
 @pytest.mark.parametrize("moves_per_launch", [0, 7])
 def test_batch_fp64_reproduces_oracle(engine_mod, moves_per_launch):
-    s = _mixture()
-    cycles = 12
+    s = _mixture(tail=True)
+    assert s.has_tail
+    cycles = 20
     o, mc, otr = _oracle_run(s, seed=77, stream=3, cycles=cycles)
     boxes, bt, moves = _engine_batch(engine_mod, [s], [77], [3], cycles, engine_mod.GRASPA_FP64,
                                      moves_per_launch=moves_per_launch)
@@ -75,8 +86,15 @@ def test_batch_fp64_reproduces_oracle(engine_mod, moves_per_launch):
         assert abs(a.pacc - b.pacc) <= 1e-10 * abs(a.pacc) + 1e-300, (i, a.pacc, b.pacc)
         assert abs(a.rosen - b.rosen) <= 1e-10 * abs(a.rosen) + 1e-300, (i, a.rosen, b.rosen)
     assert n_acc[2] > 3 and n_acc[0] + n_acc[1] > 3
-    assert n_acc.get(6, 0) >= 2              # reinsertions accepted
+    assert n_acc.get(6, 0) >= 1              # reinsertions accepted
+    assert n_acc.get(7, 0) >= 4              # identity swaps accepted
     st = bt.stats(0)
+    # identity-swap counters: attempts and acceptances per (old, new) species pair
+    from graspa_amd import engine
+    for ci in (1, 2):
+        for cj in (1, 2):
+            assert st.counters[ci - 1][engine.CNT_IDSW_TOT + cj - 1] == mc.idsw_total[ci][cj]
+            assert st.counters[ci - 1][engine.CNT_IDSW_ACC + cj - 1] == mc.idsw_acc[ci][cj]
     assert st.moves == len(otr) and st.cycles == cycles
     for ci in (1, 2):
         assert int(st.n_mol[ci - 1]) == o.comps[ci].n_mol

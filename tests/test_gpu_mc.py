@@ -95,3 +95,48 @@ def test_reinsertion_fp64_reproduces_oracle_sequence():
         opos, oq, ot, om = r["o"].comp_arrays(ci)
         na = len(q)
         assert np.allclose(pos, opos[:na], atol=1e-12)
+
+
+def test_identity_swap_fp64_reproduces_oracle_sequence():
+    """CO2/N2 mixture with identity swaps (and the other moves), tail
+    corrections on, in the reference's random-number order: the species pair,
+    molecule, growth on the old first bead, retrace, Ewald, tail and
+    acceptance must match the oracle move for move."""
+    from graspa_amd import engine, synth
+    s = synth.build(n_fw=200, L=24.0, species=("CO2", "N2"), mol_fraction=[0.5, 0.5], seed=11, T=300.0, P=5e6,
+                    cycles=20, probs=dict(translation=1.0, rotation=1.0, swap=2.0, widom=0.5, reinsertion=1.0,
+                                          identity=2.0))
+    rng = np.random.default_rng(3)
+    t = rng.uniform(-80.0, -5.0, (s.ntypes, s.ntypes))
+    s.tail = (t + t.T) / 2
+    u = (rng.uniform(size=(s.ntypes, s.ntypes)) < 0.6).astype(np.int32)
+    s.use_tail = np.maximum(u, u.T)
+    r = _run_pair(s, 20)
+    assert r["n_o"] == r["n_g"]
+    otr, gtr = r["otr"], r["gtr"]
+    n_id = 0
+    for i in range(r["n_o"]):
+        a, b = otr[i], gtr[i]
+        assert (a.cycle, a.step, a.move, a.comp, a.mol) == (b.cycle, b.step, b.move, b.comp, b.mol), i
+        assert a.accepted == b.accepted and a.overlap == b.overlap, (i, a.move)
+        if a.move == 7:
+            n_id += a.accepted
+            assert abs(a.rosen - b.rosen) <= 1e-10 * abs(a.rosen) + 1e-300, (i, a.rosen, b.rosen)
+        for k in ("hg_vdw", "hg_real", "gg_vdw", "gg_real", "hg_ewald", "gg_ewald", "tail"):
+            x, y = getattr(a.de, k), getattr(b.de, k)
+            assert abs(x - y) <= 1e-9 + 1e-10 * abs(x), (i, k, x, y)
+        if a.pacc > 0 and a.pacc < 1e300:
+            assert abs(a.pacc - b.pacc) <= 1e-9 * a.pacc, (i, a.pacc, b.pacc)
+    assert n_id >= 4
+    for ci in (1, 2):
+        for cj in (1, 2):
+            assert r["cnt"][ci - 1][engine.CNT_IDSW_TOT + cj - 1] == r["omc"].idsw_total[ci][cj]
+            assert r["cnt"][ci - 1][engine.CNT_IDSW_ACC + cj - 1] == r["omc"].idsw_acc[ci][cj]
+    assert abs(r["e"].total() - r["omc"].running.total()) <= 1e-8 * max(1.0, abs(r["omc"].running.total()))
+    for ci in (1, 2):
+        pos, q, t, m = r["box"].get_molecules(ci)
+        opos, oq, ot, om = r["o"].comp_arrays(ci)
+        na = len(q)
+        assert na == r["o"].comps[ci].n_mol * r["o"].comps[ci].molsize
+        assert np.allclose(pos, opos[:na], atol=1e-12)
+        assert np.array_equal(m, om[:na])

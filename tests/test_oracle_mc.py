@@ -5,9 +5,10 @@ equal the change of the total energy recomputed from scratch (pair sums over
 all atoms, Fourier sums from structure factors rebuilt from the atoms, minus
 the rigid exclusion of every molecule), and the running adsorbate structure
 factor must equal the rebuilt one.  A dropped or mis-signed term in any move's
-energy change (translation, rotation, insertion, deletion, reinsertion, the
-Ewald same/cross split, the exclusion correction) or a commit that stores the wrong
-molecule breaks one of these equalities.  Both drivers are checked: the
+energy change (translation, rotation, insertion, deletion, reinsertion,
+identity swap, the Ewald same/cross split, the exclusion correction, the tail
+change) or a commit that stores the wrong molecule breaks one of these
+equalities.  Both drivers are checked: the
 reference's random-number order (or_mc_cycles) and the batched form with
 counter-based streams (or_batch_cycles)."""
 import ctypes as C
@@ -16,11 +17,20 @@ import numpy as np
 import pytest
 
 
-def _system(P=5e6, n_fw=200, reinsertion=1.0):
+def _system(P=5e6, n_fw=200, reinsertion=1.0, identity=1.0, tail=True):
     from graspa_amd import synth
     s = synth.build(n_fw=n_fw, L=24.0, species=("CO2", "N2"), mol_fraction=[0.5, 0.5], seed=11, T=300.0, P=P,
                     cycles=10, probs=dict(translation=1.0, rotation=1.0, swap=2.0, widom=0.5,
-                                          reinsertion=reinsertion))
+                                          reinsertion=reinsertion, identity=identity))
+    if tail:
+        # tail corrections on a random subset of type pairs, so the tail
+        # change of insertions, deletions and identity swaps is booked too
+        rng = np.random.default_rng(3)
+        T = s.ntypes
+        t = rng.uniform(-80.0, -5.0, (T, T))
+        s.tail = (t + t.T) / 2
+        u = (rng.uniform(size=(T, T)) < 0.6).astype(np.int32)
+        s.use_tail = np.maximum(u, u.T)
     return s
 
 
@@ -33,6 +43,10 @@ def _totals(o):
         excl += o.comps[ci].n_mol * (i + sf)
     return dict(hg_vdw=e.hg_vdw, gg_vdw=e.gg_vdw, hg_real=e.hg_real, gg_real=e.gg_real,
                 ewald=f.gg_ewald + f.hg_ewald - excl), ska
+
+
+def _n_mol(o):
+    return [o.comps[c].n_mol for c in range(1, o.ncomp)]
 
 
 def _check_drift(o, mc, before, after, ska_running, ska_rebuilt):
@@ -49,32 +63,45 @@ def _check_drift(o, mc, before, after, ska_running, ska_rebuilt):
 @pytest.mark.parametrize("form", ["reference_order", "batched"])
 def test_running_energy_equals_recomputed_totals(oracle_lib, form):
     from oracle import binding as ob
+    from graspa_amd import report
     s = _system()
+    assert s.has_tail
     o = ob.OracleSystem(s, capacity=128)
     mc = ob.make_mc(s, o, production=1)
     before, ska = _totals(o)
+    tail0 = report.tail_total(s, _n_mol(o))
     f0, skf, ska, _, _, _ = o.ewald_total()
     n_max = 5000
     tr = (ob.OrTrace * n_max)()
     if form == "reference_order":
         assert ob.lib().or_mc_setup(C.byref(mc)) == 0
-        n = ob.lib().or_mc_cycles(C.byref(mc), o.box, o.comps, o.ncomp, ob._p(ska), ob._p(skf), 0, 12, tr, n_max)
+        n = ob.lib().or_mc_cycles(C.byref(mc), o.box, o.comps, o.ncomp, ob._p(ska), ob._p(skf), 0, 20, tr, n_max)
     else:
         mi = C.c_int64(0)
         n = ob.lib().or_batch_cycles(C.byref(mc), o.box, o.comps, o.ncomp, ob._p(ska), ob._p(skf), 77, 3,
-                                     C.byref(mi), 0, 12, tr, n_max)
+                                     C.byref(mi), 0, 20, tr, n_max)
         assert mi.value == n
     assert n > 0
     moves = [tr[i].move for i in range(min(n, n_max))]
     acc = [tr[i].accepted for i in range(min(n, n_max))]
-    # every move type ran and some were accepted (6: reinsertion)
-    for m in (0, 1, 2, 3, 4, 6):
+    # every move type ran and some were accepted (6: reinsertion, 7: identity swap)
+    for m in (0, 1, 2, 3, 4, 6, 7):
         assert m in moves, m
     assert sum(a for a, m in zip(acc, moves) if m == 2) > 3
     assert sum(a for a, m in zip(acc, moves) if m in (0, 1)) > 3
-    assert sum(a for a, m in zip(acc, moves) if m == 6) >= 2
+    assert sum(a for a, m in zip(acc, moves) if m == 6) >= 1
+    # identity swaps between the two species were accepted in both directions
+    idsw = np.ctypeslib.as_array(mc.idsw_acc)
+    assert idsw[1][2] >= 1 and idsw[2][1] >= 1, idsw[:3, :3]
+    # (an identity swap drawn while the box holds no adsorbate is not attempted)
+    assert 0 < int(np.ctypeslib.as_array(mc.idsw_total).sum()) <= sum(1 for m in moves if m == 7)
     after, ska_rebuilt = _totals(o)
     _check_drift(o, mc, before, after, ska, ska_rebuilt)
+    dt = report.tail_total(s, _n_mol(o)) - tail0
+    assert abs(mc.running.tail - dt) <= 1e-9 * max(1.0, abs(dt)), (mc.running.tail, dt)
+    # pseudo-atom counts follow the molecules
+    n = s.pseudo_counts(_n_mol(o))
+    assert [mc.n_pseudo[t] for t in range(len(n))] == [int(v) for v in n]
 
 
 def test_batched_form_is_deterministic_and_stream_dependent(oracle_lib):
@@ -152,3 +179,62 @@ def test_reinsertion_samples_the_boltzmann_distribution(oracle_lib):
     blocks = e.reshape(20, -1).mean(1)
     err = blocks.std(ddof=1) / np.sqrt(len(blocks))
     assert abs(e.mean() - u_ref) < 4.0 * err + 0.01 * abs(u_ref), (e.mean(), u_ref, err)
+
+
+def test_identity_swap_samples_the_semigrand_distribution(oracle_lib):
+    """One molecule that is either methane or argon among 150 framework atoms,
+    reinsertion and identity-swap moves only.  Identity swaps with symmetric
+    proposals and Pacc = (f_new / f_old) W_new / W_old leave
+    pi(s, r) ~ f_s exp(-beta U_s(r)) invariant, so the fraction of cycles
+    spent as methane is x_m Z_m / (x_m Z_m + x_a Z_a) with
+    Z_s = int exp(-beta U_s) over the box (plain Monte Carlo integration over
+    300,000 uniform points).  An inverted prefactor, a dropped retrace weight
+    or a mis-signed energy change moves the fraction far outside the bound."""
+    from graspa_amd import synth
+    from oracle import binding as ob
+    x_m = 0.3
+    s = synth.build(n_fw=150, L=25.0, species=("methane", "argon"), mol_fraction=[x_m, 1.0 - x_m], seed=5,
+                    T=200.0, P=1e3, charges=False, K=10, probs=dict(reinsertion=1.0, identity=1.0))
+    fw = np.asarray(s.fw_pos)
+    t_fw = np.asarray(s.fw_type)
+    L = s.cell[0]
+
+    def energy(p, site_t):
+        eps = np.asarray(s.eps)[t_fw, site_t]
+        sig2 = np.asarray(s.sig)[t_fw, site_t] ** 2
+        d = fw[None, :, :] - p[:, None, :]
+        d -= L * np.round(d / L)
+        rr = (d * d).sum(-1)
+        x = (sig2[None, :] / rr) ** 3
+        e = 4.0 * eps[None, :] * (x * x - x)
+        return np.where(rr < s.cut_vdw ** 2, e, 0.0).sum(1)
+
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(0, L, (300000, 3))
+    Um = np.concatenate([energy(pts[i:i + 20000], int(s.components[0].types[0])) for i in range(0, len(pts), 20000)])
+    Ua = np.concatenate([energy(pts[i:i + 20000], int(s.components[1].types[0])) for i in range(0, len(pts), 20000)])
+    u0 = min(Um.min(), Ua.min())
+    Zm = np.exp(-s.beta * (Um - u0)).mean()
+    Za = np.exp(-s.beta * (Ua - u0)).mean()
+    p_ref = x_m * Zm / (x_m * Zm + (1.0 - x_m) * Za)
+    # start as methane at the methane energy minimum of the sample
+    c = s.components[0]
+    c.init_pos = pts[np.argmin(Um)][None, :].copy()
+    c.init_molid = np.zeros(1, dtype=np.int64)
+    c.init_charge = np.zeros(1)
+    o = ob.OracleSystem(s, capacity=4)
+    mc = ob.make_mc(s, o, production=1)
+    _, skf, ska, _, _, _ = o.ewald_total()
+    assert ob.lib().or_mc_setup(C.byref(mc)) == 0
+    tr = (ob.OrTrace * 64)()
+    n_cyc = 3000
+    is_m = np.zeros(n_cyc)
+    for cyc in range(n_cyc):
+        ob.lib().or_mc_cycles(C.byref(mc), o.box, o.comps, o.ncomp, ob._p(ska), ob._p(skf), cyc, 1, tr, 64)
+        assert o.comps[1].n_mol + o.comps[2].n_mol == 1
+        is_m[cyc] = o.comps[1].n_mol
+    acc = np.ctypeslib.as_array(mc.idsw_acc)
+    assert acc[1][2] > 20 and acc[2][1] > 20, acc[:3, :3]
+    blocks = is_m.reshape(20, -1).mean(1)
+    err = blocks.std(ddof=1) / np.sqrt(len(blocks))
+    assert abs(is_m.mean() - p_ref) < 4.0 * err + 0.02, (is_m.mean(), p_ref, err)
