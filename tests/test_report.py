@@ -178,3 +178,19 @@ def test_written_drift_sections_parse_like_the_reference():
     # the energy lines keep the reference's layout: 28-column label, value, [K], [kJ/mol]
     line = [l for l in text.splitlines() if l.startswith("VDW [Host-Guest]:")][0]
     assert re.match(r"VDW \[Host-Guest\]: {11}-5000\.00000 \(-6013\.62142 \[K\], -50\.00000 \[kJ/mol\]\)$", line)
+
+
+def test_identity_swap_statistics_lines():
+    """Print_IdentitySwap_Statistics (print_statistics.cuh:214-224): one line
+    per other species with the attempts and acceptances from this one."""
+    from graspa_amd import engine
+    s = synth.build(n_fw=40, L=20.0, species=("CO2", "N2"), mol_fraction=[0.5, 0.5], seed=5)
+    cnt = np.zeros((2, engine.N_COUNTERS), np.int64)
+    cnt[0][engine.CNT_IDSW_TOT + 1], cnt[0][engine.CNT_IDSW_ACC + 1] = 40, 7
+    cnt[1][engine.CNT_IDSW_TOT + 0], cnt[1][engine.CNT_IDSW_ACC + 0] = 33, 5
+    cnt[1][engine.CNT_IDSW_TOT + 1] = 9        # to its own species: counted, not printed
+    blk = _blocks_from_series(np.ones((10, 2)), np.zeros((10, 10)), ns=2)
+    text = report.write_output(_record(s, blk, counters=cnt), __import__("io").StringIO())
+    lines = [l for l in text.splitlines() if l.startswith("Identity Swap Performed")]
+    assert lines == ["Identity Swap Performed, FROM [CO2 (1)] TO [N2 (2)]: 40 (7 Accepted)",
+                     "Identity Swap Performed, FROM [N2 (2)] TO [CO2 (1)]: 33 (5 Accepted)"]
