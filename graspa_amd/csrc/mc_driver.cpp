@@ -75,6 +75,9 @@ struct graspa_mc {
   graspa_energy running{};
   std::vector<double> cycle_seconds;
   std::vector<double> tail, use_tail_d;
+  // creation phase (InsertionMove::CreateMolecule, move_struct.h:96-106):
+  // insertions accepted with the uniform 1e-100, no move counters
+  int create = 0;
   std::vector<int> use_tail;
   int ntypes = 0;
 
@@ -296,7 +299,7 @@ int cbmc_move(graspa_mc* mc, int comp, int mol, int move, graspa_trace& tr, int 
   const int start = del ? mol * M : 0;
   const double beta = mc->p.beta;
   const int bias = mc->p.vdw_real_bias;
-  if (move == GRASPA_MOVE_INSERTION) mc->cnt[comp][4]++;
+  if (move == GRASPA_MOVE_INSERTION && !mc->create) mc->cnt[comp][4]++;
   if (move == GRASPA_MOVE_DELETION) mc->cnt[comp][6]++;
   if (move == GRASPA_MOVE_WIDOM) mc->cnt[comp][8]++;
   tr.accepted = 0; tr.overlap = 0; tr.pacc = 0.0; tr.rosen = 0.0; tr.de = graspa_energy{};
@@ -445,14 +448,14 @@ int cbmc_move(graspa_mc* mc, int comp, int mol, int move, graspa_trace& tr, int 
   double pacc = del ? pref * sp.ideal_rosen / W : pref * W / sp.ideal_rosen;
   if (!del && W <= 1e-150) pacc = 0.0;
   tr.pacc = pacc;
-  const double u = mc->uniform();
+  const double u = mc->create ? 1e-100 : mc->uniform();
   if (!(u < pacc)) return GRASPA_OK;
   tr.accepted = 1;
   if (!del) {
     if (int rc = graspa_commit_insert(b, comp, mol_pos.data(), mol_q.data(), mol_t.data(), N)) return rc;
     mc->total_molecules++;
     for (int t = 0; t < 16; ++t) mc->n_pseudo[t] += sp.pseudo_per_mol[t];
-    mc->cnt[comp][5]++;
+    if (!mc->create) mc->cnt[comp][5]++;
   } else {
     if (int rc = graspa_commit_delete(b, comp, mol)) return rc;
     mc->total_molecules--;
@@ -967,6 +970,33 @@ int graspa_mc_cycles(graspa_mc* mc, int first_cycle, int n_cycles, int productio
     mc->cycle_seconds.push_back(std::chrono::duration<double>(t1 - t0).count());
   }
   if (n_steps) *n_steps = nt;
+  return GRASPA_OK;
+}
+
+int graspa_mc_create_molecules(graspa_mc* mc, int comp, int n, int64_t max_attempts, int* n_created,
+                               graspa_energy* de)
+{
+  if (!mc || n < 0 || max_attempts < 0) return fail(GRASPA_ERR_ARG, "bad arguments");
+  if (comp < 1 || comp >= (int)mc->box->comps.size()) return fail(GRASPA_ERR_ARG, "component out of range");
+  int created = 0;
+  graspa_energy sum{};
+  mc->create = 1;
+  int rc = GRASPA_OK;
+  for (int64_t a = 0; a < max_attempts && created < n; ++a) {
+    graspa_trace tr{};
+    const int mol = created > 0 ? created - 1 : 0;     // TempVal.molecule (axpy.cu:349)
+    rc = cbmc_move(mc, comp, mol, GRASPA_MOVE_INSERTION, tr, 0);
+    if (rc) break;
+    if (tr.accepted) {
+      ++created;
+      eadd(sum, tr.de);
+    }
+  }
+  mc->create = 0;
+  if (n_created) *n_created = created;
+  if (de) *de = sum;
+  if (rc) return rc;
+  if (created < n) return fail(GRASPA_ERR_STATE, "Bad Insertions When Creating Molecules!");
   return GRASPA_OK;
 }
 

@@ -150,6 +150,7 @@ _SIGS = {
     "graspa_mc_create": (C.c_int, [C.c_void_p, C.POINTER(MCParams), C.POINTER(C.c_void_p)]),
     "graspa_mc_destroy": (None, [C.c_void_p]),
     "graspa_mc_cycles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Trace), C.c_int, _i64p]),
+    "graspa_mc_create_molecules": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int64, _i32p, C.POINTER(Energy)]),
     "graspa_mc_stats": (C.c_int, [C.c_void_p, C.POINTER(Energy), _i32p, _i64p, _f64p, _f64p]),
     "graspa_mc_cycle_times": (C.c_int, [C.c_void_p, _f64p, C.c_int]),
     "graspa_timing_enable": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
@@ -521,6 +522,15 @@ class MonteCarlo:
         _check(lib().graspa_mc_cycles(self.h, first_cycle, n_cycles, int(production), tr, max_trace, C.byref(n)),
                "graspa_mc_cycles")
         return n.value, tr
+
+    def create_molecules(self, comp, n, max_attempts=10 ** 7):
+        """CreateMolecule_InOneBox (axpy.cu:300-377) for component `comp`:
+        returns the energy change of creating n molecules."""
+        got = C.c_int()
+        e = Energy()
+        _check(lib().graspa_mc_create_molecules(self.h, comp, n, max_attempts, C.byref(got), C.byref(e)),
+               "graspa_mc_create_molecules")
+        return e
 
     def cycle_times(self, n):
         out = np.zeros(n)

@@ -27,20 +27,32 @@ def run_deck(dirpath, out_dir=None, precision=engine.GRASPA_MIXED, device=0, mov
     directory): Output/System_0/output.txt, Restart/System_0/restartfile,
     Movies/System_0/result_<cycle>.data.  Returns the RunRecord."""
     s = deck.read_deck(dirpath)
-    if any(c.create_molecules for c in s.components):
-        raise NotImplementedError("CreateNumberOfMolecules: start from an empty box or a restart file")
     out_dir = out_dir or dirpath
     name = framework_name or os.path.basename(os.path.normpath(dirpath))
     box = engine.Box(s, precision=precision, device=device, capacity=capacity)
     box.init_ewald()
-    n_mol0 = [0 if c.init_pos is None else len(c.init_pos) // c.molsize for c in s.components]
-    initial = report.recompute_totals(box, s, n_mol0)
-    bt = engine.Batch([box], [s.seed], [0], cycle_cap=max(1, s.n_prod))
-    t0 = time.perf_counter()
 
     def say(msg):
         if log is not None:
             log(msg)
+
+    if any(c.create_molecules for c in s.components):
+        # CreateMolecule_InOneBox (axpy.cu:300-377, main.cpp:338): the
+        # requested molecules are created by CBMC insertions in the
+        # reference's draw order (RandomSeed) before the cycles start; the
+        # energy summary starts from the box after creation
+        mc = engine.MonteCarlo(box)
+        try:
+            for ci, c in enumerate(s.components, start=1):
+                if c.create_molecules > 0:
+                    say("Component %d, Need to create %d full molecule" % (ci, c.create_molecules))
+                    mc.create_molecules(ci, c.create_molecules)
+        finally:
+            mc.close()
+    n_mol0 = [len(box.get_molecules(ci)[1]) // c.molsize for ci, c in enumerate(s.components, start=1)]
+    initial = report.recompute_totals(box, s, n_mol0)
+    bt = engine.Batch([box], [s.seed], [0], cycle_cap=max(1, s.n_prod))
+    t0 = time.perf_counter()
 
     try:
         for n, label in ((s.n_init, "INITIALIZATION"), (s.n_equil, "EQUILIBRATION")):

@@ -230,7 +230,8 @@ def write_deck(system: System, dirpath: str, framework_name="synth", cycles=None
                     (c.ideal_rosen, c.fugacity_coeff))
             f.write("            MolFraction %g\n" % c.mol_fraction)
             for key, v in (("TranslationProbability", c.p_translation), ("RotationProbability", c.p_rotation),
-                           ("WidomProbability", c.p_widom), ("SwapProbability", c.p_swap)):
+                           ("WidomProbability", c.p_widom), ("ReinsertionProbability", c.p_reinsertion),
+                           ("IdentityChangeProbability", c.p_identity), ("SwapProbability", c.p_swap)):
                 if v:
                     f.write("            %s %g\n" % (key, v))
             f.write("            CreateNumberOfMolecules 0\n\n")

@@ -279,6 +279,15 @@ void graspa_mc_destroy(graspa_mc* mc);
  * number of steps in *n_steps; up to max_trace records go to trace (may be NULL). */
 int  graspa_mc_cycles(graspa_mc* mc, int first_cycle, int n_cycles, int production, graspa_trace* trace,
                       int max_trace, int64_t* n_steps);
+/* CreateMolecule_InOneBox (axpy.cu:300-377) for one component: CBMC
+ * insertions in the reference's draw order, each accepted when its
+ * construction succeeds and Pacc > 1e-100 (InsertionMove::CreateMolecule,
+ * move_struct.h:96-106, no acceptance uniform, no move counters), until n
+ * molecules exist more or max_attempts insertions were tried
+ * (GRASPA_ERR_STATE then, as the reference throws).  The energy change goes to
+ * *de and into the running energy. */
+int  graspa_mc_create_molecules(graspa_mc* mc, int comp, int n, int64_t max_attempts, int* n_created,
+                                graspa_energy* de);
 /* Running energy change (SystemComponents.deltaE), molecule counts, move
  * counters [species][GRASPA_N_COUNTERS], Widom sums. */
 int  graspa_mc_stats(graspa_mc* mc, graspa_energy* running, int* n_mol, int64_t* counters, double* widom_sum_w,

@@ -119,6 +119,8 @@ def lib():
                                    C.c_int, C.c_int, C.POINTER(OrTrace), C.c_int]
         L.or_batch_cycles.argtypes = [C.POINTER(OrMC), C.POINTER(OrBox), C.POINTER(OrAtoms), C.c_int, _f64p, _f64p,
                                       C.c_uint64, C.c_uint64, _i64p, C.c_int, C.c_int, C.POINTER(OrTrace), C.c_int]
+        L.or_mc_create_molecules.argtypes = [C.POINTER(OrMC), C.POINTER(OrBox), C.POINTER(OrAtoms), C.c_int, _f64p,
+                                             _f64p, C.c_int, C.c_int, C.c_int64, C.POINTER(OrEnergy)]
         L.or_widom_batch.argtypes = [C.POINTER(OrBox), C.POINTER(OrAtoms), C.c_int, C.c_int, _f64p, _f64p, C.c_double,
                                      C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                      C.c_uint64, C.c_int64, C.c_int, C.POINTER(OrWidomOut)]

@@ -749,6 +749,9 @@ typedef struct {
   /* batched form: the uniforms of move `move` come from the Philox stream
    * (seed, stream, move) by draw index instead of the shared pool */
   int philox; uint64_t seed, stream, move;
+  /* creation phase (InsertionMove::CreateMolecule, move_struct.h:96-106):
+   * insertions accepted with the uniform 1e-100, no counters */
+  int create;
 } mc_ctx;
 
 static double mc_draw(const mc_ctx* x, int d) { return or_philox_uniform(x->seed, x->stream, x->move, (uint32_t)d); }
@@ -857,7 +860,7 @@ static void cbmc_move(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
   int64_t sel_molid = del ? (int64_t)mol : N;      /* TempVal.molecule for deletion */
   int start = del ? mol * M : 0;
   double beta = mc->beta;
-  if (move == OR_INSERTION) mc->ins_total[comp]++;
+  if (move == OR_INSERTION && !x->create) mc->ins_total[comp]++;
   if (move == OR_DELETION) mc->del_total[comp]++;
   if (move == OR_WIDOM) mc->widom_total[comp]++;
 
@@ -999,7 +1002,7 @@ static void cbmc_move(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
   double pacc = del ? pref * mc->ideal_rosen[comp] / W : pref * W / mc->ideal_rosen[comp];
   if (!del && W <= 1e-150) pacc = 0.0;
   tr->pacc = pacc;
-  double u = x->philox ? mc_draw(x, D_ACC) : or_rand_uniform();
+  double u = x->create ? 1e-100 : (x->philox ? mc_draw(x, D_ACC) : or_rand_uniform());
   if (!(u < pacc)) return;  /* rejected: energy zero */
   /* accepted */
   tr->accepted = 1;
@@ -1016,7 +1019,7 @@ static void cbmc_move(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
     A->n_mol++;
     mc->total_molecules++;
     for (int t = 0; t < b->ntypes && t < 16; t++) mc->n_pseudo[t] += mc->pseudo_per_mol[comp][t];
-    mc->ins_acc[comp]++;
+    if (!x->create) mc->ins_acc[comp]++;
   } else {
     int last = (int)((N - 1) * M);
     if (start != last) {
@@ -1508,6 +1511,36 @@ int or_mc_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
   }
   free(x.sk_tmp);
   return nt;
+}
+
+/* CreateMolecule_InOneBox (axpy.cu:300-377) for one component: CBMC
+ * insertions in the reference's draw order, each accepted when its
+ * construction succeeds and Pacc > 1e-100 (InsertionMove::CreateMolecule,
+ * move_struct.h:96-106, draws no acceptance uniform), repeated until n
+ * molecules are created or max_attempts insertions were tried.  Returns the
+ * number created (or -1 on error); the energy change is added to *de and to
+ * the running energy. */
+int or_mc_create_molecules(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp, double* sk_ads, double* sk_fw,
+                           int comp, int n, int64_t max_attempts, or_energy* de)
+{
+  if (!g_pool || comp < 1 || comp >= ncomp) return -1;
+  mc_ctx x;
+  memset(&x, 0, sizeof(x));
+  x.b = b; x.comps = comps; x.ncomp = ncomp; x.mc = mc; x.sk_ads = sk_ads; x.sk_fw = sk_fw; x.create = 1;
+  x.sk_tmp = (double*)calloc((size_t)2 * or_nvec(b), sizeof(double));
+  int created = 0;
+  for (int64_t a = 0; a < max_attempts && created < n; a++) {
+    or_trace tr;
+    memset(&tr, 0, sizeof(tr));
+    const int mol = created > 0 ? created - 1 : 0;      /* TempVal.molecule (axpy.cu:349) */
+    cbmc_move(&x, comp, mol, OR_INSERTION, &tr);
+    if (tr.accepted) {
+      created++;
+      if (de) energy_add(de, &tr.de);
+    }
+  }
+  free(x.sk_tmp);
+  return created;
 }
 
 /* Batched form of the same driver: one box, the uniforms of move t from the

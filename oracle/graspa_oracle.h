@@ -177,6 +177,12 @@ int  or_mc_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
                   double* sk_ads, double* sk_fw, int first_cycle, int n_cycles,
                   or_trace* trace, int max_trace);
 
+/* CreateMolecule_InOneBox (axpy.cu:300-377) for one component: up to n CBMC
+ * insertions accepted whenever their construction succeeds (no acceptance
+ * uniform drawn), at most max_attempts tries; returns the number created. */
+int  or_mc_create_molecules(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp, double* sk_ads, double* sk_fw,
+                            int comp, int n, int64_t max_attempts, or_energy* de);
+
 /* Batched form of the driver (the engine's device-resident GCMC): move t of
  * the box draws uniform d from or_philox_uniform(seed, stream, t, d) --
  * d0 component, d1 molecule, d2 move, d3 insertion/deletion coin, d4

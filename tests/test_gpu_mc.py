@@ -140,3 +140,49 @@ def test_identity_swap_fp64_reproduces_oracle_sequence():
         assert na == r["o"].comps[ci].n_mol * r["o"].comps[ci].molsize
         assert np.allclose(pos, opos[:na], atol=1e-12)
         assert np.array_equal(m, om[:na])
+
+
+def test_create_molecules_fp64_reproduces_oracle():
+    """CreateMolecule_InOneBox in the reference's draw order: the plain FP64
+    driver creates the same molecules as the oracle, with the same energy."""
+    import ctypes as C
+    from graspa_amd import engine, synth
+    from oracle import binding as ob
+    s = synth.build(n_fw=300, L=24.0, species=("CO2", "N2"), mol_fraction=[0.5, 0.5], seed=17, T=300.0, P=1e5,
+                    cycles=4, probs=dict(translation=1.0, rotation=1.0, swap=2.0))
+    o = ob.OracleSystem(s, capacity=64)
+    mc = ob.make_mc(s, o, production=1)
+    assert ob.lib().or_mc_setup(C.byref(mc)) == 0
+    _, skf, ska, _, _, _ = o.ewald_total()
+    ode = ob.OrEnergy()
+    for comp, n in ((1, 10), (2, 6)):
+        assert ob.lib().or_mc_create_molecules(C.byref(mc), o.box, o.comps, o.ncomp, ob._p(ska), ob._p(skf), comp, n,
+                                               100000, C.byref(ode)) == n
+    box = engine.Box(s, precision=engine.GRASPA_FP64, capacity=64)
+    box.init_ewald()
+    gm = engine.MonteCarlo(box)
+    gde = {}
+    for comp, n in ((1, 10), (2, 6)):
+        e = gm.create_molecules(comp, n)
+        for k in ("hg_vdw", "hg_real", "gg_vdw", "gg_real", "hg_ewald", "gg_ewald"):
+            gde[k] = gde.get(k, 0.0) + getattr(e, k)
+    for k, v in gde.items():
+        x = getattr(ode, k)
+        assert abs(x - v) <= 1e-9 + 1e-10 * abs(x), (k, x, v)
+    for ci in (1, 2):
+        pos, q, t, m = box.get_molecules(ci)
+        opos, oq, ot, om = o.comp_arrays(ci)
+        na = len(q)
+        assert na == o.comps[ci].n_mol * o.comps[ci].molsize
+        assert np.allclose(pos, opos[:na], atol=1e-12)
+    # the cycles that follow stay in step with the oracle
+    n_max = 20000
+    otr = (ob.OrTrace * n_max)()
+    n_o = ob.lib().or_mc_cycles(C.byref(mc), o.box, o.comps, o.ncomp, ob._p(ska), ob._p(skf), 0, 4, otr, n_max)
+    n_g, gtr = gm.cycles(0, 4, production=True, max_trace=n_max)
+    assert n_o == n_g
+    for i in range(n_o):
+        a, b = otr[i], gtr[i]
+        assert (a.move, a.comp, a.mol, a.accepted) == (b.move, b.comp, b.mol, b.accepted), i
+    gm.close()
+    box.close()

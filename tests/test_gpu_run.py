@@ -40,3 +40,30 @@ def test_run_deck(tmp_path):
     b.close()
     for k in report.TERMS:
         assert abs(e[k] - rec.drift.final_fp64[k]) <= 1e-9 * max(1.0, abs(e[k])), k
+
+
+def test_run_deck_creates_molecules(tmp_path):
+    """CreateNumberOfMolecules: the deck's molecules are created before the
+    cycles (CreateMolecule_InOneBox) and the written drift check passes."""
+    from graspa_amd import engine, run, synth
+    s = synth.build(n_fw=300, L=24.0, species=("CO2", "N2"), mol_fraction=[0.5, 0.5], seed=43, P=1e5, cycles=6,
+                    probs=dict(translation=1.0, rotation=1.0, identity=1.0))
+    d = tmp_path / "deck"
+    synth.write_deck(s, str(d), cycles=6)
+    sim = d / "simulation.input"
+    txt = sim.read_text()
+    parts = txt.split("CreateNumberOfMolecules 0")
+    assert len(parts) == 3
+    sim.write_text(parts[0] + "CreateNumberOfMolecules 9" + parts[1] + "CreateNumberOfMolecules 5" + parts[2])
+    print("run_deck create test", flush=True)
+    logs = []
+    rec = run.run_deck(str(d), precision=engine.GRASPA_MIXED, moves_per_launch=64, capacity=256, log=logs.append)
+    assert any("Need to create 9" in l for l in logs)
+    out = (d / "Output" / "System_0" / "output.txt").read_text()
+    m = re.search("ENERGY DRIFT", out)
+    val = float(re.compile(r"Total Energy:\s+([-+]?[0-9]*\.?[0-9]+)").search(out, m.end()).group(1))
+    assert abs(val) < 1e-3
+    # the initial stage is the box after creation; without insertions and
+    # deletions the 14 molecules stay (identity swaps may change the species)
+    assert rec.drift.initial["hg_vdw"] != 0.0 and rec.drift.initial["gg_vdw"] != 0.0
+    assert sum(rec.n_mol) == 14

@@ -238,3 +238,34 @@ def test_identity_swap_samples_the_semigrand_distribution(oracle_lib):
     blocks = is_m.reshape(20, -1).mean(1)
     err = blocks.std(ddof=1) / np.sqrt(len(blocks))
     assert abs(is_m.mean() - p_ref) < 4.0 * err + 0.02, (is_m.mean(), p_ref, err)
+
+
+def test_create_molecules_then_cycles_keep_the_bookkeeping(oracle_lib):
+    """CreateMolecule_InOneBox: the requested molecules of each species are
+    created from an empty box (insertions accepted whenever the construction
+    succeeds, no move counters), their energy change equals the change of the
+    recomputed totals, and the cycles that follow keep the bookkeeping."""
+    from oracle import binding as ob
+    s = _system(P=1e5)
+    o = ob.OracleSystem(s, capacity=128)
+    mc = ob.make_mc(s, o, production=1)
+    before, _ = _totals(o)
+    _, skf, ska, _, _, _ = o.ewald_total()
+    assert ob.lib().or_mc_setup(C.byref(mc)) == 0
+    de = ob.OrEnergy()
+    for comp, n in ((1, 12), (2, 7)):
+        got = ob.lib().or_mc_create_molecules(C.byref(mc), o.box, o.comps, o.ncomp, ob._p(ska), ob._p(skf), comp, n,
+                                              100000, C.byref(de))
+        assert got == n
+        assert o.comps[comp].n_mol == n
+    assert mc.ins_total[1] == 0 and mc.ins_acc[1] == 0
+    mid, ska_rebuilt = _totals(o)
+    for k in ("hg_vdw", "gg_vdw", "hg_real", "gg_real"):
+        d = mid[k] - before[k]
+        assert abs(getattr(de, k) - d) <= 1e-9 * max(1.0, abs(d)), (k, getattr(de, k), d)
+    # the running energy holds the creation; the cycles that follow add to it
+    assert mc.total_molecules == 1 + 12 + 7
+    tr = (ob.OrTrace * 5000)()
+    ob.lib().or_mc_cycles(C.byref(mc), o.box, o.comps, o.ncomp, ob._p(ska), ob._p(skf), 0, 5, tr, 5000)
+    after, ska_rebuilt = _totals(o)
+    _check_drift(o, mc, before, after, ska, ska_rebuilt)
