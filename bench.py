@@ -18,6 +18,7 @@ core) timed on a bounded sample as the CPU baseline.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -186,6 +187,50 @@ def run_batch_gcmc(n_boxes, cycles, precision, warm_cycles=1, capacity=2048):
     return out
 
 
+def gcmc_de_error(systems, cycles, seeds, capacity=2048, trace_cap=4000):
+    """|dE| of the fast form against the plain FP64 form on the same boxes and
+    draws: both forms run `cycles` cycles through the device-resident driver
+    with move traces; along each box's common sequence (the trajectories stay
+    the same while every accept/reject decision agrees) the energy change of
+    every accepted move is compared term by term."""
+    from graspa_amd import engine
+    keys = ("hg_vdw", "hg_real", "gg_vdw", "gg_real", "hg_ewald", "gg_ewald", "tail")
+    traces = {}
+    for prec in (engine.GRASPA_FP64, engine.GRASPA_MIXED):
+        boxes = []
+        for s in systems:
+            bx = engine.Box(s, precision=prec, capacity=capacity)
+            bx.init_ewald()
+            boxes.append(bx)
+        bt = engine.Batch(boxes, list(seeds), list(range(len(systems))), trace_cap=trace_cap, cycle_cap=1)
+        bt.run(cycles, production=True)
+        traces[prec] = [bt.trace(i) for i in range(len(systems))]
+        bt.close()
+        for bx in boxes:
+            bx.close()
+    n_cmp, err, same, moves, n_p, err_p = 0, 0.0, 0, 0, 0, 0.0
+    for s, ta, tb in zip(systems, traces[engine.GRASPA_FP64], traces[engine.GRASPA_MIXED]):
+        whole = len(ta) == len(tb)
+        for a, b in zip(ta, tb):
+            if (a.move, a.comp, a.mol, a.accepted) != (b.move, b.comp, b.mol, b.accepted):
+                whole = False
+                break
+            moves += 1
+            if a.accepted:
+                n_cmp += 1
+                err = max(err, max(abs(getattr(a.de, k) - getattr(b.de, k)) for k in keys))
+            if 1e-300 < a.pacc < 1e300 and 1e-300 < b.pacc < 1e300:
+                # every evaluated move, accepted or not: |ln Pacc_fast - ln Pacc_FP64| / beta
+                # is the energy error of the move (for CBMC moves a Boltzmann-weighted
+                # mixture of its trials' errors)
+                n_p += 1
+                err_p = max(err_p, abs(math.log(b.pacc) - math.log(a.pacc)) / s.beta)
+        same += int(whole)
+    return dict(boxes=len(systems), cycles=cycles, moves_in_common=moves, accepted_moves_compared=n_cmp,
+                max_abs_dE_kJmol=err * 0.01, evaluated_moves_compared=n_p, max_abs_dlnPacc_over_beta_kJmol=err_p * 0.01,
+                identical_trajectories=same)
+
+
 def run_batch_single(system, cycles, precision, warm_cycles=0, capacity=1024, seed=11):
     """One box through the device-resident driver (batch of one)."""
     import numpy as np
@@ -349,6 +394,18 @@ def main():
         line["gcmc_config2_butane_mixed"] = run_batch_single(synth.config2(), 200, engine.GRASPA_MIXED,
                                                              warm_cycles=100)
         line["served_stream"] = run_served(48, 20.0, 64, engine.GRASPA_MIXED)
+        # fixed dE error of the fast form in the batched configurations: the
+        # same boxes and draws in both forms (24 boxes of the config-3
+        # distribution, which the served stream also draws from, and the
+        # butane box)
+        line["mixed_vs_fp64_gcmc_config3"] = gcmc_de_error([synth.config3_box(b) for b in range(24)], 60,
+                                                           [3000 + b for b in range(24)])
+        line["mixed_vs_fp64_gcmc_config2"] = gcmc_de_error([synth.config2()], 300, [11], trace_cap=20000)
+        # the 3,000-atom butane framework leaves no room for a molecule (every
+        # trial overlaps), so the error is also measured on a 600-atom variant
+        # of it, where insertions and moves happen
+        line["mixed_vs_fp64_gcmc_config2_600_atom_variant"] = gcmc_de_error(
+            [synth.config2(n_fw=600, P=1e5)], 300, [11], trace_cap=20000)
         # the reference's form of the same Widom workload: one insertion at a
         # time, host-driven, plain FP64 (one insertion per cycle, as the
         # Henry-coefficient decks run it)
