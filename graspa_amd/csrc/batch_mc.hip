@@ -1680,7 +1680,13 @@ int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
   std::memset(&h, 0, sizeof(h));
   h.bc = b->bc;
   h.volume = b->volume;
-  h.fw = b->d_fw; h.fw_type = b->d_fw_type; h.n_fw = b->n_fw;
+  // the framework copy sorted by the Morton code of its screen cells: 64
+  // consecutive atoms are close in space, so in a pair pass the lanes of a
+  // wave are mostly all inside or all beyond the cutoff of a site and the
+  // pair terms are skipped for whole waves of far atoms
+  h.fw = b->d_fws ? b->d_fws : b->d_fw;
+  h.fw_type = b->d_fws ? b->d_fws_type : b->d_fw_type;
+  h.n_fw = b->n_fw;
   h.guest_cap = b->guest_cap;
   h.g = b->d_g; h.g_type = b->d_g_type; h.g_molid = b->d_g_molid; h.g_comp = b->d_g_comp;
   h.ff64 = b->d_ff64;
