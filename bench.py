@@ -71,34 +71,58 @@ def load_traffic(kernel_prefix):
     return d.get("traffic_bytes"), d
 
 
-def run_widom(system, precision, n_insert, steps, warmup, seed=2024, timing=True):
+def run_widom(system, precision, n_insert, steps, warmup, seed=2024, timing=True, autotune=False, queued=False):
+    """queued=False: one synchronous graspa_widom_batch per step (summary back
+    on the host after each batch).  queued=True: the steps are queued with
+    graspa_widom_enqueue and their summaries summed on the device, read once
+    after the last step (graspa_widom_totals)."""
     from graspa_amd import engine
     box = engine.Box(system, precision=precision)
     box.init_ewald()
     intra, selfe = engine.exclusion_energy(system, 1)
     excl = intra + selfe
+    tune = None
+    if autotune:
+        # launch shape of the pair-energy kernel, picked by timing every shape
+        # that fits on a batch of the same size (other draws), before the timed region
+        chosen, tried = box.autotune_widom(1, n_insert, seed=seed + 1, reps=2, excl=excl)
+        default = [ms for shp, ms in tried if shp == {"waves": 12, "per_cu": 2, "framework_lds": 1}]
+        tune = dict(shape=chosen, shapes_tried=len(tried), best_ms=min(ms for _, ms in tried),
+                    default_shape_ms=default[0] if default else None,
+                    slowest_ms=max(ms for _, ms in tried))
     for w in range(warmup):
         box.widom_batch(1, n_insert, seed=seed, first_index=0, excl=excl, results=False)
     if timing:
         box.timing_enable("widom", True)
     box.sync()
     _sync_torch()
+    box.widom_totals(reset=True)
     t0 = time.perf_counter()
     summ = None
     tot = dict(sum_w=0.0, n=0.0, n_first_bead=0.0, n_ok=0.0, sum_we=0.0, screened_pairs=0.0, evaluated_pairs=0.0)
     for k in range(steps):
+        if queued:
+            box.widom_enqueue(1, n_insert, seed=seed, first_index=(k + 1) * n_insert, excl=excl)
+            continue
         _, summ = box.widom_batch(1, n_insert, seed=seed, first_index=(k + 1) * n_insert, excl=excl, results=False)
         for key in tot:
             tot[key] += getattr(summ, key)
+    dev_tot = box.widom_totals(reset=True)
     box.sync()
     _sync_torch()
     t1 = time.perf_counter()
+    if queued:
+        for key in tot:
+            tot[key] = getattr(dev_tot, key)
+    else:
+        # the device-side totals must agree with the per-batch summaries
+        assert dev_tot.n == tot["n"] and dev_tot.evaluated_pairs == tot["evaluated_pairs"]
     kern_ms, launches = box.timing_read() if timing else (0.0, 0)
     if timing:
         box.timing_enable("widom", False)
     nvec, nact = box.nvec()
     box.close()
-    return dict(seconds=t1 - t0, kern_ms=kern_ms, launches=launches, nact=nact, **tot)
+    return dict(seconds=t1 - t0, kern_ms=kern_ms, launches=launches, nact=nact, autotune=tune, **tot)
 
 
 def _sync_torch():
@@ -294,6 +318,9 @@ def main():
     ap.add_argument("--gcmc-cycles", type=int, default=2000)
     ap.add_argument("--cpu-sample", type=int, default=15000)
     ap.add_argument("--quick", action="store_true", help="skip the secondary measurements")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shape of the pair kernel")
+    ap.add_argument("--sync-steps", action="store_true",
+                    help="time one synchronous graspa_widom_batch per step instead of queued batches")
     ap.add_argument("--batch-boxes", type=int, default=256)
     ap.add_argument("--batch-cycles", type=int, default=20)
     ap.add_argument("--batch-warm-cycles", type=int, default=100)
@@ -317,7 +344,8 @@ def main():
     s1 = synth.config1()
     K, O, M = s1.n_trial_pos, s1.n_trial_orient, s1.components[0].molsize
     n_fw = len(s1.fw_type)
-    r = run_widom(s1, engine.GRASPA_MIXED, args.insertions, args.steps, args.warmup)
+    r = run_widom(s1, engine.GRASPA_MIXED, args.insertions, args.steps, args.warmup, autotune=not args.no_autotune,
+                  queued=not args.sync_steps)
     n_total = args.insertions * args.steps
     value = n_total / r["seconds"]
     # dominant kernel: the pair-energy kernel of the pipeline (two launches per
@@ -366,6 +394,7 @@ def main():
                      "batch_tflops": flops / args.steps / batch_s / 1e12,
                      "reference_algorithm_flop_per_batch": ref_flops / args.steps,
                      "reference_algorithm_equivalent_tflops": ref_flops / args.steps / batch_s / 1e12},
+        "autotune": r["autotune"],
     }
     rho = s1.framework_density()
     from graspa_amd.deck import GAS_CONSTANT
@@ -375,10 +404,15 @@ def main():
         line["gcmc_batched"] = run_batch_gcmc(args.batch_boxes, args.batch_cycles, engine.GRASPA_MIXED,
                                               warm_cycles=args.batch_warm_cycles)
     if not args.quick:
-        r64 = run_widom(s1, engine.GRASPA_FP64, args.insertions, max(1, args.steps // 5), 1, timing=False)
+        rs = run_widom(s1, engine.GRASPA_MIXED, args.insertions, args.steps, 1, timing=False, queued=args.sync_steps)
+        line["widom_insertions_per_s_" + ("queued" if args.sync_steps else "synchronous_steps")] = \
+            args.insertions * args.steps / rs["seconds"]
+        r64 = run_widom(s1, engine.GRASPA_FP64, args.insertions, max(1, args.steps // 5), 1, timing=False,
+                        queued=not args.sync_steps)
         line["fp64_widom_insertions_per_s"] = args.insertions * max(1, args.steps // 5) / r64["seconds"]
         s_lo = synth.config1(n_fw=750)
-        rl = run_widom(s_lo, engine.GRASPA_MIXED, args.insertions, max(1, args.steps // 2), 1)
+        rl = run_widom(s_lo, engine.GRASPA_MIXED, args.insertions, max(1, args.steps // 2), 1,
+                       queued=not args.sync_steps)
         fl = rl["evaluated_pairs"] * PAIR_FLOP
         al = rl["kern_ms"] / max(1, rl["launches"])
         line["low_density_variant"] = {

@@ -16,6 +16,8 @@ namespace graspa {
 // one CBMC trial batch or one moved molecule (old + new) fits.
 constexpr int kMaxSites = 64;
 constexpr int kMaxTypes = 32;
+constexpr int kWaccTotals = 16;    // offset of the running Widom totals in graspa_box::d_wacc
+constexpr int kWaccSize = 32;      // doubles of graspa_box::d_wacc
 constexpr int kMaxComps = 8;
 constexpr int kBlock = 256;          // 4 wave64 per workgroup
 
@@ -136,7 +138,8 @@ struct graspa_box {
   unsigned* d_ticket = nullptr;
 
   // Widom scratch
-  double* d_wacc = nullptr;     // [0,6) batch sums, [8,10) work counters, [12] insertion counter
+  double* d_wacc = nullptr;     // [0,6) batch sums, [8,10) work counters, [12] insertion counter;
+                                // from kWaccTotals the same sums and counters over all batches since the last reset
   double* d_wrec_w = nullptr;   // per-insertion W, dE, flags
   double* d_wrec_e = nullptr;
   int* d_wrec_f = nullptr;
@@ -158,6 +161,12 @@ struct graspa_box {
   size_t wres_cap = 0;
   int n_cu = 256;
   int max_lds = 65536;          // dynamic LDS per workgroup
+  // launch shape of the Widom pair-energy kernel (graspa_widom_set_shape,
+  // graspa_widom_autotune): waves per workgroup, workgroups per CU (0: as
+  // many as are resident), framework staged in LDS when it fits
+  int ev_waves = 12;            // two 12-wave workgroups per CU with the framework in LDS
+  int ev_per_cu = 0;
+  int ev_fw_lds = 1;
 
   graspa::TimingState timing;
 };
@@ -234,6 +243,8 @@ void build_coulomb_table(double alpha, double cut2, std::vector<double>& tab);
 int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
                  uint64_t seed, uint32_t stream, int64_t first_index, int n_insert, graspa_widom_result* d_res,
                  graspa_widom_summary* summary);
+int widom_autotune(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
+                   uint64_t seed, int n_insert, int reps, graspa_eval_shape* tried, double* ms, int cap, int* n_tried);
 
 // timing
 void timing_begin(graspa_box* b, const char* family);

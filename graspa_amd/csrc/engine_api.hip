@@ -473,8 +473,8 @@ int graspa_box_create(const graspa_box_params* p, graspa_box** out)
             : hip_check(hipMemcpy(b->d_coul, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice),
                         "coulomb table upload");
   }
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_wacc, 16 * sizeof(double)), "hipMalloc widom");
-  if (!rc) rc = hip_check(hipMemset(b->d_wacc, 0, 16 * sizeof(double)), "widom init");
+  rc = rc ? rc : hip_check(hipMalloc(&b->d_wacc, kWaccSize * sizeof(double)), "hipMalloc widom");
+  if (!rc) rc = hip_check(hipMemset(b->d_wacc, 0, kWaccSize * sizeof(double)), "widom init");
   rc = rc ? rc : ensure_part(b, 4096);
   b->comps.resize(1);
   rc = rc ? rc : realloc_guests(b);
@@ -949,6 +949,64 @@ int graspa_widom_batch(graspa_box* b, int comp, int K, int O, double beta, int b
       return GRASPA_ERR_HIP;
   }
   return hip_check(hipStreamSynchronize(b->stream), "widom sync");
+}
+
+int graspa_widom_enqueue(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
+                         uint64_t seed, uint32_t stream, int64_t first_index, int n_insert)
+{
+  GRASPA_CHECK_BOX(b);
+  if (n_insert < 0) return fail(GRASPA_ERR_ARG, "n_insert < 0");
+  hipSetDevice(b->device);
+  return launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, stream, first_index, n_insert, nullptr, nullptr);
+}
+
+int graspa_widom_totals(graspa_box* b, graspa_widom_summary* totals, int reset)
+{
+  GRASPA_CHECK_BOX(b);
+  hipSetDevice(b->device);
+  double acc[kWaccSize - kWaccTotals];
+  if (hip_check(hipMemcpyAsync(acc, b->d_wacc + kWaccTotals, sizeof(acc), hipMemcpyDeviceToHost, b->stream),
+                "widom totals copy"))
+    return GRASPA_ERR_HIP;
+  if (reset && hip_check(hipMemsetAsync(b->d_wacc + kWaccTotals, 0, sizeof(acc), b->stream), "widom totals reset"))
+    return GRASPA_ERR_HIP;
+  if (hip_check(hipStreamSynchronize(b->stream), "widom sync")) return GRASPA_ERR_HIP;
+  if (totals) {
+    unsigned long long st[2];
+    std::memcpy(st, acc + 8, sizeof(st));
+    *totals = graspa_widom_summary{acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], (double)st[0], (double)st[1]};
+  }
+  return GRASPA_OK;
+}
+
+int graspa_widom_set_shape(graspa_box* b, const graspa_eval_shape* shape)
+{
+  GRASPA_CHECK_BOX(b);
+  const graspa_eval_shape s = shape ? *shape : graspa_eval_shape{12, 0, 1};
+  if ((s.waves != 4 && s.waves != 8 && s.waves != 12 && s.waves != 16) || s.per_cu < 0 || s.per_cu > 64 ||
+      (s.framework_lds != 0 && s.framework_lds != 1))
+    return fail(GRASPA_ERR_ARG, "widom shape: waves 4/8/12/16, 0 <= per_cu <= 64, framework_lds 0/1");
+  b->ev_waves = s.waves;
+  b->ev_per_cu = s.per_cu;
+  b->ev_fw_lds = s.framework_lds;
+  return GRASPA_OK;
+}
+
+int graspa_widom_get_shape(graspa_box* b, graspa_eval_shape* shape)
+{
+  GRASPA_CHECK_BOX(b);
+  if (!shape) return fail(GRASPA_ERR_ARG, "null shape");
+  *shape = graspa_eval_shape{b->ev_waves, b->ev_per_cu, b->ev_fw_lds};
+  return GRASPA_OK;
+}
+
+int graspa_widom_autotune(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
+                          uint64_t seed, int n_insert, int reps, graspa_eval_shape* tried, double* ms, int cap,
+                          int* n_tried)
+{
+  GRASPA_CHECK_BOX(b);
+  hipSetDevice(b->device);
+  return widom_autotune(b, comp, K, O, beta, bias, excl, tail_de, seed, n_insert, reps, tried, ms, cap, n_tried);
 }
 
 int graspa_timing_enable(graspa_box* b, const char* family, int on)

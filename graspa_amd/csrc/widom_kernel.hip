@@ -50,7 +50,6 @@ constexpr int kRedBlock = 256;
 constexpr int kRedPer = 8;       // records per thread in the first reduction pass
 constexpr int kInsWaves = 16;    // waves per workgroup of the chain kernel (one live insertion each)
 constexpr int kFinWaves = 8;     // waves per workgroup of the finish kernel (256 VGPRs per lane available)
-constexpr int kEvalWaves = 12;   // waves per workgroup of the pair-energy kernel (two workgroups per CU)
 constexpr int kFbWaves = 16;     // waves per workgroup of the first-bead kernel
 
 // Work lists are appended per workgroup of the screening kernels: workgroup g
@@ -58,6 +57,7 @@ constexpr int kFbWaves = 16;     // waves per workgroup of the first-bead kernel
 // g*chunk*cap_per_insertion; its length goes to counts[g].
 constexpr int kMaxScreenGroups = 1024;
 constexpr int kChainScreenMinAtoms = 4096;   // frameworks from this size screen orientation sites
+constexpr int kMaxTunePerCu = 8;       // workgroups per CU the autotuner tries at most
 constexpr int kMaxEvalGroups = 4096;   // workgroups of one pair-energy launch (work-count slots)
 
 // State of one insertion between the kernels.
@@ -444,7 +444,7 @@ __global__ __launch_bounds__(kFbWaves * 64) void widom_first_bead_kernel(const W
       }
     }
   }
-  if (lane == 0) atomicAdd(&wg_st, st);
+  if (st) atomicAdd(&wg_st, st);   // every lane's count (each lane screened its own trials)
   __syncthreads();
   if (tid == 0) {
     A.counts1[blockIdx.x] = wg_cnt;
@@ -457,8 +457,8 @@ __global__ __launch_bounds__(kFbWaves * 64) void widom_first_bead_kernel(const W
 // 1183-1352 for one trial site): wave sums of LJ and real-space Coulomb and
 // the exact overlap flag; one wave per site, 64 framework atoms per step.
 // The framework (x, y, z, q and type) is staged in LDS when it fits.
-template <bool MIXED, bool CUBIC, bool FW_LDS>
-__global__ __launch_bounds__(kEvalWaves * 64) void widom_eval_kernel(const WidomArgs A, const TrialPt* list,
+template <bool MIXED, bool CUBIC, bool FW_LDS, int W>
+__global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, const TrialPt* list,
                                                                    const unsigned* counts, int groups, int chunk,
                                                                    int per_ins, double2* energy, uint32_t* mask,
                                                                    unsigned long long* stat_slot)
@@ -474,7 +474,7 @@ __global__ __launch_bounds__(kEvalWaves * 64) void widom_eval_kernel(const Widom
   if (threadIdx.x == 0) pre[0] = 0u;
   __syncthreads();
   for (int h = 1; h < G; h <<= 1) {
-    unsigned add[(kMaxScreenGroups + kEvalWaves * 64 - 1) / (kEvalWaves * 64)];
+    unsigned add[(kMaxScreenGroups + W * 64 - 1) / (W * 64)];
     for (int k = 0, i = threadIdx.x; i < G; ++k, i += blockDim.x) add[k] = (i >= h) ? pre[i + 1 - h] : 0u;
     __syncthreads();
     for (int k = 0, i = threadIdx.x; i < G; ++k, i += blockDim.x) pre[i + 1] += add[k];
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(kEvalWaves * 64) void widom_eval_kernel(const Widom
   const int nint = A.coul_n / kCoulTabStride;
   const int npts = (int)pre[G];
   unsigned long long n_eval = 0;
-  for (int pi = blockIdx.x * kEvalWaves + wv; pi < npts; pi += gridDim.x * kEvalWaves) {
+  for (int pi = blockIdx.x * W + wv; pi < npts; pi += gridDim.x * W) {
     // sub-list g holding point pi: the last g with pre[g] <= pi
     int lo = 0, hi = G - 1;
     while (lo < hi) {
@@ -743,7 +743,7 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
       }
     }
   }
-  if (lane == 0) atomicAdd(&wg_st, st);
+  if (st) atomicAdd(&wg_st, st);   // every lane's count (each lane screened its own trials)
   __syncthreads();
   if (threadIdx.x == 0) {
     A.counts2[blockIdx.x] = wg_cnt;
@@ -879,7 +879,7 @@ __global__ __launch_bounds__(kFinWaves * 64) void widom_finish_kernel(const Wido
 // [kMaxScreenGroups, kMaxScreenGroups + n_screen)), then evaluated pairs of
 // the two pair-energy launches (kMaxEvalGroups slots each).
 __global__ void widom_stat_kernel(const unsigned long long* part, int n_screen, int n_eval1, int n_eval2,
-                                  unsigned long long* stat)
+                                  unsigned long long* stat, unsigned long long* stat_tot)
 {
   __shared__ unsigned long long s[2][64];
   unsigned long long a = 0, e = 0;
@@ -894,6 +894,8 @@ __global__ void widom_stat_kernel(const unsigned long long* part, int n_screen, 
     for (int i = 0; i < 64; ++i) { x += s[0][i]; y += s[1][i]; }
     stat[0] = x;
     stat[1] = y;
+    stat_tot[0] += x;
+    stat_tot[1] += y;
   }
 }
 
@@ -928,7 +930,8 @@ __global__ __launch_bounds__(kRedBlock) void widom_reduce_kernel(const double* r
   if (threadIdx.x < kSums) part[kSums * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(kRedBlock) void widom_final_kernel(const double* part, int nblk, double* out)
+__global__ __launch_bounds__(kRedBlock) void widom_final_kernel(const double* part, int nblk, double* out,
+                                                                double* tot)
 {
   __shared__ double s[kSums][kRedBlock];
   double acc[kSums] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -941,7 +944,11 @@ __global__ __launch_bounds__(kRedBlock) void widom_final_kernel(const double* pa
       for (int q = 0; q < kSums; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + h];
     __syncthreads();
   }
-  if (threadIdx.x < kSums) out[threadIdx.x] = s[threadIdx.x][0];
+  if (threadIdx.x < kSums) {
+    const double v = s[threadIdx.x][0];
+    out[threadIdx.x] = v;
+    tot[threadIdx.x] += v;   // running totals: batches of a box run in order on its stream
+  }
 }
 
 static size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -1011,23 +1018,72 @@ static int ensure_pipeline(graspa_box* b, int n_insert, int K, int OC)
   return rc;
 }
 
-template <bool MIXED, bool CUBIC>
+using EvalKern = void (*)(const WidomArgs, const TrialPt*, const unsigned*, int, int, int, double2*, uint32_t*,
+                          unsigned long long*);
+
+template <bool MIXED, bool CUBIC, bool FW_LDS>
+static EvalKern eval_kernel_w(int waves)
+{
+  switch (waves) {
+    case 4: return widom_eval_kernel<MIXED, CUBIC, FW_LDS, 4>;
+    case 8: return widom_eval_kernel<MIXED, CUBIC, FW_LDS, 8>;
+    case 16: return widom_eval_kernel<MIXED, CUBIC, FW_LDS, 16>;
+    default: return widom_eval_kernel<MIXED, CUBIC, FW_LDS, 12>;
+  }
+}
+
+// The pair-energy kernel of a launch shape for the box's precision form and cell.
+static EvalKern eval_kernel_for(const graspa_box* b, int waves, bool fw_lds)
+{
+  const bool mixed = b->precision == GRASPA_MIXED, cubic = b->bc.cubic != 0;
+  if (mixed && cubic) return fw_lds ? eval_kernel_w<true, true, true>(waves) : eval_kernel_w<true, true, false>(waves);
+  if (mixed) return fw_lds ? eval_kernel_w<true, false, true>(waves) : eval_kernel_w<true, false, false>(waves);
+  if (cubic) return fw_lds ? eval_kernel_w<false, true, true>(waves) : eval_kernel_w<false, true, false>(waves);
+  return fw_lds ? eval_kernel_w<false, false, true>(waves) : eval_kernel_w<false, false, false>(waves);
+}
+
+// LDS of the pair-energy kernel before the framework: FF rows of the species'
+// sites, then the real-space table (fast form).
+static size_t eval_lds_base(const graspa_box* b, int M)
+{
+  size_t off = align16((size_t)M * b->bc.ntypes * sizeof(double4));
+  if (b->precision == GRASPA_MIXED) off = align16(off + (size_t)b->coul_n * sizeof(double));
+  return off;
+}
+
+static size_t eval_lds(const graspa_box* b, size_t base, bool fw_lds)
+{
+  return base + (fw_lds ? align16((size_t)b->n_fw * (sizeof(double4) + sizeof(int))) : 0);
+}
+
+// Grid of one pair-energy launch: per_cu workgroups per CU (0: as many as
+// are resident), capped by the occupancy and the work-count slots; 0 if the
+// shape does not fit.
+static int eval_grid(graspa_box* b, EvalKern kern, int waves, size_t lds, int per_cu)
+{
+  const int occ = occupancy(b, kern, waves * 64, lds);
+  if (occ < 1) return 0;
+  const int wpc = per_cu > 0 ? std::min(per_cu, occ) : occ;
+  return std::min(kMaxEvalGroups, wpc * b->n_cu);
+}
+
+// One pair-energy launch in the box's launch shape.  A shape that stages the
+// framework in LDS falls back to reading it from L2 when it does not fit.
 static int launch_eval(graspa_box* b, const WidomArgs& A, const TrialPt* list, const unsigned* counts, int groups,
                        int chunk, int per_ins, double2* energy, uint32_t* mask, unsigned long long* stat_slot,
                        int* grid_out)
 {
-  const size_t fw_bytes = (size_t)A.n_fw * (sizeof(double4) + sizeof(int));
-  for (int pass = 0; pass < 2; ++pass) {
+  const int waves = b->ev_waves;
+  for (int pass = b->ev_fw_lds ? 0 : 1; pass < 2; ++pass) {
     const bool fw_lds = pass == 0;
-    const size_t lds = (size_t)A.lds_fw + (fw_lds ? align16(fw_bytes) : 0);
-    auto kern = fw_lds ? widom_eval_kernel<MIXED, CUBIC, true> : widom_eval_kernel<MIXED, CUBIC, false>;
-    const int occ = occupancy(b, kern, kEvalWaves * 64, lds);
-    if (occ < 1) continue;
-    const int grid = std::min(kMaxEvalGroups, occ * b->n_cu);
+    const size_t lds = eval_lds(b, (size_t)A.lds_fw, fw_lds);
+    const EvalKern kern = eval_kernel_for(b, waves, fw_lds);
+    const int grid = eval_grid(b, kern, waves, lds, b->ev_per_cu);
+    if (grid < 1) continue;
     *grid_out = grid;
     timing_begin(b, "widom");
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kEvalWaves * 64), lds, b->stream, A, list, counts,
-                       groups, chunk, per_ins, energy, mask, stat_slot);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(waves * 64), lds, b->stream, A, list, counts, groups, chunk,
+                       per_ins, energy, mask, stat_slot);
     timing_end(b, "widom");
     return hip_check(hipGetLastError(), "widom_eval_kernel launch");
   }
@@ -1142,22 +1198,16 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
       A.use_cells ? (size_t)b->cell_n[0] * b->cell_n[1] * b->cell_n[2] * b->cell_cap * sizeof(uint4) : 0;
   // pair-energy kernel: FF rows | Coulomb table (fast form) | framework (optional)
   A.lds_ffl = 0;
-  size_t off = align16((size_t)M * T * sizeof(double4));
-  A.lds_coul = (int)off;
-  if (b->precision == GRASPA_MIXED) off = align16(off + (size_t)b->coul_n * sizeof(double));
-  A.lds_fw = (int)off;
-  const bool mixed = b->precision == GRASPA_MIXED, cubic = A.bc.cubic != 0;
+  A.lds_coul = (int)align16((size_t)M * T * sizeof(double4));
+  A.lds_fw = (int)eval_lds_base(b, M);
+  const bool cubic = A.bc.cubic != 0;
 
   int rc = cubic ? launch_first_bead<true>(b, A, scr_base, tab_bytes) : launch_first_bead<false>(b, A, scr_base, tab_bytes);
   if (rc) return rc;
   int egrid[2] = {0, 0};
   auto eval = [&](const TrialPt* list, const unsigned* counts, int per_ins, double2* energy, uint32_t* mask, int e) {
     unsigned long long* slot = A.stat_part + 2 * kMaxScreenGroups + (size_t)e * kMaxEvalGroups;
-    int* g = &egrid[e];
-    if (mixed) return cubic ? launch_eval<true, true>(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, g)
-                            : launch_eval<true, false>(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, g);
-    return cubic ? launch_eval<false, true>(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, g)
-                 : launch_eval<false, false>(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, g);
+    return launch_eval(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, &egrid[e]);
   };
   rc = eval(A.list1, A.counts1, K, A.e1, A.mask1, 0);
   if (rc) return rc;
@@ -1175,10 +1225,11 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   hipLaunchKernelGGL(widom_reduce_kernel, dim3(nblk), dim3(kRedBlock), 0, b->stream, A.rec_w, A.rec_e, A.rec_f,
                      n_insert, b->d_wpart);
   if (hip_check(hipGetLastError(), "widom_reduce launch")) return GRASPA_ERR_HIP;
-  hipLaunchKernelGGL(widom_final_kernel, dim3(1), dim3(kRedBlock), 0, b->stream, b->d_wpart, nblk, b->d_wacc);
+  hipLaunchKernelGGL(widom_final_kernel, dim3(1), dim3(kRedBlock), 0, b->stream, b->d_wpart, nblk, b->d_wacc,
+                     b->d_wacc + kWaccTotals);
   if (hip_check(hipGetLastError(), "widom_final launch")) return GRASPA_ERR_HIP;
   hipLaunchKernelGGL(widom_stat_kernel, dim3(1), dim3(64), 0, b->stream, A.stat_part, A.groups, egrid[0], egrid[1],
-                     A.stat);
+                     A.stat, reinterpret_cast<unsigned long long*>(b->d_wacc + kWaccTotals + 8));
   if (hip_check(hipGetLastError(), "widom_stat launch")) return GRASPA_ERR_HIP;
   if (summary) {
     double acc[12];
@@ -1197,6 +1248,67 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
     summary->screened_pairs = (double)st[0];
     summary->evaluated_pairs = (double)st[1];
   }
+  return GRASPA_OK;
+}
+
+// Launch-shape autotuner of the pair-energy kernel.  Every shape that fits
+// (4, 8, 12 or 16 waves per workgroup; framework staged in LDS or read from
+// L2; 1 up to the occupancy, at most kMaxTunePerCu, workgroups per CU) runs
+// the given Widom batch `reps` times after one untimed run, timed with HIP
+// events on the box's stream, and the box keeps the fastest.  A shape only
+// decides which wave evaluates which trial site (one wave per site, lanes
+// over the same atoms in the same order), so every shape gives the same
+// results bit for bit.
+int widom_autotune(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
+                   uint64_t seed, int n_insert, int reps, graspa_eval_shape* tried, double* ms, int cap, int* n_tried)
+{
+  if (n_insert < 1 || reps < 1 || cap < 0) return fail(GRASPA_ERR_ARG, "widom autotune: n_insert >= 1, reps >= 1");
+  if (comp < 1 || comp >= (int)b->comps.size()) return fail(GRASPA_ERR_ARG, "widom: bad component");
+  const size_t base = eval_lds_base(b, b->comps[comp].molsize);
+  std::vector<graspa_eval_shape> cand;
+  for (const int w : {4, 8, 12, 16})
+    for (int f = 1; f >= 0; --f) {
+      const int occ = occupancy(b, eval_kernel_for(b, w, f != 0), w * 64, eval_lds(b, base, f != 0));
+      for (int p = 1; p <= std::min(occ, kMaxTunePerCu); ++p) cand.push_back(graspa_eval_shape{w, p, f});
+    }
+  if (cand.empty()) return fail(GRASPA_ERR_UNSUPPORTED, "widom: no launch shape fits the LDS (pair energies)");
+  const graspa_eval_shape keep{b->ev_waves, b->ev_per_cu, b->ev_fw_lds};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = hip_check(hipEventCreate(&e0), "widom autotune event");
+  rc = rc ? rc : hip_check(hipEventCreate(&e1), "widom autotune event");
+  double best = 0.0;
+  size_t ibest = 0;
+  for (size_t i = 0; i < cand.size() && !rc; ++i) {
+    b->ev_waves = cand[i].waves;
+    b->ev_per_cu = cand[i].per_cu;
+    b->ev_fw_lds = cand[i].framework_lds;
+    rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, 0u, 0, n_insert, nullptr, nullptr);
+    rc = rc ? rc : hip_check(hipEventRecord(e0, b->stream), "widom autotune event");
+    for (int r = 0; r < reps && !rc; ++r)
+      rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, 0u, 0, n_insert, nullptr, nullptr);
+    rc = rc ? rc : hip_check(hipEventRecord(e1, b->stream), "widom autotune event");
+    rc = rc ? rc : hip_check(hipEventSynchronize(e1), "widom autotune");
+    float t = 0.0f;
+    rc = rc ? rc : hip_check(hipEventElapsedTime(&t, e0, e1), "widom autotune timing");
+    if (rc) break;
+    const double per = (double)t / reps;
+    if ((int)i < cap) {
+      if (tried) tried[i] = cand[i];
+      if (ms) ms[i] = per;
+    }
+    if (i == 0 || per < best) {
+      best = per;
+      ibest = i;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  const graspa_eval_shape& pick = rc ? keep : cand[ibest];
+  b->ev_waves = pick.waves;
+  b->ev_per_cu = pick.per_cu;
+  b->ev_fw_lds = pick.framework_lds;
+  if (rc) return rc;
+  if (n_tried) *n_tried = (int)cand.size();
   return GRASPA_OK;
 }
 

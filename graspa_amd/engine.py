@@ -62,6 +62,14 @@ class WidomSummary(C.Structure):
                 ("evaluated_pairs", C.c_double)]
 
 
+class EvalShape(C.Structure):
+    """Launch shape of the Widom pair-energy kernel (graspa_eval_shape)."""
+    _fields_ = [("waves", C.c_int), ("per_cu", C.c_int), ("framework_lds", C.c_int)]
+
+    def as_dict(self):
+        return {"waves": self.waves, "per_cu": self.per_cu, "framework_lds": self.framework_lds}
+
+
 class Species(C.Structure):
     _fields_ = [("p_translation", C.c_double), ("p_rotation", C.c_double), ("p_widom", C.c_double),
                 ("p_swap", C.c_double), ("mol_fraction", C.c_double), ("fugacity_coeff", C.c_double),
@@ -147,6 +155,14 @@ _SIGS = {
     "graspa_widom_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
                                      C.c_double, C.c_uint64, C.c_uint32, C.c_int64, C.c_int,
                                      C.POINTER(WidomResult), C.POINTER(WidomSummary)]),
+    "graspa_widom_enqueue": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                                       C.c_double, C.c_uint64, C.c_uint32, C.c_int64, C.c_int]),
+    "graspa_widom_totals": (C.c_int, [C.c_void_p, C.POINTER(WidomSummary), C.c_int]),
+    "graspa_widom_set_shape": (C.c_int, [C.c_void_p, C.POINTER(EvalShape)]),
+    "graspa_widom_get_shape": (C.c_int, [C.c_void_p, C.POINTER(EvalShape)]),
+    "graspa_widom_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                                        C.c_double, C.c_uint64, C.c_int, C.c_int, C.POINTER(EvalShape), _f64p,
+                                        C.c_int, _i32p]),
     "graspa_mc_create": (C.c_int, [C.c_void_p, C.POINTER(MCParams), C.POINTER(C.c_void_p)]),
     "graspa_mc_destroy": (None, [C.c_void_p]),
     "graspa_mc_cycles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Trace), C.c_int, _i64p]),
@@ -424,6 +440,44 @@ class Box:
                                         excl, tail_de, seed, stream, first_index, n_insert, res, C.byref(summ)),
                "widom_batch")
         return res, summ
+
+    def widom_enqueue(self, comp, n_insert, seed, stream=0, first_index=0, excl=0.0, tail_de=0.0):
+        """Queues one Widom batch on the box's stream and returns at once; its
+        summary goes into the device-side running totals (widom_totals)."""
+        s = self.system
+        _check(lib().graspa_widom_enqueue(self.h, comp, s.n_trial_pos, s.n_trial_orient, s.beta,
+                                          int(s.vdw_real_bias), excl, tail_de, seed, stream, first_index, n_insert),
+               "widom_enqueue")
+
+    def widom_totals(self, reset=False):
+        """Waits for the queued batches; summary fields summed over every batch
+        since the last reset."""
+        summ = WidomSummary()
+        _check(lib().graspa_widom_totals(self.h, C.byref(summ), int(reset)), "widom_totals")
+        return summ
+
+    def set_eval_shape(self, waves=12, per_cu=0, framework_lds=1, default=False):
+        """Launch shape of the pair-energy kernel; default=True restores the built-in one."""
+        shp = None if default else C.byref(EvalShape(waves, per_cu, framework_lds))
+        _check(lib().graspa_widom_set_shape(self.h, shp), "widom_set_shape")
+
+    def eval_shape(self):
+        shp = EvalShape()
+        _check(lib().graspa_widom_get_shape(self.h, C.byref(shp)), "widom_get_shape")
+        return shp.as_dict()
+
+    def autotune_widom(self, comp, n_insert, seed, reps=3, excl=0.0, tail_de=0.0, cap=64):
+        """Times the Widom batch in every launch shape that fits and keeps the
+        fastest in the box.  Returns (chosen shape, [(shape, ms per batch)])."""
+        s = self.system
+        tried = (EvalShape * cap)()
+        ms = np.zeros(cap)
+        n = C.c_int(0)
+        _check(lib().graspa_widom_autotune(self.h, comp, s.n_trial_pos, s.n_trial_orient, s.beta,
+                                           int(s.vdw_real_bias), excl, tail_de, seed, n_insert, reps, tried,
+                                           ms.ctypes.data_as(_f64p), cap, C.byref(n)), "widom_autotune")
+        k = min(cap, n.value)
+        return self.eval_shape(), [(tried[i].as_dict(), float(ms[i])) for i in range(k)]
 
     # ---- measurement ----
     def timing_enable(self, family, on=True):

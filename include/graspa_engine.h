@@ -397,7 +397,42 @@ typedef struct {
   double  widom_w[GRASPA_NBLOCK][GRASPA_MAX_COMPS];      /* sum W of Widom moves               */
   double  widom_n[GRASPA_NBLOCK][GRASPA_MAX_COMPS];      /* Widom moves                        */
 } graspa_block_stats;
+/* Asynchronous form of graspa_widom_batch (no counterpart in the reference,
+ * whose Widom moves each end in a host round trip, mc_widom.h:67-119): the
+ * batch is queued on the box's stream and the call returns without waiting.
+ * Every batch of a box, queued or not, adds its summary to running totals
+ * kept on the device; graspa_widom_totals waits for the queued batches,
+ * returns the totals since the last reset and resets them when reset != 0.
+ * The arguments are those of graspa_widom_batch. */
+int  graspa_widom_enqueue(graspa_box* box, int comp, int n_trial_pos, int n_trial_orient, double beta,
+                          int vdw_real_bias, double excl, double tail_de, uint64_t seed, uint32_t stream,
+                          int64_t first_index, int n_insert);
+int  graspa_widom_totals(graspa_box* box, graspa_widom_summary* totals, int reset);
+
 int  graspa_batch_block_stats(graspa_batch* batch, int i, graspa_block_stats* out);
+
+/* Launch shape of the Widom pipeline's pair-energy kernel.  No counterpart in
+ * the reference, whose trial-energy kernel has one fixed shape (128 threads
+ * per block, one thread per (atom, site) pair, VDW_Coulomb.cu:1183-1352 as
+ * launched at mc_widom.h:109). */
+typedef struct {
+  int waves;            /* waves per workgroup: 4, 8, 12 (default) or 16          */
+  int per_cu;           /* workgroups per CU; 0 (default): as many as are resident */
+  int framework_lds;    /* 1 (default): framework staged in LDS when it fits; 0: read from L2 */
+} graspa_eval_shape;
+
+/* Sets the box's launch shape (NULL: the default); GRASPA_ERR_ARG for a
+ * shape outside the list above.  Results do not depend on the shape. */
+int  graspa_widom_set_shape(graspa_box* box, const graspa_eval_shape* shape);
+int  graspa_widom_get_shape(graspa_box* box, graspa_eval_shape* shape);
+/* Autotuner: runs the given Widom batch (arguments as graspa_widom_batch,
+ * stream 0, first index 0) in every launch shape that fits the device, reps
+ * timed runs each after one untimed run, and keeps the fastest shape in the
+ * box.  The first min(cap, *n_tried) shapes tried and their milliseconds per
+ * batch are written to tried/ms (either may be NULL). */
+int  graspa_widom_autotune(graspa_box* box, int comp, int n_trial_pos, int n_trial_orient, double beta,
+                           int vdw_real_bias, double excl, double tail_de, uint64_t seed, int n_insert, int reps,
+                           graspa_eval_shape* tried, double* ms, int cap, int* n_tried);
 
 /* ------------------------------------------------------------------------ */
 /* Measurement                                                               */
@@ -418,8 +453,8 @@ int  graspa_glibc_rand_sequence(unsigned seed, int n, int* out);
 
 /* Host helper with no counterpart in the reference: the table of
  * erfc(alpha r)/r over r^2 that the fast (GRASPA_MIXED) Widom kernel uses for
- * the real-space Coulomb term (64 intervals per octave of r^2 from 2^-7, a
- * degree-5 polynomial in r^2 - r_i^2 each, 6 doubles per interval), exposed
+ * the real-space Coulomb term (16 intervals per octave of r^2 from 2^-7, a
+ * degree-7 polynomial in r^2 - r_i^2 each, 8 doubles per interval), exposed
  * so that it can be checked without a GPU.  Returns the number of doubles
  * written, the number needed when cap is too small, or -1 for bad arguments. */
 int  graspa_coulomb_table(double alpha, double cut2, double* out, int cap);
