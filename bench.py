@@ -77,18 +77,38 @@ def run_widom(system, precision, n_insert, steps, warmup, seed=2024, timing=True
     graspa_widom_enqueue and their summaries summed on the device, read once
     after the last step (graspa_widom_totals)."""
     from graspa_amd import engine
+    setup = {}
+    t0 = time.perf_counter()
     box = engine.Box(system, precision=precision)
+    box.sync()
+    setup["box_create_ms"] = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
     box.init_ewald()
+    box.sync()
+    setup["structure_factors_ms"] = (time.perf_counter() - t0) * 1e3
     intra, selfe = engine.exclusion_energy(system, 1)
     excl = intra + selfe
+    # the first batch builds the certain-overlap voxel maps of the inserted
+    # site types on the host (kept for later batches)
+    t0 = time.perf_counter()
+    box.widom_batch(1, 64, seed=seed + 2, first_index=0, excl=excl, results=False)
+    setup["first_batch_with_voxel_maps_ms"] = (time.perf_counter() - t0) * 1e3
     tune = None
     if autotune:
         # launch shape of the pair-energy kernel, picked by timing every shape
         # that fits on a batch of the same size (other draws), before the timed region
+        default_shape = box.eval_shape()
+        t0 = time.perf_counter()
         chosen, tried = box.autotune_widom(1, n_insert, seed=seed + 1, reps=2, excl=excl)
-        default = [ms for shp, ms in tried if shp == {"waves": 12, "per_cu": 2, "framework_lds": 1}]
+        setup["autotune_ms"] = (time.perf_counter() - t0) * 1e3
+        # the built-in shape: per_cu 0 means as many workgroups per CU as are
+        # resident, i.e. the largest per_cu tried for its waves and staging
+        same = [(shp["per_cu"], ms) for shp, ms in tried
+                if shp["waves"] == default_shape["waves"] and shp["framework_lds"] == default_shape["framework_lds"]]
+        default_ms = max(same)[1] if same and default_shape["per_cu"] == 0 else \
+            next((ms for pc, ms in same if pc == default_shape["per_cu"]), None)
         tune = dict(shape=chosen, shapes_tried=len(tried), best_ms=min(ms for _, ms in tried),
-                    default_shape_ms=default[0] if default else None,
+                    default_shape=default_shape, default_shape_ms=default_ms,
                     slowest_ms=max(ms for _, ms in tried))
     for w in range(warmup):
         box.widom_batch(1, n_insert, seed=seed, first_index=0, excl=excl, results=False)
@@ -122,7 +142,7 @@ def run_widom(system, precision, n_insert, steps, warmup, seed=2024, timing=True
         box.timing_enable("widom", False)
     nvec, nact = box.nvec()
     box.close()
-    return dict(seconds=t1 - t0, kern_ms=kern_ms, launches=launches, nact=nact, autotune=tune, **tot)
+    return dict(seconds=t1 - t0, kern_ms=kern_ms, launches=launches, nact=nact, autotune=tune, setup=setup, **tot)
 
 
 def _sync_torch():
@@ -250,8 +270,10 @@ def gcmc_de_error(systems, cycles, seeds, capacity=2048, trace_cap=4000):
                 n_p += 1
                 err_p = max(err_p, abs(math.log(b.pacc) - math.log(a.pacc)) / s.beta)
         same += int(whole)
+    # no compared moves: no error figure (not 0.0)
     return dict(boxes=len(systems), cycles=cycles, moves_in_common=moves, accepted_moves_compared=n_cmp,
-                max_abs_dE_kJmol=err * 0.01, evaluated_moves_compared=n_p, max_abs_dlnPacc_over_beta_kJmol=err_p * 0.01,
+                max_abs_dE_kJmol=err * 0.01 if n_cmp else None, evaluated_moves_compared=n_p,
+                max_abs_dlnPacc_over_beta_kJmol=err_p * 0.01 if n_p else None,
                 identical_trajectories=same)
 
 
@@ -297,16 +319,52 @@ def run_served(n_jobs, rate, slots, precision, cycles_lo=50, cycles_hi=5000, see
                 admission_wait_p99_ms=float(np.percentile(wait, 99) * 1e3))
 
 
+def host_cpu():
+    """CPU model and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return dict(model=model, nproc=os.cpu_count(), usable_cores=usable,
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
+
+
 def cpu_baseline(system, n_insert):
+    """The FP64 oracle (C, -O2) on a bounded sample of the same workload: one
+    thread (like-for-like with the reference, whose CPU energy code is
+    single-threaded), and OpenMP over insertions on the cores this process
+    may use (capped by OMP_NUM_THREADS when set)."""
     from oracle import binding as ob
     o = ob.OracleSystem(system)
     _, skf, ska, _, _, _ = o.ewald_total()
+    host = host_cpu()
+    cap = int(host["omp_num_threads"]) if (host["omp_num_threads"] or "").isdigit() else host["usable_cores"]
+    n_all = max(1, min(host["usable_cores"] or 1, cap))
+    ob.lib().or_set_threads(1)
     t0 = time.perf_counter()
     o.widom_batch(1, ska, skf, seed=2024, n_insert=n_insert)
     t1 = time.perf_counter()
-    return dict(value=n_insert / (t1 - t0), unit="insertions/s", cores=1, kind="port",
-                sample="%d Widom insertions of the same workload (FP64 oracle in C, one thread, %.1f s)"
-                       % (n_insert, t1 - t0))
+    one = n_insert / (t1 - t0)
+    used = ob.lib().or_set_threads(n_all)
+    n_big = n_insert * used
+    t2 = time.perf_counter()
+    o.widom_batch(1, ska, skf, seed=2024, n_insert=n_big)
+    t3 = time.perf_counter()
+    ob.lib().or_set_threads(1)
+    return dict(value=n_big / (t3 - t2), unit="insertions/s", cores=used, kind="port",
+                sample="%d Widom insertions of the same workload, FP64 oracle in C (gcc -O2), OpenMP over "
+                       "insertions on %d threads, %.1f s" % (n_big, used, t3 - t2),
+                one_thread=dict(value=one, sample="%d insertions, one thread, %.1f s" % (n_insert, t1 - t0)),
+                host=host)
 
 
 def main():
@@ -377,7 +435,7 @@ def main():
                    "first_bead_placed_fraction": r["n_first_bead"] / max(1, r["n"]),
                    "complete_fraction": r["n_ok"] / max(1, r["n"]),
                    "mean_rosenbluth_weight": r["sum_w"] / max(1, r["n"])},
-        "roofline": {"bound": "mfma",
+        "roofline": {"bound": "valu",
                      "roof": "fp64 vector (VALU) peak: the pair terms compute in FP64 (the kernel uses no MFMA; "
                              "FP64 FMA issues at the unpacked FP32 rate on gfx950)",
                      "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -395,6 +453,9 @@ def main():
                      "reference_algorithm_flop_per_batch": ref_flops / args.steps,
                      "reference_algorithm_equivalent_tflops": ref_flops / args.steps / batch_s / 1e12},
         "autotune": r["autotune"],
+        "setup": dict(r["setup"], note="per framework, outside the timed region: box upload with the screen's "
+                                       "cell table, S(k) build, the first batch (host-built voxel maps), the "
+                                       "autotuner"),
     }
     rho = s1.framework_density()
     from graspa_amd.deck import GAS_CONSTANT

@@ -102,12 +102,14 @@ __device__ inline double lj_fp64(double rr, double a4e, double s2, double shift)
 // difference of two positions wraps to the minimum image exactly in integer
 // arithmetic (fraction in [-1/2, 1/2)).
 //
-// Mixed form of one pair.  On gfx950 FP64 FMA issues at the same lane rate as
-// unpacked FP32, so the arithmetic stays FP64 and only the transcendental steps
-// use the FP32 hardware: r^2 from the exact fixed-point difference in FP64;
-// 1/r^2 and 1/r seeded by v_rcp_f32 / v_rsq_f32 and refined by one FP64
-// Newton step (relative error ~1e-14); erfc evaluated in FP32.  The FP32 LJ
-// 12th power would amplify a 1e-7 error in r^2 twelve-fold, which this avoids.
+// Fast ("mixed") form of one pair.  On gfx950 FP64 FMA issues at the same
+// lane rate as unpacked FP32, so the arithmetic stays FP64 and only the seeds
+// of the reciprocals come from the FP32 hardware: r^2 in FP64; 1/r^2 and 1/r
+// seeded by v_rcp_f32 / v_rsq_f32 and refined by one FP64 Newton step
+// (relative error ~1e-14); erfc(alpha r)/r from a per-box FP64 polynomial
+// table (coul_table) or the FP64 exp * polynomial erfc_fast below.  An FP32
+// LJ 12th power would amplify a 1e-7 error in r^2 twelve-fold, which this
+// avoids.
 __device__ inline double r2_fixed64(uint32_t ax, uint32_t ay, uint32_t az, uint32_t bx, uint32_t by, uint32_t bz,
                                     const double* h, int cubic)
 {

@@ -1273,8 +1273,16 @@ int widom_autotune(graspa_box* b, int comp, int K, int O, double beta, int bias,
     }
   if (cand.empty()) return fail(GRASPA_ERR_UNSUPPORTED, "widom: no launch shape fits the LDS (pair energies)");
   const graspa_eval_shape keep{b->ev_waves, b->ev_per_cu, b->ev_fw_lds};
+  // the timed batches add into the running totals and overwrite the last
+  // batch's summary (d_wacc): both are saved here and restored at the end, so
+  // a tuning run between queued production batches leaves them as they were
+  double saved_acc[kWaccSize];
+  int rc = hip_check(hipMemcpyAsync(saved_acc, b->d_wacc, sizeof(saved_acc), hipMemcpyDeviceToHost, b->stream),
+                     "widom autotune save totals");
+  rc = rc ? rc : hip_check(hipStreamSynchronize(b->stream), "widom autotune save totals");
+  if (rc) return rc;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  int rc = hip_check(hipEventCreate(&e0), "widom autotune event");
+  rc = hip_check(hipEventCreate(&e0), "widom autotune event");
   rc = rc ? rc : hip_check(hipEventCreate(&e1), "widom autotune event");
   double best = 0.0;
   size_t ibest = 0;
@@ -1303,6 +1311,10 @@ int widom_autotune(graspa_box* b, int comp, int K, int O, double beta, int bias,
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
+  int rc_restore = hip_check(hipMemcpyAsync(b->d_wacc, saved_acc, sizeof(saved_acc), hipMemcpyHostToDevice, b->stream),
+                             "widom autotune restore totals");
+  rc_restore = rc_restore ? rc_restore : hip_check(hipStreamSynchronize(b->stream), "widom autotune restore totals");
+  rc = rc ? rc : rc_restore;
   const graspa_eval_shape& pick = rc ? keep : cand[ibest];
   b->ev_waves = pick.waves;
   b->ev_per_cu = pick.per_cu;

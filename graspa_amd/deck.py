@@ -323,6 +323,11 @@ def read_mixing_rules(path):
     names, eps, sig = [], [], []
     for i in range(n):
         t = _split(lines[7 + i])
+        if len(t) > 1 and t[1].lower() != "lennard-jones":
+            # lennard-jones-1264 rows carry {C12, C6, C4, C10} (read_data.cpp:820-834),
+            # not epsilon / sigma: reading them as such would give wrong energies
+            raise NotImplementedError("%s interaction %s in %s: this engine evaluates 12-6 Lennard-Jones only"
+                                      % (t[0], t[1], os.path.basename(path)))
         names.append(t[0])
         eps.append(float(t[2]))
         sig.append(float(t[3]))
@@ -511,6 +516,13 @@ _UNSUPPORTED_COMPONENT = ("CBCFProbability", "RotationSpecialProbability", "Gibb
 
 
 def _reject_unsupported(get, comp_blocks):
+    # UseLJ1264 yes|true switches every pair to the 12-6-4 polynomial with the
+    # GENERIC2_HC overrides of force_field.def (maths.cuh:454-476,
+    # read_data.cpp:506-514): refused, since the engine's pair terms are 12-6
+    t = get("UseLJ1264")
+    if t and len(t) > 1 and t[1].lower() in ("yes", "true"):
+        raise NotImplementedError("UseLJ1264 %s: the 12-6-4 polynomial pair potential is not supported by this "
+                                  "engine (12-6 Lennard-Jones only)" % t[1])
     for key in _UNSUPPORTED_NUMERIC:
         t = get(key)
         if t and len(t) > 1 and float(t[1]) > 0.0:

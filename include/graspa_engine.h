@@ -39,7 +39,8 @@ extern "C" {
 #define GRASPA_ERR_UNSUPPORTED  -5
 
 #define GRASPA_FP64   0   /* plain FP64 form: pair terms, Ewald and sums in FP64           */
-#define GRASPA_MIXED  1   /* FP32 pair terms with FP64 accumulation; FP64 Ewald and S(k)   */
+#define GRASPA_MIXED  1   /* fast form: FP64 pair arithmetic with an FP32-seeded reciprocal and a tabulated
+                             erfc(ar)/r, FP32 overlap screen; FP64 accumulation, Ewald and S(k) */
 
 typedef struct graspa_box graspa_box;
 typedef struct graspa_mc graspa_mc;
@@ -429,7 +430,10 @@ int  graspa_widom_get_shape(graspa_box* box, graspa_eval_shape* shape);
  * stream 0, first index 0) in every launch shape that fits the device, reps
  * timed runs each after one untimed run, and keeps the fastest shape in the
  * box.  The first min(cap, *n_tried) shapes tried and their milliseconds per
- * batch are written to tried/ms (either may be NULL). */
+ * batch are written to tried/ms (either may be NULL).  The running totals of
+ * queued batches (graspa_widom_totals) and the last batch's summary are
+ * saved before the timed runs and restored after them; the per-insertion
+ * records of the last batch are overwritten. */
 int  graspa_widom_autotune(graspa_box* box, int comp, int n_trial_pos, int n_trial_orient, double beta,
                            int vdw_real_bias, double excl, double tail_de, uint64_t seed, int n_insert, int reps,
                            graspa_eval_shape* tried, double* ms, int cap, int* n_tried);

@@ -109,6 +109,9 @@ def lib():
                                        C.c_int, C.c_int64, C.c_int, C.c_int64, _f64p, _u8p]
         L.or_ewald_delta.argtypes = [C.POINTER(OrBox), _f64p, _f64p, C.c_int, _f64p, _f64p, C.c_int, _f64p, _f64p,
                                      _f64p, _f64p, _f64p]
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_threads.restype = C.c_int
+        L.or_rotate_quaternion.argtypes = [_f64p, C.c_double, C.c_double, C.c_double]
         L.or_rand_uniform.restype = C.c_double
         L.or_srand.argtypes = [C.c_uint]
         L.or_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]

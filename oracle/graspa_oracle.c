@@ -9,6 +9,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -517,6 +520,9 @@ static void rotate_quaternion(double v[3], double u, double vv, double w)
   v[0] = r; v[1] = s; v[2] = c;
 }
 
+/* Exported for the uniformity test of the orientation sampler. */
+void or_rotate_quaternion(double v[3], double u, double vv, double w) { rotate_quaternion(v, u, vv, w); }
+
 /* RotationAroundAxis (mc_utilities.h:459-483) */
 static void rotate_axis(double p[3], double theta, double ax, double ay, double az)
 {
@@ -605,12 +611,18 @@ int or_widom_batch(const or_box* b, const or_atoms* comps, int ncomp, int comp,
   int chain = M - 1;
   int64_t new_molid = A->n_mol;
   if (K > 1024 || O * (chain > 0 ? chain : 1) > 1024) return -1;
+  /* insertions are independent (each draws from its own counter-based
+   * stream and writes its own record): OpenMP over insertions, per-thread
+   * work buffers; results do not depend on the thread count */
+#pragma omp parallel
+  {
   double* sp = (double*)malloc(sizeof(double) * 3 * 1024);
   int* st = (int*)malloc(sizeof(int) * 1024);
   double* sq = (double*)malloc(sizeof(double) * 1024);
   double* s4 = (double*)malloc(sizeof(double) * 4 * 1024);
   uint8_t* sov = (uint8_t*)malloc(1024);
   trial_set* ts = (trial_set*)malloc(sizeof(trial_set));
+#pragma omp for schedule(dynamic, 8)
   for (int n = 0; n < n_insert; n++) {
     uint64_t idx = (uint64_t)(first_index + n);
     or_widom_out* o = &out[n];
@@ -693,7 +705,20 @@ int or_widom_batch(const or_box* b, const or_atoms* comps, int ncomp, int comp,
     o->ok = 1;
   }
   free(sp); free(st); free(sq); free(s4); free(sov); free(ts);
+  }
   return 0;
+}
+
+/* Threads of the OpenMP loops (or_widom_batch); n < 1 keeps the default. */
+int or_set_threads(int n)
+{
+#ifdef _OPENMP
+  if (n >= 1) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
 }
 
 /* ------------------------------------------------------------------------ */

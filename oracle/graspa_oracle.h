@@ -114,6 +114,11 @@ void or_ewald_delta(const or_box* b, const double* sk_same, const double* sk_cro
                     int n_new, const double* new_pos, const double* new_q,
                     double* same, double* cross, double* sk_new);
 
+/* ---- proposals ---- */
+/* Rotate_Quaternions (mc_utilities.h:423-457): v rotated by the random
+ * rotation of the three uniforms (u, vv, w). */
+void   or_rotate_quaternion(double v[3], double u, double vv, double w);
+
 /* ---- random numbers ---- */
 double or_rand_uniform(void);               /* rand()/RAND_MAX, data_struct.cpp:10-15 */
 void   or_srand(unsigned seed);
@@ -213,6 +218,10 @@ int or_widom_batch(const or_box* b, const or_atoms* comps, int ncomp, int comp,
                    double excl_intra, double excl_self, double tail_de,
                    uint64_t seed, uint64_t stream, int64_t first_index, int n_insert,
                    or_widom_out* out);
+
+/* Threads of the OpenMP loop over insertions in or_widom_batch; n < 1 keeps
+ * the current setting.  Returns the number of threads the loop will use. */
+int or_set_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -1,5 +1,7 @@
 """Deck reader: synthetic decks written in gRASPA's formats read back to the
 same system; the reference's unit conversions."""
+import os
+
 import numpy as np
 import pytest
 
@@ -78,3 +80,40 @@ def test_switched_off_methods_are_accepted(tmp_path):
     _with_lines(tmp_path, ["GibbsVolumeChangeProbability 0.0", "RunTMMC no", "NumberOfSimulations 1"],
                 ["CBCFProbability 0.0"])
     deck.read_deck(str(tmp_path))
+
+
+@pytest.mark.parametrize("value", ["yes", "true", "YES"])
+def test_lj1264_is_refused(tmp_path, value):
+    """UseLJ1264 yes|true (read_data.cpp:506-514) switches the pair potential to
+    the 12-6-4 polynomial (maths.cuh:454-476), which the engine does not run."""
+    _with_lines(tmp_path, ["UseLJ1264 " + value])
+    with pytest.raises(NotImplementedError, match="UseLJ1264"):
+        deck.read_deck(str(tmp_path))
+
+
+def test_lj1264_off_is_accepted(tmp_path):
+    _with_lines(tmp_path, ["UseLJ1264 no"])
+    deck.read_deck(str(tmp_path))
+
+
+def test_non_lj_mixing_rule_rows_are_refused(tmp_path):
+    """A lennard-jones-1264 row holds C12, C6, C4, C10 (read_data.cpp:820-834);
+    reading its columns as epsilon and sigma would give wrong energies."""
+    _with_lines(tmp_path)
+    p = tmp_path / "force_field_mixing_rules.def"
+    lines = p.read_text().splitlines()
+    k = next(i for i, ln in enumerate(lines) if "lennard-jones" in ln)
+    t = lines[k].split()
+    lines[k] = "%s lennard-jones-1264 1.0e6 1.0e3 10.0 0.0" % t[0]
+    p.write_text("\n".join(lines) + "\n")
+    with pytest.raises(NotImplementedError, match="lennard-jones-1264"):
+        deck.read_deck(str(tmp_path))
+
+
+REF_LJ1264 = "/root/reference/Examples/TIP4PEW-MgMOF-LJ1264"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_LJ1264), reason="the reference's example decks are not mounted here")
+def test_reference_lj1264_deck_is_refused():
+    with pytest.raises(NotImplementedError):
+        deck.read_deck(REF_LJ1264)

@@ -121,3 +121,23 @@ def test_queued_batches_totals_match_synchronous(engine_mod):
         assert getattr(sync_tot, key) == host[key], key
         assert getattr(queued, key) == host[key], key
         assert getattr(empty, key) == 0.0, key
+
+
+def test_autotune_leaves_the_running_totals_unchanged(engine_mod):
+    """A tuning run between queued production batches leaves the device-side
+    running totals as they were: the timed batches are not counted."""
+    s, box, excl = _box(engine_mod, engine_mod.GRASPA_MIXED)
+    box.widom_totals(reset=True)
+    for k in range(2):
+        box.widom_enqueue(1, 300, seed=9, first_index=300 * k, excl=excl)
+    box.sync()
+    box.autotune_widom(1, 500, seed=5, reps=1, excl=excl)
+    box.widom_enqueue(1, 300, seed=9, first_index=600, excl=excl)
+    tuned = box.widom_totals(reset=True)
+    for k in range(3):
+        box.widom_enqueue(1, 300, seed=9, first_index=300 * k, excl=excl)
+    plain = box.widom_totals(reset=True)
+    box.close()
+    assert plain.n == 900.0
+    for key in ("sum_w", "sum_w2", "sum_we", "n", "n_first_bead", "n_ok", "screened_pairs", "evaluated_pairs"):
+        assert getattr(tuned, key) == getattr(plain, key), key

@@ -171,3 +171,21 @@ def test_widom_single_trial_equals_boltzmann_factor(oracle_lib):
     assert ok.sum() > 10
     w = np.exp(-s.beta * rec["energy"][ok])
     assert np.allclose(rec["rosen"][ok], w, rtol=1e-12)
+
+
+def test_oracle_widom_batch_is_independent_of_the_thread_count(oracle_lib):
+    """The oracle's batched Widom loop runs OpenMP over insertions (each with
+    its own counter-based stream): 1 and 4 threads give identical records."""
+    from graspa_amd import synth
+    from oracle import binding as ob
+    s = synth.config1(n_fw=300)
+    o = ob.OracleSystem(s)
+    _, skf, ska, _, _, _ = o.ewald_total()
+    out = []
+    for th in (1, 4):
+        ob.lib().or_set_threads(th)
+        out.append(ob.widom_records_to_numpy(o.widom_batch(1, ska, skf, seed=3, n_insert=200)))
+    ob.lib().or_set_threads(1)
+    assert out[0]["ok"].sum() > 50
+    for k in out[0].dtype.names:
+        assert np.array_equal(out[0][k], out[1][k]), k
