@@ -199,7 +199,6 @@ void or_ewald_total(const or_box* b, const or_atoms* comps, int ncomp,
   cplx* ex = (cplx*)malloc(sizeof(cplx) * (size_t)(natoms ? natoms : 1) * (kxm + 1));
   cplx* ey = (cplx*)malloc(sizeof(cplx) * (size_t)(natoms ? natoms : 1) * (kym + 1));
   cplx* ez = (cplx*)malloc(sizeof(cplx) * (size_t)(natoms ? natoms : 1) * (kzm + 1));
-  cplx* exy = (cplx*)malloc(sizeof(cplx) * (size_t)(natoms ? natoms : 1));
   double* qs = (double*)malloc(sizeof(double) * (size_t)(natoms ? natoms : 1));
   int* isfw = (int*)malloc(sizeof(int) * (size_t)(natoms ? natoms : 1));
   cplx tmp[64];
@@ -220,40 +219,57 @@ void or_ewald_total(const or_box* b, const or_atoms* comps, int ncomp,
   double alpha_sq = b->alpha * b->alpha;
   double prefactor = b->prefactor * (2.0 * M_PI / b->volume);
   double e_gg = 0, e_hh = 0, e_hg = 0;
-  int nvec = 0, active = 0;
-  for (int kx = 0; kx <= kxm; kx++) {
+  int active = 0;
+  /* k-vector index (kx, ky, kz) -> (kx (2 kym + 1) + ky + kym) (2 kzm + 1) + kz + kzm, the
+   * reference's loop order; rows of fixed (kx, ky) are independent (OpenMP over
+   * rows, partial energies per row summed in row order afterwards) */
+  const int nky = 2 * kym + 1, nkz = 2 * kzm + 1, nrows = (kxm + 1) * nky;
+  double* part = (double*)calloc((size_t)4 * nrows, sizeof(double));
+#pragma omp parallel
+  {
+  cplx* exy = (cplx*)malloc(sizeof(cplx) * (size_t)(natoms ? natoms : 1));
+#pragma omp for schedule(dynamic, 1)
+  for (int row = 0; row < nrows; row++) {
+    const int kx = row / nky, ky = row % nky - kym;
     double factor = (kx == 0) ? (1.0 * prefactor) : (2.0 * prefactor);
-    for (int ky = -kym; ky <= kym; ky++) {
-      for (int i = 0; i < natoms; i++) {
-        cplx t = ey[i + natoms * abs(ky)];
-        if (ky < 0) t.im = -t.im;
-        exy[i] = cmul(ex[i + natoms * kx], t);
-      }
-      for (int kz = -kzm; kz <= kzm; kz++) {
-        double ksqr = ksqr_cut(b, kx, ky, kz);
-        cplx ads = {0, 0}, fw = {0, 0};
-        if (ksqr > 1e-10 && ksqr < b->recip_cut) {
-          for (int i = 0; i < natoms; i++) {
-            cplx t = ez[i + natoms * abs(kz)];
-            if (kz < 0) t.im = -t.im;
-            cplx p = cmul(exy[i], t);
-            ads.re += qs[i] * p.re; ads.im += qs[i] * p.im;
-            if (isfw[i]) { fw.re += qs[i] * p.re; fw.im += qs[i] * p.im; }
-          }
-          double rksq = rksq_of(b, kx, ky, kz);
-          double temp = factor * exp((-0.25 / alpha_sq) * rksq) / rksq;
-          ads.re -= fw.re; ads.im -= fw.im;
-          e_gg += temp * (ads.re * ads.re + ads.im * ads.im);
-          e_hh += temp * (fw.re * fw.re + fw.im * fw.im);
-          e_hg += temp * (fw.re * ads.re + fw.im * ads.im) * 2.0;
-          active++;
+    for (int i = 0; i < natoms; i++) {
+      cplx t = ey[i + natoms * abs(ky)];
+      if (ky < 0) t.im = -t.im;
+      exy[i] = cmul(ex[i + natoms * kx], t);
+    }
+    for (int kz = -kzm; kz <= kzm; kz++) {
+      const int nvec = row * nkz + kz + kzm;
+      double ksqr = ksqr_cut(b, kx, ky, kz);
+      cplx ads = {0, 0}, fw = {0, 0};
+      if (ksqr > 1e-10 && ksqr < b->recip_cut) {
+        for (int i = 0; i < natoms; i++) {
+          cplx t = ez[i + natoms * abs(kz)];
+          if (kz < 0) t.im = -t.im;
+          cplx p = cmul(exy[i], t);
+          ads.re += qs[i] * p.re; ads.im += qs[i] * p.im;
+          if (isfw[i]) { fw.re += qs[i] * p.re; fw.im += qs[i] * p.im; }
         }
-        if (sk_fw) { sk_fw[2 * nvec] = fw.re; sk_fw[2 * nvec + 1] = fw.im; }
-        if (sk_ads) { sk_ads[2 * nvec] = ads.re; sk_ads[2 * nvec + 1] = ads.im; }
-        nvec++;
+        double rksq = rksq_of(b, kx, ky, kz);
+        double temp = factor * exp((-0.25 / alpha_sq) * rksq) / rksq;
+        ads.re -= fw.re; ads.im -= fw.im;
+        part[4 * row] += temp * (ads.re * ads.re + ads.im * ads.im);
+        part[4 * row + 1] += temp * (fw.re * fw.re + fw.im * fw.im);
+        part[4 * row + 2] += temp * (fw.re * ads.re + fw.im * ads.im) * 2.0;
+        part[4 * row + 3] += 1.0;
       }
+      if (sk_fw) { sk_fw[2 * nvec] = fw.re; sk_fw[2 * nvec + 1] = fw.im; }
+      if (sk_ads) { sk_ads[2 * nvec] = ads.re; sk_ads[2 * nvec + 1] = ads.im; }
     }
   }
+  free(exy);
+  }
+  for (int row = 0; row < nrows; row++) {
+    e_gg += part[4 * row];
+    e_hh += part[4 * row + 1];
+    e_hg += part[4 * row + 2];
+    active += (int)part[4 * row + 3];
+  }
+  free(part);
   memset(fourier, 0, sizeof(*fourier));
   fourier->gg_ewald = e_gg; fourier->hh_ewald = e_hh; fourier->hg_ewald = e_hg;
   if (n_active) *n_active = active;
@@ -265,22 +281,31 @@ void or_ewald_total(const or_box* b, const or_atoms* comps, int ncomp,
     int na = comps[c].n_mol * comps[c].molsize;
     for (int i = 0; i < na; i++) se += pself * comps[c].charge[i] * comps[c].charge[i];
     int M = comps[c].molsize;
+    /* pairs (i, j > i) of each molecule; the framework is one molecule of
+     * all its atoms, so this is O(N^2) there: OpenMP over i, a partial sum
+     * per i, summed in i order (the same result for any thread count) */
+    double* pi = (double*)calloc((size_t)(M > 0 ? M : 1), sizeof(double));
     for (int m = 0; m < comps[c].n_mol && M > 0; m++) {
-      int a0 = m * M;
+      const int a0 = m * M;
+#pragma omp parallel for schedule(dynamic, 64)
       for (int i = a0; i < a0 + M - 1; i++) {
+        double acc = 0.0;
         for (int j = i + 1; j < a0 + M; j++) {
           double d[3] = {comps[c].pos[3 * i] - comps[c].pos[3 * j], comps[c].pos[3 * i + 1] - comps[c].pos[3 * j + 1],
                          comps[c].pos[3 * i + 2] - comps[c].pos[3 * j + 2]};
           or_pbc(d, b->cell, b->inv_cell, b->cubic);
           double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-          ie += b->prefactor * comps[c].charge[i] * comps[c].charge[j] * erf(b->alpha * r) / r;
+          acc += b->prefactor * comps[c].charge[i] * comps[c].charge[j] * erf(b->alpha * r) / r;
         }
+        pi[i - a0] = acc;
       }
+      for (int i = 0; i < M - 1; i++) ie += pi[i];
     }
+    free(pi);
     if (self_per_comp) self_per_comp[c] = se;
     if (intra_per_comp) intra_per_comp[c] = ie;
   }
-  free(ex); free(ey); free(ez); free(exy); free(qs); free(isfw);
+  free(ex); free(ey); free(ez); free(qs); free(isfw);
 }
 
 /* ewald_preparation.h:261-298, 351-366: intra and self exclusion of one rigid
