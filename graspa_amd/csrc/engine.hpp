@@ -240,9 +240,13 @@ int coulomb_table_octaves(double cut2);
 void build_coulomb_table(double alpha, double cut2, std::vector<double>& tab);
 
 // widom_kernel.hip
+// Bytes of the no-allocation form's fixed scratch block (one per device).
+constexpr size_t kWidomScratchBytes = (size_t)32 << 20;
+// no_alloc: the pipeline's scratch comes from that block, and no buffer or
+// voxel map is allocated (maps built by earlier calls are used).
 int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
                  uint64_t seed, uint32_t stream, int64_t first_index, int n_insert, graspa_widom_result* d_res,
-                 graspa_widom_summary* summary);
+                 graspa_widom_summary* summary, bool no_alloc);
 int widom_autotune(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
                    uint64_t seed, int n_insert, int reps, graspa_eval_shape* tried, double* ms, int cap, int* n_tried);
 

@@ -940,7 +940,8 @@ int graspa_widom_batch(graspa_box* b, int comp, int K, int O, double beta, int b
     }
     d_res = b->d_wres;
   }
-  int rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, stream, first_index, n_insert, d_res, summary);
+  int rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, stream, first_index, n_insert, d_res, summary,
+                        false);
   if (rc) return rc;
   if (results && n_insert > 0) {
     if (hip_check(hipMemcpyAsync(results, d_res, (size_t)n_insert * sizeof(graspa_widom_result), hipMemcpyDeviceToHost,
@@ -957,7 +958,18 @@ int graspa_widom_enqueue(graspa_box* b, int comp, int K, int O, double beta, int
   GRASPA_CHECK_BOX(b);
   if (n_insert < 0) return fail(GRASPA_ERR_ARG, "n_insert < 0");
   hipSetDevice(b->device);
-  return launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, stream, first_index, n_insert, nullptr, nullptr);
+  return launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, stream, first_index, n_insert, nullptr, nullptr,
+                      false);
+}
+
+int graspa_widom_enqueue_noalloc(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl,
+                                 double tail_de, uint64_t seed, uint32_t stream, int64_t first_index, int n_insert)
+{
+  GRASPA_CHECK_BOX(b);
+  if (n_insert < 0) return fail(GRASPA_ERR_ARG, "n_insert < 0");
+  hipSetDevice(b->device);
+  return launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, stream, first_index, n_insert, nullptr, nullptr,
+                      true);
 }
 
 int graspa_widom_totals(graspa_box* b, graspa_widom_summary* totals, int reset)

@@ -35,6 +35,8 @@
 // The semantics are Insertion_Body (mc_swap_utilities.h:3-133) with
 // Widom_Move_FirstBead_PARTIAL / Widom_Move_Chain_PARTIAL (mc_widom.h:305-614).
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "device_math.hpp"
 #include "engine.hpp"
@@ -1071,7 +1073,7 @@ static int eval_grid(graspa_box* b, EvalKern kern, int waves, size_t lds, int pe
 // framework in LDS falls back to reading it from L2 when it does not fit.
 static int launch_eval(graspa_box* b, const WidomArgs& A, const TrialPt* list, const unsigned* counts, int groups,
                        int chunk, int per_ins, double2* energy, uint32_t* mask, unsigned long long* stat_slot,
-                       int* grid_out)
+                       int* grid_out, bool timed)
 {
   const int waves = b->ev_waves;
   for (int pass = b->ev_fw_lds ? 0 : 1; pass < 2; ++pass) {
@@ -1081,10 +1083,10 @@ static int launch_eval(graspa_box* b, const WidomArgs& A, const TrialPt* list, c
     const int grid = eval_grid(b, kern, waves, lds, b->ev_per_cu);
     if (grid < 1) continue;
     *grid_out = grid;
-    timing_begin(b, "widom");
+    if (timed) timing_begin(b, "widom");
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(waves * 64), lds, b->stream, A, list, counts, groups, chunk,
                        per_ins, energy, mask, stat_slot);
-    timing_end(b, "widom");
+    if (timed) timing_end(b, "widom");
     return hip_check(hipGetLastError(), "widom_eval_kernel launch");
   }
   return fail(GRASPA_ERR_UNSUPPORTED, "widom: no launch shape fits the LDS (pair energies)");
@@ -1102,9 +1104,122 @@ static int launch_first_bead(graspa_box* b, const WidomArgs& A, size_t base_lds,
   return hip_check(hipGetLastError(), "widom_first_bead_kernel launch");
 }
 
+// Scratch of one batch: the pipeline's masks, per-insertion state, live
+// lists, work lists and their energies, sub-list lengths, work counts, the
+// per-insertion records and the reduction partials.
+struct WidomBufs {
+  uint32_t* mask1;
+  uint32_t* mask2;
+  InsState* state;
+  int* live;
+  unsigned long long* stat_part;
+  double2* e1;
+  TrialPt* list1;
+  double2* e2;
+  TrialPt* list2;
+  unsigned* counts;
+  double* rec_w;
+  double* rec_e;
+  int* rec_f;
+  double* part;
+};
+
+// Screening grid of a batch: up to kMaxScreenGroups workgroups, each owning
+// a contiguous range of `chunk` insertions (four wave steps of kFbWaves x 64 /
+// K insertions each); the work lists and the live list have a sub-list of
+// chunk insertions per workgroup.  The first-bead, chain and finish kernels
+// hold one workgroup per CU, so the group count is rounded down to a
+// multiple of the CU count: a few groups beyond it would take a second round
+// of the whole kernel.
+static void widom_grid(const graspa_box* b, int K, int n_insert, int* groups, int* chunk)
+{
+  const long long per_group = 4LL * kFbWaves * (64 / K);
+  long long raw = std::max<long long>(1, ((long long)n_insert + per_group - 1) / per_group);
+  const long long ncu = std::max(1, b->n_cu);
+  if (raw > ncu) raw = raw / ncu * ncu;
+  const long long cap = kMaxScreenGroups >= ncu ? kMaxScreenGroups / ncu * ncu : kMaxScreenGroups;
+  *groups = (int)std::min<long long>(raw, cap);
+  *chunk = (n_insert + *groups - 1) / *groups;
+}
+
+// The box's own buffers, grown to the batch.
+static int box_bufs(graspa_box* b, int n_insert, int K, int OC, int n_slots, WidomBufs* w)
+{
+  if (ensure_widom_records(b, n_insert)) return GRASPA_ERR_HIP;
+  if (ensure_pipeline(b, n_slots, K, OC)) return GRASPA_ERR_HIP;
+  w->mask1 = static_cast<uint32_t*>(b->d_wmask1);
+  w->mask2 = static_cast<uint32_t*>(b->d_wmask2);
+  w->state = static_cast<InsState*>(b->d_wstate);
+  w->live = static_cast<int*>(b->d_wlive);
+  w->stat_part = static_cast<unsigned long long*>(b->d_wstat);
+  w->e1 = static_cast<double2*>(b->d_we1);
+  w->list1 = static_cast<TrialPt*>(b->d_wlist1);
+  w->e2 = static_cast<double2*>(b->d_we2);
+  w->list2 = static_cast<TrialPt*>(b->d_wlist2);
+  w->counts = static_cast<unsigned*>(b->d_wcounts);
+  w->rec_w = b->d_wrec_w;
+  w->rec_e = b->d_wrec_e;
+  w->rec_f = b->d_wrec_f;
+  w->part = b->d_wpart;
+  return GRASPA_OK;
+}
+
+// Bytes of scratch a batch needs when every buffer is carved from one block
+// (256-byte aligned pieces), and the carve-up itself when base is not null.
+static size_t carve_bufs(char* base, int n_insert, int K, int OC, int n_slots, WidomBufs* w)
+{
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* p = base ? base + off : nullptr;
+    off += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  const size_t n = (size_t)n_slots;
+  const size_t nblk = ((size_t)n_insert + kRedBlock * kRedPer - 1) / (kRedBlock * kRedPer);
+  WidomBufs v;
+  v.mask1 = reinterpret_cast<uint32_t*>(take(n * sizeof(uint32_t)));
+  v.mask2 = reinterpret_cast<uint32_t*>(take(n * sizeof(uint32_t)));
+  v.state = reinterpret_cast<InsState*>(take(n * sizeof(InsState)));
+  v.live = reinterpret_cast<int*>(take(n * sizeof(int)));
+  v.stat_part = reinterpret_cast<unsigned long long*>(
+      take((2 * kMaxScreenGroups + 2 * kMaxEvalGroups) * sizeof(unsigned long long)));
+  v.e1 = reinterpret_cast<double2*>(take(n * K * sizeof(double2)));
+  v.list1 = reinterpret_cast<TrialPt*>(take(n * K * sizeof(TrialPt)));
+  v.e2 = reinterpret_cast<double2*>(take(n * std::max(OC, 1) * sizeof(double2)));
+  v.list2 = reinterpret_cast<TrialPt*>(take(n * std::max(OC, 1) * sizeof(TrialPt)));
+  v.counts = reinterpret_cast<unsigned*>(take(3 * kMaxScreenGroups * sizeof(unsigned)));
+  v.rec_w = reinterpret_cast<double*>(take((size_t)n_insert * sizeof(double)));
+  v.rec_e = reinterpret_cast<double*>(take((size_t)n_insert * sizeof(double)));
+  v.rec_f = reinterpret_cast<int*>(take((size_t)n_insert * sizeof(int)));
+  v.part = reinterpret_cast<double*>(take(6 * std::max<size_t>(nblk, 1) * sizeof(double)));
+  if (w) *w = v;
+  return off;
+}
+
+// The no-allocation form's scratch: one block per device, its address taken
+// once per device with hipGetSymbolAddress (host code never uses the
+// symbol's name as an address).  Shared by every call on the device.
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) char widom_scratch[kWidomScratchBytes];
+static std::mutex scratch_mu;
+static char* scratch_ptr[kMaxDevices];   // per device, null until looked up
+
+static int device_scratch(int device, char** out)
+{
+  if (device < 0 || device >= kMaxDevices) return fail(GRASPA_ERR_ARG, "widom scratch: device index out of range");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[device]) {
+    void* p = nullptr;
+    if (hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(widom_scratch)), "widom scratch address")) return GRASPA_ERR_HIP;
+    scratch_ptr[device] = static_cast<char*>(p);
+  }
+  *out = scratch_ptr[device];
+  return GRASPA_OK;
+}
+
 int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
                  uint64_t seed, uint32_t stream, int64_t first_index, int n_insert, graspa_widom_result* d_res,
-                 graspa_widom_summary* summary)
+                 graspa_widom_summary* summary, bool no_alloc)
 {
   if (comp < 1 || comp >= (int)b->comps.size()) return fail(GRASPA_ERR_ARG, "widom: bad component");
   const Component& c = b->comps[comp];
@@ -1116,22 +1231,22 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
     if (summary) *summary = graspa_widom_summary{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     return GRASPA_OK;
   }
-  if (ensure_widom_records(b, n_insert)) return GRASPA_ERR_HIP;
   const int chain = c.molsize - 1;
-  // screening grid: up to kMaxScreenGroups workgroups, each owning a contiguous range of insertions
-  // (four wave steps of kFbWaves x 64 / K insertions each); the work lists and the
-  // live list have a sub-list of chunk insertions per workgroup
-  // The first-bead, chain and finish kernels hold one workgroup per CU, so
-  // the group count is rounded down to a multiple of the CU count: a few
-  // groups beyond it would take a second round of the whole kernel.
-  const long long per_group = 4LL * kFbWaves * (64 / K);
-  long long raw = std::max<long long>(1, ((long long)n_insert + per_group - 1) / per_group);
-  const long long ncu = std::max(1, b->n_cu);
-  if (raw > ncu) raw = raw / ncu * ncu;
-  const long long cap = kMaxScreenGroups >= ncu ? kMaxScreenGroups / ncu * ncu : kMaxScreenGroups;
-  const int groups = (int)std::min<long long>(raw, cap);
-  const int chunk = (n_insert + groups - 1) / groups;
-  if (ensure_pipeline(b, groups * chunk, K, O * chain)) return GRASPA_ERR_HIP;
+  int groups = 0, chunk = 0;
+  widom_grid(b, K, n_insert, &groups, &chunk);
+  WidomBufs W{};
+  if (no_alloc) {
+    const size_t need = carve_bufs(nullptr, n_insert, K, O * chain, groups * chunk, nullptr);
+    if (need > (size_t)kWidomScratchBytes)
+      return fail(GRASPA_ERR_ARG, "widom (no allocation): the batch needs " + std::to_string(need) +
+                                      " bytes of scratch, above the fixed " + std::to_string(kWidomScratchBytes) +
+                                      "; use fewer insertions per call");
+    char* base = nullptr;
+    if (int rc = device_scratch(b->device, &base)) return rc;
+    carve_bufs(base, n_insert, K, O * chain, groups * chunk, &W);
+  } else if (int rc = box_bufs(b, n_insert, K, O * chain, groups * chunk, &W)) {
+    return rc;
+  }
   WidomArgs A{};
   A.bc = b->bc;
   A.fw = b->d_fws; A.fw_type = b->d_fws_type; A.fw_fixT = b->d_fw_fixT; A.n_fw = b->n_fw;
@@ -1169,25 +1284,34 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   A.kstride = 1 + std::max(b->bc.kmax[0], std::max(b->bc.kmax[1], b->bc.kmax[2]));
   A.coul = b->d_coul;
   A.coul_n = b->coul_n;
-  A.mask1 = static_cast<uint32_t*>(b->d_wmask1);
-  A.mask2 = static_cast<uint32_t*>(b->d_wmask2);
-  A.e1 = static_cast<double2*>(b->d_we1);
-  A.e2 = static_cast<double2*>(b->d_we2);
-  A.list1 = static_cast<TrialPt*>(b->d_wlist1);
-  A.list2 = static_cast<TrialPt*>(b->d_wlist2);
-  A.state = static_cast<InsState*>(b->d_wstate);
-  A.counts1 = static_cast<unsigned*>(b->d_wcounts);
+  A.mask1 = W.mask1;
+  A.mask2 = W.mask2;
+  A.e1 = W.e1;
+  A.e2 = W.e2;
+  A.list1 = W.list1;
+  A.list2 = W.list2;
+  A.state = W.state;
+  A.counts1 = W.counts;
   A.counts2 = A.counts1 + kMaxScreenGroups;
   A.live_counts = A.counts2 + kMaxScreenGroups;
-  A.live = static_cast<int*>(b->d_wlive);
-  A.stat_part = static_cast<unsigned long long*>(b->d_wstat);
-  for (int a = 0; a < c.molsize; ++a)
-    if (int rc = ensure_dead_map(b, c.type[a], &A.dmap[a])) return rc;
+  A.live = W.live;
+  A.stat_part = W.stat_part;
+  for (int a = 0; a < c.molsize; ++a) {
+    if (no_alloc) {
+      // maps built by an earlier call are used; none is built here (a trial
+      // a map would decide goes through the cell screen instead, with the
+      // same outcome)
+      const int t = c.type[a];
+      A.dmap[a] = (t >= 0 && t < (int)b->d_dmap.size()) ? b->d_dmap[t] : nullptr;
+    } else if (int rc = ensure_dead_map(b, c.type[a], &A.dmap[a])) {
+      return rc;
+    }
+  }
   for (int d = 0; d < 3; ++d) A.dnv[d] = b->dmap_nv[d];
   A.groups = groups;
   A.chunk = chunk;
   A.res = d_res;
-  A.rec_w = b->d_wrec_w; A.rec_e = b->d_wrec_e; A.rec_f = b->d_wrec_f;
+  A.rec_w = W.rec_w; A.rec_e = W.rec_e; A.rec_f = W.rec_f;
   A.stat = reinterpret_cast<unsigned long long*>(b->d_wacc + 8);
   const int T = A.bc.ntypes, M = A.M;
   // screening kernels: thresholds | cell table (optional) | per-wave areas
@@ -1207,7 +1331,7 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   int egrid[2] = {0, 0};
   auto eval = [&](const TrialPt* list, const unsigned* counts, int per_ins, double2* energy, uint32_t* mask, int e) {
     unsigned long long* slot = A.stat_part + 2 * kMaxScreenGroups + (size_t)e * kMaxEvalGroups;
-    return launch_eval(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, &egrid[e]);
+    return launch_eval(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, &egrid[e], !no_alloc);
   };
   rc = eval(A.list1, A.counts1, K, A.e1, A.mask1, 0);
   if (rc) return rc;
@@ -1223,9 +1347,9 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   if (rc) return rc;
   const int nblk = (n_insert + kRedBlock * kRedPer - 1) / (kRedBlock * kRedPer);
   hipLaunchKernelGGL(widom_reduce_kernel, dim3(nblk), dim3(kRedBlock), 0, b->stream, A.rec_w, A.rec_e, A.rec_f,
-                     n_insert, b->d_wpart);
+                     n_insert, W.part);
   if (hip_check(hipGetLastError(), "widom_reduce launch")) return GRASPA_ERR_HIP;
-  hipLaunchKernelGGL(widom_final_kernel, dim3(1), dim3(kRedBlock), 0, b->stream, b->d_wpart, nblk, b->d_wacc,
+  hipLaunchKernelGGL(widom_final_kernel, dim3(1), dim3(kRedBlock), 0, b->stream, W.part, nblk, b->d_wacc,
                      b->d_wacc + kWaccTotals);
   if (hip_check(hipGetLastError(), "widom_final launch")) return GRASPA_ERR_HIP;
   hipLaunchKernelGGL(widom_stat_kernel, dim3(1), dim3(64), 0, b->stream, A.stat_part, A.groups, egrid[0], egrid[1],
@@ -1290,10 +1414,10 @@ int widom_autotune(graspa_box* b, int comp, int K, int O, double beta, int bias,
     b->ev_waves = cand[i].waves;
     b->ev_per_cu = cand[i].per_cu;
     b->ev_fw_lds = cand[i].framework_lds;
-    rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, 0u, 0, n_insert, nullptr, nullptr);
+    rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, 0u, 0, n_insert, nullptr, nullptr, false);
     rc = rc ? rc : hip_check(hipEventRecord(e0, b->stream), "widom autotune event");
     for (int r = 0; r < reps && !rc; ++r)
-      rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, 0u, 0, n_insert, nullptr, nullptr);
+      rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, 0u, 0, n_insert, nullptr, nullptr, false);
     rc = rc ? rc : hip_check(hipEventRecord(e1, b->stream), "widom autotune event");
     rc = rc ? rc : hip_check(hipEventSynchronize(e1), "widom autotune");
     float t = 0.0f;

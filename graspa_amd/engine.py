@@ -157,6 +157,8 @@ _SIGS = {
                                      C.POINTER(WidomResult), C.POINTER(WidomSummary)]),
     "graspa_widom_enqueue": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
                                        C.c_double, C.c_uint64, C.c_uint32, C.c_int64, C.c_int]),
+    "graspa_widom_enqueue_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                                       C.c_double, C.c_uint64, C.c_uint32, C.c_int64, C.c_int]),
     "graspa_widom_totals": (C.c_int, [C.c_void_p, C.POINTER(WidomSummary), C.c_int]),
     "graspa_widom_set_shape": (C.c_int, [C.c_void_p, C.POINTER(EvalShape)]),
     "graspa_widom_get_shape": (C.c_int, [C.c_void_p, C.POINTER(EvalShape)]),
@@ -448,6 +450,15 @@ class Box:
         _check(lib().graspa_widom_enqueue(self.h, comp, s.n_trial_pos, s.n_trial_orient, s.beta,
                                           int(s.vdw_real_bias), excl, tail_de, seed, stream, first_index, n_insert),
                "widom_enqueue")
+
+    def widom_enqueue_noalloc(self, comp, n_insert, seed, stream=0, first_index=0, excl=0.0, tail_de=0.0):
+        """widom_enqueue with the pipeline's scratch in the library's fixed
+        per-device block: allocates nothing (graspa_widom_enqueue_noalloc)."""
+        s = self.system
+        _check(lib().graspa_widom_enqueue_noalloc(self.h, comp, s.n_trial_pos, s.n_trial_orient, s.beta,
+                                                  int(s.vdw_real_bias), excl, tail_de, seed, stream, first_index,
+                                                  n_insert),
+               "widom_enqueue_noalloc")
 
     def widom_totals(self, reset=False):
         """Waits for the queued batches; summary fields summed over every batch

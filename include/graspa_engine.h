@@ -409,6 +409,23 @@ int  graspa_widom_enqueue(graspa_box* box, int comp, int n_trial_pos, int n_tria
                           int vdw_real_bias, double excl, double tail_de, uint64_t seed, uint32_t stream,
                           int64_t first_index, int n_insert);
 int  graspa_widom_totals(graspa_box* box, graspa_widom_summary* totals, int reset);
+/* graspa_widom_enqueue without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops that forbid hipMalloc).  The
+ * pipeline's scratch (trial masks, per-insertion state, work lists, energies,
+ * records, partial sums) comes from one fixed 32 MiB block per device in the
+ * library's static device memory instead of the box's buffers, and no
+ * certain-overlap voxel map is built (maps that earlier calls built are
+ * used; without one the cell screen decides the same trials).  A batch that
+ * needs more scratch than the block (about 18,000 CO2 insertions at 10 x 10
+ * trials on 256 CUs) is refused with GRASPA_ERR_ARG before any GPU work.
+ * Results are those of graspa_widom_enqueue: the summary goes into the box's
+ * running totals (graspa_widom_totals).  Every call on a device shares the
+ * block: calls on one device that overlap in time (other streams, other
+ * host threads) race and must be serialised by the caller. */
+int  graspa_widom_enqueue_noalloc(graspa_box* box, int comp, int n_trial_pos, int n_trial_orient, double beta,
+                                  int vdw_real_bias, double excl, double tail_de, uint64_t seed, uint32_t stream,
+                                  int64_t first_index, int n_insert);
 
 int  graspa_batch_block_stats(graspa_batch* batch, int i, graspa_block_stats* out);
 
