@@ -68,9 +68,10 @@ struct BSpecies {
   double p_ident;                             // cumulative, between p_reins and p_swap
   double mol_fraction, fugacity, ideal_rosen, excl;   // excl = ExclusionIntra + ExclusionAtom
   int base, molsize, capacity, charged;
-  int pseudo[16];
-  double q[kBMaxMol];               // charges and types of the species' atoms (rigid: the same in
-  int type[kBMaxMol];               // every molecule)
+  int pseudo[GRASPA_MAX_PSEUDO];
+  double q[kBMaxMol];               // charges and types of the species' atoms (the same in every
+  int type[kBMaxMol];               // molecule)
+  const double* chain;              // flexible chain model (kChainDoubles), null for a rigid molecule
 };
 
 // Read-only description of one box.
@@ -121,7 +122,7 @@ struct BState {
   int sk_cur, error, production, _pad;
   int n_mol[kMaxComps];
   int64_t total_mol;
-  int64_t n_pseudo[16];
+  int64_t n_pseudo[GRASPA_MAX_PSEUDO];
   double max_trans[kMaxComps][3], max_rot[kMaxComps][3];
   int64_t cnt[kMaxComps][GRASPA_N_COUNTERS];
   int64_t upd_tot[kMaxComps][2], upd_acc[kMaxComps][2];
@@ -560,9 +561,9 @@ __device__ double tail_delta(const BBox& B, const BState& S, const int comp, con
 {
   const int T = B.bc.ntypes;
   double e = 0.0;
-  for (int i = 0; i < T && i < 16; ++i) {
+  for (int i = 0; i < T && i < GRASPA_MAX_PSEUDO; ++i) {
     const int64_t di = (int64_t)B.sp[comp].pseudo[i] * sign;
-    for (int j = i; j < T && j < 16; ++j) {
+    for (int j = i; j < T && j < GRASPA_MAX_PSEUDO; ++j) {
       const int64_t dj = (int64_t)B.sp[comp].pseudo[j] * sign;
       const int ij = i * T + j;
       if (!B.use_tail[ij]) continue;
@@ -581,9 +582,9 @@ __device__ double tail_identity(const BBox& B, const BState& S, const int nc, co
 {
   const int T = B.bc.ntypes;
   double e = 0.0;
-  for (int i = 0; i < T && i < 16; ++i) {
+  for (int i = 0; i < T && i < GRASPA_MAX_PSEUDO; ++i) {
     const int64_t di = (int64_t)B.sp[nc].pseudo[i] - B.sp[oc].pseudo[i];
-    for (int j = i; j < T && j < 16; ++j) {
+    for (int j = i; j < T && j < GRASPA_MAX_PSEUDO; ++j) {
       const int64_t dj = (int64_t)B.sp[nc].pseudo[j] - B.sp[oc].pseudo[j];
       const int ij = i * T + j;
       if (!B.use_tail[ij]) continue;
@@ -763,7 +764,59 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
   }
   __syncthreads();
   if (L.fail) return;
-  if (chain > 0) {
+  if (chain > 0 && sp.chain) {
+    // flexible chain, bead by bead: O trial positions of bead i from the
+    // bonded densities given the beads placed (a deletion retraces its own
+    // bead as trial 0), screened, evaluated and selected on the Boltzmann
+    // factor of their external energy
+    for (int i = 1; i < M; ++i) {
+      if (tid < O) {
+        BSite s;
+        if (del && tid == 0) {
+          const int j = L.start + i;
+          s.x = B.g[j]; s.y = B.g[cap + j]; s.z = B.g[2 * cap + j];
+        } else {
+          double p[3];
+          flex_trial(L.molp, i, sp.chain, B.beta, B.seed, B.stream, (uint64_t)S.move, tid, p);
+          s.x = p[0]; s.y = p[1]; s.z = p[2];
+        }
+        s.q = sp.q[i];
+        s.type = sp.type[i];
+        s._p = 0;
+        L.site[tid] = s;
+      }
+      __syncthreads();
+      if (B.cell_cap > 0) screen_pass<CUBIC>(B, L, thr, 0, O);
+      else if (tid == 0) L.smask = 0ull;
+      if (tid == 0) list_survivors(L, O, 1);
+      __syncthreads();
+      if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, L.n_eval, ex_lo, ex_hi);
+      if (tid < O) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) L.t4[tid][t] = L.e[tid][t];
+      }
+      __syncthreads();
+      if (tid < 64) {
+        double W = L.W;
+        const double u = philox_uniform(B.seed, B.stream, (uint64_t)S.move, kFlexDraw - 8 + i);
+        const int sel = select_wave0(B, L, O, (uint32_t)L.ovl, u, del, true, &W);
+        if (tid == 0) {
+          if (sel < 0) {
+            L.fail = 1;
+          } else {
+            L.W = W;
+            L.en[kHGV] += L.t4[sel][0]; L.en[kHGR] += L.t4[sel][1];
+            L.en[kGGV] += L.t4[sel][2]; L.en[kGGR] += L.t4[sel][3];
+            L.molp[i][0] = L.site[sel].x; L.molp[i][1] = L.site[sel].y; L.molp[i][2] = L.site[sel].z;
+            L.molq[i] = L.site[sel].q;
+            L.molt[i] = L.site[sel].type;
+          }
+        }
+      }
+      __syncthreads();
+      if (L.fail) return;
+    }
+  } else if (chain > 0) {
     // orientations (get_random_trial_orientation, mc_widom.h:215-303): the
     // template vectors of the component's first slot as it is now, rotated
     for (int t = tid; t < O * chain; t += kBT) {
@@ -886,11 +939,11 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
       if (acc) {
         if (!del) {
           S.total_mol++;
-          for (int t = 0; t < 16; ++t) S.n_pseudo[t] += sp.pseudo[t];
+          for (int t = 0; t < GRASPA_MAX_PSEUDO; ++t) S.n_pseudo[t] += sp.pseudo[t];
           S.cnt[comp][5]++;
         } else {
           S.total_mol--;
-          for (int t = 0; t < 16; ++t) S.n_pseudo[t] -= sp.pseudo[t];
+          for (int t = 0; t < GRASPA_MAX_PSEUDO; ++t) S.n_pseudo[t] -= sp.pseudo[t];
           S.cnt[comp][7]++;
           for (int i = 0; i < 10; ++i) L.en[i] = -L.en[i];
         }
@@ -1331,7 +1384,7 @@ __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* 
     if (acc) {
       S.cnt[oc][GRASPA_CNT_IDSW_ACC + nc - 1]++;
       if (nc != oc)
-        for (int t = 0; t < 16; ++t) S.n_pseudo[t] += sn.pseudo[t] - so.pseudo[t];
+        for (int t = 0; t < GRASPA_MAX_PSEUDO; ++t) S.n_pseudo[t] += sn.pseudo[t] - so.pseudo[t];
       add_energy(S.running, L.en);
       if (n_old + n_new > 0) S.sk_cur ^= 1;
     }
@@ -1743,7 +1796,15 @@ int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
       d.q[a] = cc.q[a];
       d.type[a] = cc.type[a];
     }
-    for (int t = 0; t < 16; ++t) d.pseudo[t] = (int)sp.pseudo_per_mol[t];
+    for (int t = 0; t < GRASPA_MAX_PSEUDO; ++t) d.pseudo[t] = (int)sp.pseudo_per_mol[t];
+    d.chain = cc.d_chain;
+    if (cc.d_chain) {
+      // flexible chains grow by insertion / deletion / Widom; reinsertion and
+      // identity swaps of a flexible molecule are not implemented
+      if (sp.p_reinsertion > sp.p_widom || sp.p_identity > std::max(sp.p_reinsertion, sp.p_widom))
+        return fail(GRASPA_ERR_UNSUPPORTED, "flexible chains: no reinsertion or identity-swap moves");
+      if (d.charged) return fail(GRASPA_ERR_UNSUPPORTED, "flexible chains must be uncharged");
+    }
     if (sp.p_swap > 1e-10) h.valid[h.n_valid++] = c;
   }
   if (h.n_valid == 0) return fail(GRASPA_ERR_ARG, "no species with a move probability");
@@ -1779,7 +1840,7 @@ int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
       st.max_rot[c][d] = p->species[c - 1].max_rotation[d];
     }
   }
-  for (int t = 0; t < 16; ++t) st.n_pseudo[t] = p->n_pseudo[t];
+  for (int t = 0; t < GRASPA_MAX_PSEUDO; ++t) st.n_pseudo[t] = p->n_pseudo[t];
   st.blk_size = 1;
   if (hip_check(hipMemset(bt->d_blk + i, 0, sizeof(graspa_block_stats)), "block statistics")) return GRASPA_ERR_HIP;
   hipSetDevice(b->device);
@@ -2050,6 +2111,20 @@ int graspa_batch_box_stats(graspa_batch* bt, int i, graspa_energy* running, int*
   if (n_load_cycles) *n_load_cycles = s.load_n;
   if (moves) *moves = s.move;
   if (cycles) *cycles = s.cycle;
+  return GRASPA_OK;
+}
+
+int graspa_batch_max_steps(graspa_batch* bt, int i, double* max_translation, double* max_rotation)
+{
+  if (!bt || i < 0 || i >= bt->n || !bt->boxes[i]) return fail(GRASPA_ERR_ARG, "bad box index or empty slot");
+  hipSetDevice(bt->device);
+  if (int rc = download_states(bt)) return rc;
+  const BState& s = bt->hs[i];
+  for (int c = 1; c < bt->hb[i].n_comp; ++c)
+    for (int d = 0; d < 3; ++d) {
+      if (max_translation) max_translation[3 * (c - 1) + d] = s.max_trans[c][d];
+      if (max_rotation) max_rotation[3 * (c - 1) + d] = s.max_rot[c][d];
+    }
   return GRASPA_OK;
 }
 

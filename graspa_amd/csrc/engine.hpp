@@ -16,9 +16,15 @@ namespace graspa {
 // one CBMC trial batch or one moved molecule (old + new) fits.
 constexpr int kMaxSites = 64;
 constexpr int kMaxTypes = 32;
+static_assert(kMaxTypes == GRASPA_MAX_PSEUDO, "pseudo-atom counts cover every type a box may have");
 constexpr int kWaccTotals = 16;    // offset of the running Widom totals in graspa_box::d_wacc
 constexpr int kWaccSize = 32;      // doubles of graspa_box::d_wacc
 constexpr int kMaxComps = 8;
+// flexible chain model of a species: per bead i < kChainBeads, {bond,
+// theta0, k_bend, t0, t1, t2, t3} (graspa_chain, packed)
+constexpr int kChainBeads = 4;
+constexpr int kChainPer = 7;
+constexpr int kChainDoubles = kChainBeads * kChainPer;
 constexpr int kBlock = 256;          // 4 wave64 per workgroup
 
 struct SiteBatch {
@@ -62,6 +68,9 @@ struct Component {
   std::vector<double> q;
   std::vector<int> type;
   std::vector<int64_t> molid;
+  // flexible chain model on the device (kChainDoubles doubles, move_math.hpp
+  // layout); null for a rigid molecule
+  double* d_chain = nullptr;
 };
 
 struct TimingState {

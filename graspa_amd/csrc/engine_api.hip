@@ -507,6 +507,7 @@ void graspa_box_destroy(graspa_box* b)
   hipFree(b->d_ktemp); hipFree(b->d_sk_fw); hipFree(b->d_sk_ads); hipFree(b->d_sk_stage);
   hipFree(b->d_kact); hipFree(b->d_ktemp_act); hipFree(b->d_coul);
   free_dead_maps(b);
+  for (Component& c : b->comps) hipFree(c.d_chain);
   hipFree(b->d_part);
   if (b->h_part) hipHostFree(b->h_part);
   hipFree(b->d_wacc); hipFree(b->d_wres);
@@ -587,6 +588,39 @@ int graspa_box_add_component(graspa_box* b, int molsize, int capacity, const dou
   b->comps.push_back(c);
   *comp_out = (int)b->comps.size() - 1;
   return realloc_guests(b);
+}
+
+int graspa_box_set_chain(graspa_box* b, int comp, const graspa_chain* chain)
+{
+  GRASPA_CHECK_BOX(b);
+  if (comp < 1 || comp >= (int)b->comps.size()) return fail(GRASPA_ERR_ARG, "bad component");
+  Component& c = b->comps[comp];
+  hipSetDevice(b->device);
+  if (!chain) {
+    if (c.d_chain) {
+      hipStreamSynchronize(b->stream);
+      hipFree(c.d_chain);
+    }
+    c.d_chain = nullptr;
+    return GRASPA_OK;
+  }
+  if (c.molsize > kChainBeads) return fail(GRASPA_ERR_UNSUPPORTED, "flexible chains of at most 4 beads");
+  for (int a = 0; a < c.molsize; ++a)
+    if (c.q[a] != 0.0) return fail(GRASPA_ERR_UNSUPPORTED, "flexible chains must be uncharged");
+  double m[kChainDoubles] = {0.0};
+  for (int i = 0; i < kChainBeads; ++i) {
+    double* d = m + kChainPer * i;
+    d[0] = chain->bond[i];
+    d[1] = chain->theta0[i];
+    d[2] = chain->k_bend[i];
+    for (int t = 0; t < 4; ++t) d[3 + t] = chain->torsion[i][t];
+    if (i >= 1 && i < c.molsize && !(d[0] > 0.0)) return fail(GRASPA_ERR_ARG, "chain: bond lengths must be > 0");
+    if (i >= 2 && i < c.molsize && !(d[2] > 0.0 && d[1] > 0.0 && d[1] < 3.14159265358979323846))
+      return fail(GRASPA_ERR_ARG, "chain: bends need k_bend > 0 and 0 < theta0 < pi");
+  }
+  if (!c.d_chain && hip_check(hipMalloc(&c.d_chain, sizeof(m)), "hipMalloc chain")) return GRASPA_ERR_HIP;
+  hipStreamSynchronize(b->stream);
+  return hip_check(hipMemcpy(c.d_chain, m, sizeof(m), hipMemcpyHostToDevice), "chain upload");
 }
 
 int graspa_box_set_molecules(graspa_box* b, int comp, int n_mol, const double* xyz, const double* q, const int* type,

@@ -66,7 +66,7 @@ struct graspa_mc {
   std::vector<double> pool;   // 3 * pool_size
   size_t pool_off = 0;
   int64_t total_molecules = 1;
-  int64_t n_pseudo[16] = {0};
+  int64_t n_pseudo[GRASPA_MAX_PSEUDO] = {0};
   // trans_tot, trans_acc, rot_tot, rot_acc, ins_tot, ins_acc, del_tot, del_acc, widom, reins_tot, reins_acc,
   // identity swaps to each species (attempted, accepted)
   int64_t cnt[8][GRASPA_N_COUNTERS] = {{0}};
@@ -198,9 +198,9 @@ double tail_delta(const graspa_mc* mc, int comp, int sign)
   const int T = mc->ntypes;
   double e = 0.0;
   const int64_t* d = mc->p.species[comp - 1].pseudo_per_mol;
-  for (int i = 0; i < T && i < 16; ++i) {
+  for (int i = 0; i < T && i < GRASPA_MAX_PSEUDO; ++i) {
     const int64_t di = d[i] * sign;
-    for (int j = i; j < T && j < 16; ++j) {
+    for (int j = i; j < T && j < GRASPA_MAX_PSEUDO; ++j) {
       const int64_t dj = d[j] * sign;
       const int ij = i * T + j;
       if (!mc->use_tail[ij]) continue;
@@ -454,12 +454,12 @@ int cbmc_move(graspa_mc* mc, int comp, int mol, int move, graspa_trace& tr, int 
   if (!del) {
     if (int rc = graspa_commit_insert(b, comp, mol_pos.data(), mol_q.data(), mol_t.data(), N)) return rc;
     mc->total_molecules++;
-    for (int t = 0; t < 16; ++t) mc->n_pseudo[t] += sp.pseudo_per_mol[t];
+    for (int t = 0; t < GRASPA_MAX_PSEUDO; ++t) mc->n_pseudo[t] += sp.pseudo_per_mol[t];
     if (!mc->create) mc->cnt[comp][5]++;
   } else {
     if (int rc = graspa_commit_delete(b, comp, mol)) return rc;
     mc->total_molecules--;
-    for (int t = 0; t < 16; ++t) mc->n_pseudo[t] -= sp.pseudo_per_mol[t];
+    for (int t = 0; t < GRASPA_MAX_PSEUDO; ++t) mc->n_pseudo[t] -= sp.pseudo_per_mol[t];
     mc->cnt[comp][7]++;
     en.hg_vdw = -en.hg_vdw; en.hg_real = -en.hg_real; en.gg_vdw = -en.gg_vdw; en.gg_real = -en.gg_real;
     en.gg_ewald = -en.gg_ewald; en.hg_ewald = -en.hg_ewald; en.tail = -en.tail;
@@ -801,9 +801,9 @@ int identity_swap_move(graspa_mc* mc, graspa_trace& tr)
     const int64_t* pn = mc->p.species[nc - 1].pseudo_per_mol;
     const int64_t* po = mc->p.species[oc - 1].pseudo_per_mol;
     double e = 0.0;
-    for (int i = 0; i < T && i < 16; ++i) {
+    for (int i = 0; i < T && i < GRASPA_MAX_PSEUDO; ++i) {
       const int64_t di = pn[i] - po[i];
-      for (int j = i; j < T && j < 16; ++j) {
+      for (int j = i; j < T && j < GRASPA_MAX_PSEUDO; ++j) {
         const int64_t dj = pn[j] - po[j];
         const int ij = i * T + j;
         if (!mc->use_tail[ij]) continue;
@@ -831,7 +831,7 @@ int identity_swap_move(graspa_mc* mc, graspa_trace& tr)
     if (cn.n_mol + 1 > cn.capacity) return fail(GRASPA_ERR_CAPACITY, "Need to allocate more space");
     if (int rc = graspa_commit_delete(b, oc, oldmol)) return rc;
     if (int rc = graspa_commit_insert(b, nc, newp.data(), newq.data(), newt.data(), nnew)) return rc;
-    for (int t = 0; t < 16; ++t) mc->n_pseudo[t] += sn.pseudo_per_mol[t] - sq.pseudo_per_mol[t];
+    for (int t = 0; t < GRASPA_MAX_PSEUDO; ++t) mc->n_pseudo[t] += sn.pseudo_per_mol[t] - sq.pseudo_per_mol[t];
   } else {
     if (int rc = graspa_commit_move(b, oc, oldmol, newp.data())) return rc;
   }
@@ -876,6 +876,9 @@ int graspa_mc_create(graspa_box* box, const graspa_mc_params* p, graspa_mc** out
     return fail(GRASPA_ERR_ARG, "n_species must match the box's adsorbate components");
   if (p->n_trial_pos < 1 || p->n_trial_orient < 1) return fail(GRASPA_ERR_ARG, "trial counts must be >= 1");
   if (p->pool_size < 16) return fail(GRASPA_ERR_ARG, "pool_size too small");
+  for (size_t c = 1; c < box->comps.size(); ++c)
+    if (box->comps[c].d_chain)
+      return fail(GRASPA_ERR_UNSUPPORTED, "flexible chains run in the batched driver only (graspa_batch_*)");
   graspa_mc* mc = new graspa_mc();
   mc->box = box;
   mc->p = *p;
@@ -889,7 +892,7 @@ int graspa_mc_create(graspa_box* box, const graspa_mc_params* p, graspa_mc** out
     mc->tail.assign(p->tail, p->tail + T * T);
     mc->use_tail.assign(p->use_tail, p->use_tail + T * T);
   }
-  for (int t = 0; t < 16; ++t) mc->n_pseudo[t] = p->n_pseudo[t];
+  for (int t = 0; t < GRASPA_MAX_PSEUDO; ++t) mc->n_pseudo[t] = p->n_pseudo[t];
   mc->total_molecules = 1;
   for (size_t c = 1; c < box->comps.size(); ++c) mc->total_molecules += box->comps[c].n_mol;
   // RandomNumber::Setup (data_struct.h:1337-1345): srand, fill, then the

@@ -127,4 +127,113 @@ __device__ inline float screen_threshold(const double* f, double overlap, double
   return (float)(overlap_r2(f[0], f[1], f[2], overlap, cut2) * (1.0 - 1.0e-5));
 }
 
+// ---------------------------------------------------------------------------
+// Flexible linear chains.  No counterpart in the reference, which grows rigid
+// molecules only (read_data.cpp:2104-2116).  Bead i >= 1 sits at bond length
+// from bead i - 1; the angle (i-2, i-1, i) has U = k/2 (theta - theta0)^2;
+// the dihedral (i-3 .. i) has U = t0 + t1 (1 + cos phi) + t2 (1 - cos 2 phi)
+// + t3 (1 + cos 3 phi), phi = pi trans.  A species' model is kChainDoubles
+// doubles (engine.hpp).
+// ---------------------------------------------------------------------------
+// Draws of bead i, trial j of a move: the block at kFlexDraw + ((i - 1) * 32
+// + j) * kFlexBlock.  Bead 1: draws 0, 1; bend attempt a: 3a, 3a + 1 (a
+// Box-Muller normal), 3a + 2 (acceptance), a < kFlexBendTries; dihedral
+// attempt a: 512 + 2a (angle), 513 + 2a (acceptance), a < kFlexTorsTries;
+// bead 2's azimuth: kFlexBlock - 1.  Step i's selection uniform: draw
+// kFlexDraw - 8 + i.
+constexpr uint32_t kFlexDraw = 4096;
+constexpr uint32_t kFlexBlock = 1024;
+constexpr int kFlexBendTries = 160;
+constexpr int kFlexTorsTries = 250;
+
+__device__ inline double flex_u(uint64_t seed, uint32_t stream, uint64_t move, int i, int j, int d)
+{
+  return philox_uniform(seed, stream, move, kFlexDraw + (uint32_t)((i - 1) * 32 + j) * kFlexBlock + (uint32_t)d);
+}
+
+// Trial position of bead i (1 <= i < kChainBeads) given beads 0 .. i-1 in
+// prev: bead 1 uniform on the bond sphere; the bend angle from
+// sin(theta) exp(-beta U_bend) (normal proposal carrying the Boltzmann factor,
+// accepted with probability sin(theta)); the dihedral of bead 3 from
+// exp(-beta U_tors) (uniform proposal accepted with exp(-beta (U - U_lb)),
+// U_lb = t0 + min(0, 2 t1) + min(0, 2 t2) + min(0, 2 t3)), bead 2's azimuth
+// uniform.  Position: C + bond (-cos th bc + sin th (cos phi m + sin phi n)),
+// bc = unit(C - B), n = unit((B - A) x bc) (bead 2: the unit normal built
+// from the axis least aligned with bc), m = n x bc.  A sampler that runs out
+// of attempts returns theta0 or pi.
+__device__ inline void flex_trial(const double (*prev)[3], const int i, const double* __restrict__ chain,
+                                  const double beta, const uint64_t seed, const uint32_t stream, const uint64_t move,
+                                  const int j, double out[3])
+{
+  const double pi = 3.14159265358979323846;
+  const double* cb = chain + kChainPer * i;
+  const double bl = cb[0];
+  const double* C = prev[i - 1];
+  if (i == 1) {
+    const double z = 2.0 * flex_u(seed, stream, move, i, j, 0) - 1.0;
+    const double ph = 2.0 * pi * flex_u(seed, stream, move, i, j, 1);
+    const double r = sqrt(fmax(0.0, 1.0 - z * z));
+    out[0] = C[0] + bl * r * cos(ph);
+    out[1] = C[1] + bl * r * sin(ph);
+    out[2] = C[2] + bl * z;
+    return;
+  }
+  const double* Bp = prev[i - 2];
+  double bc[3] = {C[0] - Bp[0], C[1] - Bp[1], C[2] - Bp[2]};
+  double r = sqrt(bc[0] * bc[0] + bc[1] * bc[1] + bc[2] * bc[2]);
+  bc[0] /= r; bc[1] /= r; bc[2] /= r;
+  double n[3];
+  if (i == 2) {
+    double ax[3] = {0.0, 0.0, 0.0};
+    const double fx = fabs(bc[0]), fy = fabs(bc[1]), fz = fabs(bc[2]);
+    ax[(fx <= fy && fx <= fz) ? 0 : (fy <= fz ? 1 : 2)] = 1.0;
+    n[0] = bc[1] * ax[2] - bc[2] * ax[1];
+    n[1] = bc[2] * ax[0] - bc[0] * ax[2];
+    n[2] = bc[0] * ax[1] - bc[1] * ax[0];
+  } else {
+    const double* A = prev[i - 3];
+    const double ab[3] = {Bp[0] - A[0], Bp[1] - A[1], Bp[2] - A[2]};
+    n[0] = ab[1] * bc[2] - ab[2] * bc[1];
+    n[1] = ab[2] * bc[0] - ab[0] * bc[2];
+    n[2] = ab[0] * bc[1] - ab[1] * bc[0];
+  }
+  r = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  n[0] /= r; n[1] /= r; n[2] /= r;
+  const double m[3] = {n[1] * bc[2] - n[2] * bc[1], n[2] * bc[0] - n[0] * bc[2], n[0] * bc[1] - n[1] * bc[0]};
+  // bend angle
+  const double th0 = cb[1];
+  const double sd = 1.0 / sqrt(beta * cb[2]);
+  double th = th0;
+  for (int a = 0; a < kFlexBendTries; ++a) {
+    const double u1 = flex_u(seed, stream, move, i, j, 3 * a);
+    const double u2 = flex_u(seed, stream, move, i, j, 3 * a + 1);
+    const double ua = flex_u(seed, stream, move, i, j, 3 * a + 2);
+    const double t = th0 + sd * sqrt(-2.0 * log(u1)) * cos(2.0 * pi * u2);
+    if (t > 0.0 && t < pi && ua < sin(t)) {
+      th = t;
+      break;
+    }
+  }
+  double ph;
+  if (i == 2) {
+    ph = 2.0 * pi * flex_u(seed, stream, move, i, j, kFlexBlock - 1);
+  } else {
+    const double* t = cb + 3;
+    const double lb = t[0] + fmin(0.0, 2.0 * t[1]) + fmin(0.0, 2.0 * t[2]) + fmin(0.0, 2.0 * t[3]);
+    ph = pi;
+    for (int a = 0; a < kFlexTorsTries; ++a) {
+      const double p = 2.0 * pi * flex_u(seed, stream, move, i, j, 512 + 2 * a);
+      const double ua = flex_u(seed, stream, move, i, j, 513 + 2 * a);
+      const double u = t[0] + t[1] * (1.0 + cos(p)) + t[2] * (1.0 - cos(2.0 * p)) + t[3] * (1.0 + cos(3.0 * p));
+      if (ua < exp(-beta * (u - lb))) {
+        ph = p;
+        break;
+      }
+    }
+  }
+  const double d0 = -bl * cos(th), d1 = bl * sin(th) * cos(ph), d2 = bl * sin(th) * sin(ph);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) out[k] = C[k] + d0 * bc[k] + d1 * m[k] + d2 * n[k];
+}
+
 }  // namespace graspa

@@ -1225,6 +1225,8 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   const Component& c = b->comps[comp];
   if (K < 1 || K > kMaxTrials || O < 1 || O > kMaxTrials) return fail(GRASPA_ERR_ARG, "widom: 1 <= K,O <= 32");
   if (c.molsize < 1 || c.molsize > kMaxMol) return fail(GRASPA_ERR_UNSUPPORTED, "widom: molecule larger than 8 sites");
+  if (c.d_chain)
+    return fail(GRASPA_ERR_UNSUPPORTED, "widom: flexible chains take Widom moves in the batched driver (graspa_batch_*)");
   if (O * (c.molsize - 1) > kMaxPts) return fail(GRASPA_ERR_UNSUPPORTED, "widom: orientations x chain sites above 64");
   if (b->bc.ntypes > kMaxTypes) return fail(GRASPA_ERR_UNSUPPORTED, "widom: too many pseudo-atom types");
   if (n_insert <= 0) {

@@ -46,6 +46,21 @@ def _split(line: str) -> List[str]:
 
 
 @dataclass
+class Chain:
+    """Bonded model of a flexible linear chain.  No counterpart in the
+    reference, which grows rigid molecules only (read_data.cpp:2104-2116).
+    Entry i belongs to bead i: bond[i] to bead i - 1 (i >= 1), the harmonic
+    bend U = k_bend[i]/2 (theta - theta0[i])^2 of the angle (i-2, i-1, i)
+    (i >= 2) and the dihedral U = t0 + t1 (1 + cos phi) + t2 (1 - cos 2 phi)
+    + t3 (1 + cos 3 phi) of (i-3, i-2, i-1, i) (i >= 3, phi = pi trans).
+    Lengths in A, angles in rad, energies in internal units (10 J/mol)."""
+    bond: np.ndarray
+    theta0: np.ndarray
+    k_bend: np.ndarray
+    torsion: np.ndarray               # (M, 4)
+
+
+@dataclass
 class Component:
     """One adsorbate species (read_data.cpp:2044-2195 and 2298-2696)."""
     name: str
@@ -65,6 +80,8 @@ class Component:
     p_identity: float = 0.0
     p_swap: float = 0.0
     create_molecules: int = 0
+    # flexible linear chain (None: rigid, as every reference molecule)
+    chain: Optional[Chain] = None
     # initial configuration (restart file); (n_mol*M, 3) and MolIDs
     init_pos: Optional[np.ndarray] = None
     init_molid: Optional[np.ndarray] = None

@@ -70,12 +70,18 @@ class EvalShape(C.Structure):
         return {"waves": self.waves, "per_cu": self.per_cu, "framework_lds": self.framework_lds}
 
 
+class ChainModel(C.Structure):
+    """graspa_chain: bonded model of a flexible linear chain."""
+    _fields_ = [("bond", C.c_double * 4), ("theta0", C.c_double * 4), ("k_bend", C.c_double * 4),
+                ("torsion", (C.c_double * 4) * 4)]
+
+
 class Species(C.Structure):
     _fields_ = [("p_translation", C.c_double), ("p_rotation", C.c_double), ("p_widom", C.c_double),
                 ("p_swap", C.c_double), ("mol_fraction", C.c_double), ("fugacity_coeff", C.c_double),
                 ("ideal_rosen", C.c_double), ("excl_intra", C.c_double), ("excl_self", C.c_double),
                 ("max_translation", C.c_double * 3), ("max_rotation", C.c_double * 3),
-                ("pseudo_per_mol", C.c_int64 * 16), ("p_reinsertion", C.c_double), ("p_identity", C.c_double)]
+                ("pseudo_per_mol", C.c_int64 * 32), ("p_reinsertion", C.c_double), ("p_identity", C.c_double)]
 
 
 class MCParams(C.Structure):
@@ -83,7 +89,7 @@ class MCParams(C.Structure):
                 ("n_trial_orient", C.c_int), ("vdw_real_bias", C.c_int), ("use_max_step", C.c_int),
                 ("max_step_per_cycle", C.c_int), ("has_tail", C.c_int), ("tail", _f64p), ("use_tail", _i32p),
                 ("beta", C.c_double), ("pressure", C.c_double), ("seed", C.c_uint), ("pool_size", C.c_int),
-                ("n_pseudo", C.c_int64 * 16)]
+                ("n_pseudo", C.c_int64 * 32)]
 
 
 class BatchBox(C.Structure):
@@ -187,6 +193,8 @@ _SIGS = {
     "graspa_batch_trace": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "graspa_batch_kernel_time": (C.c_int, [C.c_void_p, _f64p, _i64p]),
     "graspa_batch_sync": (C.c_int, [C.c_void_p]),
+    "graspa_batch_max_steps": (C.c_int, [C.c_void_p, C.c_int, _f64p, _f64p]),
+    "graspa_box_set_chain": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "graspa_batch_block_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(BlockStats)]),
     "graspa_batch_set_box": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "graspa_batch_set_cycles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
@@ -310,6 +318,14 @@ class Box:
             _check(lib().graspa_box_add_component(h, c.molsize, cap, _p(tx), _p(tt, _i32p), _p(tq), C.byref(cid)),
                    "add_component")
             self.comp_ids.append(cid.value)
+            ch = getattr(c, "chain", None)
+            if ch is not None:
+                m = ChainModel()
+                for i in range(min(4, c.molsize)):
+                    m.bond[i], m.theta0[i], m.k_bend[i] = float(ch.bond[i]), float(ch.theta0[i]), float(ch.k_bend[i])
+                    for t in range(4):
+                        m.torsion[i][t] = float(ch.torsion[i][t])
+                _check(lib().graspa_box_set_chain(h, cid.value, C.byref(m)), "set_chain")
             if c.init_pos is not None and len(c.init_pos):
                 nmol = len(c.init_pos) // c.molsize
                 q = c.init_charge if c.init_charge is not None else np.tile(c.charges, nmol)
@@ -566,7 +582,7 @@ def make_mc_params(system, box, pool_size=333334):
     p.pool_size = pool_size
     nm = [len(c.init_pos) // c.molsize if (c.init_pos is not None) else 0 for c in s.components]
     n = s.pseudo_counts(nm)
-    for t in range(min(16, len(n))):
+    for t in range(min(32, len(n))):
         p.n_pseudo[t] = int(n[t])
     return p
 
@@ -743,6 +759,13 @@ class Batch:
 
     def sync(self):
         _check(lib().graspa_batch_sync(self.h), "batch_sync")
+
+    def max_steps(self, i):
+        """(max translation, max rotation) of box i, [n_species, 3] each."""
+        ns = len(self.boxes[i].system.components)
+        mt, mr = np.zeros(3 * ns), np.zeros(3 * ns)
+        _check(lib().graspa_batch_max_steps(self.h, i, _p(mt), _p(mr)), "batch_max_steps")
+        return mt.reshape(ns, 3), mr.reshape(ns, 3)
 
     def close(self):
         if getattr(self, "h", None):

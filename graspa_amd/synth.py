@@ -153,10 +153,38 @@ def config1(seed=0, n_fw=1536, L=24.0):
     return s
 
 
-def config2(seed=2000, n_fw=3000, L=30.0, P=1e4):
-    """Rigid united-atom n-butane CBMC GCMC, uncharged, 16 x 8 trials."""
-    return build(n_fw=n_fw, L=L, species=("butane",), seed=seed, T=300.0, P=P, charges=False, K=16, O=8,
-                 probs=dict(translation=1.0, rotation=1.0, swap=2.0))
+# TraPPE-UA n-butane bonded terms: bond 1.54 A, bend 114 deg with k/kB 62,500
+# K/rad^2, dihedral cosine series (0, 355.03, -68.19, 791.32) K
+BUTANE_BEND_K = 62500.0
+BUTANE_TORSION_K = (0.0, 355.03, -68.19, 791.32)
+
+
+def butane_chain():
+    """The flexible n-butane model (bond, bend, dihedral) in internal units."""
+    from .deck import K_TO_INTERNAL, Chain
+    M = 4
+    bond = np.full(M, 1.54)
+    bond[0] = 0.0
+    theta0 = np.zeros(M)
+    theta0[2:] = math.radians(114.0)
+    kb = np.zeros(M)
+    kb[2:] = BUTANE_BEND_K / K_TO_INTERNAL
+    tors = np.zeros((M, 4))
+    tors[3] = np.array(BUTANE_TORSION_K) / K_TO_INTERNAL
+    return Chain(bond=bond, theta0=theta0, k_bend=kb, torsion=tors)
+
+
+def config2(seed=2000, n_fw=3000, L=30.0, P=1e4, flexible=False):
+    """United-atom n-butane CBMC GCMC, uncharged, 16 x 8 trials.  flexible=True
+    grows the chain bead by bead with the TraPPE bend and dihedral (the
+    workload as stated); flexible=False is the rigid all-trans molecule, the
+    form the reference can run."""
+    s = build(n_fw=n_fw, L=L, species=("butane",), seed=seed, T=300.0, P=P, charges=False, K=16, O=8,
+              probs=dict(translation=1.0, rotation=1.0, swap=2.0))
+    if flexible:
+        s.components[0].chain = butane_chain()
+        s.components[0].rigid = False
+    return s
 
 
 def config3_box(b, seed_base=3000, cutoff=12.0):

@@ -38,6 +38,8 @@ extern "C" {
 #define GRASPA_ERR_STATE        -4
 #define GRASPA_ERR_UNSUPPORTED  -5
 
+#define GRASPA_MAX_PSEUDO       32  /* pseudo-atom types a box may have */
+
 #define GRASPA_FP64   0   /* plain FP64 form: pair terms, Ewald and sums in FP64           */
 #define GRASPA_MIXED  1   /* fast form: FP64 pair arithmetic with an FP32-seeded reciprocal and a tabulated
                              erfc(ar)/r, FP32 overlap screen; FP64 accumulation, Ewald and S(k) */
@@ -119,6 +121,28 @@ int  graspa_box_set_framework(graspa_box* box, int n_atoms, const double* xyz, c
  * until a molecule is stored there, as in the reference.  *comp_out is >= 1. */
 int  graspa_box_add_component(graspa_box* box, int molsize, int capacity_molecules, const double* template_xyz,
                               const int* template_type, const double* template_charge, int* comp_out);
+
+/* Bonded model of a flexible linear chain of at most 4 beads, grown bead by
+ * bead in CBMC moves.  No counterpart in the reference, which grows rigid
+ * molecules only (read_data.cpp:2104-2116).  Entry i belongs to bead i. */
+typedef struct {
+  double bond[4];          /* length of the bond to bead i - 1 (i >= 1), Angstrom                    */
+  double theta0[4];        /* equilibrium angle (i-2, i-1, i) (i >= 2), rad                           */
+  double k_bend[4];        /* U = k_bend/2 (theta - theta0)^2, internal energy units per rad^2        */
+  double torsion[4][4];    /* dihedral (i-3 .. i) (i >= 3): t0 + t1 (1 + cos phi) + t2 (1 - cos 2 phi)
+                              + t3 (1 + cos 3 phi), phi = pi trans, internal energy units            */
+} graspa_chain;
+/* Makes component comp a flexible chain (chain != NULL) or rigid again
+ * (NULL).  A flexible component has at most 4 beads (no intramolecular pair
+ * beyond 1-4) and no charges.  Its insertions, deletions and Widom moves in
+ * the batched driver (graspa_batch_*) grow bead i from n_trial_orient
+ * positions drawn from the bend and dihedral Boltzmann densities given the
+ * beads placed (deletion: trial 0 is the molecule's own bead) and select one
+ * on the Boltzmann factor of its external energy; W is the product of the
+ * first bead's and each step's trial averages.  Translation and rotation move
+ * the chain rigidly.  The parity driver (graspa_mc_*), the Widom pipeline and
+ * reinsertion / identity-swap moves refuse flexible components. */
+int  graspa_box_set_chain(graspa_box* box, int comp, const graspa_chain* chain);
 
 /* Replaces all molecules of a component (RestartFileParser, read_data.cpp:3000-3221). */
 int  graspa_box_set_molecules(graspa_box* box, int comp, int n_mol, const double* xyz, const double* charge,
@@ -225,7 +249,7 @@ typedef struct {
   double mol_fraction, fugacity_coeff, ideal_rosen;
   double excl_intra, excl_self;                        /* rigid exclusion (ewald_preparation.h:351-366) */
   double max_translation[3], max_rotation[3];
-  int64_t pseudo_per_mol[16];                          /* pseudo-atom counts of one molecule            */
+  int64_t pseudo_per_mol[GRASPA_MAX_PSEUDO];                          /* pseudo-atom counts of one molecule            */
   double p_reinsertion;            /* cumulative, between p_widom and p_swap (a value <= p_widom: none)  */
   double p_identity;               /* cumulative, between p_reinsertion and p_swap (<= p_reinsertion: none) */
 } graspa_species;
@@ -242,7 +266,7 @@ typedef struct {
   double  beta, pressure;          /* internal units                                          */
   unsigned seed;                   /* srand seed (RandomSeed)                                  */
   int     pool_size;               /* device random pool entries (main.cpp:145: 333334)       */
-  int64_t n_pseudo[16];            /* pseudo-atom counts of the whole box                      */
+  int64_t n_pseudo[GRASPA_MAX_PSEUDO];            /* pseudo-atom counts of the whole box                      */
 } graspa_mc_params;
 
 /* One record per MC step. */
@@ -372,6 +396,11 @@ int  graspa_batch_progress(graspa_batch* batch, int64_t* moves, int64_t* cycles,
 int  graspa_batch_kernel_time(graspa_batch* batch, double* total_ms, int64_t* launches);
 /* Copies the device state of every box back to its graspa_box. */
 int  graspa_batch_sync(graspa_batch* batch);
+/* Current maximum translation and rotation steps of box i, [n_species][3]
+ * each (Components::MaxTranslation / MaxRotation as adapted by
+ * Update_Max_Translation / Update_Max_Rotation, mc_utilities.h:612-665, every
+ * 500 cycles); either pointer may be NULL. */
+int  graspa_batch_max_steps(graspa_batch* batch, int i, double* max_translation, double* max_rotation);
 
 /* Block statistics of the current production run of a box, the data behind
  * the reference's block averages (axpy.cu:420-453, 176-185; Gather_Averages_*

@@ -61,19 +61,21 @@ class OrMC(C.Structure):
                 ("p_swap", C.c_double * 8), ("mol_fraction", C.c_double * 8), ("fugacity_coeff", C.c_double * 8),
                 ("ideal_rosen", C.c_double * 8), ("excl_intra", C.c_double * 8), ("excl_self", C.c_double * 8),
                 ("max_translation", (C.c_double * 3) * 8), ("max_rotation", (C.c_double * 3) * 8),
-                ("pseudo_per_mol", (C.c_int64 * 16) * 8), ("has_charge", C.c_int * 8),
+                ("pseudo_per_mol", (C.c_int64 * 32) * 8), ("has_charge", C.c_int * 8),
                 ("n_trial_pos", C.c_int), ("n_trial_orient", C.c_int), ("vdw_real_bias", C.c_int),
                 ("use_max_step", C.c_int), ("max_step_per_cycle", C.c_int), ("has_tail", C.c_int),
                 ("temperature", C.c_double), ("pressure_pa", C.c_double), ("beta", C.c_double),
                 ("pressure", C.c_double), ("seed", C.c_uint), ("pool_size", C.c_int), ("production", C.c_int),
-                ("n_pseudo", C.c_int64 * 16), ("total_molecules", C.c_int64),
+                ("n_pseudo", C.c_int64 * 32), ("total_molecules", C.c_int64),
                 ("trans_total", C.c_int64 * 8), ("trans_acc", C.c_int64 * 8), ("rot_total", C.c_int64 * 8),
                 ("rot_acc", C.c_int64 * 8), ("ins_total", C.c_int64 * 8), ("ins_acc", C.c_int64 * 8),
                 ("del_total", C.c_int64 * 8), ("del_acc", C.c_int64 * 8), ("widom_total", C.c_int64 * 8),
                 ("widom_rosen_sum", C.c_double * 8), ("widom_rosen_n", C.c_double * 8), ("running", OrEnergy),
                 ("p_reinsertion", C.c_double * 8), ("reins_total", C.c_int64 * 8), ("reins_acc", C.c_int64 * 8),
                 ("p_identity", C.c_double * 8), ("idsw_total", (C.c_int64 * 8) * 8),
-                ("idsw_acc", (C.c_int64 * 8) * 8)]
+                ("idsw_acc", (C.c_int64 * 8) * 8),
+                ("flexible", C.c_int * 8), ("bond", (C.c_double * 8) * 8), ("bend_theta0", (C.c_double * 8) * 8),
+                ("bend_k", (C.c_double * 8) * 8), ("torsion", ((C.c_double * 4) * 8) * 8)]
 
 
 class OrWidomOut(C.Structure):
@@ -112,6 +114,8 @@ def lib():
         L.or_set_threads.argtypes = [C.c_int]
         L.or_set_threads.restype = C.c_int
         L.or_rotate_quaternion.argtypes = [_f64p, C.c_double, C.c_double, C.c_double]
+        L.or_flex_trial.argtypes = [C.POINTER(OrMC), C.c_int, C.c_int, _f64p, C.c_uint64, C.c_uint64, C.c_uint64,
+                                    C.c_int, _f64p]
         L.or_rand_uniform.restype = C.c_double
         L.or_srand.argtypes = [C.c_uint]
         L.or_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
@@ -300,6 +304,15 @@ def make_mc(system, oracle_sys, production=1, pool_size=333334):
         for t in c.types:
             mc.pseudo_per_mol[k][int(t)] += 1
         mc.has_charge[k] = int(c.has_charge)
+        ch = getattr(c, "chain", None)
+        if ch is not None:
+            mc.flexible[k] = 1
+            for i in range(c.molsize):
+                mc.bond[k][i] = float(ch.bond[i])
+                mc.bend_theta0[k][i] = float(ch.theta0[i])
+                mc.bend_k[k][i] = float(ch.k_bend[i])
+                for t in range(4):
+                    mc.torsion[k][i][t] = float(ch.torsion[i][t])
     mc.n_trial_pos = s.n_trial_pos
     mc.n_trial_orient = s.n_trial_orient
     mc.vdw_real_bias = int(s.vdw_real_bias)
@@ -315,7 +328,7 @@ def make_mc(system, oracle_sys, production=1, pool_size=333334):
     mc.production = production
     nm = [oracle_sys.comps[1 + i].n_mol for i in range(len(s.components))]
     n = s.pseudo_counts(nm)
-    for t in range(min(16, len(n))):
+    for t in range(min(32, len(n))):
         mc.n_pseudo[t] = int(n[t])
     mc.total_molecules = 1 + sum(nm)
     return mc

@@ -615,6 +615,119 @@ static double sum_exp(const trial_set* ts)
 }
 
 /* ------------------------------------------------------------------------ */
+/* Flexible linear chains (no reference counterpart: the reference grows     */
+/* rigid molecules only, read_data.cpp:2104-2116)                            */
+/* ------------------------------------------------------------------------ */
+
+/* Draws of bead i, trial j of a batched move: the block starting at
+ * D_FLEX + ((i - 1) * 32 + j) * 1024.  Bead 1: draws 0, 1 (cos of the polar
+ * angle, azimuth).  Bend angle (beads 2, 3): attempt a takes draws 3a, 3a + 1
+ * (a Box-Muller normal) and 3a + 2 (acceptance), a < 160.  Dihedral (bead 3):
+ * attempt a takes 512 + 2a (angle) and 513 + 2a (acceptance), a < 250.
+ * Azimuth of bead 2: draw 1023.  A sampler that runs out of attempts returns
+ * theta0 (bend) or pi (dihedral); for the butane model at 200-400 K that has
+ * probability below 1e-30.  The uniform of step i's selection is draw
+ * D_FLEX - 8 + i. */
+enum { D_FLEX = 4096, FLEX_BLOCK = 1024, FLEX_BEND_TRIES = 160, FLEX_TORS_TRIES = 250 };
+
+static double flex_u(uint64_t seed, uint64_t stream, uint64_t move, int i, int j, int d)
+{
+  return or_philox_uniform(seed, stream, move, (uint32_t)(D_FLEX + ((i - 1) * 32 + j) * FLEX_BLOCK + d));
+}
+
+/* Bend angle from sin(theta) exp(-beta k/2 (theta - theta0)^2) on (0, pi):
+ * a normal proposal N(theta0, 1/sqrt(beta k)) carries the Boltzmann factor,
+ * so accepting it with probability sin(theta) leaves the density above. */
+static double flex_bend(const or_mc* mc, int comp, int i, uint64_t seed, uint64_t stream, uint64_t move, int j)
+{
+  const double pi = 3.14159265358979323846;
+  const double th0 = mc->bend_theta0[comp][i];
+  const double sd = 1.0 / sqrt(mc->beta * mc->bend_k[comp][i]);
+  for (int a = 0; a < FLEX_BEND_TRIES; a++) {
+    const double u1 = flex_u(seed, stream, move, i, j, 3 * a);
+    const double u2 = flex_u(seed, stream, move, i, j, 3 * a + 1);
+    const double ua = flex_u(seed, stream, move, i, j, 3 * a + 2);
+    const double th = th0 + sd * sqrt(-2.0 * log(u1)) * cos(2.0 * pi * u2);
+    if (th > 0.0 && th < pi && ua < sin(th)) return th;
+  }
+  return th0;
+}
+
+/* Dihedral from exp(-beta U(phi)) on [0, 2 pi): uniform proposals accepted
+ * with exp(-beta (U - U_lb)), U_lb = t0 + min(0, 2 t1) + min(0, 2 t2) +
+ * min(0, 2 t3) <= U everywhere. */
+static double flex_dihedral(const or_mc* mc, int comp, int i, uint64_t seed, uint64_t stream, uint64_t move,
+                            int j)
+{
+  const double pi = 3.14159265358979323846;
+  const double* t = mc->torsion[comp][i];
+  const double lb = t[0] + fmin(0.0, 2.0 * t[1]) + fmin(0.0, 2.0 * t[2]) + fmin(0.0, 2.0 * t[3]);
+  for (int a = 0; a < FLEX_TORS_TRIES; a++) {
+    const double phi = 2.0 * pi * flex_u(seed, stream, move, i, j, 512 + 2 * a);
+    const double ua = flex_u(seed, stream, move, i, j, 513 + 2 * a);
+    const double u = t[0] + t[1] * (1.0 + cos(phi)) + t[2] * (1.0 - cos(2.0 * phi)) + t[3] * (1.0 + cos(3.0 * phi));
+    if (ua < exp(-mc->beta * (u - lb))) return phi;
+  }
+  return pi;
+}
+
+static void unit3(double v[3])
+{
+  const double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] /= r; v[1] /= r; v[2] /= r;
+}
+
+static void cross3(const double a[3], const double b[3], double out[3])
+{
+  out[0] = a[1] * b[2] - a[2] * b[1];
+  out[1] = a[2] * b[0] - a[0] * b[2];
+  out[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* Bead i from beads i-1 (C), i-2 (B), i-3 (A) and (bond, bend, dihedral):
+ * C + bond (-cos th bc + sin th (cos phi m + sin phi n)), bc = (C - B)/|C - B|,
+ * n = unit(AB x bc), m = n x bc; phi is then the IUPAC dihedral A-B-C-D.
+ * Bead 2 has no A: n is any unit vector normal to bc (built from the axis
+ * least aligned with bc) and phi a uniform azimuth. */
+void or_flex_trial(const or_mc* mc, int comp, int i, const double* prev, uint64_t seed, uint64_t stream,
+                   uint64_t move, int j, double out[3])
+{
+  const double pi = 3.14159265358979323846;
+  const double bl = mc->bond[comp][i];
+  const double* C = prev + 3 * (i - 1);
+  if (i == 1) {
+    const double z = 2.0 * flex_u(seed, stream, move, i, j, 0) - 1.0;
+    const double ph = 2.0 * pi * flex_u(seed, stream, move, i, j, 1);
+    const double r = sqrt(fmax(0.0, 1.0 - z * z));
+    out[0] = C[0] + bl * r * cos(ph);
+    out[1] = C[1] + bl * r * sin(ph);
+    out[2] = C[2] + bl * z;
+    return;
+  }
+  const double* B = prev + 3 * (i - 2);
+  double bc[3] = {C[0] - B[0], C[1] - B[1], C[2] - B[2]};
+  unit3(bc);
+  double n[3], m[3];
+  if (i == 2) {
+    double ax[3] = {0.0, 0.0, 0.0};
+    const double fx = fabs(bc[0]), fy = fabs(bc[1]), fz = fabs(bc[2]);
+    ax[(fx <= fy && fx <= fz) ? 0 : (fy <= fz ? 1 : 2)] = 1.0;
+    cross3(bc, ax, n);
+  } else {
+    const double* A = prev + 3 * (i - 3);
+    const double ab[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    cross3(ab, bc, n);
+  }
+  unit3(n);
+  cross3(n, bc, m);
+  const double th = flex_bend(mc, comp, i, seed, stream, move, j);
+  const double ph = (i == 2) ? 2.0 * pi * flex_u(seed, stream, move, i, j, FLEX_BLOCK - 1)
+                             : flex_dihedral(mc, comp, i, seed, stream, move, j);
+  const double d0 = -bl * cos(th), d1 = bl * sin(th) * cos(ph), d2 = bl * sin(th) * sin(ph);
+  for (int k = 0; k < 3; k++) out[k] = C[k] + d0 * bc[k] + d1 * m[k] + d2 * n[k];
+}
+
+/* ------------------------------------------------------------------------ */
 /* Batched Widom insertions (counter-based streams)                          */
 /* ------------------------------------------------------------------------ */
 
@@ -962,7 +1075,43 @@ static void cbmc_move(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
   int mol_t[64];
   mol_pos[0] = sp[3 * fbt]; mol_pos[1] = sp[3 * fbt + 1]; mol_pos[2] = sp[3 * fbt + 2];
   mol_q[0] = sq[fbt]; mol_t[0] = st[fbt];
-  if (chain > 0) {
+  if (chain > 0 && mc->flexible[comp]) {
+    /* flexible chain: bead i of O trial positions from the bonded
+     * distributions given the beads already placed (deletion: trial 0 is the
+     * molecule's own bead), selected on the Boltzmann factor of its external
+     * energy; the step weight is that factor's trial average */
+    for (int i = 1; i < M; i++) {
+      for (int t = 0; t < O; t++) {
+        if (del && t == 0) {
+          for (int d = 0; d < 3; d++) sp[3 * t + d] = A->pos[3 * (start + i) + d];
+        } else {
+          or_flex_trial(mc, comp, i, mol_pos, x->seed, x->stream, x->move, t, &sp[3 * t]);
+        }
+        st[t] = A->type[start + i]; sq[t] = A->charge[start + i];
+      }
+      or_site_energies(b, x->comps, x->ncomp, O, sp, st, sq, comp, sel_molid, -1, -1, s4, sov);
+      sum_trials(s4, sov, O, 1, beta, mc->vdw_real_bias, &ts);
+      int osel = 0;
+      double rc;
+      if (ts.n_keep == 0) goto fail;
+      if (!del) {
+        osel = select_trial(ts.rosen, ts.n_keep, mc_draw(x, D_FLEX - 8 + i));
+        rc = sum_exp(&ts);
+        if (rc < 1e-150) goto fail;
+      } else {
+        rc = sum_exp(&ts);
+      }
+      double Wc = rc / (double)O;
+      if (!mc->vdw_real_bias) Wc *= exp(-beta * (ts.e4[osel][1] + ts.e4[osel][3]));
+      W *= Wc;
+      if (W <= 1e-150) goto fail;
+      int ot = ts.keep[osel];
+      en.hg_vdw += ts.e4[osel][0]; en.hg_real += ts.e4[osel][1];
+      en.gg_vdw += ts.e4[osel][2]; en.gg_real += ts.e4[osel][3];
+      for (int d = 0; d < 3; d++) mol_pos[3 * i + d] = sp[3 * ot + d];
+      mol_q[i] = sq[ot]; mol_t[i] = st[ot];
+    }
+  } else if (chain > 0) {
     /* Widom_Move_Chain_PARTIAL (mc_widom.h:509-614), get_random_trial_orientation
      * (mc_widom.h:215-303): template vectors from the component's first
      * molecule slot as it is now. */
@@ -1068,7 +1217,7 @@ static void cbmc_move(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
     }
     A->n_mol++;
     mc->total_molecules++;
-    for (int t = 0; t < b->ntypes && t < 16; t++) mc->n_pseudo[t] += mc->pseudo_per_mol[comp][t];
+    for (int t = 0; t < b->ntypes && t < OR_MAX_PSEUDO; t++) mc->n_pseudo[t] += mc->pseudo_per_mol[comp][t];
     if (!x->create) mc->ins_acc[comp]++;
   } else {
     int last = (int)((N - 1) * M);
@@ -1081,7 +1230,7 @@ static void cbmc_move(mc_ctx* x, int comp, int mol, int move, or_trace* tr)
     }
     A->n_mol--;
     mc->total_molecules--;
-    for (int t = 0; t < b->ntypes && t < 16; t++) mc->n_pseudo[t] -= mc->pseudo_per_mol[comp][t];
+    for (int t = 0; t < b->ntypes && t < OR_MAX_PSEUDO; t++) mc->n_pseudo[t] -= mc->pseudo_per_mol[comp][t];
     mc->del_acc[comp]++;
     /* take_negative */
     en.hg_vdw = -en.hg_vdw; en.hg_real = -en.hg_real; en.gg_vdw = -en.gg_vdw; en.gg_real = -en.gg_real;
@@ -1442,8 +1591,8 @@ static void identity_swap_move(mc_ctx* x, or_trace* tr)
   }
   /* ---- tail ---- */
   if (mc->has_tail) {
-    int64_t d[16];
-    for (int t = 0; t < 16; t++) d[t] = mc->pseudo_per_mol[nc][t] - mc->pseudo_per_mol[oc][t];
+    int64_t d[OR_MAX_PSEUDO];
+    for (int t = 0; t < OR_MAX_PSEUDO; t++) d[t] = mc->pseudo_per_mol[nc][t] - mc->pseudo_per_mol[oc][t];
     de.tail = or_tail_delta(b, mc->n_pseudo, d, 1);
   }
   Wn *= exp(-beta * de.tail);
@@ -1478,7 +1627,7 @@ static void identity_swap_move(mc_ctx* x, or_trace* tr)
       AN->molid[loc + i] = nnew;
     }
     AN->n_mol++;
-    for (int t = 0; t < b->ntypes && t < 16; t++) mc->n_pseudo[t] += mc->pseudo_per_mol[nc][t] - mc->pseudo_per_mol[oc][t];
+    for (int t = 0; t < b->ntypes && t < OR_MAX_PSEUDO; t++) mc->n_pseudo[t] += mc->pseudo_per_mol[nc][t] - mc->pseudo_per_mol[oc][t];
   } else {
     for (int i = 0; i < Mo; i++)
       for (int d = 0; d < 3; d++) AO->pos[3 * (so + i) + d] = new_pos[3 * i + d];
@@ -1504,7 +1653,9 @@ int or_mc_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
                  double* sk_ads, double* sk_fw, int first_cycle, int n_cycles,
                  or_trace* trace, int max_trace)
 {
-  if (!g_pool) return -1;
+  if (!g_pool || b->ntypes > OR_MAX_PSEUDO) return -1;
+  for (int c = 1; c < ncomp && c < 8; c++)
+    if (mc->flexible[c]) return -1;   /* flexible chains: batched form only */
   mc_ctx x;
   memset(&x, 0, sizeof(x));
   x.b = b; x.comps = comps; x.ncomp = ncomp; x.mc = mc; x.sk_ads = sk_ads; x.sk_fw = sk_fw;
@@ -1573,7 +1724,7 @@ int or_mc_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
 int or_mc_create_molecules(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp, double* sk_ads, double* sk_fw,
                            int comp, int n, int64_t max_attempts, or_energy* de)
 {
-  if (!g_pool || comp < 1 || comp >= ncomp) return -1;
+  if (!g_pool || comp < 1 || comp >= ncomp || mc->flexible[comp] || b->ntypes > OR_MAX_PSEUDO) return -1;
   mc_ctx x;
   memset(&x, 0, sizeof(x));
   x.b = b; x.comps = comps; x.ncomp = ncomp; x.mc = mc; x.sk_ads = sk_ads; x.sk_fw = sk_fw; x.create = 1;
@@ -1608,11 +1759,18 @@ int or_batch_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
   memset(&x, 0, sizeof(x));
   x.b = b; x.comps = comps; x.ncomp = ncomp; x.mc = mc; x.sk_ads = sk_ads; x.sk_fw = sk_fw;
   x.philox = 1; x.seed = seed; x.stream = stream;
-  x.sk_tmp = (double*)calloc((size_t)2 * or_nvec(b), sizeof(double));
+  if (b->ntypes > OR_MAX_PSEUDO) return -1;
   int valid[8], n_valid = 0;
-  for (int c = 1; c < ncomp && c < 8; c++)
+  for (int c = 1; c < ncomp && c < 8; c++) {
     if (mc->p_swap[c] > 1e-10) valid[n_valid++] = c;
-  if (n_valid == 0) { free(x.sk_tmp); return -1; }
+    /* flexible chains: at most 4 beads (no intramolecular pair beyond 1-4),
+     * uncharged, grown by insertion / deletion / Widom only */
+    if (mc->flexible[c] && (comps[c].molsize > 4 || mc->has_charge[c] || mc->p_reinsertion[c] > mc->p_widom[c] ||
+                            mc->p_identity[c] > mc->p_reinsertion[c]))
+      return -1;
+  }
+  if (n_valid == 0) return -1;
+  x.sk_tmp = (double*)calloc((size_t)2 * or_nvec(b), sizeof(double));
   int nt = 0;
   for (int cyc = first_cycle; cyc < first_cycle + n_cycles; cyc++) {
     int64_t steps = 20;

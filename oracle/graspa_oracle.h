@@ -28,6 +28,8 @@
 
 #include <stdint.h>
 
+#define OR_MAX_PSEUDO 32   /* pseudo-atom types of a box */
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -142,7 +144,7 @@ typedef struct {
   double  mol_fraction[8], fugacity_coeff[8], ideal_rosen[8];
   double  excl_intra[8], excl_self[8];
   double  max_translation[8][3], max_rotation[8][3];
-  int64_t pseudo_per_mol[8][16];     /* pseudo-atom counts per molecule, by type */
+  int64_t pseudo_per_mol[8][OR_MAX_PSEUDO];     /* pseudo-atom counts per molecule, by type */
   int     has_charge[8];
   int     n_trial_pos, n_trial_orient;
   int     vdw_real_bias;
@@ -153,7 +155,7 @@ typedef struct {
   int     pool_size;
   int     production;                 /* 1 = record Widom statistics */
   /* running state */
-  int64_t n_pseudo[16];
+  int64_t n_pseudo[OR_MAX_PSEUDO];
   int64_t total_molecules;            /* includes the framework molecule */
   int64_t trans_total[8], trans_acc[8], rot_total[8], rot_acc[8];
   int64_t ins_total[8], ins_acc[8], del_total[8], del_acc[8], widom_total[8];
@@ -168,6 +170,15 @@ typedef struct {
    * from component [old] to component [new] (IdentitySwap_Total_TO / _Acc_TO) */
   double  p_identity[8];
   int64_t idsw_total[8][8], idsw_acc[8][8];
+  /* flexible linear chains (no reference counterpart: the reference accepts
+   * rigid molecules only, read_data.cpp:2104-2116).  Bead i >= 1 is bonded to
+   * bead i - 1 at bond[c][i]; the angle (i-2, i-1, i) has the harmonic bend
+   * U = bend_k/2 (theta - bend_theta0)^2 (i >= 2); the dihedral (i-3 .. i)
+   * has U = t0 + t1 (1 + cos phi) + t2 (1 - cos 2 phi) + t3 (1 + cos 3 phi)
+   * with phi = 180 degrees trans (i >= 3); energies in internal units.
+   * Batched form only. */
+  int     flexible[8];
+  double  bond[8][8], bend_theta0[8][8], bend_k[8][8], torsion[8][8][4];
 } or_mc;
 
 /* Allocates the device-style random pool and seeds it (data_struct.h:1287-1346).
@@ -200,6 +211,15 @@ int  or_mc_create_molecules(or_mc* mc, const or_box* b, or_atoms* comps, int nco
 int  or_batch_cycles(or_mc* mc, const or_box* b, or_atoms* comps, int ncomp,
                      double* sk_ads, double* sk_fw, uint64_t seed, uint64_t stream, int64_t* move_index,
                      int first_cycle, int n_cycles, or_trace* trace, int max_trace);
+
+/* One trial position of bead i (1 <= i < molsize <= 4) of flexible component
+ * comp given beads 0 .. i-1 in prev, from the batched form's draws of trial j
+ * of move `move` of stream (seed, stream): bead 1 uniform on the bond
+ * sphere, bead 2 with its bend angle from sin(theta) exp(-beta U_bend) and a
+ * uniform azimuth, bead 3 with bend angle and dihedral from their Boltzmann
+ * densities (the two are independent given beads 0..2). */
+void or_flex_trial(const or_mc* mc, int comp, int i, const double* prev, uint64_t seed, uint64_t stream,
+                   uint64_t move, int j, double out[3]);
 
 /* ---- batched Widom with per-insertion counter-based streams ---- */
 typedef struct {
