@@ -6,7 +6,9 @@ orientations each) into a synthetic 1,536-atom framework in a 24 A cubic box,
 empty box, Ewald alpha 0.265 / kmax 7, in the engine's fast form (FP64 pair
 arithmetic with an FP32-seeded reciprocal and a tabulated erfc(ar)/r, an FP32
 overlap screen, FP64 Ewald).  value = insertions/s over K timed steps with all
-inputs resident in HBM.
+inputs resident in HBM; a step is one replay of the batch captured as a HIP
+graph (graspa_widom_graph_launch), the same batches as queued enqueues, which
+are timed beside it (--no-graph makes them the headline).
 
 Also reported on the same line: the plain FP64 form of the same batch, a
 lower-density variant of the framework (750 atoms) where most constructions
@@ -71,11 +73,14 @@ def load_traffic(kernel_prefix):
     return d.get("traffic_bytes"), d
 
 
-def run_widom(system, precision, n_insert, steps, warmup, seed=2024, timing=True, autotune=False, queued=False):
+def run_widom(system, precision, n_insert, steps, warmup, seed=2024, timing=True, autotune=False, queued=False,
+              graph=False):
     """queued=False: one synchronous graspa_widom_batch per step (summary back
     on the host after each batch).  queued=True: the steps are queued with
     graspa_widom_enqueue and their summaries summed on the device, read once
-    after the last step (graspa_widom_totals)."""
+    after the last step (graspa_widom_totals).  graph=True (with queued): the
+    batch is captured once as a HIP graph before the timed region and each
+    step is one replay (graspa_widom_graph_launch), the same batches."""
     from graspa_amd import engine
     setup = {}
     t0 = time.perf_counter()
@@ -112,6 +117,10 @@ def run_widom(system, precision, n_insert, steps, warmup, seed=2024, timing=True
                     slowest_ms=max(ms for _, ms in tried))
     for w in range(warmup):
         box.widom_batch(1, n_insert, seed=seed, first_index=0, excl=excl, results=False)
+    if graph:
+        t0 = time.perf_counter()
+        box.widom_graph(1, n_insert, seed=seed, first_index=n_insert, excl=excl)
+        setup["graph_capture_ms"] = (time.perf_counter() - t0) * 1e3
     if timing:
         box.timing_enable("widom", True)
     box.sync()
@@ -121,6 +130,9 @@ def run_widom(system, precision, n_insert, steps, warmup, seed=2024, timing=True
     summ = None
     tot = dict(sum_w=0.0, n=0.0, n_first_bead=0.0, n_ok=0.0, sum_we=0.0, screened_pairs=0.0, evaluated_pairs=0.0)
     for k in range(steps):
+        if queued and graph:
+            box.widom_graph_launch(1)
+            continue
         if queued:
             box.widom_enqueue(1, n_insert, seed=seed, first_index=(k + 1) * n_insert, excl=excl)
             continue
@@ -299,19 +311,125 @@ def run_batch_single(system, cycles, precision, warm_cycles=0, capacity=1024, se
     return out
 
 
-def run_served(n_jobs, rate, slots, precision, cycles_lo=50, cycles_hi=5000, seed=4000):
+# The reference's CO2-in-MFI run (Examples/CO2-MFI/output.txt:171-212): 5,000
+# initialization cycles, 104,868 moves in 7.967520 s of work
+REFERENCE_CO2_MFI = dict(moves=24719 + 25130 + 13735 + 13766 + 27518, seconds=7.967520, final_molecules=26)
+
+
+def run_co2_mfi(n_seeds=16, cycles=5000):
+    """The reference's CO2-MFI deck (MFI 2x2x2 with CIF charges, rigid CO2,
+    298 K, 1e4 Pa, translation : rotation : reinsertion : swap 1:1:1:1) on the
+    device-resident driver: one box, then n_seeds independent boxes in one
+    launch, in both forms; moves/s, per-cycle latency, final loading and
+    acceptance ratios beside the reference's own run."""
+    import numpy as np
+    from graspa_amd import deck, engine
+    s = deck.load_system(os.path.join(ROOT, "tests", "golden", "co2_mfi.npz"))
+    ref = REFERENCE_CO2_MFI
+    out = dict(deck="Examples/CO2-MFI (tests/golden/co2_mfi.npz)", cycles=cycles,
+               reference=dict(ref, moves_per_s=ref["moves"] / ref["seconds"]))
+
+    def leg(n, prec):
+        boxes = []
+        for _ in range(n):
+            b = engine.Box(s, precision=prec, capacity=256)
+            b.init_ewald()
+            boxes.append(b)
+        bt = engine.Batch(boxes, [101 + i for i in range(n)], list(range(n)), cycle_cap=cycles)
+        t0 = time.perf_counter()
+        moves = bt.run(cycles, production=False)
+        t1 = time.perf_counter()
+        lat = np.concatenate([bt.cycle_times(i, cycles) for i in range(n)])
+        st = [bt.stats(i) for i in range(n)]
+        c = np.array([x.counters[0] for x in st], dtype=np.float64).sum(0)
+        r = dict(boxes=n, moves=int(moves), seconds=t1 - t0, moves_per_s=moves / (t1 - t0),
+                 cycle_p50_ms=float(np.percentile(lat, 50) * 1e3), cycle_p99_ms=float(np.percentile(lat, 99) * 1e3),
+                 final_molecules_mean=float(np.mean([x.n_mol[0] for x in st])),
+                 acceptance=dict(translation=c[1] / max(c[0], 1), rotation=c[3] / max(c[2], 1),
+                                 insertion=c[5] / max(c[4], 1), deletion=c[7] / max(c[6], 1),
+                                 reinsertion=c[10] / max(c[9], 1)))
+        bt.close()
+        for b in boxes:
+            b.close()
+        return r
+
+    # the reference-order parity driver (host-driven, one move at a time, plain
+    # FP64: the reference's own form), over the first 1,000 cycles
+    out["parity_driver_fp64_1000_cycles"] = run_gcmc(s, 1000)
+    out["parity_driver_fp64_1000_cycles"]["over_reference"] = (
+        out["parity_driver_fp64_1000_cycles"]["moves_per_s"] / out["reference"]["moves_per_s"])
+    for name, prec in (("fp64", engine.GRASPA_FP64), ("mixed", engine.GRASPA_MIXED)):
+        one = leg(1, prec)
+        many = leg(n_seeds, prec)
+        one["over_reference"] = one["moves_per_s"] / out["reference"]["moves_per_s"]
+        many["over_reference"] = many["moves_per_s"] / out["reference"]["moves_per_s"]
+        out[name + "_one_box"] = one
+        out[name + "_%d_boxes" % n_seeds] = many
+    return out
+
+
+def run_batch_steady(n_boxes, cycles, precision, window=100, capacity=2048):
+    """The 256-box distribution run for its stated cycles from empty boxes:
+    moves/s over the whole run and over its last half (steady loading), the
+    per-cycle latency of every box's cycles in the last half, loadings and
+    the spread of the boxes' busy time (the launch lasts as long as its
+    slowest box)."""
+    import numpy as np
+    from graspa_amd import engine, synth
+    systems = [synth.config3_box(b) for b in range(n_boxes)]
+    boxes = []
+    for s in systems:
+        bx = engine.Box(s, precision=precision, capacity=capacity)
+        bx.init_ewald()
+        boxes.append(bx)
+    bt = engine.Batch(boxes, [3000 + b for b in range(n_boxes)], list(range(n_boxes)), cycle_cap=window)
+    done, moves, secs, rows = 0, 0, 0.0, []
+    while done < cycles:
+        n = min(window, cycles - done)
+        k0, _ = bt.kernel_time()
+        t0 = time.perf_counter()
+        mv = bt.run(n, production=True)
+        t1 = time.perf_counter()
+        k1, _ = bt.kernel_time()
+        done += n
+        rows.append((done, mv, t1 - t0, k1 - k0))
+    half = [r for r in rows if r[0] > cycles // 2]
+    lat = np.concatenate([bt.cycle_times(i, window) for i in range(n_boxes)])
+    nm = np.array([bt.stats(i).n_mol.sum() for i in range(n_boxes)])
+    per_box = np.array([bt.stats(i).moves for i in range(n_boxes)])
+    out = dict(boxes=n_boxes, cycles_per_box=cycles, moves=int(sum(r[1] for r in rows)),
+               seconds=float(sum(r[2] for r in rows)), kernel_ms=float(sum(r[3] for r in rows)),
+               moves_per_s=sum(r[1] for r in rows) / sum(r[2] for r in rows),
+               steady_moves_per_s=sum(r[1] for r in half) / sum(r[2] for r in half),
+               steady_cycle_p50_ms=float(np.percentile(lat, 50) * 1e3),
+               steady_cycle_p99_ms=float(np.percentile(lat, 99) * 1e3),
+               mean_molecules_per_box=float(nm.mean()), max_molecules_per_box=int(nm.max()),
+               moves_per_box_max=int(per_box.max()), moves_per_box_median=float(np.median(per_box)))
+    bt.close()
+    for bx in boxes:
+        bx.close()
+    return out
+
+
+def run_served(n_jobs, rate, slots, precision, cycles_lo=50, cycles_hi=5000, seed=4000, time_slice_us=0.0,
+               moves_per_launch=256):
     """Served stream: Poisson arrivals of config-3 style jobs, continuous
-    batching; per-cycle latency over every job's cycles and aggregate moves/s."""
+    batching; per-cycle latency over every job's cycles and aggregate moves/s.
+    time_slice_us > 0: every box runs about that long per launch at its own
+    pace (graspa_batch_set_time_slice) instead of moves_per_launch moves."""
     import numpy as np
     from graspa_amd import serve
     jobs = serve.poisson_jobs(n_jobs, rate, seed=seed, cycles_lo=cycles_lo, cycles_hi=cycles_hi)
-    srv = serve.Server(n_slots=slots, precision=precision, moves_per_launch=256, cycle_cap=cycles_hi)
+    srv = serve.Server(n_slots=slots, precision=precision, moves_per_launch=moves_per_launch, cycle_cap=cycles_hi,
+                       time_slice_us=time_slice_us)
     done = srv.run(jobs)
     srv.close()
     lat = np.concatenate([j.cycle_seconds for j in done])
     moves = sum(j.stats.moves for j in done)
     wait = np.array([j.admitted - j.arrival for j in done])
     return dict(jobs=n_jobs, arrival_rate_per_s=rate, slots=slots, cycles_range=[cycles_lo, cycles_hi],
+                time_slice_us=time_slice_us, moves_per_launch=moves_per_launch,
+                peak_device_bytes=int(srv.peak_bytes),
                 total_cycles=int(sum(j.cycles for j in done)), moves=int(moves), seconds=srv.elapsed,
                 moves_per_s=moves / srv.elapsed, launches=srv.launches,
                 cycle_p50_ms=float(np.percentile(lat, 50) * 1e3), cycle_p99_ms=float(np.percentile(lat, 99) * 1e3),
@@ -379,6 +497,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shape of the pair kernel")
     ap.add_argument("--sync-steps", action="store_true",
                     help="time one synchronous graspa_widom_batch per step instead of queued batches")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time queued batches (graspa_widom_enqueue) instead of replays of a captured graph")
     ap.add_argument("--batch-boxes", type=int, default=256)
     ap.add_argument("--batch-cycles", type=int, default=20)
     ap.add_argument("--batch-warm-cycles", type=int, default=100)
@@ -402,10 +522,16 @@ def main():
     s1 = synth.config1()
     K, O, M = s1.n_trial_pos, s1.n_trial_orient, s1.components[0].molsize
     n_fw = len(s1.fw_type)
+    # the pair kernel's launch time (HIP events on the box's stream) over
+    # queued batches; the headline steps are the same batches replayed from a
+    # captured HIP graph (measured faster, same totals), unless --no-graph
     r = run_widom(s1, engine.GRASPA_MIXED, args.insertions, args.steps, args.warmup, autotune=not args.no_autotune,
                   queued=not args.sync_steps)
     n_total = args.insertions * args.steps
-    value = n_total / r["seconds"]
+    use_graph = not (args.no_graph or args.sync_steps)
+    rv = (run_widom(s1, engine.GRASPA_MIXED, args.insertions, args.steps, args.warmup,
+                    autotune=not args.no_autotune, timing=False, queued=True, graph=True) if use_graph else r)
+    value = n_total / rv["seconds"]
     # dominant kernel: the pair-energy kernel of the pipeline (two launches per
     # batch: first-bead sites, then orientation sites), timed with HIP events
     flops = widom_flops(r["screened_pairs"], r["evaluated_pairs"], M, r["nact"], r["n_ok"])
@@ -423,7 +549,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r["seconds"] / args.steps * 1e3,
+        "ms_per_step": rv["seconds"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -453,7 +579,7 @@ def main():
                      "reference_algorithm_flop_per_batch": ref_flops / args.steps,
                      "reference_algorithm_equivalent_tflops": ref_flops / args.steps / batch_s / 1e12},
         "autotune": r["autotune"],
-        "setup": dict(r["setup"], note="per framework, outside the timed region: box upload with the screen's "
+        "setup": dict(rv["setup"], note="per framework, outside the timed region: box upload with the screen's "
                                        "cell table, S(k) build, the first batch (host-built voxel maps), the "
                                        "autotuner"),
     }
@@ -465,6 +591,11 @@ def main():
         line["gcmc_batched"] = run_batch_gcmc(args.batch_boxes, args.batch_cycles, engine.GRASPA_MIXED,
                                               warm_cycles=args.batch_warm_cycles)
     if not args.quick:
+        # the same queued steps replayed from a captured HIP graph
+        line["widom_insertions_per_s_queued"] = n_total / r["seconds"]
+        line["widom_insertions_per_s_graph_replays"] = n_total / rv["seconds"] if use_graph else None
+        line["widom_graph_same_totals_as_queued"] = (rv["sum_w"] == r["sum_w"] and rv["n_ok"] == r["n_ok"]
+                                                     and rv["evaluated_pairs"] == r["evaluated_pairs"])
         rs = run_widom(s1, engine.GRASPA_MIXED, args.insertions, args.steps, 1, timing=False, queued=args.sync_steps)
         line["widom_insertions_per_s_" + ("queued" if args.sync_steps else "synchronous_steps")] = \
             args.insertions * args.steps / rs["seconds"]
@@ -488,7 +619,15 @@ def main():
                                                                    args.gcmc_cycles, engine.GRASPA_FP64)
         line["gcmc_config2_butane_mixed"] = run_batch_single(synth.config2(), 200, engine.GRASPA_MIXED,
                                                              warm_cycles=100)
-        line["served_stream"] = run_served(48, 20.0, 64, engine.GRASPA_MIXED)
+        # served stream: 500 us time slices per launch (each box at its own
+        # pace), and the same jobs with a fixed 256 moves per box per launch
+        line["served_stream"] = run_served(48, 20.0, 64, engine.GRASPA_MIXED, time_slice_us=500.0,
+                                           moves_per_launch=4096)
+        line["served_stream_fixed_256_moves"] = run_served(48, 20.0, 64, engine.GRASPA_MIXED)
+        # GCMC where the moves do work: the reference's CO2-MFI deck, and the
+        # 256-box distribution run for its 1,000 cycles to steady loading
+        line["gcmc_co2_mfi"] = run_co2_mfi()
+        line["gcmc_batched_steady"] = run_batch_steady(args.batch_boxes, 1000, engine.GRASPA_MIXED)
         # fixed dE error of the fast form in the batched configurations: the
         # same boxes and draws in both forms (24 boxes of the config-3
         # distribution, which the served stream also draws from, and the
@@ -511,6 +650,10 @@ def main():
         line["widom_batched_over_plain_fp64"] = value / wp["moves_per_s"]
         if "gcmc_batched" in line:
             line["gcmc_batched"]["over_single_box_fp64"] = line["gcmc_batched"]["moves_per_s"] / g["moves_per_s"]
+        # at loading: the steady 256-box run over one CO2-MFI box (plain FP64,
+        # device driver), both with molecules in the box
+        st = line["gcmc_batched_steady"]
+        st["over_single_box_fp64_co2_mfi"] = st["steady_moves_per_s"] / line["gcmc_co2_mfi"]["fp64_one_box"]["moves_per_s"]
         line["cpu_baseline"] = cpu_baseline(s1, args.cpu_sample)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line))

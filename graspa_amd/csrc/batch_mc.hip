@@ -52,6 +52,10 @@ constexpr int kBMaxTrial = 32;         // K and O
 constexpr int kBMaxMol = 8;            // atoms per molecule
 constexpr int kBMaxSites = 64;         // probe sites of one pair pass
 constexpr int kBGroup = 8;             // sites per pass over the atoms
+// neighbour list of a pair pass (cubic cells): per wave a segment of the
+// framework atoms and one of the guest atoms within the pass's reach
+constexpr int kLocFw = 256;            // framework entries per wave
+constexpr int kLocG = 128;             // guest entries per wave
 constexpr int kBDrawTrial = 16;        // first trial draw
 constexpr int kBMaxDraws = kBDrawTrial + 9 * kBMaxTrial;   // K trials, O orientations, O retrace orientations
 
@@ -134,6 +138,7 @@ struct BState {
   int64_t trace_n;
   int64_t cyc_first;                // first cycle of the current run
   int64_t prod_first, blk_size;     // first cycle and block length of the production run
+  uint64_t busy;                    // device-clock ticks spent in launches
 };
 static_assert(sizeof(BState) % 8 == 0, "BState is copied as 64-bit words");
 static_assert(kMaxComps == GRASPA_MAX_COMPS, "block statistics are sized for kMaxComps components");
@@ -166,7 +171,38 @@ struct BMove {
   int type, tmove, comp, mol, M, start, charged, n_eval;
   int fail, accept, overlap;
   int nc, qo, qn, _pad;             // identity swap: new species, old / new species charged
+  // guest atoms mirrored in LDS for the pair passes: gl_n atoms in SoA
+  // (x, y, z, q, then types); component c's atoms [base, base + gl_cap[c])
+  // at gl_off[c] (gl_cap 0: not mirrored)
+  int gl_n, gl_off[kMaxComps], gl_cap[kMaxComps];
+  // neighbour list of the current pair pass: centre and squared reach,
+  // per-wave entry counts (framework, guests) and their prefix sums
+  double lc[3], lr2;
+  int loc_on, loc_over;
+  int lcnt[2][kBW], lpre[2][kBW + 1];
 };
+
+// Keeps the LDS mirror of guest atom j of species c equal to the store.
+__device__ inline void mirror_put(const BBox& B, const BMove& L, double* gml, const int c, const int j, const double x,
+                                  const double y, const double z, const double q, const int t)
+{
+  const int k = j - B.sp[c].base;
+  if (gml && k < L.gl_cap[c]) {
+    const int n = L.gl_n, m = L.gl_off[c] + k;
+    gml[m] = x; gml[n + m] = y; gml[2 * n + m] = z; gml[3 * n + m] = q;
+    reinterpret_cast<int*>(gml + 4 * n)[m] = t;
+  }
+}
+
+__device__ inline void mirror_pos(const BBox& B, const BMove& L, double* gml, const int c, const int j, const double x,
+                                  const double y, const double z)
+{
+  const int k = j - B.sp[c].base;
+  if (gml && k < L.gl_cap[c]) {
+    const int n = L.gl_n, m = L.gl_off[c] + k;
+    gml[m] = x; gml[n + m] = y; gml[2 * n + m] = z;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // pair energies of the probe sites L.site[0 .. n_sites)
@@ -290,19 +326,110 @@ __device__ void block_sums(const BBox& B, const BState& S, const int bk, const i
 template <bool MIXED, bool CUBIC>
 __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double* __restrict__ ffl,
                           const double2* __restrict__ ct, const int nint, const double* __restrict__ fwl,
-                          const int n_sites, const int ex_lo, const int ex_hi)
+                          const double* __restrict__ gml, int* __restrict__ loc, const int n_sites, const int ex_lo,
+                          const int ex_hi)
 {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const BoxConst& bc = B.bc;
   const int T = bc.ntypes;
+  // Neighbour list (cubic cells): when the sites lie within R of their
+  // centre c and the sphere of radius r_cut + R about c is well inside the
+  // cell, only atoms with |a - c| < r_cut + R (minimum image) can be within
+  // the cutoff of a site (triangle inequality), so the pass runs over those.
+  if (CUBIC && loc) {
+    if (wave == 0) {
+      // centre and radius of the sites (minimum image about site 0), one
+      // lane per site
+      const BSite& s0 = L.site[L.idx[0]];
+      double dx = 0.0, dy = 0.0, dz = 0.0;
+      if (lane < n_sites) {
+        const BSite& sk = L.site[L.idx[lane]];
+        dx = sk.x - s0.x; dy = sk.y - s0.y; dz = sk.z - s0.z;
+        pbc_rint<true>(dx, dy, dz, bc.cell, bc.inv);
+      }
+      const double inv_n = 1.0 / (double)n_sites;
+      const double cx = wave_sum(dx) * inv_n, cy = wave_sum(dy) * inv_n, cz = wave_sum(dz) * inv_n;
+      const double ex = dx - cx, ey = dy - cy, ez = dz - cz;
+      const double r2 = wave_max(lane < n_sites ? ex * ex + ey * ey + ez * ez : 0.0);
+      if (lane == 0) {
+        const double rcut = sqrt(bc.use_charges ? fmax(bc.cut_vdw2, bc.cut_coul2) : bc.cut_vdw2);
+        const double reach = rcut + sqrt(r2) + 1e-6;
+        const double lmin = fmin(bc.cell[0], fmin(bc.cell[4], bc.cell[8]));
+        L.lc[0] = s0.x + cx; L.lc[1] = s0.y + cy; L.lc[2] = s0.z + cz;
+        L.lr2 = reach * reach;
+        // worth it when the sphere holds under half the cell's volume
+        L.loc_on = (4.18879020478639 * reach * reach * reach < 0.5 * B.volume && reach < 0.5 * lmin) ? 1 : 0;
+        L.loc_over = 0;
+      }
+    }
+    __syncthreads();
+    if (L.loc_on) {
+      // wave w filters a contiguous share of the atoms in index order into
+      // its own segments (deterministic order)
+      const unsigned long long lt = (1ull << lane) - 1ull;
+      const double cx = L.lc[0], cy = L.lc[1], cz = L.lc[2], lr2 = L.lr2;
+      const int nf = B.n_fw, cap = B.guest_cap;
+      const int share = (nf + kBW - 1) / kBW;
+      int cnt = 0;
+      for (int i0 = wave * share; i0 < min(nf, (wave + 1) * share); i0 += 64) {
+        const int i = i0 + lane;
+        bool in = false;
+        if (i < min(nf, (wave + 1) * share)) {
+          double dx = (fwl ? fwl[i] : B.fw[i]) - cx, dy = (fwl ? fwl[nf + i] : B.fw[nf + i]) - cy;
+          double dz = (fwl ? fwl[2 * nf + i] : B.fw[2 * nf + i]) - cz;
+          pbc_rint<true>(dx, dy, dz, bc.cell, bc.inv);
+          in = dx * dx + dy * dy + dz * dz < lr2;
+        }
+        const unsigned long long m = __ballot(in);
+        const int k = cnt + __popcll(m & lt);
+        if (in && k < kLocFw) loc[wave * kLocFw + k] = i;
+        cnt += __popcll(m);
+      }
+      int gcnt = 0;
+      for (int c = 1; c < B.n_comp; ++c) {
+        const int lo = B.sp[c].base, hi = lo + S.n_mol[c] * B.sp[c].molsize;
+        const int gs = (hi - lo + kBW - 1) / kBW;
+        const int a0 = lo + wave * gs, a1 = min(hi, lo + (wave + 1) * gs);
+        for (int i0 = a0; i0 < a1; i0 += 64) {
+          const int i = i0 + lane;
+          bool in = false;
+          if (i < a1 && !(i >= ex_lo && i < ex_hi)) {
+            double dx = B.g[i] - cx, dy = B.g[cap + i] - cy, dz = B.g[2 * cap + i] - cz;
+            pbc_rint<true>(dx, dy, dz, bc.cell, bc.inv);
+            in = dx * dx + dy * dy + dz * dz < lr2;
+          }
+          const unsigned long long m = __ballot(in);
+          const int k = gcnt + __popcll(m & lt);
+          if (in && k < kLocG) loc[kBW * kLocFw + wave * kLocG + k] = i;
+          gcnt += __popcll(m);
+        }
+      }
+      if (lane == 0) {
+        L.lcnt[0][wave] = cnt;
+        L.lcnt[1][wave] = gcnt;
+        if (cnt > kLocFw || gcnt > kLocG) L.loc_over = 1;
+      }
+      __syncthreads();
+      if (tid < 2) {
+        int a = 0;
+        L.lpre[tid][0] = 0;
+        for (int w = 0; w < kBW; ++w) L.lpre[tid][w + 1] = (a += L.lcnt[tid][w]);
+      }
+      __syncthreads();
+    }
+  }
+  const bool use_loc = CUBIC && loc && L.loc_on && !L.loc_over;
+  // G groups of at most kBGroup sites, the sites and the waves split evenly
+  // between them: group g holds sites [g n / G, (g+1) n / G) and waves
+  // [g kBW / G, (g+1) kBW / G), so every wave is busy
   const int ngrp = (n_sites + kBGroup - 1) / kBGroup;    // <= kBW
-  const int per = (n_sites + ngrp - 1) / ngrp;           // <= kBGroup
-  const int wpg = kBW / ngrp;                            // waves per group
-  const int grp = wave / wpg, r = wave - grp * wpg;
-  const bool active = grp < ngrp;
-  const int g0 = grp * per;
-  const int cnt = active ? max(0, min(per, n_sites - g0)) : 0;
-  const int first = r * 64 + lane, stride = wpg * 64;
+  int grp = 0;
+  while (grp + 1 < ngrp && wave >= (grp + 1) * kBW / ngrp) ++grp;
+  const int w0 = grp * kBW / ngrp, w1 = (grp + 1) * kBW / ngrp;
+  const int g0 = grp * n_sites / ngrp;
+  const int cnt = (grp + 1) * n_sites / ngrp - g0;
+  const int r = wave - w0;
+  const int first = r * 64 + lane, stride = (w1 - w0) * 64;
   const int nf = B.n_fw, cap = B.guest_cap;
   uint32_t ovb = 0;
 #pragma unroll 1
@@ -310,42 +437,91 @@ __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double
     double av[kBGroup], ar[kBGroup];
 #pragma unroll
     for (int j = 0; j < kBGroup; ++j) { av[j] = 0.0; ar[j] = 0.0; }
-    if (cnt > 0) {
-      if (part == 0 && fwl) {
-        // framework staged in LDS for the launch: SoA x, y, z, q, then types
-        const int* __restrict__ ftl = reinterpret_cast<const int*>(fwl + 4 * nf);
+    if (use_loc) {
+      // the neighbour list: entry k of the part lies in the segment of the
+      // last wave whose prefix is <= k
+      const int* pre = L.lpre[part];
+      const int n_in = pre[kBW];
+      const int* seg = loc + (part == 0 ? 0 : kBW * kLocFw);
+      const int segcap = part == 0 ? kLocFw : kLocG;
+      const int* __restrict__ ftl = fwl ? reinterpret_cast<const int*>(fwl + 4 * nf) : nullptr;
+      const int gn = L.gl_n;
+      const int* __restrict__ gtl = gml ? reinterpret_cast<const int*>(gml + 4 * gn) : nullptr;
 #pragma unroll 1
-        for (int i = first; i < nf; i += stride) {
-          const double ax = fwl[i], ay = fwl[nf + i], az = fwl[2 * nf + i], aq = fwl[3 * nf + i];
-          const int at = ftl[i];
-#pragma unroll
-          for (int j = 0; j < kBGroup; ++j)
-            if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j],
-                                                 ar[j], ovb, 1u << j);
-        }
-      } else if (part == 0) {
-#pragma unroll 1
-        for (int i = first; i < nf; i += stride) {
-          const double ax = B.fw[i], ay = B.fw[nf + i], az = B.fw[2 * nf + i], aq = B.fw[3 * nf + i];
-          const int at = B.fw_type[i];
-#pragma unroll
-          for (int j = 0; j < kBGroup; ++j)
-            if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j],
-                                                 ar[j], ovb, 1u << j);
-        }
-      } else {
-        for (int c = 1; c < B.n_comp; ++c) {
-          const int lo = B.sp[c].base, hi = lo + S.n_mol[c] * B.sp[c].molsize;
-#pragma unroll 1
-          for (int i = lo + first; i < hi; i += stride) {
-            if (i >= ex_lo && i < ex_hi) continue;
-            const double ax = B.g[i], ay = B.g[cap + i], az = B.g[2 * cap + i], aq = B.g[3 * cap + i];
-            const int at = B.g_type[i];
-#pragma unroll
-            for (int j = 0; j < kBGroup; ++j)
-              if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at,
-                                                   av[j], ar[j], ovb, 1u << j);
+      for (int k = first; k < n_in; k += stride) {
+        int w = 0;
+        while (w + 1 < kBW && pre[w + 1] <= k) ++w;
+        const int i = seg[w * segcap + (k - pre[w])];
+        double ax, ay, az, aq;
+        int at;
+        if (part == 0) {
+          if (fwl) { ax = fwl[i]; ay = fwl[nf + i]; az = fwl[2 * nf + i]; aq = fwl[3 * nf + i]; at = ftl[i]; }
+          else { ax = B.fw[i]; ay = B.fw[nf + i]; az = B.fw[2 * nf + i]; aq = B.fw[3 * nf + i]; at = B.fw_type[i]; }
+        } else {
+          // guest i of species c: from the mirror when the species fits there
+          int c = 1;
+          while (c + 1 < B.n_comp && i >= B.sp[c + 1].base) ++c;
+          const int lo = B.sp[c].base;
+          if (gml && S.n_mol[c] * B.sp[c].molsize <= L.gl_cap[c]) {
+            const int m = i - lo + L.gl_off[c];
+            ax = gml[m]; ay = gml[gn + m]; az = gml[2 * gn + m]; aq = gml[3 * gn + m]; at = gtl[m];
+          } else {
+            ax = B.g[i]; ay = B.g[cap + i]; az = B.g[2 * cap + i]; aq = B.g[3 * cap + i]; at = B.g_type[i];
           }
+        }
+#pragma unroll
+        for (int j = 0; j < kBGroup; ++j)
+          if (j < cnt)
+            pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j], ar[j],
+                                    ovb, 1u << j);
+      }
+    } else if (part == 0 && fwl) {
+      // framework staged in LDS for the launch: SoA x, y, z, q, then types
+      const int* __restrict__ ftl = reinterpret_cast<const int*>(fwl + 4 * nf);
+#pragma unroll 1
+      for (int i = first; i < nf; i += stride) {
+        const double ax = fwl[i], ay = fwl[nf + i], az = fwl[2 * nf + i], aq = fwl[3 * nf + i];
+        const int at = ftl[i];
+#pragma unroll
+        for (int j = 0; j < kBGroup; ++j)
+          if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j], ar[j],
+                                          ovb, 1u << j);
+      }
+    } else if (part == 0) {
+#pragma unroll 1
+      for (int i = first; i < nf; i += stride) {
+        const double ax = B.fw[i], ay = B.fw[nf + i], az = B.fw[2 * nf + i], aq = B.fw[3 * nf + i];
+        const int at = B.fw_type[i];
+#pragma unroll
+        for (int j = 0; j < kBGroup; ++j)
+          if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j], ar[j],
+                                          ovb, 1u << j);
+      }
+    } else {
+      for (int c = 1; c < B.n_comp; ++c) {
+        const int lo = B.sp[c].base, n_at = S.n_mol[c] * B.sp[c].molsize, hi = lo + n_at;
+        // from the LDS mirror when the species' atoms all fit in it
+        const bool lds = gml && n_at <= L.gl_cap[c];
+        const int gn = L.gl_n, go = L.gl_off[c] - lo;
+        const int* __restrict__ gtl = reinterpret_cast<const int*>(gml + 4 * gn);
+#pragma unroll 1
+        for (int i = lo + first; i < hi; i += stride) {
+          if (i >= ex_lo && i < ex_hi) continue;
+          double ax, ay, az, aq;
+          int at;
+          if (lds) {
+            const int m = i + go;
+            ax = gml[m]; ay = gml[gn + m]; az = gml[2 * gn + m]; aq = gml[3 * gn + m];
+            at = gtl[m];
+          } else {
+            ax = B.g[i]; ay = B.g[cap + i]; az = B.g[2 * cap + i]; aq = B.g[3 * cap + i];
+            at = B.g_type[i];
+          }
+#pragma unroll
+          for (int j = 0; j < kBGroup; ++j)
+            if (j < cnt)
+              pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j], ar[j],
+                                          ovb, 1u << j);
         }
       }
     }
@@ -359,10 +535,14 @@ __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double
   if (lane == 0 && ovl) atomicOr(&L.ovl, ovl);
   __syncthreads();
   if (tid < 4 * n_sites) {
+    // site slot -> its group (the group boundaries as above), summed over the
+    // group's waves in wave order
     const int slot = tid >> 2, t = tid & 3, part = t >> 1, k = t & 1;
-    const int g = slot / per, j = slot - g * per;
+    int g = 0;
+    while (g + 1 < ngrp && slot >= (g + 1) * n_sites / ngrp) ++g;
+    const int j = slot - g * n_sites / ngrp;
     double s = 0.0;
-    for (int w = g * wpg; w < (g + 1) * wpg; ++w) s += L.part[part][w][2 * j + k];
+    for (int w = g * kBW / ngrp; w < (g + 1) * kBW / ngrp; ++w) s += L.part[part][w][2 * j + k];
     L.e[L.idx[slot]][t] = s;
   }
   __syncthreads();
@@ -434,7 +614,7 @@ __device__ void ewald_pass(const BBox& B, const BState& S, BMove& L, double2* __
   // kEwU k-vectors per thread per step, their loads issued together (one
   // workgroup per box leaves few waves to hide memory latency); a thread
   // still visits its k-vectors in ascending order, so the sums are the same
-  constexpr int kEwU = 4;
+  constexpr int kEwU = 8;
   const int na = B.n_active;
   for (int j0 = tid; j0 < na; j0 += kBT * kEwU) {
     int4 kv[kEwU];
@@ -610,7 +790,7 @@ enum { kHGV = 1, kGGV = 2, kHGR = 4, kGGR = 5, kHGE = 7, kGGE = 8, kTail = 9 };
 // ---------------------------------------------------------------------------
 template <bool MIXED, bool CUBIC>
 __device__ void single_body(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr, const double2* ct,
-                            int nint, const double* fwl, double2* eik)
+                            int nint, const double* fwl, double* gml, int* loc, double2* eik)
 {
   const int tid = threadIdx.x;
   const int M = L.M, start = L.start, cap = B.guest_cap, comp = L.comp;
@@ -652,7 +832,7 @@ __device__ void single_body(const BBox& B, BState& S, BMove& L, const double* ff
   if (tid < 2 * M) L.idx[tid] = (unsigned char)tid;
   if (tid == 0) L.ovl = 0ull;
   __syncthreads();
-  pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, 2 * M, start, start + M);
+  pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, gml, loc, 2 * M, start, start + M);
   if (tid == 0) {
     const unsigned long long newmask = ((1ull << M) - 1ull) << M;
     L.overlap = (L.ovl & newmask) ? 1 : 0;
@@ -692,6 +872,7 @@ __device__ void single_body(const BBox& B, BState& S, BMove& L, const double* ff
     const int j = start + tid;
     const BSite s = L.site[M + tid];
     B.g[j] = s.x; B.g[cap + j] = s.y; B.g[2 * cap + j] = s.z;
+    mirror_pos(B, L, gml, comp, j, s.x, s.y, s.z);
   }
 }
 
@@ -712,7 +893,7 @@ __device__ inline void list_survivors(BMove& L, const int n, const int g)
 
 template <bool MIXED, bool CUBIC>
 __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr, const double2* ct,
-                     int nint, const double* fwl, double2* eik)
+                     int nint, const double* fwl, double* gml, int* loc, double2* eik)
 {
   const int tid = threadIdx.x;
   const int M = L.M, comp = L.comp, cap = B.guest_cap;
@@ -740,7 +921,7 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
   else if (tid == 0) L.smask = 0ull;
   if (tid == 0) list_survivors(L, K, 1);
   __syncthreads();
-  if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, L.n_eval, ex_lo, ex_hi);
+  if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, gml, loc, L.n_eval, ex_lo, ex_hi);
   if (tid < K) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) L.t4[tid][t] = L.e[tid][t];
@@ -790,7 +971,7 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
       else if (tid == 0) L.smask = 0ull;
       if (tid == 0) list_survivors(L, O, 1);
       __syncthreads();
-      if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, L.n_eval, ex_lo, ex_hi);
+      if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, gml, loc, L.n_eval, ex_lo, ex_hi);
       if (tid < O) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) L.t4[tid][t] = L.e[tid][t];
@@ -843,7 +1024,7 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
     else if (tid == 0) L.smask = 0ull;
     if (tid == 0) list_survivors(L, O * chain, chain);
     __syncthreads();
-    if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, L.n_eval, ex_lo, ex_hi);
+    if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, gml, loc, L.n_eval, ex_lo, ex_hi);
     if (tid < O) {
       double e4[4] = {0.0, 0.0, 0.0, 0.0};
       for (int a = 0; a < chain; ++a)
@@ -963,6 +1144,7 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
         B.g_type[j] = L.molt[tid];
         B.g_molid[j] = N;
         B.g_comp[j] = comp;
+        mirror_put(B, L, gml, comp, j, L.molp[tid][0], L.molp[tid][1], L.molp[tid][2], L.molq[tid], L.molt[tid]);
       }
     } else {
       // the last molecule moves into the hole, whose MolIDs stay
@@ -970,9 +1152,12 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
       const int last = sp.base + (N - 1) * M;
       if (L.start != last && tid < M) {
         const int j = L.start + tid, k = last + tid;
-        B.g[j] = B.g[k]; B.g[cap + j] = B.g[cap + k]; B.g[2 * cap + j] = B.g[2 * cap + k];
-        B.g[3 * cap + j] = B.g[3 * cap + k];
-        B.g_type[j] = B.g_type[k];
+        const double x = B.g[k], y = B.g[cap + k], z = B.g[2 * cap + k], q = B.g[3 * cap + k];
+        const int t = B.g_type[k];
+        B.g[j] = x; B.g[cap + j] = y; B.g[2 * cap + j] = z;
+        B.g[3 * cap + j] = q;
+        B.g_type[j] = t;
+        mirror_put(B, L, gml, comp, j, x, y, z, q, t);
       }
     }
     __syncthreads();
@@ -986,8 +1171,8 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
 // excluded.
 template <bool MIXED, bool CUBIC>
 __device__ uint32_t trial_energies(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr,
-                                   const double2* ct, int nint, const double* fwl, const int n_trials, const int chain,
-                                   const int ex_lo, const int ex_hi)
+                                   const double2* ct, int nint, const double* fwl, double* gml, int* loc,
+                                   const int n_trials, const int chain, const int ex_lo, const int ex_hi)
 {
   const int tid = threadIdx.x;
   const int ns = n_trials * chain;
@@ -995,7 +1180,7 @@ __device__ uint32_t trial_energies(const BBox& B, BState& S, BMove& L, const dou
   else if (tid == 0) L.smask = 0ull;
   if (tid == 0) list_survivors(L, ns, chain);
   __syncthreads();
-  if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, L.n_eval, ex_lo, ex_hi);
+  if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, gml, loc, L.n_eval, ex_lo, ex_hi);
   if (tid < n_trials) {
     double e4[4] = {0.0, 0.0, 0.0, 0.0};
     for (int a = 0; a < chain; ++a)
@@ -1045,7 +1230,7 @@ __device__ double retrace_weight(const BBox& B, BMove& L, const int n, const uin
 // kBDrawTrial + 3K + 3o, retrace orientation o at kBDrawTrial + 3K + 3O + 3o.
 template <bool MIXED, bool CUBIC>
 __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr,
-                            const double2* ct, int nint, const double* fwl, double2* eik)
+                            const double2* ct, int nint, const double* fwl, double* gml, int* loc, double2* eik)
 {
   const int tid = threadIdx.x;
   const int M = L.M, comp = L.comp, cap = B.guest_cap;
@@ -1064,7 +1249,7 @@ __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ff
     L.site[tid] = s;
   }
   __syncthreads();
-  uint32_t ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, K, 1, ex_lo, ex_hi);
+  uint32_t ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, K, 1, ex_lo, ex_hi);
   if (tid < 64) {
     double W = 1.0, others = 0.0;
     const int sel = select_wave0(B, L, K, ov, L.ud[5], false, false, &W, &others);
@@ -1099,7 +1284,7 @@ __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ff
       L.site[t] = s;
     }
     __syncthreads();
-    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, O, chain, ex_lo, ex_hi);
+    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, O, chain, ex_lo, ex_hi);
     if (tid < 64) {
       double W = L.W;
       const int sel = select_wave0(B, L, O, ov, L.ud[6], false, true, &W);
@@ -1131,7 +1316,7 @@ __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ff
     for (int t = 0; t < 4; ++t) L.eo[t] = 0.0;
   }
   __syncthreads();
-  ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, 1, 1, ex_lo, ex_hi);
+  ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, 1, 1, ex_lo, ex_hi);
   if (tid == 0) L.Wo = retrace_weight(B, L, 1, ov, L.stored, K);   // (own weight + StoredR) / K
   __syncthreads();
   // retrace: orientations about the old first bead, trial 0 its own
@@ -1156,7 +1341,7 @@ __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ff
       L.site[t] = s;
     }
     __syncthreads();
-    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, O, chain, ex_lo, ex_hi);
+    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, O, chain, ex_lo, ex_hi);
     if (tid == 0) L.Wo *= retrace_weight(B, L, O, ov, 0.0, O);
     __syncthreads();
   }
@@ -1203,6 +1388,7 @@ __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ff
   if (L.accept && tid < M) {
     const int j = L.start + tid;
     B.g[j] = L.molp[tid][0]; B.g[cap + j] = L.molp[tid][1]; B.g[2 * cap + j] = L.molp[tid][2];
+    mirror_pos(B, L, gml, comp, j, L.molp[tid][0], L.molp[tid][1], L.molp[tid][2]);
   }
 }
 
@@ -1219,7 +1405,7 @@ __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ff
 // at kBDrawTrial + 3K + 3o, retrace orientation o at kBDrawTrial + 3K + 3O + 3o.
 template <bool MIXED, bool CUBIC>
 __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr,
-                              const double2* ct, int nint, const double* fwl, double2* eik)
+                              const double2* ct, int nint, const double* fwl, double* gml, int* loc, double2* eik)
 {
   const int tid = threadIdx.x;
   const int oc = L.comp, nc = L.nc, cap = B.guest_cap;
@@ -1238,7 +1424,7 @@ __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* 
     for (int t = 0; t < 4; ++t) L.eo[t] = 0.0;
   }
   __syncthreads();
-  uint32_t ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, 1, 1, ex_lo, ex_hi);
+  uint32_t ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, 1, 1, ex_lo, ex_hi);
   if (tid < 64) {
     double W = 1.0;
     const int sel = select_wave0(B, L, 1, ov, 0.5, false, false, &W);
@@ -1273,7 +1459,7 @@ __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* 
       L.site[t] = s;
     }
     __syncthreads();
-    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, O, chain, ex_lo, ex_hi);
+    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, O, chain, ex_lo, ex_hi);
     if (tid < 64) {
       double W = L.W;
       const int sel = select_wave0(B, L, O, ov, L.ud[6], false, true, &W);
@@ -1304,7 +1490,7 @@ __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* 
     L.site[0] = s;
   }
   __syncthreads();
-  ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, 1, 1, ex_lo, ex_hi);
+  ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, 1, 1, ex_lo, ex_hi);
   if (tid == 0) L.Wo = retrace_weight(B, L, 1, ov, 0.0, 1);
   __syncthreads();
   if (Mo > 1) {
@@ -1329,7 +1515,7 @@ __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* 
       L.site[t] = s;
     }
     __syncthreads();
-    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, O, chain, ex_lo, ex_hi);
+    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, O, chain, ex_lo, ex_hi);
     if (tid == 0) L.Wo *= retrace_weight(B, L, O, ov, 0.0, O);
     __syncthreads();
   }
@@ -1396,9 +1582,12 @@ __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* 
     const int last = so.base + (S.n_mol[oc] - 1) * Mo;
     if (L.start != last && tid < Mo) {
       const int j = L.start + tid, k = last + tid;
-      B.g[j] = B.g[k]; B.g[cap + j] = B.g[cap + k]; B.g[2 * cap + j] = B.g[2 * cap + k];
-      B.g[3 * cap + j] = B.g[3 * cap + k];
-      B.g_type[j] = B.g_type[k];
+      const double x = B.g[k], y = B.g[cap + k], z = B.g[2 * cap + k], q = B.g[3 * cap + k];
+      const int t = B.g_type[k];
+      B.g[j] = x; B.g[cap + j] = y; B.g[2 * cap + j] = z;
+      B.g[3 * cap + j] = q;
+      B.g_type[j] = t;
+      mirror_put(B, L, gml, oc, j, x, y, z, q, t);
     }
     const int nn = S.n_mol[nc];
     if (tid >= 64 && tid < 64 + Mn) {
@@ -1408,6 +1597,7 @@ __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* 
       B.g_type[j] = sn.type[a];
       B.g_molid[j] = nn;
       B.g_comp[j] = nc;
+      mirror_put(B, L, gml, nc, j, L.molp[a][0], L.molp[a][1], L.molp[a][2], sn.q[a], sn.type[a]);
     }
     __syncthreads();
     if (tid == 0) {
@@ -1417,6 +1607,7 @@ __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* 
   } else if (tid < Mo) {
     const int j = L.start + tid;
     B.g[j] = L.molp[tid][0]; B.g[cap + j] = L.molp[tid][1]; B.g[2 * cap + j] = L.molp[tid][2];
+    mirror_pos(B, L, gml, oc, j, L.molp[tid][0], L.molp[tid][1], L.molp[tid][2]);
   }
 }
 
@@ -1435,7 +1626,8 @@ __device__ inline void update_max(double* v, int64_t& acc, int64_t& tot, double 
 
 template <bool MIXED, bool CUBIC>
 __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ boxes, BState* __restrict__ states,
-                                                       const int max_moves, const int dyn_bytes)
+                                                       const int max_moves, const int64_t* __restrict__ budgets,
+                                                       const int dyn_bytes)
 {
   extern __shared__ double4 dyn4[];
   __shared__ BState S;
@@ -1474,6 +1666,49 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
     int* ftl = reinterpret_cast<int*>(fwl + 4 * nf);
     for (int i = tid; i < nf; i += kBT) ftl[i] = B.fw_type[i];
   }
+  // the guest atoms go to an LDS mirror in what the framework leaves: every
+  // species gets an equal share, at most its capacity of molecules; the pair
+  // passes read a species from the mirror while its atoms fit there
+  // then the pair passes' neighbour-list segments (cubic cells), then the
+  // guest mirror
+  const int l_off = fw_in_lds ? fw_off + 4 * nf + (nf + 1) / 2 : fw_off;
+  constexpr int kLocDoubles = kBW * (kLocFw + kLocG) / 2;
+  int* loc = (CUBIC && (size_t)(l_off + kLocDoubles) * 8 <= (size_t)dyn_bytes) ? reinterpret_cast<int*>(dyn + l_off)
+                                                                               : nullptr;
+  const int g_off = loc ? l_off + kLocDoubles : l_off;
+  const int g_room = dyn_bytes / 8 > g_off ? (int)((dyn_bytes / 8 - g_off) * 8 / 36) : 0;
+  if (tid == 0) {
+    const int ns = B.n_comp - 1, share = ns > 0 ? g_room / ns : 0;
+    int n = 0;
+    for (int c = 0; c < kMaxComps; ++c) {
+      int cp = 0;
+      if (c >= 1 && c < B.n_comp) {
+        const int M = B.sp[c].molsize;
+        cp = min(B.sp[c].capacity, share / M) * M;
+      }
+      L.gl_off[c] = n;
+      L.gl_cap[c] = cp;
+      n += cp;
+    }
+    L.gl_n = n;
+  }
+  __syncthreads();
+  double* gml = L.gl_n > 0 ? dyn + g_off : nullptr;
+  if (gml) {
+    const int gn = L.gl_n;
+    int* gtl = reinterpret_cast<int*>(gml + 4 * gn);
+    for (int i = tid; i < gn; i += kBT) {
+      int c = 1;
+      while (c + 1 < B.n_comp && i >= L.gl_off[c + 1]) ++c;
+      const int k = i - L.gl_off[c];
+      if (k < S.n_mol[c] * B.sp[c].molsize) {
+        const int j = B.sp[c].base + k, cap = B.guest_cap;
+        gml[i] = B.g[j]; gml[gn + i] = B.g[cap + j]; gml[2 * gn + i] = B.g[2 * cap + j];
+        gml[3 * gn + i] = B.g[3 * cap + j];
+        gtl[i] = B.g_type[j];
+      }
+    }
+  }
   for (int i = tid; i < nff; i += kBT) ffl[i] = B.ff64[i];
   for (int i = tid; i < T * T; i += kBT) {
     const int st = i / T, at = i - st * T;
@@ -1483,7 +1718,9 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
     for (int i = tid; i < B.coul_n; i += kBT) dyn[nff4 + nthr + i] = B.coul[i];
   __syncthreads();
   const int nd = kBDrawTrial + 3 * B.K + 6 * B.O;
-  for (int mv = 0; mv < max_moves; ++mv) {
+  const int64_t limit = budgets ? budgets[blockIdx.x] : (int64_t)max_moves;
+  const uint64_t t_launch = wall_clock64();
+  for (int64_t mv = 0; mv < limit; ++mv) {
     if (S.cycle >= S.target || S.error) break;
     // this move's uniforms, one Philox block (two uniforms) per thread
     if (2 * tid < nd) {
@@ -1572,10 +1809,10 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
     }
     __syncthreads();
     const int type = L.type;
-    if (type == kMvTrans || type == kMvRot) single_body<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
-    else if (type == kMvReins) reinsertion<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
-    else if (type == kMvIdent) identity_swap<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
-    else if (type != kMvNone) cbmc<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, eik);
+    if (type == kMvTrans || type == kMvRot) single_body<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, eik);
+    else if (type == kMvReins) reinsertion<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, eik);
+    else if (type == kMvIdent) identity_swap<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, eik);
+    else if (type != kMvNone) cbmc<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, eik);
     __syncthreads();
     if (tid == 0) {
       const int comp = L.comp;
@@ -1625,6 +1862,8 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
       if (tid == 0) cyc_block = -1;
     }
   }
+  if (threadIdx.x == 0) S.busy += wall_clock64() - t_launch;
+  __syncthreads();
   {
     const uint64_t* src = reinterpret_cast<const uint64_t*>(&S);
     uint64_t* dst = reinterpret_cast<uint64_t*>(&states[blockIdx.x]);
@@ -1660,12 +1899,27 @@ struct graspa_batch {
   double tick_hz = 1.0e8;
   double kernel_ms = 0.0;
   int64_t launches = 0;
+  // the launch in flight (graspa_batch_launch_async): its events
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool in_flight = false;
+  // time slices (graspa_batch_set_time_slice): per-box move budgets of a
+  // launch from each box's measured time per move
+  double slice_us = 0.0;
+  std::vector<int64_t> hbud;
+  int64_t* d_bud = nullptr;
+  std::vector<double> us_per_move;   // running estimate per slot (0: none yet)
+  std::vector<int64_t> seen_moves;
+  std::vector<uint64_t> seen_busy;
 };
 
 namespace {
 
+int finish_launch(graspa_batch* bt);
+int copy_states_back(graspa_batch* bt);
+
 int upload_batch(graspa_batch* bt)
 {
+  if (int rc = finish_launch(bt)) return rc;
   if (hip_check(hipMemcpyAsync(bt->d_box, bt->hb.data(), bt->n * sizeof(BBox), hipMemcpyHostToDevice, bt->stream),
                 "batch upload"))
     return GRASPA_ERR_HIP;
@@ -1676,6 +1930,24 @@ int upload_batch(graspa_batch* bt)
 }
 
 int download_states(graspa_batch* bt)
+{
+  if (int rc = finish_launch(bt)) return rc;
+  if (int rc = copy_states_back(bt)) return rc;
+  // time per move of each box over the moves since the last look
+  for (int i = 0; i < bt->n; ++i) {
+    const BState& st = bt->hs[i];
+    const int64_t dm = st.move - bt->seen_moves[i];
+    if (bt->boxes[i] && dm > 0) {
+      const double us = (double)(st.busy - bt->seen_busy[i]) / bt->tick_hz * 1e6 / (double)dm;
+      bt->us_per_move[i] = bt->us_per_move[i] > 0.0 ? 0.5 * (bt->us_per_move[i] + us) : us;
+    }
+    bt->seen_moves[i] = st.move;
+    bt->seen_busy[i] = st.busy;
+  }
+  return GRASPA_OK;
+}
+
+int copy_states_back(graspa_batch* bt)
 {
   if (hip_check(hipMemcpyAsync(bt->hs.data(), bt->d_state, bt->n * sizeof(BState), hipMemcpyDeviceToHost, bt->stream),
                 "batch state copy"))
@@ -1688,6 +1960,9 @@ void free_batch(graspa_batch* bt)
   if (!bt) return;
   hipSetDevice(bt->device);
   if (bt->stream) hipStreamSynchronize(bt->stream);
+  if (bt->ev0) hipEventDestroy(bt->ev0);
+  hipFree(bt->d_bud);
+  if (bt->ev1) hipEventDestroy(bt->ev1);
   hipFree(bt->d_box); hipFree(bt->d_state); hipFree(bt->d_tail); hipFree(bt->d_use_tail);
   hipFree(bt->d_trace); hipFree(bt->d_cyc); hipFree(bt->d_blk);
   for (double2* p : bt->skc) hipFree(p);
@@ -1853,6 +2128,9 @@ int fill_slot(graspa_batch* bt, int i, const graspa_batch_box& in)
   bt->hb[i] = h;
   bt->hs[i] = st;
   bt->boxes[i] = b;
+  bt->us_per_move[i] = 0.0;
+  bt->seen_moves[i] = 0;
+  bt->seen_busy[i] = 0;
   bt->lds = std::max(bt->lds, need);
   bt->cubic = 1;
   for (int k = 0; k < bt->n; ++k)
@@ -1873,27 +2151,44 @@ int launch_batch(graspa_batch* bt, int max_moves)
   if (avail < bt->lds) return fail(GRASPA_ERR_UNSUPPORTED, "batch workspace exceeds LDS");
   const size_t lds = avail;
   (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  hipEventRecord(e0, bt->stream);
-  hipLaunchKernelGGL(kern, dim3(bt->n), dim3(kBT), lds, bt->stream, bt->d_box, bt->d_state, max_moves, (int)lds);
-  hipError_t le = hipGetLastError();
-  hipEventRecord(e1, bt->stream);
-  hipError_t se = hipEventSynchronize(e1);
+  if (!bt->ev0 && hip_check(hipEventCreate(&bt->ev0), "batch event")) return GRASPA_ERR_HIP;
+  if (!bt->ev1 && hip_check(hipEventCreate(&bt->ev1), "batch event")) return GRASPA_ERR_HIP;
+  hipEventRecord(bt->ev0, bt->stream);
+  const int64_t* bud = nullptr;
+  if (bt->slice_us > 0.0) {
+    // budgets for a launch of about slice_us: each box's moves at its
+    // measured time per move (a new box starts from 20 us per move)
+    for (int i = 0; i < bt->n; ++i) {
+      const double c = bt->us_per_move[i] > 0.0 ? bt->us_per_move[i] : 20.0;
+      bt->hbud[i] = std::max<int64_t>(1, std::min<int64_t>(max_moves, (int64_t)(bt->slice_us / c)));
+    }
+    if (hip_check(hipMemcpyAsync(bt->d_bud, bt->hbud.data(), bt->n * sizeof(int64_t), hipMemcpyHostToDevice,
+                                 bt->stream), "batch budgets"))
+      return GRASPA_ERR_HIP;
+    bud = bt->d_bud;
+  }
+  hipLaunchKernelGGL(kern, dim3(bt->n), dim3(kBT), lds, bt->stream, bt->d_box, bt->d_state, max_moves, bud, (int)lds);
+  if (hip_check(hipGetLastError(), "batch_mc_kernel launch")) return GRASPA_ERR_HIP;
+  hipEventRecord(bt->ev1, bt->stream);
+  bt->in_flight = true;
+  return GRASPA_OK;
+}
+
+// Waits for the launch in flight and adds its kernel time.
+int finish_launch(graspa_batch* bt)
+{
+  if (!bt->in_flight) return GRASPA_OK;
+  bt->in_flight = false;
+  if (hip_check(hipEventSynchronize(bt->ev1), "batch_mc_kernel")) return GRASPA_ERR_HIP;
   float ms = 0.0f;
-  if (le == hipSuccess && se == hipSuccess) hipEventElapsedTime(&ms, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  if (hip_check(le, "batch_mc_kernel launch")) return GRASPA_ERR_HIP;
-  if (hip_check(se, "batch_mc_kernel")) return GRASPA_ERR_HIP;
-  bt->kernel_ms += ms;
+  if (hipEventElapsedTime(&ms, bt->ev0, bt->ev1) == hipSuccess) bt->kernel_ms += ms;
   bt->launches++;
   return GRASPA_OK;
 }
 
 int launch_once(graspa_batch* bt, int max_moves)
 {
+  if (bt->in_flight) return fail(GRASPA_ERR_STATE, "a launch of the batch is in flight (graspa_batch_wait)");
   if (bt->precision == GRASPA_MIXED)
     return bt->cubic ? launch_batch<true, true>(bt, max_moves) : launch_batch<true, false>(bt, max_moves);
   return bt->cubic ? launch_batch<false, true>(bt, max_moves) : launch_batch<false, false>(bt, max_moves);
@@ -1915,6 +2210,10 @@ int graspa_batch_create(const graspa_batch_box* boxes, int n_boxes, int trace_ca
   bt->skc.assign(n_boxes, nullptr);
   bt->hb.assign(n_boxes, BBox{});
   bt->hs.assign(n_boxes, BState{});
+  bt->hbud.assign(n_boxes, 0);
+  bt->us_per_move.assign(n_boxes, 0.0);
+  bt->seen_moves.assign(n_boxes, 0);
+  bt->seen_busy.assign(n_boxes, 0);
   for (int i = 0; i < n_boxes; ++i) {
     bt->hs[i].error = 0;
     bt->hs[i].target = 0;   // empty slots never run
@@ -1953,6 +2252,7 @@ int graspa_batch_create(const graspa_batch_box* boxes, int n_boxes, int trace_ca
   rc = rc ? rc : hip_check(hipMemset(bt->d_blk, 0, n_boxes * sizeof(graspa_block_stats)), "block statistics");
   rc = rc ? rc : hip_check(hipMalloc(&bt->d_box, n_boxes * sizeof(BBox)), "hipMalloc batch");
   rc = rc ? rc : hip_check(hipMalloc(&bt->d_state, n_boxes * sizeof(BState)), "hipMalloc batch");
+  rc = rc ? rc : hip_check(hipMalloc(&bt->d_bud, n_boxes * sizeof(int64_t)), "hipMalloc batch");
   for (int i = 0; boxes && i < n_boxes && !rc; ++i)
     if (boxes[i].box) rc = fill_slot(bt, i, boxes[i]);
   rc = rc ? rc : upload_batch(bt);
@@ -1964,6 +2264,7 @@ int graspa_batch_create(const graspa_batch_box* boxes, int n_boxes, int trace_ca
 int graspa_batch_set_box(graspa_batch* bt, int slot, const graspa_batch_box* box)
 {
   if (!bt || slot < 0 || slot >= bt->n) return fail(GRASPA_ERR_ARG, "bad slot");
+  if (int rc = finish_launch(bt)) return rc;
   hipSetDevice(bt->device);
   if (int rc = download_states(bt)) return rc;
   if (!box || !box->box) {
@@ -1996,6 +2297,7 @@ static int start_production(graspa_batch* bt, int slot, int n_cycles)
 int graspa_batch_set_cycles(graspa_batch* bt, int slot, int n_cycles, int production)
 {
   if (!bt || slot < 0 || slot >= bt->n || n_cycles < 0) return fail(GRASPA_ERR_ARG, "bad arguments");
+  if (int rc = finish_launch(bt)) return rc;
   if (!bt->boxes[slot]) return fail(GRASPA_ERR_STATE, "empty slot");
   hipSetDevice(bt->device);
   if (int rc = download_states(bt)) return rc;
@@ -2009,11 +2311,30 @@ int graspa_batch_set_cycles(graspa_batch* bt, int slot, int n_cycles, int produc
   return hip_check(hipMemcpy(&bt->d_state[slot], &s, sizeof(BState), hipMemcpyHostToDevice), "batch state upload");
 }
 
-int graspa_batch_launch(graspa_batch* bt, int moves_per_launch, int* n_running)
+int graspa_batch_set_time_slice(graspa_batch* bt, double us)
+{
+  if (!bt || !(us >= 0.0)) return fail(GRASPA_ERR_ARG, "bad arguments");
+  bt->slice_us = us;
+  return GRASPA_OK;
+}
+
+int graspa_batch_launch_async(graspa_batch* bt, int moves_per_launch)
 {
   if (!bt) return fail(GRASPA_ERR_ARG, "null batch");
   hipSetDevice(bt->device);
-  if (int rc = launch_once(bt, moves_per_launch > 0 ? moves_per_launch : 4096)) return rc;
+  return launch_once(bt, moves_per_launch > 0 ? moves_per_launch : 4096);
+}
+
+int graspa_batch_launch(graspa_batch* bt, int moves_per_launch, int* n_running)
+{
+  if (int rc = graspa_batch_launch_async(bt, moves_per_launch)) return rc;
+  return graspa_batch_wait(bt, n_running);
+}
+
+int graspa_batch_wait(graspa_batch* bt, int* n_running)
+{
+  if (!bt) return fail(GRASPA_ERR_ARG, "null batch");
+  hipSetDevice(bt->device);
   if (int rc = download_states(bt)) return rc;
   int run = 0;
   for (int i = 0; i < bt->n; ++i) {

@@ -117,9 +117,9 @@ int ensure_dead_map(graspa_box* b, int site_type, const uint32_t** out)
     int nv[3];
     build_dead_map(b, site_type, bits, nv);
     uint32_t* d = nullptr;
-    if (hip_check(hipMalloc(&d, bits.size() * sizeof(uint32_t)), "hipMalloc dead map")) return GRASPA_ERR_HIP;
+    if (hip_check(box_malloc(b, &d, bits.size() * sizeof(uint32_t)), "hipMalloc dead map")) return GRASPA_ERR_HIP;
     if (hip_check(hipMemcpy(d, bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice), "dead map upload")) {
-      (void)hipFree(d);
+      box_free(b, d);
       return GRASPA_ERR_HIP;
     }
     b->d_dmap[site_type] = d;
@@ -132,7 +132,7 @@ int ensure_dead_map(graspa_box* b, int site_type, const uint32_t** out)
 void free_dead_maps(graspa_box* b)
 {
   for (uint32_t*& d : b->d_dmap) {
-    (void)hipFree(d);
+    box_free(b, d);
     d = nullptr;
   }
 }

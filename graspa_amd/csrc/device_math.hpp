@@ -199,24 +199,6 @@ __device__ inline float r2_fixed32(uint32_t ax, uint32_t ay, uint32_t az, uint32
 // ---------------------------------------------------------------------------
 // wave64 reductions
 // ---------------------------------------------------------------------------
-__device__ inline double wave_sum(double v)
-{
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
-}
-__device__ inline uint32_t wave_or(uint32_t v)
-{
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v |= __shfl_xor(v, m, 64);
-  return v;
-}
-__device__ inline double wave_max(double v)
-{
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
-  return v;
-}
 // Value of v in lane l (l uniform across the wave).
 __device__ inline double readlane_d(double v, int l)
 {
@@ -224,6 +206,53 @@ __device__ inline double readlane_d(double v, int l)
   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// v of the lane the DPP control CTRL names (a lane of a row ROW_MASK leaves
+// out reads 0), both 32-bit halves moved by the data-parallel primitives.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline double dpp_d(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over the 64 lanes (every lane active), in every lane: pairs and quads
+// by quad permutes, rows of 16 by rotations by 4 and 8, then the row sums
+// carried into the last row by the row broadcasts of lanes 15 and 31, and
+// lane 63 read back.
+__device__ inline double wave_sum(double v)
+{
+  v += dpp_d<0xB1>(v);         // quad_perm [1,0,3,2]
+  v += dpp_d<0x4E>(v);         // quad_perm [2,3,0,1]
+  v += dpp_d<0x124>(v);        // row_ror:4
+  v += dpp_d<0x128>(v);        // row_ror:8
+  v += dpp_d<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+  v += dpp_d<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+  return readlane_d(v, 63);
+}
+__device__ inline uint32_t wave_or(uint32_t v)
+{
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v |= __shfl_xor(v, m, 64);
+  return v;
+}
+// Maximum over the 64 lanes (every lane active), in every lane; the masked
+// broadcast steps fold in -inf, which leaves a maximum unchanged.
+__device__ inline double wave_max(double v)
+{
+  constexpr double ninf = -__builtin_huge_val();
+  v = fmax(v, dpp_d<0xB1>(v));
+  v = fmax(v, dpp_d<0x4E>(v));
+  v = fmax(v, dpp_d<0x124>(v));
+  v = fmax(v, dpp_d<0x128>(v));
+  const double b15 = dpp_d<0x142, 0xa>(v);
+  v = fmax(v, ((threadIdx.x >> 4) & 1) ? b15 : ninf);
+  const double b31 = dpp_d<0x143, 0xc>(v);
+  v = fmax(v, ((threadIdx.x >> 5) & 1) ? b31 : ninf);
+  return readlane_d(v, 63);
 }
 // Orders this wave's LDS writes before its later LDS reads by other lanes
 // (a wave executes its LDS operations in order; this keeps the compiler from

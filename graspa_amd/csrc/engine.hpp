@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/graspa_engine.h"
@@ -173,12 +174,51 @@ struct graspa_box {
   // launch shape of the Widom pair-energy kernel (graspa_widom_set_shape,
   // graspa_widom_autotune): waves per workgroup, workgroups per CU (0: as
   // many as are resident), framework staged in LDS when it fits
+  // a captured Widom batch (graspa_widom_graph_create): the instantiated
+  // graph, the device-side first insertion index its replays advance, and
+  // the bytes of the captured arguments (a replay is refused when the box's
+  // arguments no longer match them)
+  hipGraph_t wg_graph = nullptr;
+  hipGraphExec_t wg_exec = nullptr;
+  int64_t* d_wg_first = nullptr;
+  std::vector<unsigned char> wg_key;
+  int wg_comp = 0, wg_K = 0, wg_O = 0, wg_bias = 0, wg_n = 0;
+  double wg_beta = 0.0, wg_excl = 0.0, wg_tail = 0.0;
+  uint64_t wg_seed = 0;
+  uint32_t wg_stream = 0;
   int ev_waves = 12;            // two 12-wave workgroups per CU with the framework in LDS
   int ev_per_cu = 0;
   int ev_fw_lds = 1;
 
   graspa::TimingState timing;
+  // device allocations and their total (box_malloc / box_free)
+  std::unordered_map<void*, size_t> dev_alloc;
+  size_t dev_bytes = 0;
 };
+
+// Device allocations of a box, recorded so that graspa_box_footprint can
+// report what the box holds.
+template <typename T>
+inline hipError_t box_malloc(graspa_box* b, T** p, size_t bytes)
+{
+  const hipError_t e = hipMalloc(reinterpret_cast<void**>(p), bytes);
+  if (e == hipSuccess && *p) {
+    b->dev_alloc[static_cast<void*>(*p)] = bytes;
+    b->dev_bytes += bytes;
+  }
+  return e;
+}
+
+inline void box_free(graspa_box* b, void* p)
+{
+  if (!p) return;
+  auto it = b->dev_alloc.find(p);
+  if (it != b->dev_alloc.end()) {
+    b->dev_bytes -= it->second;
+    b->dev_alloc.erase(it);
+  }
+  (void)hipFree(p);
+}
 
 namespace graspa {
 
@@ -251,11 +291,26 @@ void build_coulomb_table(double alpha, double cut2, std::vector<double>& tab);
 // widom_kernel.hip
 // Bytes of the no-allocation form's fixed scratch block (one per device).
 constexpr size_t kWidomScratchBytes = (size_t)32 << 20;
-// no_alloc: the pipeline's scratch comes from that block, and no buffer or
-// voxel map is allocated (maps built by earlier calls are used).
+// How a Widom batch is launched.  no_alloc: the pipeline's scratch comes
+// from that block, and no buffer or voxel map is allocated (maps built by
+// earlier calls are used).  first_dev: the batch's first insertion index is
+// read from device memory (graph replays) instead of first_index.
+// args_only: nothing is launched or allocated; key receives the bytes of the
+// kernels' arguments and launch shape (the box's buffers as they are; an
+// error if they are too small).
+struct WidomLaunch {
+  bool no_alloc = false;
+  const int64_t* first_dev = nullptr;
+  bool args_only = false;
+  std::vector<unsigned char>* key = nullptr;
+};
 int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
                  uint64_t seed, uint32_t stream, int64_t first_index, int n_insert, graspa_widom_result* d_res,
-                 graspa_widom_summary* summary, bool no_alloc);
+                 graspa_widom_summary* summary, const WidomLaunch& how);
+int widom_graph_create(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
+                       uint64_t seed, uint32_t stream, int64_t first_index, int n_insert);
+int widom_graph_launch(graspa_box* b, int n_replays);
+void widom_graph_free(graspa_box* b);
 int widom_autotune(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
                    uint64_t seed, int n_insert, int reps, graspa_eval_shape* tried, double* ms, int cap, int* n_tried);
 

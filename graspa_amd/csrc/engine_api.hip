@@ -143,15 +143,15 @@ static int realloc_guests(graspa_box* b)
     b->comps[c].base = cap;
     cap += b->comps[c].capacity * b->comps[c].molsize;
   }
-  hipFree(b->d_g); hipFree(b->d_g_type); hipFree(b->d_g_molid); hipFree(b->d_g_comp); hipFree(b->d_g_fix);
+  box_free(b, b->d_g); box_free(b, b->d_g_type); box_free(b, b->d_g_molid); box_free(b, b->d_g_comp); box_free(b, b->d_g_fix);
   b->d_g = nullptr; b->d_g_type = b->d_g_molid = b->d_g_comp = nullptr; b->d_g_fix = nullptr;
   b->guest_cap = cap;
   const size_t n = (size_t)std::max(cap, 1);
-  if (hip_check(hipMalloc(&b->d_g, 4 * n * sizeof(double)), "hipMalloc guests")) return GRASPA_ERR_HIP;
-  if (hip_check(hipMalloc(&b->d_g_type, n * sizeof(int)), "hipMalloc guests")) return GRASPA_ERR_HIP;
-  if (hip_check(hipMalloc(&b->d_g_molid, n * sizeof(int)), "hipMalloc guests")) return GRASPA_ERR_HIP;
-  if (hip_check(hipMalloc(&b->d_g_comp, n * sizeof(int)), "hipMalloc guests")) return GRASPA_ERR_HIP;
-  if (hip_check(hipMalloc(&b->d_g_fix, n * sizeof(uint4)), "hipMalloc guests")) return GRASPA_ERR_HIP;
+  if (hip_check(box_malloc(b, &b->d_g, 4 * n * sizeof(double)), "hipMalloc guests")) return GRASPA_ERR_HIP;
+  if (hip_check(box_malloc(b, &b->d_g_type, n * sizeof(int)), "hipMalloc guests")) return GRASPA_ERR_HIP;
+  if (hip_check(box_malloc(b, &b->d_g_molid, n * sizeof(int)), "hipMalloc guests")) return GRASPA_ERR_HIP;
+  if (hip_check(box_malloc(b, &b->d_g_comp, n * sizeof(int)), "hipMalloc guests")) return GRASPA_ERR_HIP;
+  if (hip_check(box_malloc(b, &b->d_g_fix, n * sizeof(uint4)), "hipMalloc guests")) return GRASPA_ERR_HIP;
   hipMemsetAsync(b->d_g, 0, 4 * n * sizeof(double), b->stream);
   hipMemsetAsync(b->d_g_type, 0, n * sizeof(int), b->stream);
   hipMemsetAsync(b->d_g_molid, 0xff, n * sizeof(int), b->stream);
@@ -169,12 +169,12 @@ static int realloc_guests(graspa_box* b)
 static int ensure_part(graspa_box* b, size_t n_doubles)
 {
   if (n_doubles <= b->part_cap) return GRASPA_OK;
-  hipFree(b->d_part);
+  box_free(b, b->d_part);
   hipHostFree(b->h_part);
   b->d_part = nullptr;
   b->h_part = nullptr;
   size_t cap = std::max<size_t>(n_doubles, 4096);
-  if (hip_check(hipMalloc(&b->d_part, cap * sizeof(double)), "hipMalloc partials")) return GRASPA_ERR_HIP;
+  if (hip_check(box_malloc(b, &b->d_part, cap * sizeof(double)), "hipMalloc partials")) return GRASPA_ERR_HIP;
   if (hip_check(hipHostMalloc((void**)&b->h_part, cap * sizeof(double), hipHostMallocDefault), "hipHostMalloc"))
     return GRASPA_ERR_HIP;
   b->part_cap = cap;
@@ -185,16 +185,16 @@ int graspa::ensure_widom_records(graspa_box* b, int n_insert)
 {
   if ((size_t)n_insert <= b->wrec_cap) return GRASPA_OK;
   hipStreamSynchronize(b->stream);
-  hipFree(b->d_wrec_w); hipFree(b->d_wrec_e); hipFree(b->d_wrec_f); hipFree(b->d_wpart);
+  box_free(b, b->d_wrec_w); box_free(b, b->d_wrec_e); box_free(b, b->d_wrec_f); box_free(b, b->d_wpart);
   b->d_wrec_w = b->d_wrec_e = b->d_wpart = nullptr;
   b->d_wrec_f = nullptr;
   b->wrec_cap = 0;
   const size_t n = (size_t)n_insert;
   const size_t nblk = (n + 2047) / 2048;
-  int rc = hip_check(hipMalloc(&b->d_wrec_w, n * sizeof(double)), "hipMalloc widom records");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_wrec_e, n * sizeof(double)), "hipMalloc widom records");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_wrec_f, n * sizeof(int)), "hipMalloc widom records");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_wpart, 6 * nblk * sizeof(double)), "hipMalloc widom records");
+  int rc = hip_check(box_malloc(b, &b->d_wrec_w, n * sizeof(double)), "hipMalloc widom records");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_wrec_e, n * sizeof(double)), "hipMalloc widom records");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_wrec_f, n * sizeof(int)), "hipMalloc widom records");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_wpart, 6 * nblk * sizeof(double)), "hipMalloc widom records");
   if (!rc) b->wrec_cap = n;
   return rc;
 }
@@ -219,7 +219,7 @@ static uint32_t spread3(uint32_t v)
 int graspa::build_screen_cells(graspa_box* b)
 {
   constexpr int kCellCapMax = 16;
-  hipFree(b->d_fws); hipFree(b->d_fws_type); hipFree(b->d_ctab);
+  box_free(b, b->d_fws); box_free(b, b->d_fws_type); box_free(b, b->d_ctab);
   b->d_fws = nullptr; b->d_fws_type = nullptr; b->d_ctab = nullptr;
   b->use_cells = 0;
   b->cell_cap = 0;
@@ -272,8 +272,8 @@ int graspa::build_screen_cells(graspa_box* b)
     soa[3 * n + k] = b->fw_q[i];
     ty[k] = b->fw_type[i];
   }
-  int rc = hip_check(hipMalloc(&b->d_fws, 4 * m * sizeof(double)), "hipMalloc fw");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_fws_type, m * sizeof(int)), "hipMalloc fw");
+  int rc = hip_check(box_malloc(b, &b->d_fws, 4 * m * sizeof(double)), "hipMalloc fw");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_fws_type, m * sizeof(int)), "hipMalloc fw");
   rc = rc ? rc : hip_check(hipMemcpy(b->d_fws, soa.data(), 4 * m * sizeof(double), hipMemcpyHostToDevice), "fw upload");
   rc = rc ? rc : hip_check(hipMemcpy(b->d_fws_type, ty.data(), m * sizeof(int), hipMemcpyHostToDevice), "fw upload");
   if (rc || !grid_ok) return rc;
@@ -285,7 +285,7 @@ int graspa::build_screen_cells(graspa_box* b)
     const int cl = cell_of[i];
     tab[(size_t)cl * cap + fill[cl]++] = make_uint4(fx[3 * i], fx[3 * i + 1], fx[3 * i + 2], (uint32_t)b->fw_type[i]);
   }
-  rc = hip_check(hipMalloc(&b->d_ctab, tab.size() * sizeof(uint4)), "hipMalloc cells");
+  rc = hip_check(box_malloc(b, &b->d_ctab, tab.size() * sizeof(uint4)), "hipMalloc cells");
   rc = rc ? rc : hip_check(hipMemcpy(b->d_ctab, tab.data(), tab.size() * sizeof(uint4), hipMemcpyHostToDevice), "cells upload");
   if (rc) return rc;
   b->use_cells = 1;
@@ -389,8 +389,8 @@ int graspa_box_create(const graspa_box_params* p, graspa_box** out)
     ff64[4 * i + 3] = p->epsilon[i];
     for (int k = 0; k < 4; ++k) ff32[4 * i + k] = (float)ff64[4 * i + k];
   }
-  int rc = hip_check(hipMalloc(&b->d_ff64, ff64.size() * sizeof(double)), "hipMalloc ff");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_ff32, ff32.size() * sizeof(float)), "hipMalloc ff");
+  int rc = hip_check(box_malloc(b, &b->d_ff64, ff64.size() * sizeof(double)), "hipMalloc ff");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_ff32, ff32.size() * sizeof(float)), "hipMalloc ff");
   rc = rc ? rc : hip_check(hipMemcpy(b->d_ff64, ff64.data(), ff64.size() * sizeof(double), hipMemcpyHostToDevice), "ff upload");
   rc = rc ? rc : hip_check(hipMemcpy(b->d_ff32, ff32.data(), ff32.size() * sizeof(float), hipMemcpyHostToDevice), "ff upload");
   // Ewald: per-k factor prefactor*2pi/V * (1 or 2) * exp(-k^2/4a^2)/k^2, -1 where the k-vector
@@ -429,10 +429,10 @@ int graspa_box_create(const graspa_box_params* p, graspa_box** out)
     b->n_active = nact;
   }
   const size_t nv = (size_t)std::max(b->nvec, 1);
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_ktemp, nv * sizeof(double)), "hipMalloc ewald");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_sk_fw, nv * sizeof(double2)), "hipMalloc ewald");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_sk_ads, nv * sizeof(double2)), "hipMalloc ewald");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_sk_stage, nv * sizeof(double2)), "hipMalloc ewald");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_ktemp, nv * sizeof(double)), "hipMalloc ewald");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_sk_fw, nv * sizeof(double2)), "hipMalloc ewald");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_sk_ads, nv * sizeof(double2)), "hipMalloc ewald");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_sk_stage, nv * sizeof(double2)), "hipMalloc ewald");
   if (!rc) {
     hipMemcpy(b->d_ktemp, b->ktemp.data(), b->nvec * sizeof(double), hipMemcpyHostToDevice);
     hipMemset(b->d_sk_fw, 0, nv * sizeof(double2));
@@ -454,8 +454,8 @@ int graspa_box_create(const graspa_box_params* p, graspa_box** out)
       tact.push_back(b->ktemp[k]);
     }
     const size_t na = std::max<size_t>(kact.size(), 1);
-    rc = rc ? rc : hip_check(hipMalloc(&b->d_kact, na * sizeof(int4)), "hipMalloc ewald");
-    rc = rc ? rc : hip_check(hipMalloc(&b->d_ktemp_act, na * sizeof(double)), "hipMalloc ewald");
+    rc = rc ? rc : hip_check(box_malloc(b, &b->d_kact, na * sizeof(int4)), "hipMalloc ewald");
+    rc = rc ? rc : hip_check(box_malloc(b, &b->d_ktemp_act, na * sizeof(double)), "hipMalloc ewald");
     if (!rc && !kact.empty()) {
       rc = hip_check(hipMemcpy(b->d_kact, kact.data(), kact.size() * sizeof(int4), hipMemcpyHostToDevice), "k upload");
       rc = rc ? rc
@@ -468,22 +468,22 @@ int graspa_box_create(const graspa_box_params* p, graspa_box** out)
     std::vector<double> ct(kCoulTabStride, 0.0);
     if (bc.use_charges && bc.alpha > 0.0 && bc.cut_coul2 > 0.0) build_coulomb_table(bc.alpha, bc.cut_coul2, ct);
     b->coul_n = (int)ct.size();
-    rc = rc ? rc : hip_check(hipMalloc(&b->d_coul, ct.size() * sizeof(double)), "hipMalloc coulomb table");
+    rc = rc ? rc : hip_check(box_malloc(b, &b->d_coul, ct.size() * sizeof(double)), "hipMalloc coulomb table");
     rc = rc ? rc
             : hip_check(hipMemcpy(b->d_coul, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice),
                         "coulomb table upload");
   }
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_wacc, kWaccSize * sizeof(double)), "hipMalloc widom");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_wacc, kWaccSize * sizeof(double)), "hipMalloc widom");
   if (!rc) rc = hip_check(hipMemset(b->d_wacc, 0, kWaccSize * sizeof(double)), "widom init");
   rc = rc ? rc : ensure_part(b, 4096);
   b->comps.resize(1);
   rc = rc ? rc : realloc_guests(b);
   if (!rc) {
     // empty framework arrays so kernels always get valid pointers
-    rc = hip_check(hipMalloc(&b->d_fw, 4 * sizeof(double)), "hipMalloc fw");
-    rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_type, sizeof(int)), "hipMalloc fw");
-    rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_fix, sizeof(uint4)), "hipMalloc fw");
-    rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_fixT, sizeof(uint4)), "hipMalloc fw");
+    rc = hip_check(box_malloc(b, &b->d_fw, 4 * sizeof(double)), "hipMalloc fw");
+    rc = rc ? rc : hip_check(box_malloc(b, &b->d_fw_type, sizeof(int)), "hipMalloc fw");
+    rc = rc ? rc : hip_check(box_malloc(b, &b->d_fw_fix, sizeof(uint4)), "hipMalloc fw");
+    rc = rc ? rc : hip_check(box_malloc(b, &b->d_fw_fixT, sizeof(uint4)), "hipMalloc fw");
     rc = rc ? rc : build_screen_cells(b);
   }
   if (rc) {
@@ -500,20 +500,21 @@ void graspa_box_destroy(graspa_box* b)
   hipSetDevice(b->device);
   if (b->stream) hipStreamSynchronize(b->stream);
   timing_clear(b);
-  hipFree(b->d_ff64); hipFree(b->d_ff32);
-  hipFree(b->d_fw); hipFree(b->d_fw_type); hipFree(b->d_fw_fix); hipFree(b->d_fw_fixT);
-  hipFree(b->d_fws); hipFree(b->d_fws_type); hipFree(b->d_ctab);
-  hipFree(b->d_g); hipFree(b->d_g_type); hipFree(b->d_g_molid); hipFree(b->d_g_comp); hipFree(b->d_g_fix);
-  hipFree(b->d_ktemp); hipFree(b->d_sk_fw); hipFree(b->d_sk_ads); hipFree(b->d_sk_stage);
-  hipFree(b->d_kact); hipFree(b->d_ktemp_act); hipFree(b->d_coul);
+  widom_graph_free(b);
+  box_free(b, b->d_ff64); box_free(b, b->d_ff32);
+  box_free(b, b->d_fw); box_free(b, b->d_fw_type); box_free(b, b->d_fw_fix); box_free(b, b->d_fw_fixT);
+  box_free(b, b->d_fws); box_free(b, b->d_fws_type); box_free(b, b->d_ctab);
+  box_free(b, b->d_g); box_free(b, b->d_g_type); box_free(b, b->d_g_molid); box_free(b, b->d_g_comp); box_free(b, b->d_g_fix);
+  box_free(b, b->d_ktemp); box_free(b, b->d_sk_fw); box_free(b, b->d_sk_ads); box_free(b, b->d_sk_stage);
+  box_free(b, b->d_kact); box_free(b, b->d_ktemp_act); box_free(b, b->d_coul);
   free_dead_maps(b);
-  for (Component& c : b->comps) hipFree(c.d_chain);
-  hipFree(b->d_part);
+  for (Component& c : b->comps) box_free(b, c.d_chain);
+  box_free(b, b->d_part);
   if (b->h_part) hipHostFree(b->h_part);
-  hipFree(b->d_wacc); hipFree(b->d_wres);
-  hipFree(b->d_wrec_w); hipFree(b->d_wrec_e); hipFree(b->d_wrec_f); hipFree(b->d_wpart);
-  hipFree(b->d_wmask1); hipFree(b->d_wmask2); hipFree(b->d_we1); hipFree(b->d_we2);
-  hipFree(b->d_wlist1); hipFree(b->d_wlist2); hipFree(b->d_wstate); hipFree(b->d_wcounts); hipFree(b->d_wlive); hipFree(b->d_wstat);
+  box_free(b, b->d_wacc); box_free(b, b->d_wres);
+  box_free(b, b->d_wrec_w); box_free(b, b->d_wrec_e); box_free(b, b->d_wrec_f); box_free(b, b->d_wpart);
+  box_free(b, b->d_wmask1); box_free(b, b->d_wmask2); box_free(b, b->d_we1); box_free(b, b->d_we2);
+  box_free(b, b->d_wlist1); box_free(b, b->d_wlist2); box_free(b, b->d_wstate); box_free(b, b->d_wcounts); box_free(b, b->d_wlive); box_free(b, b->d_wstat);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
 }
@@ -526,7 +527,7 @@ int graspa_box_set_framework(graspa_box* b, int n, const double* xyz, const doub
     if (type[i] < 0 || type[i] >= b->bc.ntypes) return fail(GRASPA_ERR_ARG, "framework type out of range");
   hipSetDevice(b->device);
   hipStreamSynchronize(b->stream);
-  hipFree(b->d_fw); hipFree(b->d_fw_type); hipFree(b->d_fw_fix); hipFree(b->d_fw_fixT);
+  box_free(b, b->d_fw); box_free(b, b->d_fw_type); box_free(b, b->d_fw_fix); box_free(b, b->d_fw_fixT);
   b->d_fw = nullptr; b->d_fw_type = nullptr; b->d_fw_fix = nullptr; b->d_fw_fixT = nullptr;
   free_dead_maps(b);
   b->n_fw = n;
@@ -550,10 +551,10 @@ int graspa_box_set_framework(graspa_box* b, int n, const double* xyz, const doub
     fix[i] = make_uint4(f[0], f[1], f[2], qb);
     fixT[i] = make_uint4(f[0], f[1], f[2], (uint32_t)type[i]);
   }
-  int rc = hip_check(hipMalloc(&b->d_fw, 4 * m * sizeof(double)), "hipMalloc fw");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_type, m * sizeof(int)), "hipMalloc fw");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_fix, m * sizeof(uint4)), "hipMalloc fw");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_fw_fixT, m * sizeof(uint4)), "hipMalloc fw");
+  int rc = hip_check(box_malloc(b, &b->d_fw, 4 * m * sizeof(double)), "hipMalloc fw");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_fw_type, m * sizeof(int)), "hipMalloc fw");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_fw_fix, m * sizeof(uint4)), "hipMalloc fw");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_fw_fixT, m * sizeof(uint4)), "hipMalloc fw");
   rc = rc ? rc : hip_check(hipMemcpy(b->d_fw_fixT, fixT.data(), m * sizeof(uint4), hipMemcpyHostToDevice), "fw upload");
   rc = rc ? rc : hip_check(hipMemcpy(b->d_fw, soa.data(), 4 * m * sizeof(double), hipMemcpyHostToDevice), "fw upload");
   if (!rc && n > 0) rc = hip_check(hipMemcpy(b->d_fw_type, type, n * sizeof(int), hipMemcpyHostToDevice), "fw upload");
@@ -590,6 +591,14 @@ int graspa_box_add_component(graspa_box* b, int molsize, int capacity, const dou
   return realloc_guests(b);
 }
 
+int graspa_box_footprint(graspa_box* b, uint64_t* bytes)
+{
+  GRASPA_CHECK_BOX(b);
+  if (!bytes) return fail(GRASPA_ERR_ARG, "null output");
+  *bytes = (uint64_t)b->dev_bytes;
+  return GRASPA_OK;
+}
+
 int graspa_box_set_chain(graspa_box* b, int comp, const graspa_chain* chain)
 {
   GRASPA_CHECK_BOX(b);
@@ -599,7 +608,7 @@ int graspa_box_set_chain(graspa_box* b, int comp, const graspa_chain* chain)
   if (!chain) {
     if (c.d_chain) {
       hipStreamSynchronize(b->stream);
-      hipFree(c.d_chain);
+      box_free(b, c.d_chain);
     }
     c.d_chain = nullptr;
     return GRASPA_OK;
@@ -618,7 +627,7 @@ int graspa_box_set_chain(graspa_box* b, int comp, const graspa_chain* chain)
     if (i >= 2 && i < c.molsize && !(d[2] > 0.0 && d[1] > 0.0 && d[1] < 3.14159265358979323846))
       return fail(GRASPA_ERR_ARG, "chain: bends need k_bend > 0 and 0 < theta0 < pi");
   }
-  if (!c.d_chain && hip_check(hipMalloc(&c.d_chain, sizeof(m)), "hipMalloc chain")) return GRASPA_ERR_HIP;
+  if (!c.d_chain && hip_check(box_malloc(b, &c.d_chain, sizeof(m)), "hipMalloc chain")) return GRASPA_ERR_HIP;
   hipStreamSynchronize(b->stream);
   return hip_check(hipMemcpy(c.d_chain, m, sizeof(m), hipMemcpyHostToDevice), "chain upload");
 }
@@ -966,16 +975,16 @@ int graspa_widom_batch(graspa_box* b, int comp, int K, int O, double beta, int b
   graspa_widom_result* d_res = nullptr;
   if (results) {
     if ((size_t)n_insert > b->wres_cap) {
-      hipFree(b->d_wres);
+      box_free(b, b->d_wres);
       b->d_wres = nullptr;
-      if (hip_check(hipMalloc(&b->d_wres, (size_t)n_insert * sizeof(graspa_widom_result)), "hipMalloc widom results"))
+      if (hip_check(box_malloc(b, &b->d_wres, (size_t)n_insert * sizeof(graspa_widom_result)), "hipMalloc widom results"))
         return GRASPA_ERR_HIP;
       b->wres_cap = n_insert;
     }
     d_res = b->d_wres;
   }
   int rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, stream, first_index, n_insert, d_res, summary,
-                        false);
+                        WidomLaunch());
   if (rc) return rc;
   if (results && n_insert > 0) {
     if (hip_check(hipMemcpyAsync(results, d_res, (size_t)n_insert * sizeof(graspa_widom_result), hipMemcpyDeviceToHost,
@@ -993,7 +1002,7 @@ int graspa_widom_enqueue(graspa_box* b, int comp, int K, int O, double beta, int
   if (n_insert < 0) return fail(GRASPA_ERR_ARG, "n_insert < 0");
   hipSetDevice(b->device);
   return launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, stream, first_index, n_insert, nullptr, nullptr,
-                      false);
+                      WidomLaunch());
 }
 
 int graspa_widom_enqueue_noalloc(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl,
@@ -1002,8 +1011,25 @@ int graspa_widom_enqueue_noalloc(graspa_box* b, int comp, int K, int O, double b
   GRASPA_CHECK_BOX(b);
   if (n_insert < 0) return fail(GRASPA_ERR_ARG, "n_insert < 0");
   hipSetDevice(b->device);
+  WidomLaunch noalloc;
+  noalloc.no_alloc = true;
   return launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, stream, first_index, n_insert, nullptr, nullptr,
-                      true);
+                      noalloc);
+}
+
+int graspa_widom_graph_create(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl,
+                              double tail_de, uint64_t seed, uint32_t stream, int64_t first_index, int n_insert)
+{
+  GRASPA_CHECK_BOX(b);
+  hipSetDevice(b->device);
+  return widom_graph_create(b, comp, K, O, beta, bias, excl, tail_de, seed, stream, first_index, n_insert);
+}
+
+int graspa_widom_graph_launch(graspa_box* b, int n_replays)
+{
+  GRASPA_CHECK_BOX(b);
+  hipSetDevice(b->device);
+  return widom_graph_launch(b, n_replays);
 }
 
 int graspa_widom_totals(graspa_box* b, graspa_widom_summary* totals, int reset)

@@ -161,7 +161,7 @@ __device__ inline double flex_u(uint64_t seed, uint32_t stream, uint64_t move, i
 // bc = unit(C - B), n = unit((B - A) x bc) (bead 2: the unit normal built
 // from the axis least aligned with bc), m = n x bc.  A sampler that runs out
 // of attempts returns theta0 or pi.
-__device__ inline void flex_trial(const double (*prev)[3], const int i, const double* __restrict__ chain,
+__device__ __noinline__ void flex_trial(const double (*prev)[3], const int i, const double* __restrict__ chain,
                                   const double beta, const uint64_t seed, const uint32_t stream, const uint64_t move,
                                   const int j, double out[3])
 {

@@ -109,6 +109,7 @@ struct WidomArgs {
   uint64_t seed;
   uint32_t stream;
   int64_t first_index;
+  const int64_t* first_dev;  // when set, the first index is read here (graph replays)
   int n_insert;
   int charged;
   int n_kact;
@@ -146,6 +147,9 @@ struct WidomArgs {
   int lds_thr, lds_tab;                 // screening kernels
   int lds_ffl, lds_coul, lds_fw;        // pair-energy kernel
 };
+
+// First insertion index of the batch.
+__device__ inline int64_t first_of(const WidomArgs& A) { return A.first_dev ? *A.first_dev : A.first_index; }
 
 __device__ inline double draw(const WidomArgs& A, uint64_t idx, uint32_t d)
 {
@@ -388,7 +392,7 @@ __global__ __launch_bounds__(kFbWaves * 64) void widom_first_bead_kernel(const W
     bool hit = false, undecided = false;
     if (act) {
       double u[3];
-      draw3(A, (uint64_t)(A.first_index + ins), 3 * trial, u);
+      draw3(A, (uint64_t)(first_of(A) + ins), 3 * trial, u);
       p[0] = A.bc.cell[0] * u[0];
       p[1] = A.bc.cell[4] * u[1];
       p[2] = A.bc.cell[8] * u[2];
@@ -676,7 +680,7 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
   unsigned long long st = 0;
   for (unsigned li = wv; li < n_live; li += kInsWaves) {
     const int ins = A.live[i0 + li];
-    const uint64_t idx = (uint64_t)(A.first_index + ins);
+    const uint64_t idx = (uint64_t)(first_of(A) + ins);
     const uint32_t dead = A.mask1[ins];
     double ev = 0.0, er = 0.0;
     if (lane < K && !((dead >> lane) & 1u)) {
@@ -768,7 +772,7 @@ __global__ __launch_bounds__(kFinWaves * 64) void widom_finish_kernel(const Wido
   const unsigned n_live = A.live_counts[blockIdx.x];
   for (unsigned li = wv; li < n_live; li += kFinWaves) {
     const int ins = A.live[blockIdx.x * A.chunk + li];
-    const uint64_t idx = (uint64_t)(A.first_index + ins);
+    const uint64_t idx = (uint64_t)(first_of(A) + ins);
     const InsState S = A.state[ins];
     bool ok = S.ok != 0;
     double Wtot = S.wfb, e_v = S.ev, e_r = S.er;
@@ -994,24 +998,24 @@ static int ensure_pipeline(graspa_box* b, int n_insert, int K, int OC)
   if (n <= b->wp_n && need_e1 <= b->wp_e1 && need_e2 <= b->wp_e2) return GRASPA_OK;
   const size_t cn = std::max(n, b->wp_n), c1 = std::max(need_e1, b->wp_e1), c2 = std::max(need_e2, b->wp_e2);
   hipStreamSynchronize(b->stream);
-  hipFree(b->d_wmask1); hipFree(b->d_wmask2); hipFree(b->d_we1); hipFree(b->d_we2);
-  hipFree(b->d_wlist1); hipFree(b->d_wlist2); hipFree(b->d_wstate); hipFree(b->d_wlive);
+  box_free(b, b->d_wmask1); box_free(b, b->d_wmask2); box_free(b, b->d_we1); box_free(b, b->d_we2);
+  box_free(b, b->d_wlist1); box_free(b, b->d_wlist2); box_free(b, b->d_wstate); box_free(b, b->d_wlive);
   b->d_wmask1 = b->d_wmask2 = nullptr;
   b->d_we1 = b->d_we2 = b->d_wlist1 = b->d_wlist2 = b->d_wstate = b->d_wlive = nullptr;
   b->wp_n = b->wp_e1 = b->wp_e2 = 0;
-  int rc = hip_check(hipMalloc(&b->d_wmask1, cn * sizeof(uint32_t)), "hipMalloc widom pipeline");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_wmask2, cn * sizeof(uint32_t)), "hipMalloc widom pipeline");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_wstate, cn * sizeof(InsState)), "hipMalloc widom pipeline");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_wlive, cn * sizeof(int)), "hipMalloc widom pipeline");
+  int rc = hip_check(box_malloc(b, &b->d_wmask1, cn * sizeof(uint32_t)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_wmask2, cn * sizeof(uint32_t)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_wstate, cn * sizeof(InsState)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_wlive, cn * sizeof(int)), "hipMalloc widom pipeline");
   if (!rc && !b->d_wstat)
-    rc = hip_check(hipMalloc(&b->d_wstat, (2 * kMaxScreenGroups + 2 * kMaxEvalGroups) * sizeof(unsigned long long)),
+    rc = hip_check(box_malloc(b, &b->d_wstat, (2 * kMaxScreenGroups + 2 * kMaxEvalGroups) * sizeof(unsigned long long)),
                    "hipMalloc widom pipeline");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_we1, c1 * sizeof(double2)), "hipMalloc widom pipeline");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_wlist1, c1 * sizeof(TrialPt)), "hipMalloc widom pipeline");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_we2, c2 * sizeof(double2)), "hipMalloc widom pipeline");
-  rc = rc ? rc : hip_check(hipMalloc(&b->d_wlist2, c2 * sizeof(TrialPt)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_we1, c1 * sizeof(double2)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_wlist1, c1 * sizeof(TrialPt)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_we2, c2 * sizeof(double2)), "hipMalloc widom pipeline");
+  rc = rc ? rc : hip_check(box_malloc(b, &b->d_wlist2, c2 * sizeof(TrialPt)), "hipMalloc widom pipeline");
   if (!rc && !b->d_wcounts)
-    rc = hip_check(hipMalloc(&b->d_wcounts, 3 * kMaxScreenGroups * sizeof(unsigned)), "hipMalloc widom pipeline");
+    rc = hip_check(box_malloc(b, &b->d_wcounts, 3 * kMaxScreenGroups * sizeof(unsigned)), "hipMalloc widom pipeline");
   if (!rc) {
     b->wp_n = cn;
     b->wp_e1 = c1;
@@ -1219,8 +1223,9 @@ static int device_scratch(int device, char** out)
 
 int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
                  uint64_t seed, uint32_t stream, int64_t first_index, int n_insert, graspa_widom_result* d_res,
-                 graspa_widom_summary* summary, bool no_alloc)
+                 graspa_widom_summary* summary, const WidomLaunch& how)
 {
+  const bool no_alloc = how.no_alloc || how.args_only;
   if (comp < 1 || comp >= (int)b->comps.size()) return fail(GRASPA_ERR_ARG, "widom: bad component");
   const Component& c = b->comps[comp];
   if (K < 1 || K > kMaxTrials || O < 1 || O > kMaxTrials) return fail(GRASPA_ERR_ARG, "widom: 1 <= K,O <= 32");
@@ -1237,7 +1242,25 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   int groups = 0, chunk = 0;
   widom_grid(b, K, n_insert, &groups, &chunk);
   WidomBufs W{};
-  if (no_alloc) {
+  if (how.args_only) {
+    const size_t ns = (size_t)groups * chunk, OC = (size_t)std::max(O * chain, 1);
+    if (ns > b->wp_n || ns * K > b->wp_e1 || ns * OC > b->wp_e2 || (size_t)n_insert > b->wrec_cap)
+      return fail(GRASPA_ERR_STATE, "widom: the box's buffers are smaller than the batch");
+    W.mask1 = static_cast<uint32_t*>(b->d_wmask1);
+    W.mask2 = static_cast<uint32_t*>(b->d_wmask2);
+    W.state = static_cast<InsState*>(b->d_wstate);
+    W.live = static_cast<int*>(b->d_wlive);
+    W.stat_part = static_cast<unsigned long long*>(b->d_wstat);
+    W.e1 = static_cast<double2*>(b->d_we1);
+    W.list1 = static_cast<TrialPt*>(b->d_wlist1);
+    W.e2 = static_cast<double2*>(b->d_we2);
+    W.list2 = static_cast<TrialPt*>(b->d_wlist2);
+    W.counts = static_cast<unsigned*>(b->d_wcounts);
+    W.rec_w = b->d_wrec_w;
+    W.rec_e = b->d_wrec_e;
+    W.rec_f = b->d_wrec_f;
+    W.part = b->d_wpart;
+  } else if (how.no_alloc) {
     const size_t need = carve_bufs(nullptr, n_insert, K, O * chain, groups * chunk, nullptr);
     if (need > (size_t)kWidomScratchBytes)
       return fail(GRASPA_ERR_ARG, "widom (no allocation): the batch needs " + std::to_string(need) +
@@ -1249,7 +1272,8 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   } else if (int rc = box_bufs(b, n_insert, K, O * chain, groups * chunk, &W)) {
     return rc;
   }
-  WidomArgs A{};
+  WidomArgs A;
+  std::memset(&A, 0, sizeof(A));   // padding too: the argument bytes key a captured graph
   A.bc = b->bc;
   A.fw = b->d_fws; A.fw_type = b->d_fws_type; A.fw_fixT = b->d_fw_fixT; A.n_fw = b->n_fw;
   A.use_cells = b->use_cells;
@@ -1278,7 +1302,8 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   for (int a = 0; a < c.molsize; ++a) qabs += std::fabs(c.q[a]);
   A.charged = (b->bc.use_charges && qabs > 1e-6) ? 1 : 0;
   A.K = K; A.O = O; A.beta = beta; A.bias = bias; A.excl = excl; A.tail_de = tail_de;
-  A.seed = seed; A.stream = stream; A.first_index = first_index; A.n_insert = n_insert;
+  A.seed = seed; A.stream = stream; A.first_index = how.first_dev ? 0 : first_index; A.n_insert = n_insert;
+  A.first_dev = how.first_dev;
   A.n_kact = b->n_active;
   A.kact = b->d_kact;
   A.ktemp_act = b->d_ktemp_act;
@@ -1327,13 +1352,22 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   A.lds_coul = (int)align16((size_t)M * T * sizeof(double4));
   A.lds_fw = (int)eval_lds_base(b, M);
   const bool cubic = A.bc.cubic != 0;
+  if (how.key) {
+    const unsigned char* a = reinterpret_cast<const unsigned char*>(&A);
+    const int shape[4] = {b->ev_waves, b->ev_per_cu, b->ev_fw_lds, groups};
+    const unsigned char* sh = reinterpret_cast<const unsigned char*>(shape);
+    how.key->assign(a, a + sizeof(A));
+    how.key->insert(how.key->end(), sh, sh + sizeof(shape));
+  }
+  if (how.args_only) return GRASPA_OK;
 
   int rc = cubic ? launch_first_bead<true>(b, A, scr_base, tab_bytes) : launch_first_bead<false>(b, A, scr_base, tab_bytes);
   if (rc) return rc;
   int egrid[2] = {0, 0};
   auto eval = [&](const TrialPt* list, const unsigned* counts, int per_ins, double2* energy, uint32_t* mask, int e) {
     unsigned long long* slot = A.stat_part + 2 * kMaxScreenGroups + (size_t)e * kMaxEvalGroups;
-    return launch_eval(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, &egrid[e], !no_alloc);
+    return launch_eval(b, A, list, counts, A.groups, A.chunk, per_ins, energy, mask, slot, &egrid[e],
+                       !no_alloc && !how.first_dev);
   };
   rc = eval(A.list1, A.counts1, K, A.e1, A.mask1, 0);
   if (rc) return rc;
@@ -1416,10 +1450,10 @@ int widom_autotune(graspa_box* b, int comp, int K, int O, double beta, int bias,
     b->ev_waves = cand[i].waves;
     b->ev_per_cu = cand[i].per_cu;
     b->ev_fw_lds = cand[i].framework_lds;
-    rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, 0u, 0, n_insert, nullptr, nullptr, false);
+    rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, 0u, 0, n_insert, nullptr, nullptr, WidomLaunch());
     rc = rc ? rc : hip_check(hipEventRecord(e0, b->stream), "widom autotune event");
     for (int r = 0; r < reps && !rc; ++r)
-      rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, 0u, 0, n_insert, nullptr, nullptr, false);
+      rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, 0u, 0, n_insert, nullptr, nullptr, WidomLaunch());
     rc = rc ? rc : hip_check(hipEventRecord(e1, b->stream), "widom autotune event");
     rc = rc ? rc : hip_check(hipEventSynchronize(e1), "widom autotune");
     float t = 0.0f;
@@ -1447,6 +1481,99 @@ int widom_autotune(graspa_box* b, int comp, int K, int O, double beta, int bias,
   b->ev_fw_lds = pick.framework_lds;
   if (rc) return rc;
   if (n_tried) *n_tried = (int)cand.size();
+  return GRASPA_OK;
+}
+
+// Advances a captured batch's first insertion index past the batch (the
+// last node of the graph).
+__global__ void widom_advance_kernel(int64_t* first, int n)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) *first += n;
+}
+
+void widom_graph_free(graspa_box* b)
+{
+  if (b->wg_exec) (void)hipGraphExecDestroy(b->wg_exec);
+  if (b->wg_graph) (void)hipGraphDestroy(b->wg_graph);
+  if (b->d_wg_first) (void)box_free(b, b->d_wg_first);
+  b->wg_exec = nullptr;
+  b->wg_graph = nullptr;
+  b->d_wg_first = nullptr;
+  b->wg_key.clear();
+  b->wg_n = 0;
+}
+
+// Captures one queued batch (launch_widom with its first index in device
+// memory, then the advance kernel) from the box's stream into a graph.  The
+// box's buffers and voxel maps are grown and built first by an uncaptured
+// argument pass, so nothing is allocated while the stream is captured.
+int widom_graph_create(graspa_box* b, int comp, int K, int O, double beta, int bias, double excl, double tail_de,
+                       uint64_t seed, uint32_t stream, int64_t first_index, int n_insert)
+{
+  if (n_insert < 1) return fail(GRASPA_ERR_ARG, "widom graph: n_insert >= 1");
+  if (hip_check(hipStreamSynchronize(b->stream), "widom graph")) return GRASPA_ERR_HIP;
+  widom_graph_free(b);
+  if (comp < 1 || comp >= (int)b->comps.size()) return fail(GRASPA_ERR_ARG, "widom: bad component");
+  const Component& c = b->comps[comp];
+  int groups = 0, chunk = 0;
+  if (K < 1 || K > kMaxTrials) return fail(GRASPA_ERR_ARG, "widom: 1 <= K,O <= 32");
+  widom_grid(b, K, n_insert, &groups, &chunk);
+  WidomBufs W{};
+  if (int rc = box_bufs(b, n_insert, K, O * std::max(c.molsize - 1, 0), groups * chunk, &W)) return rc;
+  for (int a = 0; a < c.molsize; ++a) {
+    const uint32_t* m = nullptr;
+    if (int rc = ensure_dead_map(b, c.type[a], &m)) return rc;
+  }
+  if (hip_check(box_malloc(b, &b->d_wg_first, sizeof(int64_t)), "hipMalloc widom graph")) return GRASPA_ERR_HIP;
+  if (hip_check(hipMemcpy(b->d_wg_first, &first_index, sizeof(int64_t), hipMemcpyHostToDevice), "widom graph")) {
+    widom_graph_free(b);
+    return GRASPA_ERR_HIP;
+  }
+  WidomLaunch how;
+  how.first_dev = b->d_wg_first;
+  how.key = &b->wg_key;
+  if (hip_check(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "widom graph capture")) {
+    widom_graph_free(b);
+    return GRASPA_ERR_HIP;
+  }
+  int rc = launch_widom(b, comp, K, O, beta, bias, excl, tail_de, seed, stream, first_index, n_insert, nullptr, nullptr,
+                        how);
+  if (!rc) {
+    hipLaunchKernelGGL(widom_advance_kernel, dim3(1), dim3(64), 0, b->stream, b->d_wg_first, n_insert);
+    rc = hip_check(hipGetLastError(), "widom_advance launch");
+  }
+  hipGraph_t g = nullptr;
+  const int rc_end = hip_check(hipStreamEndCapture(b->stream, &g), "widom graph capture");
+  b->wg_graph = g;
+  rc = rc ? rc : rc_end;
+  if (!rc) rc = hip_check(hipGraphInstantiate(&b->wg_exec, b->wg_graph, nullptr, nullptr, 0), "widom graph instantiate");
+  if (rc) {
+    widom_graph_free(b);
+    return rc;
+  }
+  b->wg_comp = comp; b->wg_K = K; b->wg_O = O; b->wg_bias = bias; b->wg_n = n_insert;
+  b->wg_beta = beta; b->wg_excl = excl; b->wg_tail = tail_de; b->wg_seed = seed; b->wg_stream = stream;
+  return GRASPA_OK;
+}
+
+// Queues n replays of the captured batch, after checking that the box's
+// kernel arguments for that batch are still the captured ones (molecules,
+// structure-factor buffer, framework, buffers, launch shape).
+int widom_graph_launch(graspa_box* b, int n_replays)
+{
+  if (!b->wg_exec) return fail(GRASPA_ERR_STATE, "widom graph: none captured");
+  if (n_replays < 0) return fail(GRASPA_ERR_ARG, "widom graph: n_replays >= 0");
+  std::vector<unsigned char> key;
+  WidomLaunch how;
+  how.first_dev = b->d_wg_first;
+  how.args_only = true;
+  how.key = &key;
+  int rc = launch_widom(b, b->wg_comp, b->wg_K, b->wg_O, b->wg_beta, b->wg_bias, b->wg_excl, b->wg_tail, b->wg_seed,
+                        b->wg_stream, 0, b->wg_n, nullptr, nullptr, how);
+  if (rc || key != b->wg_key)
+    return fail(GRASPA_ERR_STATE, "widom graph: the box changed since the capture; capture again");
+  for (int r = 0; r < n_replays; ++r)
+    if (hip_check(hipGraphLaunch(b->wg_exec, b->stream), "widom graph launch")) return GRASPA_ERR_HIP;
   return GRASPA_OK;
 }
 

@@ -166,6 +166,9 @@ _SIGS = {
     "graspa_widom_enqueue_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
                                        C.c_double, C.c_uint64, C.c_uint32, C.c_int64, C.c_int]),
     "graspa_widom_totals": (C.c_int, [C.c_void_p, C.POINTER(WidomSummary), C.c_int]),
+    "graspa_widom_graph_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+                                            C.c_double, C.c_uint64, C.c_uint32, C.c_int64, C.c_int]),
+    "graspa_widom_graph_launch": (C.c_int, [C.c_void_p, C.c_int]),
     "graspa_widom_set_shape": (C.c_int, [C.c_void_p, C.POINTER(EvalShape)]),
     "graspa_widom_get_shape": (C.c_int, [C.c_void_p, C.POINTER(EvalShape)]),
     "graspa_widom_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
@@ -199,6 +202,10 @@ _SIGS = {
     "graspa_batch_set_box": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "graspa_batch_set_cycles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "graspa_batch_launch": (C.c_int, [C.c_void_p, C.c_int, _i32p]),
+    "graspa_batch_launch_async": (C.c_int, [C.c_void_p, C.c_int]),
+    "graspa_batch_wait": (C.c_int, [C.c_void_p, _i32p]),
+    "graspa_batch_set_time_slice": (C.c_int, [C.c_void_p, C.c_double]),
+    "graspa_box_footprint": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "graspa_batch_progress": (C.c_int, [C.c_void_p, _i64p, _i64p, _i64p]),
 }
 
@@ -476,6 +483,24 @@ class Box:
                                                   n_insert),
                "widom_enqueue_noalloc")
 
+    def footprint(self):
+        """Device bytes the box holds now (graspa_box_footprint)."""
+        v = C.c_uint64()
+        _check(lib().graspa_box_footprint(self.h, C.byref(v)), "box_footprint")
+        return int(v.value)
+
+    def widom_graph(self, comp, n_insert, seed, stream=0, first_index=0, excl=0.0, tail_de=0.0):
+        """Captures one queued Widom batch as a HIP graph kept in the box
+        (graspa_widom_graph_create); replay r runs first_index + r n_insert."""
+        s = self.system
+        _check(lib().graspa_widom_graph_create(self.h, comp, s.n_trial_pos, s.n_trial_orient, s.beta,
+                                               int(s.vdw_real_bias), excl, tail_de, seed, stream, first_index,
+                                               n_insert), "widom_graph_create")
+
+    def widom_graph_launch(self, n_replays=1):
+        """Queues replays of the captured batch (graspa_widom_graph_launch)."""
+        _check(lib().graspa_widom_graph_launch(self.h, int(n_replays)), "widom_graph_launch")
+
     def widom_totals(self, reset=False):
         """Waits for the queued batches; summary fields summed over every batch
         since the last reset."""
@@ -700,6 +725,20 @@ class Batch:
         """One launch; returns the number of boxes still short of their cycles."""
         r = C.c_int()
         _check(lib().graspa_batch_launch(self.h, moves_per_launch, C.byref(r)), "batch_launch")
+        return r.value
+
+    def launch_async(self, moves_per_launch=0):
+        """Queues one launch and returns at once (graspa_batch_launch_async)."""
+        _check(lib().graspa_batch_launch_async(self.h, moves_per_launch), "batch_launch_async")
+
+    def set_time_slice(self, us):
+        """Per-box move budgets of about `us` microseconds per launch (0: off)."""
+        _check(lib().graspa_batch_set_time_slice(self.h, float(us)), "batch_set_time_slice")
+
+    def wait(self):
+        """Finishes the launch in flight; returns the boxes still short of their cycles."""
+        r = C.c_int()
+        _check(lib().graspa_batch_wait(self.h, C.byref(r)), "batch_wait")
         return r.value
 
     def progress(self):

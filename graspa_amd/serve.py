@@ -3,10 +3,11 @@
 Jobs (a framework, its adsorbates, T, P and a cycle count) arrive over time.
 The server keeps a fixed set of batch slots on one GPU; every job that has
 arrived is admitted into a free slot as soon as one is free and the resident
-footprint of the admitted jobs (framework, guest store, structure factors,
-tables) stays under the memory budget.  Between launches of the batch kernel
-the server retires finished jobs and admits waiting ones, so jobs that arrive
-while others run share the device with them.  Each job draws its moves from its
+footprint of the admitted jobs (the boxes' device memory as the engine
+reports it) stays under the memory budget.  Launches are queued
+asynchronously and the boxes of arriving jobs are built while a launch runs;
+between launches the server retires finished jobs and admits the built ones,
+so jobs that arrive while others run share the device with them.  Each job draws its moves from its
 own counter-based stream (seed, stream), so its trajectory is the same alone
 and in any mix.
 
@@ -36,6 +37,8 @@ class Job:
     capacity: int = 1024          # molecules per species
     # filled by the server
     admitted: Optional[float] = None
+    prepared: bool = False
+    footprint: int = 0            # device bytes of its box (graspa_box_footprint)
     finished: Optional[float] = None
     stats: Optional[engine.BatchStats] = None
     cycle_seconds: Optional[np.ndarray] = None
@@ -57,10 +60,17 @@ def footprint_bytes(system, capacity=1024):
 
 
 class Server:
-    """Continuous-batching GCMC server on one GPU."""
+    """Continuous-batching GCMC server on one GPU.
+
+    Each launch of the batch kernel is queued asynchronously; while it runs,
+    the host builds the boxes of jobs that have arrived (framework upload,
+    screen cells, structure factors on the box's own stream) so that they are
+    ready to enter the batch when the launch ends.  A job is admitted into a
+    free slot when its box's real device footprint (graspa_box_footprint)
+    fits the memory budget beside the resident and the prepared boxes."""
 
     def __init__(self, n_slots=64, precision=engine.GRASPA_MIXED, memory_budget=8 << 30, moves_per_launch=512,
-                 trace_cap=0, cycle_cap=8192, device=0):
+                 trace_cap=0, cycle_cap=8192, device=0, prepare_ahead=4, time_slice_us=0.0):
         self.n_slots = n_slots
         self.precision = precision
         self.budget = memory_budget
@@ -68,25 +78,56 @@ class Server:
         self.trace_cap = trace_cap
         self.cycle_cap = cycle_cap
         self.device = device
+        self.prepare_ahead = prepare_ahead
         self.batch = engine.Batch([], [], [], trace_cap=trace_cap, cycle_cap=cycle_cap, n_slots=n_slots)
+        if time_slice_us > 0:
+            # every box runs about this long per launch, at its own pace
+            self.batch.set_time_slice(time_slice_us)
         self.slots: List[Optional[Job]] = [None] * n_slots
         self._boxes: List[Optional[engine.Box]] = [None] * n_slots
         self._params = [None] * n_slots
-        self.resident = 0
+        self._fp = [0] * n_slots
+        self.ready = []               # (job, box, params, footprint) built and waiting for a slot
+        self.resident = 0             # bytes of the boxes in slots
+        self.prepared = 0             # bytes of the boxes built and waiting
         self.launches = 0
         self.moves = 0
+        self.peak_bytes = 0
+        self.max_concurrent = 0       # most jobs resident in one launch
 
-    def _admit(self, job, slot, now):
+    def _prepare(self, job):
+        """Builds the job's box (host set-up, uploads and S(k) on the box's stream)."""
+        est = footprint_bytes(job.system, job.capacity)
+        if est > self.budget:
+            raise ValueError("job %d needs about %d bytes, more than the memory budget" % (job.job_id, est))
         box = engine.Box(job.system, precision=self.precision, device=self.device, capacity=job.capacity)
         box.init_ewald()
         params = engine.make_mc_params(job.system, box)
-        self.batch.set_box(slot, box, params, job.seed, job.stream)
-        self.batch.set_cycles(slot, job.cycles, production=True)
-        self.slots[slot] = job
-        self._boxes[slot] = box
-        self._params[slot] = params
-        self.resident += footprint_bytes(job.system, job.capacity)
-        job.admitted = now
+        fp = box.footprint()
+        self.ready.append((job, box, params, fp))
+        self.prepared += fp
+        job.prepared = True
+
+    def _admit(self, now):
+        """Moves prepared boxes into free slots while their footprint fits."""
+        while self.ready:
+            free = [i for i, s in enumerate(self.slots) if s is None]
+            job, box, params, fp = self.ready[0]
+            if not free or self.resident + fp > self.budget:
+                break
+            self.ready.pop(0)
+            self.prepared -= fp
+            slot = free[0]
+            self.batch.set_box(slot, box, params, job.seed, job.stream)
+            self.batch.set_cycles(slot, job.cycles, production=True)
+            self.slots[slot] = job
+            self._boxes[slot] = box
+            self._params[slot] = params
+            self._fp[slot] = fp
+            self.resident += fp
+            job.admitted = now
+            job.footprint = fp
+        self.peak_bytes = max(self.peak_bytes, self.resident + self.prepared)
 
     def _retire(self, slot, now):
         job = self.slots[slot]
@@ -100,7 +141,14 @@ class Server:
         self._boxes[slot] = None
         self._params[slot] = None
         self.slots[slot] = None
-        self.resident -= footprint_bytes(job.system, job.capacity)
+        self.resident -= self._fp[slot]
+        self._fp[slot] = 0
+
+    def _can_prepare(self, job):
+        est = footprint_bytes(job.system, job.capacity)
+        n_free = sum(1 for s in self.slots if s is None)
+        return (len(self.ready) < n_free + self.prepare_ahead
+                and self.resident + self.prepared + est <= self.budget)
 
     def run(self, jobs, time_scale=1.0):
         """Serves `jobs` (sorted by arrival, seconds scaled by time_scale)
@@ -108,23 +156,27 @@ class Server:
         pending = sorted(jobs, key=lambda j: j.arrival)
         done = []
         t0 = time.perf_counter()
-        while pending or any(s is not None for s in self.slots):
+        while pending or self.ready or any(s is not None for s in self.slots):
             now = time.perf_counter() - t0
-            # admission: arrived jobs, free slot, memory budget
-            while pending and pending[0].arrival * time_scale <= now:
-                free = [i for i, s in enumerate(self.slots) if s is None]
-                need = footprint_bytes(pending[0].system, pending[0].capacity)
-                if need > self.budget:
-                    raise ValueError("job %d needs %d bytes, more than the memory budget" % (pending[0].job_id, need))
-                if not free or self.resident + need > self.budget:
-                    break
-                self._admit(pending.pop(0), free[0], now)
+            self._admit(now)
             if all(s is None for s in self.slots):
+                if pending and pending[0].arrival * time_scale <= now:
+                    # the device is idle: build the next box now
+                    self._prepare(pending.pop(0))
+                    continue
+                if self.ready:
+                    raise ValueError("job %d does not fit the memory budget" % self.ready[0][0].job_id)
                 # nothing resident: wait for the next arrival
                 time.sleep(max(0.0, min(0.01, pending[0].arrival * time_scale - now)))
                 continue
             before = self.batch.total_moves()
-            self.batch.launch(self.mpl)
+            self.max_concurrent = max(self.max_concurrent, sum(1 for x in self.slots if x is not None))
+            self.batch.launch_async(self.mpl)
+            # while the launch runs: build the boxes of arrived jobs
+            while pending and pending[0].arrival * time_scale <= time.perf_counter() - t0 and \
+                    self._can_prepare(pending[0]):
+                self._prepare(pending.pop(0))
+            self.batch.wait()
             self.launches += 1
             self.moves += self.batch.total_moves() - before
             now = time.perf_counter() - t0
@@ -140,6 +192,9 @@ class Server:
             if b is not None:
                 b.close()
                 self._boxes[i] = None
+        for _, box, _, _ in self.ready:
+            box.close()
+        self.ready = []
         self.batch.close()
 
 

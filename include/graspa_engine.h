@@ -122,6 +122,12 @@ int  graspa_box_set_framework(graspa_box* box, int n_atoms, const double* xyz, c
 int  graspa_box_add_component(graspa_box* box, int molsize, int capacity_molecules, const double* template_xyz,
                               const int* template_type, const double* template_charge, int* comp_out);
 
+/* Device memory the box holds now (framework copies and screen cells, guest
+ * store, tables, structure factors, voxel maps, Widom buffers grown so far),
+ * in bytes.  No counterpart in the reference; a job server admits boxes by
+ * it.  A batch holding the box adds its own per-slot arrays. */
+int  graspa_box_footprint(graspa_box* box, uint64_t* bytes);
+
 /* Bonded model of a flexible linear chain of at most 4 beads, grown bead by
  * bead in CBMC moves.  No counterpart in the reference, which grows rigid
  * molecules only (read_data.cpp:2104-2116).  Entry i belongs to bead i. */
@@ -371,6 +377,19 @@ void graspa_batch_destroy(graspa_batch* batch);
 int  graspa_batch_set_box(graspa_batch* batch, int slot, const graspa_batch_box* box);
 int  graspa_batch_set_cycles(graspa_batch* batch, int slot, int n_cycles, int production);
 int  graspa_batch_launch(graspa_batch* batch, int moves_per_launch, int* n_running);
+/* graspa_batch_launch in two halves, so the host can prepare boxes while the
+ * device runs: launch_async queues the launch and returns; wait finishes it
+ * (*n_running as above).  Other calls on the batch wait for a launch in
+ * flight first. */
+int  graspa_batch_launch_async(graspa_batch* batch, int moves_per_launch);
+int  graspa_batch_wait(graspa_batch* batch, int* n_running);
+/* Time slices for continuous batching (no counterpart in the reference):
+ * with us > 0 each launch gives every box a move budget of about us
+ * microseconds at its own measured time per move (device clock, updated after
+ * every launch), at most moves_per_launch, so fast boxes are not held idle
+ * until the slowest box of the launch ends; us = 0 (the default) gives every
+ * box moves_per_launch moves.  A box's trajectory does not depend on it. */
+int  graspa_batch_set_time_slice(graspa_batch* batch, double us);
 /* Advances every box by n_cycles cycles (production != 0 records Widom and
  * loading statistics); launches of at most moves_per_launch moves per box
  * (<= 0: a default) until all boxes are done.  *moves = moves made. */
@@ -455,6 +474,25 @@ int  graspa_widom_totals(graspa_box* box, graspa_widom_summary* totals, int rese
 int  graspa_widom_enqueue_noalloc(graspa_box* box, int comp, int n_trial_pos, int n_trial_orient, double beta,
                                   int vdw_real_bias, double excl, double tail_de, uint64_t seed, uint32_t stream,
                                   int64_t first_index, int n_insert);
+
+/* A queued Widom batch captured as a HIP graph (no counterpart in the
+ * reference, which launches each trial kernel from the host,
+ * mc_widom.h:440-566).  graspa_widom_graph_create captures one
+ * graspa_widom_enqueue batch with the given arguments (the box's current
+ * launch shape, buffers and voxel maps, grown and built before the capture)
+ * and keeps it in the box; its first insertion index lives in device memory
+ * and the graph's last kernel advances it by n_insert, so replay r runs the
+ * batch at first_index + r * n_insert, the same batches as enqueueing with
+ * those indices.  graspa_widom_graph_launch queues n_replays replays on the
+ * box's stream and returns at once; summaries go to the running totals
+ * (graspa_widom_totals).  A replay is refused with GRASPA_ERR_STATE when the
+ * box's arguments for the batch differ from the captured ones (molecules
+ * committed, structure factors swapped, framework, buffers or launch shape
+ * changed); capturing again fixes that.  A new capture replaces the old. */
+int  graspa_widom_graph_create(graspa_box* box, int comp, int n_trial_pos, int n_trial_orient, double beta,
+                               int vdw_real_bias, double excl, double tail_de, uint64_t seed, uint32_t stream,
+                               int64_t first_index, int n_insert);
+int  graspa_widom_graph_launch(graspa_box* box, int n_replays);
 
 int  graspa_batch_block_stats(graspa_batch* batch, int i, graspa_block_stats* out);
 

@@ -26,18 +26,21 @@ def _alone(engine_mod, job, prec):
     return st
 
 
-@pytest.mark.parametrize("prec", [1])
-def test_served_jobs_equal_alone(engine_mod, prec):
+@pytest.mark.parametrize("prec,slice_us", [(1, 0.0), (1, 200.0), (0, 200.0)])
+def test_served_jobs_equal_alone(engine_mod, prec, slice_us):
     from graspa_amd import serve
     jobs = [serve.Job(job_id=j, system=_job_system(j), cycles=4 + 3 * j, seed=900 + j, stream=j,
                       arrival=0.002 * j, capacity=128) for j in range(5)]
-    print("served case jobs=5 slots=2", flush=True)
-    srv = serve.Server(n_slots=2, precision=prec, moves_per_launch=23, trace_cap=0, cycle_cap=64)
+    print("served case jobs=5 slots=2 precision=%d slice=%g" % (prec, slice_us), flush=True)
+    srv = serve.Server(n_slots=2, precision=prec, moves_per_launch=23, trace_cap=0, cycle_cap=64,
+                       time_slice_us=slice_us)
     done = srv.run(jobs)
     srv.close()
     assert sorted(j.job_id for j in done) == list(range(5))
-    # at least one launch ran two jobs at once (continuous batching, not a queue)
-    assert srv.launches < sum(j.stats.moves for j in done) / 23
+    # launches ran two jobs at once (continuous batching, not a queue)
+    assert srv.max_concurrent == 2
+    if slice_us == 0.0:
+        assert srv.launches < sum(j.stats.moves for j in done) / 23
     for j in done:
         ref = _alone(engine_mod, serve.Job(job_id=j.job_id, system=_job_system(j.job_id), cycles=j.cycles,
                                            seed=j.seed, stream=j.stream, capacity=128), prec)
@@ -50,3 +53,24 @@ def test_served_jobs_equal_alone(engine_mod, prec):
             assert getattr(st.running, k) == getattr(ref.running, k), (j.job_id, k)
         assert np.array_equal(st.widom_w, ref.widom_w)
         assert len(j.cycle_seconds) == j.cycles and np.all(j.cycle_seconds > 0)
+
+
+def test_box_footprint_reports_device_memory(engine_mod):
+    """graspa_box_footprint counts the box's device allocations: at least the
+    framework copies and the guest store, growing when a Widom batch grows the
+    pipeline's buffers, and the same for a box built again from the same job."""
+    from graspa_amd import synth
+    s = synth.config1(n_fw=750)
+    box = engine_mod.Box(s, precision=engine_mod.GRASPA_MIXED, capacity=64)
+    box.init_ewald()
+    f0 = box.footprint()
+    n_fw = len(s.fw_type)
+    assert f0 >= n_fw * (32 + 4 + 16 + 16 + 32 + 4) + 64 * 3 * (32 + 12 + 16)
+    box.widom_batch(1, 20000, seed=3, results=False)
+    f1 = box.footprint()
+    assert f1 > f0 + 20000 * 10 * 16
+    box.close()
+    again = engine_mod.Box(s, precision=engine_mod.GRASPA_MIXED, capacity=64)
+    again.init_ewald()
+    assert again.footprint() == f0
+    again.close()

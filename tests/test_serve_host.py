@@ -1,6 +1,6 @@
 """Host logic of the served stream (no GPU): admission only of jobs that have
-arrived, only into free slots, only while the resident footprint fits the
-memory budget; slots are refilled as jobs finish.  The device batch is replaced
+arrived, only into free slots, only while the resident and prepared boxes'
+footprint fits the memory budget; slots are refilled as jobs finish.  The device batch is replaced
 by a stand-in that advances each slot by a fixed number of cycles per launch."""
 import numpy as np
 
@@ -10,9 +10,13 @@ from graspa_amd import serve, synth
 class _FakeBox:
     def __init__(self, system, precision=1, device=0, capacity=1024):
         self.system = system
+        self.capacity = capacity
 
     def init_ewald(self):
         return None
+
+    def footprint(self):
+        return serve.footprint_bytes(self.system, self.capacity)
 
     def close(self):
         pass
@@ -35,11 +39,13 @@ class _FakeBatch:
     def set_cycles(self, slot, n, production=True):
         self.tgt[slot] = self.cyc[slot] + n
 
-    def launch(self, mpl):
+    def launch_async(self, mpl):
         _FakeBatch.log.append(sum(self.on))
         for i in range(self.n):
             if self.on[i] and self.cyc[i] < self.tgt[i]:
                 self.cyc[i] += 1
+
+    def wait(self):
         return int(sum(1 for i in range(self.n) if self.on[i] and self.cyc[i] < self.tgt[i]))
 
     def total_moves(self):
@@ -75,8 +81,10 @@ def test_admission_respects_slots_budget_and_arrivals(monkeypatch):
         assert j.stats == j.cycles
         assert j.admitted <= j.finished
     # at most two jobs resident at once under this budget, and some launches ran two
-    assert max(_FakeBatch.log) == 2
-    assert srv.resident == 0
+    assert max(_FakeBatch.log) == 2 and srv.max_concurrent == 2
+    assert srv.resident == 0 and srv.prepared == 0 and not srv.ready
+    # the resident plus the prepared boxes never exceeded the budget
+    assert srv.peak_bytes <= budget
     # jobs were admitted in arrival order
     order = sorted(done, key=lambda j: j.admitted)
     assert [j.job_id for j in order] == sorted(j.job_id for j in order)
