@@ -82,6 +82,10 @@ def run_widom(system, precision, n_insert, steps, warmup, seed=2024, timing=True
     batch is captured once as a HIP graph before the timed region and each
     step is one replay (graspa_widom_graph_launch), the same batches."""
     from graspa_amd import engine
+    # torch's first import takes about a second; done here so that the GPU
+    # does not sit idle between the warmup and the timed batches (after such a
+    # gap the pair kernel ran 13% slower for the next several milliseconds)
+    _sync_torch()
     setup = {}
     t0 = time.perf_counter()
     box = engine.Box(system, precision=precision)
@@ -397,6 +401,28 @@ def run_batch_steady(n_boxes, cycles, precision, window=100, capacity=2048):
     lat = np.concatenate([bt.cycle_times(i, window) for i in range(n_boxes)])
     nm = np.array([bt.stats(i).n_mol.sum() for i in range(n_boxes)])
     per_box = np.array([bt.stats(i).moves for i in range(n_boxes)])
+    busy = bt.busy_seconds()
+    # algorithmic work of the moves made (the reference's algorithm on the
+    # same moves: every pair of the moved or trial sites with every framework
+    # and guest atom, 52 FLOP each; 16 FLOP per (active k, site) and 12 per
+    # active k of the Fourier change)
+    flop = 0.0
+    for i, sy in enumerate(systems):
+        st = bt.stats(i)
+        n_act = boxes[i].nvec()[1]
+        nf = len(sy.fw_type)
+        load = st.loading()
+        ng = float(sum(load[c] * sy.components[c].molsize for c in range(len(sy.components))))
+        for c, comp in enumerate(sy.components):
+            M = comp.molsize
+            cnt = st.counters[c]
+            body = cnt[0] + cnt[2]                     # translations, rotations with a molecule
+            swap = cnt[4] + cnt[6] + cnt[8]            # insertions, deletions, Widom
+            flop += body * (2 * M * (nf + ng) * PAIR_FLOP + n_act * (16.0 * 2 * M + 12.0))
+            flop += swap * ((sy.n_trial_pos + sy.n_trial_orient * (M - 1)) * (nf + ng) * PAIR_FLOP
+                            + n_act * (16.0 * M + 12.0))
+    kern_s = sum(r[3] for r in rows) * 1e-3
+    achieved = flop / kern_s / 1e12
     out = dict(boxes=n_boxes, cycles_per_box=cycles, moves=int(sum(r[1] for r in rows)),
                seconds=float(sum(r[2] for r in rows)), kernel_ms=float(sum(r[3] for r in rows)),
                moves_per_s=sum(r[1] for r in rows) / sum(r[2] for r in rows),
@@ -404,7 +430,15 @@ def run_batch_steady(n_boxes, cycles, precision, window=100, capacity=2048):
                steady_cycle_p50_ms=float(np.percentile(lat, 50) * 1e3),
                steady_cycle_p99_ms=float(np.percentile(lat, 99) * 1e3),
                mean_molecules_per_box=float(nm.mean()), max_molecules_per_box=int(nm.max()),
-               moves_per_box_max=int(per_box.max()), moves_per_box_median=float(np.median(per_box)))
+               moves_per_box_max=int(per_box.max()), moves_per_box_median=float(np.median(per_box)),
+               busy_fraction=float(busy.sum() / (n_boxes * kern_s)),
+               busiest_box_seconds=float(busy.max()), median_box_seconds=float(np.median(busy)),
+               roofline=dict(bound="valu", unit="TFLOP/s", achieved=achieved, peak=FP64_PEAK_TFLOPS,
+                             frac=achieved / FP64_PEAK_TFLOPS, algorithmic_flop=flop,
+                             note="algorithmic FLOP of the moves made (every pair of the moved or trial "
+                                  "sites with every atom) over the batch kernel's time; the launch lasts as "
+                                  "long as its busiest box (busy_fraction = summed box time / boxes x kernel "
+                                  "time)"))
     bt.close()
     for bx in boxes:
         bx.close()

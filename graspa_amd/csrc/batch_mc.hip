@@ -1653,7 +1653,7 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
   double* ffl = dyn;
   float* thr = reinterpret_cast<float*>(dyn + nff4);
   double2* ct = reinterpret_cast<double2*>(dyn + nff4 + nthr);
-  const int nct = MIXED ? ((B.coul_n + 3) & ~3) : 0;
+  const int nct = MIXED ? ((coul_lds_doubles(B.coul_n) + 3) & ~3) : 0;
   double2* eik = reinterpret_cast<double2*>(dyn + nff4 + nthr + nct);
   const int nint = B.coul_n / kCoulTabStride;
   // the framework goes to LDS for the launch when it fits after the tables
@@ -1715,7 +1715,8 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
     thr[i] = screen_threshold(B.ff64 + 4 * (at * T + st), B.bc.overlap, B.bc.cut_vdw2);
   }
   if (MIXED)
-    for (int i = tid; i < B.coul_n; i += kBT) dyn[nff4 + nthr + i] = B.coul[i];
+    for (int i = tid; i < B.coul_n / 2; i += kBT)
+      ct[(i / (kCoulTabStride / 2)) * kCoulLdsRow + i % (kCoulTabStride / 2)] = reinterpret_cast<const double2*>(B.coul)[i];
   __syncthreads();
   const int nd = kBDrawTrial + 3 * B.K + 6 * B.O;
   const int64_t limit = budgets ? budgets[blockIdx.x] : (int64_t)max_moves;
@@ -1978,7 +1979,7 @@ size_t box_lds(const graspa_box* b, int precision)
   const int ks = 1 + std::max(b->bc.kmax[0], std::max(b->bc.kmax[1], b->bc.kmax[2]));
   const size_t nff4 = ((size_t)4 * T * T + 3) & ~(size_t)3;
   const size_t nthr = (((size_t)T * T + 7) & ~(size_t)7) / 2;
-  const size_t nct = precision == GRASPA_MIXED ? (((size_t)b->coul_n + 3) & ~(size_t)3) : 0;
+  const size_t nct = precision == GRASPA_MIXED ? (((size_t)coul_lds_doubles(b->coul_n) + 3) & ~(size_t)3) : 0;
   return (nff4 + nthr + nct) * sizeof(double) + (size_t)2 * kBMaxMol * 3 * ks * sizeof(double2);
 }
 
@@ -2490,6 +2491,15 @@ int graspa_batch_progress(graspa_batch* bt, int64_t* moves, int64_t* cycles, int
     if (cycles) cycles[i] = on ? bt->hs[i].cycle : 0;
     if (targets) targets[i] = on ? bt->hs[i].target : 0;
   }
+  return GRASPA_OK;
+}
+
+int graspa_batch_busy(graspa_batch* bt, double* seconds)
+{
+  if (!bt || !seconds) return fail(GRASPA_ERR_ARG, "bad arguments");
+  hipSetDevice(bt->device);
+  if (int rc = download_states(bt)) return rc;
+  for (int i = 0; i < bt->n; ++i) seconds[i] = bt->boxes[i] ? (double)bt->hs[i].busy / bt->tick_hz : 0.0;
   return GRASPA_OK;
 }
 

@@ -285,6 +285,15 @@ constexpr int kCoulTabM = 1 << kCoulTabMBits;
 constexpr int kCoulTabDeg = 7;
 constexpr int kCoulTabStride = 8;    // doubles per interval (kCoulTabDeg + 1)
 static_assert(kCoulTabStride == kCoulTabDeg + 1 && kCoulTabStride % 2 == 0, "table rows are read as double2");
+// The pair kernels stage the table in LDS with rows kCoulLdsRow double2 (80 B)
+// apart, the last double2 of a row unused.  Lanes of a wave read rows of
+// different intervals at once: with rows 64 B apart a given coefficient of
+// every interval falls in one of only two 16-byte bank groups of the 32 banks,
+// with 80 B rows in one of eight.
+constexpr int kCoulLdsRow = 5;
+static_assert(kCoulLdsRow >= kCoulTabStride / 2, "an LDS row holds a table row");
+// doubles of the LDS copy of a table of coul_n doubles
+constexpr int coul_lds_doubles(int coul_n) { return coul_n / kCoulTabStride * 2 * kCoulLdsRow; }
 int coulomb_table_octaves(double cut2);
 void build_coulomb_table(double alpha, double cut2, std::vector<double>& tab);
 

@@ -83,7 +83,7 @@ __device__ inline double coul_table(const double2* __restrict__ ct, const int ni
   idx = min(max(idx, 0), nint - 1);
   const double si = (double)__uint_as_float(bits & ~((1u << kShift) - 1u));
   const double d = rr - si;
-  const double2* c = ct + (kCoulTabStride / 2) * idx;
+  const double2* c = ct + kCoulLdsRow * idx;     // the LDS copy's row stride
   double2 cc[kCoulTabStride / 2];
 #pragma unroll
   for (int k = 0; k < kCoulTabStride / 2; ++k) cc[k] = c[k];

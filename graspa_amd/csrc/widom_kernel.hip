@@ -496,7 +496,8 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
   }
   double2* ct = reinterpret_cast<double2*>(dyn + A.lds_coul);
   if (MIXED)
-    for (int i = threadIdx.x; i < A.coul_n / 2; i += blockDim.x) ct[i] = reinterpret_cast<const double2*>(A.coul)[i];
+    for (int i = threadIdx.x; i < A.coul_n / 2; i += blockDim.x)
+      ct[(i / (kCoulTabStride / 2)) * kCoulLdsRow + i % (kCoulTabStride / 2)] = reinterpret_cast<const double2*>(A.coul)[i];
   double4* fw4 = reinterpret_cast<double4*>(dyn + A.lds_fw);
   int* fwt = reinterpret_cast<int*>(dyn + A.lds_fw + (size_t)n * sizeof(double4));
   if (FW_LDS) {
@@ -1053,7 +1054,7 @@ static EvalKern eval_kernel_for(const graspa_box* b, int waves, bool fw_lds)
 static size_t eval_lds_base(const graspa_box* b, int M)
 {
   size_t off = align16((size_t)M * b->bc.ntypes * sizeof(double4));
-  if (b->precision == GRASPA_MIXED) off = align16(off + (size_t)b->coul_n * sizeof(double));
+  if (b->precision == GRASPA_MIXED) off = align16(off + (size_t)coul_lds_doubles(b->coul_n) * sizeof(double));
   return off;
 }
 

@@ -205,6 +205,7 @@ _SIGS = {
     "graspa_batch_launch_async": (C.c_int, [C.c_void_p, C.c_int]),
     "graspa_batch_wait": (C.c_int, [C.c_void_p, _i32p]),
     "graspa_batch_set_time_slice": (C.c_int, [C.c_void_p, C.c_double]),
+    "graspa_batch_busy": (C.c_int, [C.c_void_p, _f64p]),
     "graspa_box_footprint": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "graspa_batch_progress": (C.c_int, [C.c_void_p, _i64p, _i64p, _i64p]),
 }
@@ -790,6 +791,12 @@ class Batch:
         if k < 0:
             _check(k, "batch_trace")
         return buf[:k]
+
+    def busy_seconds(self):
+        """Device-clock seconds each slot's box has spent in launches."""
+        out = np.zeros(self.n)
+        _check(lib().graspa_batch_busy(self.h, _p(out)), "batch_busy")
+        return out
 
     def kernel_time(self):
         t, n = C.c_double(), C.c_int64()

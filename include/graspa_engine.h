@@ -411,6 +411,10 @@ int  graspa_batch_trace(graspa_batch* batch, int i, graspa_trace* out, int max_n
 /* Moves and cycles made and cycle targets of every slot ([n_boxes] each, 0 for
  * empty slots); any pointer may be NULL. */
 int  graspa_batch_progress(graspa_batch* batch, int64_t* moves, int64_t* cycles, int64_t* targets);
+/* Device-clock seconds each slot's box has spent in launches since it was
+ * put into the slot ([n_boxes], 0 for empty slots): a launch lasts as long as
+ * its busiest box, so these against the kernel time show the load balance. */
+int  graspa_batch_busy(graspa_batch* batch, double* seconds);
 /* Summed kernel time (HIP events) and launches of the batch since creation. */
 int  graspa_batch_kernel_time(graspa_batch* batch, double* total_ms, int64_t* launches);
 /* Copies the device state of every box back to its graspa_box. */

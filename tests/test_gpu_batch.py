@@ -149,6 +149,11 @@ def test_batch_alone_equals_in_mix(engine_mod, prec):
     sa, sm = alone.stats(0), mix.stats(1)
     assert np.array_equal(sa.n_mol, sm.n_mol)
     assert np.array_equal(sa.counters, sm.counters)
+    # every box's busy time is inside the launches that ran it
+    busy = mix.busy_seconds()
+    kern_ms, launches = mix.kernel_time()
+    assert launches > 1 and np.all(busy > 0), busy
+    assert busy.max() <= kern_ms * 1e-3 * 1.01 + 1e-4, (busy, kern_ms)
     alone.close()
     mix.close()
 
