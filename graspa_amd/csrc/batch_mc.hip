@@ -237,11 +237,23 @@ __device__ inline void wave_sum_group(const double av[kBGroup], const double ar[
   if ((lane & (64 / NV - 1)) == 0) out[idx] = v[0];
 }
 
+// OR over the 64 lanes (every lane active), uniform result: the steps of
+// wave_sum on both 32-bit halves (data-parallel moves within the SIMD, no LDS
+// round trips as the lane shuffles take), lane 63 read back.
+__device__ inline uint32_t wave_or32(uint32_t v)
+{
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xf, 0xf, false);    // quad_perm [1,0,3,2]
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xf, 0xf, false);    // quad_perm [2,3,0,1]
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xf, 0xf, false);   // row_ror:4
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xf, 0xf, false);   // row_ror:8
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);   // row_bcast:15
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);   // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
 __device__ inline unsigned long long wave_or64(unsigned long long v)
 {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v |= __shfl_xor(v, m, 64);
-  return v;
+  const uint32_t lo = wave_or32((uint32_t)v), hi = wave_or32((uint32_t)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
 }
 
 // Semantics of one pair: Calculate_Multiple_Trial_Energy_VDWReal

@@ -68,7 +68,13 @@ __device__ inline void axis_table(const BoxConst& bc, double px, double py, doub
   sincos(s, &sn, &cs);
   const double2 e1 = make_double2(cs, sn);
   if (km >= 1) out[1] = e1;
-  for (int k = 2; k <= km; ++k) out[k] = cmul2(out[k - 1], e1);
+  // the running power stays in registers (out may be LDS: no store-to-load
+  // round trip per step)
+  double2 ek = e1;
+  for (int k = 2; k <= km; ++k) {
+    ek = cmul2(ek, e1);
+    out[k] = ek;
+  }
 }
 
 // erfc(alpha r)/r from the per-box table (coulomb_table.cpp): the interval
@@ -83,7 +89,9 @@ __device__ inline double coul_table(const double2* __restrict__ ct, const int ni
   idx = min(max(idx, 0), nint - 1);
   const double si = (double)__uint_as_float(bits & ~((1u << kShift) - 1u));
   const double d = rr - si;
-  const double2* c = ct + kCoulLdsRow * idx;     // the LDS copy's row stride
+  // the LDS copy's row stride; idx < 2^24, and the 24-bit multiply issues at
+  // full rate where a 32-bit one takes four cycles
+  const double2* c = ct + __umul24((unsigned)idx, (unsigned)kCoulLdsRow);
   double2 cc[kCoulTabStride / 2];
 #pragma unroll
   for (int k = 0; k < kCoulTabStride / 2; ++k) cc[k] = c[k];
