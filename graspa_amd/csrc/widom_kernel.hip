@@ -204,42 +204,49 @@ __device__ inline void orient_pos(const WidomArgs& A, uint64_t idx, int o, int a
 // thresholds th[atom type]; stops at the first hit.  Guests are not screened:
 // their overlaps are found by the exact evaluation.
 template <bool CUBIC>
-__device__ inline bool screen_lane(const WidomArgs& A, const uint4* __restrict__ tab, const float* __restrict__ th,
-                                   const uint4 P, unsigned long long& st)
+__device__ inline bool screen_row(const WidomArgs& A, const uint4* __restrict__ tab, const float* __restrict__ th,
+                                  const uint4 P, const int r, unsigned long long& st)
 {
+  // row r (0..8) of the 3 x 3 x 3 cells around P's cell: dz = r / 3 - 1, dy = r % 3 - 1
   const int C = A.cell_cap, n0 = A.cell_n[0], n1 = A.cell_n[1], n2 = A.cell_n[2];
   const int cx = (int)(((uint64_t)P.x * (uint64_t)n0) >> 32);
   const int cy = (int)(((uint64_t)P.y * (uint64_t)n1) >> 32);
   const int cz = (int)(((uint64_t)P.z * (uint64_t)n2) >> 32);
   const int xm = cx == 0 ? n0 - 1 : cx - 1, xp = cx == n0 - 1 ? 0 : cx + 1;
-  for (int dz = -1; dz <= 1; ++dz) {
-    int z = cz + dz;
-    z = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
-    for (int dy = -1; dy <= 1; ++dy) {
-      int y = cy + dy;
-      y = y < 0 ? y + n1 : (y >= n1 ? y - n1 : y);
-      const int row = __mul24(n0, y + __mul24(n1, z));
-      const int cb[3] = {__mul24(row + xm, C), __mul24(row + cx, C), __mul24(row + xp, C)};
-      // a row of three cells per step, up to four slots each, loaded together
-      // (slots fill from the front, empty ones have type ~0u)
-      for (int k0 = 0; k0 < C; k0 += 4) {
-        uint4 at[12];
+  int z = cz + r / 3 - 1;
+  z = z < 0 ? z + n2 : (z >= n2 ? z - n2 : z);
+  int y = cy + r % 3 - 1;
+  y = y < 0 ? y + n1 : (y >= n1 ? y - n1 : y);
+  const int row = __mul24(n0, y + __mul24(n1, z));
+  const int cb[3] = {__mul24(row + xm, C), __mul24(row + cx, C), __mul24(row + xp, C)};
+  // a row of three cells per step, up to four slots each, loaded together
+  // (slots fill from the front, empty ones have type ~0u)
+  for (int k0 = 0; k0 < C; k0 += 4) {
+    uint4 at[12];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) at[4 * c + j] = (k0 + j < C) ? tab[cb[c] + k0 + j] : make_uint4(0u, 0u, 0u, ~0u);
-        bool hit = false;
+      for (int j = 0; j < 4; ++j) at[4 * c + j] = (k0 + j < C) ? tab[cb[c] + k0 + j] : make_uint4(0u, 0u, 0u, ~0u);
+    bool hit = false;
 #pragma unroll
-        for (int q = 0; q < 12; ++q) {
-          if (at[q].w != ~0u) {
-            ++st;
-            hit = hit || (r2_fixed32<CUBIC>(at[q].x, at[q].y, at[q].z, P.x, P.y, P.z, A.bc.h32) < th[at[q].w]);
-          }
-        }
-        if (hit) return true;
+    for (int q = 0; q < 12; ++q) {
+      if (at[q].w != ~0u) {
+        ++st;
+        hit = hit || (r2_fixed32<CUBIC>(at[q].x, at[q].y, at[q].z, P.x, P.y, P.z, A.bc.h32) < th[at[q].w]);
       }
     }
+    if (hit) return true;
   }
+  return false;
+}
+
+// The whole 3 x 3 x 3 neighbourhood of P on one lane, row by row.
+template <bool CUBIC>
+__device__ inline bool screen_lane(const WidomArgs& A, const uint4* __restrict__ tab, const float* __restrict__ th,
+                                   const uint4 P, unsigned long long& st)
+{
+  for (int r = 0; r < 9; ++r)
+    if (screen_row<CUBIC>(A, tab, th, P, r, st)) return true;
   return false;
 }
 
@@ -415,9 +422,22 @@ __global__ __launch_bounds__(kFbWaves * 64) void widom_first_bead_kernel(const W
     }
     __syncthreads();
     if (tid == 0) q_n[(step + 1) & 1] = 0u;
+    // the queued trials, nine lanes each (lane 9t + r of a wave scans row r
+    // of queued trial t's neighbourhood; 7 trials per wave), rows joined by
+    // a ballot: a single lane per trial would walk the 27 cells in series
     const unsigned nq = *qn;
-    for (unsigned q = tid; q < nq; q += kFbWaves * 64)
-      q_hit[q_t[q]] = screen_lane<CUBIC>(A, tab, thr, q_p[q], st) ? 1 : 0;
+    {
+      const int tw = lane / 9, tr = lane - 9 * tw;
+      constexpr int kPerRound = kFbWaves * 7;
+      const int rounds = ((int)nq + kPerRound - 1) / kPerRound;
+      for (int rd = 0; rd < rounds; ++rd) {
+        const int q = rd * kPerRound + wv * 7 + tw;
+        const bool sact = tw < 7 && q < (int)nq;
+        const bool h = sact && screen_row<CUBIC>(A, tab, thr, q_p[q], tr, st);
+        const unsigned long long hb = __ballot(h);
+        if (sact && tr == 0) q_hit[q_t[q]] = ((hb >> (9 * tw)) & 0x1ffull) != 0ull ? 1 : 0;
+      }
+    }
     __syncthreads();
     if (undecided) hit = q_hit[tid] != 0;
     const unsigned long long hits = __ballot(act && hit);
