@@ -479,6 +479,9 @@ __global__ __launch_bounds__(kFbWaves * 64) void widom_first_bead_kernel(const W
   }
 }
 
+// Atoms of the pair kernel's LDS framework copy: whole steps of 128 (two per lane).
+__host__ __device__ constexpr int eval_fw_pad(int n) { return (n + 127) & ~127; }
+
 // B / D: full energies of the trial sites of a work list (VDW_Coulomb.cu:
 // 1183-1352 for one trial site): wave sums of LJ and real-space Coulomb and
 // the exact overlap flag; one wave per site, 64 framework atoms per step.
@@ -518,12 +521,18 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
   if (MIXED)
     for (int i = threadIdx.x; i < A.coul_n / 2; i += blockDim.x)
       ct[(i / (kCoulTabStride / 2)) * kCoulLdsRow + i % (kCoulTabStride / 2)] = reinterpret_cast<const double2*>(A.coul)[i];
+  // the LDS copy is padded to whole steps of two atoms per lane with atoms at
+  // NaN positions: every comparison with their r^2 is false, so they add
+  // nothing, flag nothing and the steps need no bound check
+  const int npad = eval_fw_pad(n);
   double4* fw4 = reinterpret_cast<double4*>(dyn + A.lds_fw);
-  int* fwt = reinterpret_cast<int*>(dyn + A.lds_fw + (size_t)n * sizeof(double4));
+  int* fwt = reinterpret_cast<int*>(dyn + A.lds_fw + (size_t)npad * sizeof(double4));
   if (FW_LDS) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      fw4[i] = make_double4(A.fw[i], A.fw[n + i], A.fw[2 * n + i], A.fw[3 * n + i]);
-      fwt[i] = A.fw_type[i];
+    const double qnan = __builtin_nan("");
+    for (int i = threadIdx.x; i < npad; i += blockDim.x) {
+      fw4[i] = i < n ? make_double4(A.fw[i], A.fw[n + i], A.fw[2 * n + i], A.fw[3 * n + i])
+                     : make_double4(qnan, qnan, qnan, 0.0);
+      fwt[i] = i < n ? A.fw_type[i] : 0;
     }
   }
   __syncthreads();
@@ -548,9 +557,10 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
     if (MIXED) {
       // straight-line pair terms with selects for the cutoffs, two atoms per
       // lane per step (independent chains) under one wave-uniform branch that
-      // skips atom pairs beyond both cutoffs for all lanes
+      // skips atom pairs beyond both cutoffs for all lanes.  r sums q_atom g(r^2)
+      // (one FMA on the cutoff-selected g); the prefactor and the site's charge
+      // multiply the wave's sum once
       const double cmax = coul ? fmax(bc.cut_vdw2, bc.cut_coul2) : bc.cut_vdw2;
-      const double qs = bc.prefactor * sq;
       auto pair_fast = [&](double rr, double aq, int ty, double& v_, double& r_, bool& o_) {
         const double4 ff = fr[ty];
         const double e = lj_mixed(rr, ff.x, ff.y, ff.z);
@@ -559,12 +569,27 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
         o_ = o_ || (inv && ((e > bc.overlap) || (rr < 0.01)));
         if (coul) {
           const double g = coul_table(ct, nint, rr);
-          r_ += (rr < bc.cut_coul2) ? (qs * aq) * g : 0.0;
+          r_ = fma(aq, (rr < bc.cut_coul2) ? g : 0.0, r_);
         }
       };
       double v1 = 0.0, r1 = 0.0;
       bool o1 = false;
-      for (int j = lane; j < n; j += 128) {
+      if (FW_LDS) {
+        for (int j = lane; j < npad; j += 128) {
+          const double4 a0 = fw4[j], a1 = fw4[j + 64];
+          const int ty0 = fwt[j], ty1 = fwt[j + 64];
+          double dx0 = a0.x - px, dy0 = a0.y - py, dz0 = a0.z - pz;
+          double dx1 = a1.x - px, dy1 = a1.y - py, dz1 = a1.z - pz;
+          pbc_rint<CUBIC>(dx0, dy0, dz0, bc.cell, bc.inv);
+          pbc_rint<CUBIC>(dx1, dy1, dz1, bc.cell, bc.inv);
+          const double rr0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+          const double rr1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+          if (__any(rr0 < cmax || rr1 < cmax)) {
+            pair_fast(rr0, a0.w, ty0, v, r, o);
+            pair_fast(rr1, a1.w, ty1, v1, r1, o1);
+          }
+        }
+      } else for (int j = lane; j < n; j += 128) {
         const int k = j + 64;
         const bool has1 = k < n;
         double ax0, ay0, az0, aq0, ax1 = 0.0, ay1 = 0.0, az1 = 0.0, aq1 = 0.0;
@@ -646,7 +671,7 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
         }
         if (coul && rr < bc.cut_coul2) {
           if (MIXED) {
-            r += (bc.prefactor * gq * sq) * coul_table(ct, nint, rr);
+            r = fma(gq, coul_table(ct, nint, rr), r);    // scaled with the framework sum below
           } else {
             const double rd = sqrt(rr);
             r += bc.prefactor * 1.0 * (gq * sq * erfc(bc.alpha * rd)) / rd;
@@ -656,7 +681,7 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
     }
     n_eval += npair;
     const double sv = wave_sum(v);
-    const double sr = coul ? wave_sum(r) : 0.0;
+    const double sr = coul ? (MIXED ? (bc.prefactor * sq) * wave_sum(r) : wave_sum(r)) : 0.0;
     const bool ov = __any(o) != 0;
     if (lane == 0) {
       energy[P.slot] = make_double2(sv, sr);
@@ -1080,7 +1105,7 @@ static size_t eval_lds_base(const graspa_box* b, int M)
 
 static size_t eval_lds(const graspa_box* b, size_t base, bool fw_lds)
 {
-  return base + (fw_lds ? align16((size_t)b->n_fw * (sizeof(double4) + sizeof(int))) : 0);
+  return base + (fw_lds ? align16((size_t)eval_fw_pad(b->n_fw) * (sizeof(double4) + sizeof(int))) : 0);
 }
 
 // Grid of one pair-energy launch: per_cu workgroups per CU (0: as many as
