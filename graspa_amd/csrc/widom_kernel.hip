@@ -145,7 +145,7 @@ struct WidomArgs {
   unsigned long long* stat;   // screened pairs, evaluated pairs (sums of stat_part)
   // LDS layout (bytes from the start of the dynamic area)
   int lds_thr, lds_tab;                 // screening kernels
-  int lds_ffl, lds_coul, lds_fw;        // pair-energy kernel
+  int lds_ffl, lds_coul, lds_fw;        // pair-energy kernel (lds_coul is 0)
 };
 
 // First insertion index of the batch.
@@ -517,7 +517,9 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
     const double* f = A.ff64 + 4 * (ty * T + A.site_type[a]);
     ffl[i] = make_double4(f[0], f[1], f[2], 0.0);
   }
-  double2* ct = reinterpret_cast<double2*>(dyn + A.lds_coul);
+  // the table sits at LDS offset 0 (lds_coul == 0): a row's address is one
+  // multiply of its index
+  double2* ct = reinterpret_cast<double2*>(dyn);
   if (MIXED)
     for (int i = threadIdx.x; i < A.coul_n / 2; i += blockDim.x)
       ct[(i / (kCoulTabStride / 2)) * kCoulLdsRow + i % (kCoulTabStride / 2)] = reinterpret_cast<const double2*>(A.coul)[i];
@@ -1096,11 +1098,13 @@ static EvalKern eval_kernel_for(const graspa_box* b, int waves, bool fw_lds)
 
 // LDS of the pair-energy kernel before the framework: FF rows of the species'
 // sites, then the real-space table (fast form).
+static size_t eval_lds_coul_bytes(const graspa_box* b)
+{
+  return b->precision == GRASPA_MIXED ? align16((size_t)coul_lds_doubles(b->coul_n) * sizeof(double)) : 0;
+}
 static size_t eval_lds_base(const graspa_box* b, int M)
 {
-  size_t off = align16((size_t)M * b->bc.ntypes * sizeof(double4));
-  if (b->precision == GRASPA_MIXED) off = align16(off + (size_t)coul_lds_doubles(b->coul_n) * sizeof(double));
-  return off;
+  return align16(eval_lds_coul_bytes(b) + (size_t)M * b->bc.ntypes * sizeof(double4));
 }
 
 static size_t eval_lds(const graspa_box* b, size_t base, bool fw_lds)
@@ -1393,9 +1397,9 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   A.lds_tab = (int)scr_base;
   const size_t tab_bytes =
       A.use_cells ? (size_t)b->cell_n[0] * b->cell_n[1] * b->cell_n[2] * b->cell_cap * sizeof(uint4) : 0;
-  // pair-energy kernel: FF rows | Coulomb table (fast form) | framework (optional)
-  A.lds_ffl = 0;
-  A.lds_coul = (int)align16((size_t)M * T * sizeof(double4));
+  // pair-energy kernel: Coulomb table (fast form) | FF rows | framework (optional)
+  A.lds_coul = 0;
+  A.lds_ffl = (int)eval_lds_coul_bytes(b);
   A.lds_fw = (int)eval_lds_base(b, M);
   const bool cubic = A.bc.cubic != 0;
   if (how.key) {
