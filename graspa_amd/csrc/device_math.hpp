@@ -132,7 +132,7 @@ __device__ inline double lj_mixed(double rr, double a4e, double s2, double shift
   y = y * (2.0 - rr * y);
   const double sr = s2 * y;
   const double s3 = sr * sr * sr;
-  return a4e * (s3 * (s3 - 1.0)) - shift;
+  return fma(a4e, fma(s3, s3, -s3), -shift);      // s3 (s3 - 1) in one rounding
 }
 
 // exp(y) for -708 < y <= 0: Cody-Waite reduction y = n ln2 + r with a

@@ -37,6 +37,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 #include "device_math.hpp"
 #include "engine.hpp"
@@ -87,6 +88,7 @@ struct WidomArgs {
   int n_fw;
   int use_cells;
   int screen_chain;             // cell screen of the orientation sites (large frameworks only)
+  int iso;                      // cubic cell with equal edges: the fast form's pair kernel works in cell fractions
   int cell_n[3];
   int cell_cap;
   const uint4* ctab;       // [cells * cell_cap] fixed-point x,y,z + type, ~0u = empty
@@ -479,6 +481,15 @@ __global__ __launch_bounds__(kFbWaves * 64) void widom_first_bead_kernel(const W
   }
 }
 
+// a * b rounded on its own: never fused with the operation that uses it (the
+// cell fractions of the pair kernel must round alike whether they are staged
+// in LDS or computed as the framework is read).
+__device__ inline double mul_unfused(double a, double b)
+{
+#pragma clang fp contract(off)
+  return a * b;
+}
+
 // Atoms of the pair kernel's LDS framework copy: whole steps of 128 (two per lane).
 __host__ __device__ constexpr int eval_fw_pad(int n) { return (n + 127) & ~127; }
 
@@ -531,8 +542,13 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
   int* fwt = reinterpret_cast<int*>(dyn + A.lds_fw + (size_t)npad * sizeof(double4));
   if (FW_LDS) {
     const double qnan = __builtin_nan("");
+    // fast form in an equal-edged cube: x, y, z in cell fractions
+    const double sc = (MIXED && A.iso) ? bc.inv[0] : 1.0;
     for (int i = threadIdx.x; i < npad; i += blockDim.x) {
-      fw4[i] = i < n ? make_double4(A.fw[i], A.fw[n + i], A.fw[2 * n + i], A.fw[3 * n + i])
+      // (products rounded on their own, as the path that reads the framework
+      // from memory rounds them)
+      fw4[i] = i < n ? make_double4(mul_unfused(A.fw[i], sc), mul_unfused(A.fw[n + i], sc), mul_unfused(A.fw[2 * n + i], sc),
+                                    A.fw[3 * n + i])
                      : make_double4(qnan, qnan, qnan, 0.0);
       fwt[i] = i < n ? A.fw_type[i] : 0;
     }
@@ -576,50 +592,65 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
       };
       double v1 = 0.0, r1 = 0.0;
       bool o1 = false;
-      if (FW_LDS) {
-        for (int j = lane; j < npad; j += 128) {
-          const double4 a0 = fw4[j], a1 = fw4[j + 64];
-          const int ty0 = fwt[j], ty1 = fwt[j + 64];
-          double dx0 = a0.x - px, dy0 = a0.y - py, dz0 = a0.z - pz;
-          double dx1 = a1.x - px, dy1 = a1.y - py, dz1 = a1.z - pz;
-          pbc_rint<CUBIC>(dx0, dy0, dz0, bc.cell, bc.inv);
-          pbc_rint<CUBIC>(dx1, dy1, dz1, bc.cell, bc.inv);
-          const double rr0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
-          const double rr1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-          if (__any(rr0 < cmax || rr1 < cmax)) {
-            pair_fast(rr0, a0.w, ty0, v, r, o);
-            pair_fast(rr1, a1.w, ty1, v1, r1, o1);
+      // squared distance to the site: in an equal-edged cube from cell
+      // fractions (13 instructions: per axis a difference, its nearest
+      // integer removed; then L^2 times the sum), else from Cartesian
+      // coordinates through pbc_rint (15)
+      const double Lc = bc.cell[0], L2 = Lc * Lc, iL = bc.inv[0];
+      const double fx = mul_unfused(px, iL), fy = mul_unfused(py, iL), fz = mul_unfused(pz, iL);
+      auto run = [&](auto iso_c) {
+        constexpr bool ISO = decltype(iso_c)::value;
+        // (the sums as explicit FMAs: both loops below round them alike, so
+        // every launch shape gives the same sums)
+        auto dist2 = [&](double ax, double ay, double az) {
+          if constexpr (ISO) {
+            double dx = ax - fx, dy = ay - fy, dz = az - fz;
+            dx -= rint(dx); dy -= rint(dy); dz -= rint(dz);
+            return L2 * fma(dz, dz, fma(dy, dy, dx * dx));
+          } else {
+            double dx = ax - px, dy = ay - py, dz = az - pz;
+            pbc_rint<CUBIC>(dx, dy, dz, bc.cell, bc.inv);
+            return fma(dz, dz, fma(dy, dy, dx * dx));
           }
-        }
-      } else for (int j = lane; j < n; j += 128) {
-        const int k = j + 64;
-        const bool has1 = k < n;
-        double ax0, ay0, az0, aq0, ax1 = 0.0, ay1 = 0.0, az1 = 0.0, aq1 = 0.0;
-        int ty0, ty1 = 0;
+        };
         if (FW_LDS) {
-          const double4 a0 = fw4[j];
-          ax0 = a0.x; ay0 = a0.y; az0 = a0.z; aq0 = a0.w; ty0 = fwt[j];
-          if (has1) {
-            const double4 a1 = fw4[k];
-            ax1 = a1.x; ay1 = a1.y; az1 = a1.z; aq1 = a1.w; ty1 = fwt[k];
+          for (int j = lane; j < npad; j += 128) {
+            const double4 a0 = fw4[j], a1 = fw4[j + 64];
+            const int ty0 = fwt[j], ty1 = fwt[j + 64];
+            const double rr0 = dist2(a0.x, a0.y, a0.z);
+            const double rr1 = dist2(a1.x, a1.y, a1.z);
+            if (__any(rr0 < cmax || rr1 < cmax)) {
+              pair_fast(rr0, a0.w, ty0, v, r, o);
+              pair_fast(rr1, a1.w, ty1, v1, r1, o1);
+            }
           }
         } else {
-          ax0 = A.fw[j]; ay0 = A.fw[n + j]; az0 = A.fw[2 * n + j]; aq0 = A.fw[3 * n + j]; ty0 = A.fw_type[j];
-          if (has1) {
-            ax1 = A.fw[k]; ay1 = A.fw[n + k]; az1 = A.fw[2 * n + k]; aq1 = A.fw[3 * n + k]; ty1 = A.fw_type[k];
+          const double sc = ISO ? iL : 1.0;
+          for (int j = lane; j < n; j += 128) {
+            const int k = j + 64;
+            const bool has1 = k < n;
+            const double ax0 = mul_unfused(A.fw[j], sc), ay0 = mul_unfused(A.fw[n + j], sc);
+            const double az0 = mul_unfused(A.fw[2 * n + j], sc);
+            const double aq0 = A.fw[3 * n + j];
+            const int ty0 = A.fw_type[j];
+            double ax1 = 0.0, ay1 = 0.0, az1 = 0.0, aq1 = 0.0;
+            int ty1 = 0;
+            if (has1) {
+              ax1 = mul_unfused(A.fw[k], sc); ay1 = mul_unfused(A.fw[n + k], sc); az1 = mul_unfused(A.fw[2 * n + k], sc);
+              aq1 = A.fw[3 * n + k];
+              ty1 = A.fw_type[k];
+            }
+            const double rr0 = dist2(ax0, ay0, az0);
+            const double rr1 = has1 ? dist2(ax1, ay1, az1) : 1e300;   // 1e300: contributes nothing
+            if (__any(rr0 < cmax || rr1 < cmax)) {
+              pair_fast(rr0, aq0, ty0, v, r, o);
+              pair_fast(rr1, aq1, ty1, v1, r1, o1);
+            }
           }
         }
-        double dx0 = ax0 - px, dy0 = ay0 - py, dz0 = az0 - pz;
-        double dx1 = ax1 - px, dy1 = ay1 - py, dz1 = az1 - pz;
-        pbc_rint<CUBIC>(dx0, dy0, dz0, bc.cell, bc.inv);
-        pbc_rint<CUBIC>(dx1, dy1, dz1, bc.cell, bc.inv);
-        const double rr0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
-        const double rr1 = has1 ? dx1 * dx1 + dy1 * dy1 + dz1 * dz1 : 1e300;   // 1e300: contributes nothing
-        if (__any(rr0 < cmax || rr1 < cmax)) {
-          pair_fast(rr0, aq0, ty0, v, r, o);
-          pair_fast(rr1, aq1, ty1, v1, r1, o1);
-        }
-      }
+      };
+      if (CUBIC && A.iso) run(std::true_type{});
+      else run(std::false_type{});
       v += v1;
       r += r1;
       o = o || o1;
@@ -1351,6 +1382,7 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   double qabs = 0.0;
   for (int a = 0; a < c.molsize; ++a) qabs += std::fabs(c.q[a]);
   A.charged = (b->bc.use_charges && qabs > 1e-6) ? 1 : 0;
+  A.iso = (b->bc.cubic && b->bc.cell[0] == b->bc.cell[4] && b->bc.cell[0] == b->bc.cell[8]) ? 1 : 0;
   A.K = K; A.O = O; A.beta = beta; A.bias = bias; A.excl = excl; A.tail_de = tail_de;
   A.seed = seed; A.stream = stream; A.first_index = how.first_dev ? 0 : first_index; A.n_insert = n_insert;
   A.first_dev = how.first_dev;
