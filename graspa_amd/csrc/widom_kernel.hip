@@ -319,6 +319,60 @@ __device__ inline void wave_select(double ev, double er, uint32_t ovm, int n, do
   W_out = W;
 }
 
+// wave_select for the insertion of this lane's 16-lane row (its trials in
+// lanes 0 .. n-1 of the row, n <= 16): four insertions per wave.  The same
+// arithmetic in the same order: the row maximum by the same maximum steps, the
+// ordered sums over the survivors left to right by a scan along the row (after
+// step t, lane t holds ((s_0 + s_1) + ...) + s_t; screened trials add 0.0),
+// and the cumulative pick is the first survivor whose running sum is not below
+// u times the total, else the last survivor, as the sequential loop stops.
+__device__ inline void row_select(double ev, double er, bool alive, int n, double u, double beta, int bias,
+                                  int& sel_out, double& W_out)
+{
+  const int lane = threadIdx.x & 63, rl = lane & 15, rb0 = lane & 48;
+  double lw = -INFINITY;
+  if (alive) {
+    double tot = ev;
+    if (bias) tot += er;
+    lw = -beta * tot;
+  }
+  double mx = lw;
+  mx = fmax(mx, dpp_d<0xB1>(mx));         // quad_perm [1,0,3,2]
+  mx = fmax(mx, dpp_d<0x4E>(mx));         // quad_perm [2,3,0,1]
+  mx = fmax(mx, dpp_d<0x124>(mx));        // row_ror:4
+  mx = fmax(mx, dpp_d<0x128>(mx));        // row_ror:8
+  const double shf = alive ? exp(lw - mx) : -1.0;
+  const double ew = alive ? exp(lw) : 0.0;
+  const double vs = alive ? shf : 0.0, ve = alive ? ew : 0.0;
+  double cs = vs, ce = ve;
+  for (int t = 1; t < n; ++t) {
+    const double ps = dpp_d<0x111>(cs);   // row_shr:1: lane t reads lane t - 1
+    const double pe = dpp_d<0x111>(ce);
+    if (rl == t) {
+      cs = ps + vs;
+      ce = pe + ve;
+    }
+  }
+  const double ssum = __shfl(cs, rb0 + n - 1, 64);
+  const double rb = __shfl(ce, rb0 + n - 1, 64);
+  const uint32_t arow = (uint32_t)((__ballot(alive) >> rb0) & 0xffffull);
+  const int nk = __popc(arow);
+  const double ws = u * ssum;
+  const uint32_t crow = (uint32_t)((__ballot(alive && !(cs < ws)) >> rb0) & 0xffffull);
+  int sel = -1;
+  if (nk > 0) sel = crow ? __ffs((int)crow) - 1 : 31 - __clz((int)arow);
+  double W = 0.0;
+  const double er_sel = __shfl(er, rb0 + (sel >= 0 ? sel : 0), 64);
+  if (nk > 0 && !(rb < 1e-150)) {
+    W = rb / (double)n;
+    if (!bias) W *= exp(-beta * er_sel);
+  } else {
+    sel = -1;
+  }
+  sel_out = sel;
+  W_out = W;
+}
+
 // Screen thresholds and the cell table of a screening workgroup, in LDS.
 template <bool TAB_LDS>
 __device__ inline const uint4* stage_screen(const WidomArgs& A, char* dyn, float*& thr)
@@ -757,75 +811,131 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
   TrialPt* sub = A.list2 + (size_t)i0 * O * chain;
   const unsigned long long lt = (1ull << lane) - 1ull;
   unsigned long long st = 0;
-  for (unsigned li = wv; li < n_live; li += kInsWaves) {
-    const int ins = A.live[i0 + li];
-    const uint64_t idx = (uint64_t)(first_of(A) + ins);
-    const uint32_t dead = A.mask1[ins];
-    double ev = 0.0, er = 0.0;
-    if (lane < K && !((dead >> lane) & 1u)) {
-      const double2 e = A.e1[(size_t)ins * K + lane];
-      ev = e.x;
-      er = e.y;
-    }
-    int sel;
-    double Wfb;
-    wave_select(ev, er, dead, K, draw(A, idx, 3 * K), A.beta, A.bias, sel, Wfb);
-    const bool fb_ok = sel >= 0 && Wfb > 1e-150;
-    InsState S;
-    S.sel = fb_ok ? sel : -1;
-    S.ok = fb_ok ? 1 : 0;
-    S.wfb = Wfb;
-    S.ev = fb_ok ? readlane_d(ev, sel) : 0.0;
-    S.er = fb_ok ? readlane_d(er, sel) : 0.0;
-    S.fb[0] = S.fb[1] = S.fb[2] = 0.0;
-    if (fb_ok) {
+  // orientations of a live insertion whose first bead was placed (the whole
+  // wave; lane o * chain + a is orientation o's chain atom a)
+  auto orient = [&](const int ins, const uint64_t idx, const double fb0, const double fb1, const double fb2) {
+    const int o = lane / chain, a = lane - o * chain;
+    const bool act = lane < O * chain;
+    double p[3] = {0.0, 0.0, 0.0};
+    bool hit = false;
+    if (act) {
       double u[3];
-      draw3(A, idx, 3 * sel, u);
-      S.fb[0] = A.bc.cell[0] * u[0];
-      S.fb[1] = A.bc.cell[4] * u[1];
-      S.fb[2] = A.bc.cell[8] * u[2];
+      draw3(A, idx, 3 * K + 1 + 3 * o, u);
+      double vec[3] = {A.tvec[a][0], A.tvec[a][1], A.tvec[a][2]};
+      quat_rotate(vec, u[0], u[1], u[2]);
+      p[0] = fb0 + vec[0]; p[1] = fb1 + vec[1]; p[2] = fb2 + vec[2];
+      const uint4 P = to_fixed4(A.bc, p);
+      if (A.dmap[1 + a]) hit = dead_voxel(A.dmap[1 + a], A.dnv, P);
+      if (!hit && A.screen_chain) hit = screen_lane<CUBIC>(A, A.ctab, thr + (1 + a) * T, P, st);
     }
-    if (lane == 0) {
-      A.state[ins] = S;
-      A.mask2[ins] = 0u;
-    }
-    if (fb_ok && chain > 0) {
-      const int o = lane / chain, a = lane - o * chain;
-      const bool act = lane < O * chain;
-      double p[3] = {0.0, 0.0, 0.0};
-      bool hit = false;
-      if (act) {
-        double u[3];
-        draw3(A, idx, 3 * K + 1 + 3 * o, u);
-        double vec[3] = {A.tvec[a][0], A.tvec[a][1], A.tvec[a][2]};
-        quat_rotate(vec, u[0], u[1], u[2]);
-        p[0] = S.fb[0] + vec[0]; p[1] = S.fb[1] + vec[1]; p[2] = S.fb[2] + vec[2];
-        const uint4 P = to_fixed4(A.bc, p);
-        if (A.dmap[1 + a]) hit = dead_voxel(A.dmap[1 + a], A.dnv, P);
-        if (!hit && A.screen_chain) hit = screen_lane<CUBIC>(A, A.ctab, thr + (1 + a) * T, P, st);
+    const unsigned long long hits = __ballot(act && hit);
+    const unsigned long long am = (1ull << chain) - 1ull;
+    uint32_t dead2 = 0u;
+    for (int oo = 0; oo < O; ++oo)
+      if ((hits >> (oo * chain)) & am) dead2 |= 1u << oo;
+    if (lane == 0) A.mask2[ins] = dead2;
+    const bool alive = act && !((dead2 >> o) & 1u);
+    const unsigned long long al = __ballot(alive);
+    if (al) {
+      unsigned b = 0;
+      if (lane == 0) b = atomicAdd(&wg_cnt, (unsigned)__popcll(al));
+      b = (unsigned)__builtin_amdgcn_readfirstlane((int)b);
+      if (alive) {
+        TrialPt q;
+        q.x = p[0]; q.y = p[1]; q.z = p[2];
+        q.site = 1 + a;
+        q.slot = (ins * O + o) * chain + a;
+        q.ins = ins;
+        q.bit = o;
+        sub[b + __popcll(al & lt)] = q;
       }
-      const unsigned long long hits = __ballot(act && hit);
-      const unsigned long long am = (1ull << chain) - 1ull;
-      uint32_t dead2 = 0u;
-      for (int oo = 0; oo < O; ++oo)
-        if ((hits >> (oo * chain)) & am) dead2 |= 1u << oo;
-      if (lane == 0) A.mask2[ins] = dead2;
-      const bool alive = act && !((dead2 >> o) & 1u);
-      const unsigned long long al = __ballot(alive);
-      if (al) {
-        unsigned b = 0;
-        if (lane == 0) b = atomicAdd(&wg_cnt, (unsigned)__popcll(al));
-        b = (unsigned)__builtin_amdgcn_readfirstlane((int)b);
-        if (alive) {
-          TrialPt q;
-          q.x = p[0]; q.y = p[1]; q.z = p[2];
-          q.site = 1 + a;
-          q.slot = (ins * O + o) * chain + a;
-          q.ins = ins;
-          q.bit = o;
-          sub[b + __popcll(al & lt)] = q;
+    }
+  };
+  if (K <= 16) {
+    // first-bead selections of four live insertions per wave, one per
+    // 16-lane row; then the orientations of those whose bead was placed
+    const int row = lane >> 4, rl = lane & 15;
+    for (unsigned b0 = wv * 4; b0 < n_live; b0 += kInsWaves * 4) {
+      const unsigned li = b0 + row;
+      const bool has = li < n_live;
+      const int ins = has ? A.live[i0 + li] : 0;
+      const uint64_t idx = (uint64_t)(first_of(A) + ins);
+      const uint32_t dead = has ? A.mask1[ins] : ~0u;
+      const bool alive = has && rl < K && !((dead >> rl) & 1u);
+      double ev = 0.0, er = 0.0;
+      if (alive) {
+        const double2 e = A.e1[(size_t)ins * K + rl];
+        ev = e.x;
+        er = e.y;
+      }
+      int sel;
+      double Wfb;
+      row_select(ev, er, alive, K, draw(A, idx, 3 * K), A.beta, A.bias, sel, Wfb);
+      const bool fb_ok = has && sel >= 0 && Wfb > 1e-150;
+      const int sl = row * 16 + (fb_ok ? sel : 0);
+      InsState S;
+      S.sel = fb_ok ? sel : -1;
+      S.ok = fb_ok ? 1 : 0;
+      S.wfb = Wfb;
+      const double ev_s = __shfl(ev, sl, 64), er_s = __shfl(er, sl, 64);
+      S.ev = fb_ok ? ev_s : 0.0;
+      S.er = fb_ok ? er_s : 0.0;
+      S.fb[0] = S.fb[1] = S.fb[2] = 0.0;
+      if (fb_ok) {
+        double u[3];
+        draw3(A, idx, 3 * sel, u);
+        S.fb[0] = A.bc.cell[0] * u[0];
+        S.fb[1] = A.bc.cell[4] * u[1];
+        S.fb[2] = A.bc.cell[8] * u[2];
+      }
+      if (has && rl == 0) {
+        A.state[ins] = S;
+        A.mask2[ins] = 0u;
+      }
+      if (chain > 0) {
+        const unsigned long long okm = __ballot(fb_ok && rl == 0);
+        for (int r = 0; r < 4; ++r) {
+          if (!((okm >> (16 * r)) & 1ull)) continue;
+          const int ins_r = __builtin_amdgcn_readlane(ins, 16 * r);
+          orient(ins_r, (uint64_t)(first_of(A) + ins_r), readlane_d(S.fb[0], 16 * r), readlane_d(S.fb[1], 16 * r),
+                 readlane_d(S.fb[2], 16 * r));
         }
       }
+    }
+  } else {
+    for (unsigned li = wv; li < n_live; li += kInsWaves) {
+      const int ins = A.live[i0 + li];
+      const uint64_t idx = (uint64_t)(first_of(A) + ins);
+      const uint32_t dead = A.mask1[ins];
+      double ev = 0.0, er = 0.0;
+      if (lane < K && !((dead >> lane) & 1u)) {
+        const double2 e = A.e1[(size_t)ins * K + lane];
+        ev = e.x;
+        er = e.y;
+      }
+      int sel;
+      double Wfb;
+      wave_select(ev, er, dead, K, draw(A, idx, 3 * K), A.beta, A.bias, sel, Wfb);
+      const bool fb_ok = sel >= 0 && Wfb > 1e-150;
+      InsState S;
+      S.sel = fb_ok ? sel : -1;
+      S.ok = fb_ok ? 1 : 0;
+      S.wfb = Wfb;
+      S.ev = fb_ok ? readlane_d(ev, sel) : 0.0;
+      S.er = fb_ok ? readlane_d(er, sel) : 0.0;
+      S.fb[0] = S.fb[1] = S.fb[2] = 0.0;
+      if (fb_ok) {
+        double u[3];
+        draw3(A, idx, 3 * sel, u);
+        S.fb[0] = A.bc.cell[0] * u[0];
+        S.fb[1] = A.bc.cell[4] * u[1];
+        S.fb[2] = A.bc.cell[8] * u[2];
+      }
+      if (lane == 0) {
+        A.state[ins] = S;
+        A.mask2[ins] = 0u;
+      }
+      if (fb_ok && chain > 0) orient(ins, idx, S.fb[0], S.fb[1], S.fb[2]);
     }
   }
   if (st) atomicAdd(&wg_st, st);   // every lane's count (each lane screened its own trials)
