@@ -959,113 +959,143 @@ __global__ __launch_bounds__(kFinWaves * 64) void widom_finish_kernel(const Wido
   double2* eik = reinterpret_cast<double2*>(dyn + (size_t)wv * wave_bytes);
   double (*mol)[3] = reinterpret_cast<double (*)[3]>(dyn + (size_t)wv * wave_bytes + (size_t)M * 3 * KS * sizeof(double2));
   const unsigned n_live = A.live_counts[blockIdx.x];
-  for (unsigned li = wv; li < n_live; li += kFinWaves) {
-    const int ins = A.live[blockIdx.x * A.chunk + li];
-    const uint64_t idx = (uint64_t)(first_of(A) + ins);
-    const InsState S = A.state[ins];
-    bool ok = S.ok != 0;
-    double Wtot = S.wfb, e_v = S.ev, e_r = S.er;
-    int orient = 0;
-    // selects, not an index into the loaded struct (which would place it in scratch)
-    if (lane < 3) mol[0][lane] = lane == 0 ? S.fb[0] : (lane == 1 ? S.fb[1] : S.fb[2]);
-    if (ok && chain > 0) {
-      const uint32_t dead2 = A.mask2[ins];
-      double cv = 0.0, cr = 0.0;
-      if (lane < O && !((dead2 >> lane) & 1u)) {
-        for (int a = 0; a < chain; ++a) {
-          const double2 e = A.e2[((size_t)ins * O + lane) * chain + a];
-          cv += e.x;
-          cr += e.y;
-        }
-      }
-      int so;
-      double Wch;
-      wave_select(cv, cr, dead2, O, draw(A, idx, 3 * K + 3 * O + 1), A.beta, A.bias, so, Wch);
-      if (so < 0 || !(S.wfb * Wch > 1e-150)) {
-        ok = false;
-      } else {
-        Wtot = S.wfb * Wch;
-        orient = so;
-        e_v += readlane_d(cv, so);
-        e_r += readlane_d(cr, so);
-        if (lane < chain) {
-          double p[3];
-          orient_pos(A, idx, so, lane, S.fb, p);
-          mol[1 + lane][0] = p[0]; mol[1 + lane][1] = p[1]; mol[1 + lane][2] = p[2];
-        }
+  // 64 live insertions per wave at a time, one per lane: those whose first
+  // bead was not placed get their record (W = 0) from their lane; the others
+  // are finished by the whole wave, one after another
+  for (unsigned l0 = wv * 64; l0 < n_live; l0 += kFinWaves * 64) {
+    const unsigned lk = l0 + lane;
+    const bool has = lk < n_live;
+    const int ins_l = has ? A.live[blockIdx.x * A.chunk + lk] : 0;
+    const bool placed = has && A.state[ins_l].ok != 0;
+    if (has && !placed) {
+      A.rec_w[ins_l] = 0.0;
+      A.rec_e[ins_l] = 0.0;
+      A.rec_f[ins_l] = 0;
+      if (A.res) {
+        graspa_widom_result r;
+        r.rosen = 0.0;
+        r.energy = 0.0;
+        r.vdw = 0.0;
+        r.real = 0.0;
+        r.ewald = 0.0;
+        r.first_bead = -1;
+        r.orientation = -1;
+        r.ok = 0;
+        r._pad = 0;
+        A.res[ins_l] = r;
       }
     }
-    wave_sync();
-    double ew = 0.0;
-    if (ok && A.charged) {
-      for (int t = lane; t < M * 3; t += 64) {
-        const int site = t / 3, axis = t % 3;
-        double s = bc.inv[axis] * mol[site][0] + bc.inv[3 + axis] * mol[site][1] + bc.inv[6 + axis] * mol[site][2];
-        s *= 2 * 3.14159265358979323846;
-        double sn, cs;
-        sincos(s, &sn, &cs);
-        const double2 e1 = make_double2(cs, sn);
-        double2* e = eik + (size_t)t * KS;
-        e[0] = make_double2(1.0, 0.0);
-        if (bc.kmax[axis] >= 1) e[1] = e1;
-        double2 cur = e1;
-        for (int k = 2; k <= bc.kmax[axis]; ++k) {
-          cur = cmul2(cur, e1);
-          e[k] = cur;
+    unsigned long long pend = __ballot(placed);
+    while (pend) {
+      const int pl = __ffsll((long long)pend) - 1;
+      pend &= pend - 1ull;
+      const int ins = __builtin_amdgcn_readlane(ins_l, pl);
+      const uint64_t idx = (uint64_t)(first_of(A) + ins);
+      const InsState S = A.state[ins];
+      bool ok = S.ok != 0;
+      double Wtot = S.wfb, e_v = S.ev, e_r = S.er;
+      int orient = 0;
+      // selects, not an index into the loaded struct (which would place it in scratch)
+      if (lane < 3) mol[0][lane] = lane == 0 ? S.fb[0] : (lane == 1 ? S.fb[1] : S.fb[2]);
+      if (ok && chain > 0) {
+        const uint32_t dead2 = A.mask2[ins];
+        double cv = 0.0, cr = 0.0;
+        if (lane < O && !((dead2 >> lane) & 1u)) {
+          for (int a = 0; a < chain; ++a) {
+            const double2 e = A.e2[((size_t)ins * O + lane) * chain + a];
+            cv += e.x;
+            cr += e.y;
+          }
+        }
+        int so;
+        double Wch;
+        wave_select(cv, cr, dead2, O, draw(A, idx, 3 * K + 3 * O + 1), A.beta, A.bias, so, Wch);
+        if (so < 0 || !(S.wfb * Wch > 1e-150)) {
+          ok = false;
+        } else {
+          Wtot = S.wfb * Wch;
+          orient = so;
+          e_v += readlane_d(cv, so);
+          e_r += readlane_d(cr, so);
+          if (lane < chain) {
+            double p[3];
+            orient_pos(A, idx, so, lane, S.fb, p);
+            mol[1 + lane][0] = p[0]; mol[1 + lane][1] = p[1]; mol[1 + lane][2] = p[2];
+          }
         }
       }
       wave_sync();
-      double es = 0.0, ec = 0.0;
-      for (int kk = lane; kk < A.n_kact; kk += 64) {
-        const int4 kv = A.kact[kk];
-        const double t = A.ktemp_act[kk];
-        const int aky = abs(kv.y), akz = abs(kv.z);
-        double2 cn = make_double2(0.0, 0.0);
-        for (int i = 0; i < M; ++i) {
-          const double2* e = eik + (size_t)(3 * i) * KS;
-          double2 ey = e[KS + aky];
-          if (kv.y < 0) ey.y = -ey.y;
-          const double2 exy = cmul2(e[kv.x], ey);
-          double2 ez = e[2 * KS + akz];
-          if (kv.z < 0) ez.y = -ez.y;
-          const double2 pr = cmul2(exy, ez);
-          cn.x += A.site_q[i] * pr.x;
-          cn.y += A.site_q[i] * pr.y;
+      double ew = 0.0;
+      if (ok && A.charged) {
+        for (int t = lane; t < M * 3; t += 64) {
+          const int site = t / 3, axis = t % 3;
+          double s = bc.inv[axis] * mol[site][0] + bc.inv[3 + axis] * mol[site][1] + bc.inv[6 + axis] * mol[site][2];
+          s *= 2 * 3.14159265358979323846;
+          double sn, cs;
+          sincos(s, &sn, &cs);
+          const double2 e1 = make_double2(cs, sn);
+          double2* e = eik + (size_t)t * KS;
+          e[0] = make_double2(1.0, 0.0);
+          if (bc.kmax[axis] >= 1) e[1] = e1;
+          double2 cur = e1;
+          for (int k = 2; k <= bc.kmax[axis]; ++k) {
+            cur = cmul2(cur, e1);
+            e[k] = cur;
+          }
         }
-        const double2 ov = A.sk_ads[kv.w];
-        const double2 nv = make_double2(ov.x + cn.x, ov.y + cn.y);
-        es += t * (nv.x * nv.x + nv.y * nv.y) - t * (ov.x * ov.x + ov.y * ov.y);
-        const double2 cvk = A.sk_fw[kv.w];
-        ec += t * (cvk.x * cn.x + cvk.y * cn.y);
+        wave_sync();
+        double es = 0.0, ec = 0.0;
+        for (int kk = lane; kk < A.n_kact; kk += 64) {
+          const int4 kv = A.kact[kk];
+          const double t = A.ktemp_act[kk];
+          const int aky = abs(kv.y), akz = abs(kv.z);
+          double2 cn = make_double2(0.0, 0.0);
+          for (int i = 0; i < M; ++i) {
+            const double2* e = eik + (size_t)(3 * i) * KS;
+            double2 ey = e[KS + aky];
+            if (kv.y < 0) ey.y = -ey.y;
+            const double2 exy = cmul2(e[kv.x], ey);
+            double2 ez = e[2 * KS + akz];
+            if (kv.z < 0) ez.y = -ez.y;
+            const double2 pr = cmul2(exy, ez);
+            cn.x += A.site_q[i] * pr.x;
+            cn.y += A.site_q[i] * pr.y;
+          }
+          const double2 ov = A.sk_ads[kv.w];
+          const double2 nv = make_double2(ov.x + cn.x, ov.y + cn.y);
+          es += t * (nv.x * nv.x + nv.y * nv.y) - t * (ov.x * ov.x + ov.y * ov.y);
+          const double2 cvk = A.sk_fw[kv.w];
+          ec += t * (cvk.x * cn.x + cvk.y * cn.y);
+        }
+        es = wave_sum(es);
+        ec = wave_sum(ec);
+        ew = (es - A.excl) + 2.0 * ec;
       }
-      es = wave_sum(es);
-      ec = wave_sum(ec);
-      ew = (es - A.excl) + 2.0 * ec;
+      if (lane == 0) {
+        double Wf = 0.0, etot = 0.0;
+        if (ok) {
+          Wf = Wtot * exp(-A.beta * ew) * exp(-A.beta * A.tail_de);
+          etot = e_v + e_r + ew + A.tail_de;
+        }
+        A.rec_w[ins] = Wf;
+        A.rec_e[ins] = etot;
+        A.rec_f[ins] = (S.ok ? 1 : 0) | (ok ? 2 : 0);
+        if (A.res) {
+          graspa_widom_result r;
+          r.rosen = Wf;
+          r.energy = etot;
+          r.vdw = ok ? e_v : 0.0;
+          r.real = ok ? e_r : 0.0;
+          r.ewald = ok ? ew : 0.0;
+          r.first_bead = ok ? S.sel : -1;
+          r.orientation = ok ? orient : -1;
+          r.ok = ok ? 1 : 0;
+          r._pad = 0;
+          A.res[ins] = r;
+        }
+      }
+      wave_sync();
     }
-    if (lane == 0) {
-      double Wf = 0.0, etot = 0.0;
-      if (ok) {
-        Wf = Wtot * exp(-A.beta * ew) * exp(-A.beta * A.tail_de);
-        etot = e_v + e_r + ew + A.tail_de;
-      }
-      A.rec_w[ins] = Wf;
-      A.rec_e[ins] = etot;
-      A.rec_f[ins] = (S.ok ? 1 : 0) | (ok ? 2 : 0);
-      if (A.res) {
-        graspa_widom_result r;
-        r.rosen = Wf;
-        r.energy = etot;
-        r.vdw = ok ? e_v : 0.0;
-        r.real = ok ? e_r : 0.0;
-        r.ewald = ok ? ew : 0.0;
-        r.first_bead = ok ? S.sel : -1;
-        r.orientation = ok ? orient : -1;
-        r.ok = ok ? 1 : 0;
-        r._pad = 0;
-        A.res[ins] = r;
-      }
-    }
-    wave_sync();
   }
 }
 
