@@ -633,16 +633,16 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
       // (one FMA on the cutoff-selected g); the prefactor and the site's charge
       // multiply the wave's sum once
       const double cmax = coul ? fmax(bc.cut_vdw2, bc.cut_coul2) : bc.cut_vdw2;
+      // the force-field row and the table row are read only by lanes inside
+      // the cutoffs (lanes outside issue no LDS access)
       auto pair_fast = [&](double rr, double aq, int ty, double& v_, double& r_, bool& o_) {
-        const double4 ff = fr[ty];
-        const double e = lj_mixed(rr, ff.x, ff.y, ff.z);
-        const bool inv = rr < bc.cut_vdw2;
-        v_ += inv ? e : 0.0;
-        o_ = o_ || (inv && ((e > bc.overlap) || (rr < 0.01)));
-        if (coul) {
-          const double g = coul_table(ct, nint, rr);
-          r_ = fma(aq, (rr < bc.cut_coul2) ? g : 0.0, r_);
+        if (rr < bc.cut_vdw2) {
+          const double4 ff = fr[ty];
+          const double e = lj_mixed(rr, ff.x, ff.y, ff.z);
+          v_ += e;
+          o_ = o_ || (e > bc.overlap) || (rr < 0.01);
         }
+        if (coul && rr < bc.cut_coul2) r_ = fma(aq, coul_table(ct, nint, rr), r_);
       };
       double v1 = 0.0, r1 = 0.0;
       bool o1 = false;
