@@ -196,6 +196,25 @@ def test_widom_fp64_matches_oracle(engine_mod):
     assert summ.n == 300 and summ.n_ok == ref["ok"].sum()
 
 
+@pytest.mark.parametrize("K", [16, 20])
+def test_widom_fp64_matches_oracle_trial_counts(engine_mod, K):
+    """The first-bead selection takes four insertions per wave (one per 16-lane
+    row, ordered sums by a scan along the row) for K <= 16 trials and one
+    insertion per wave beyond: both must make the oracle's selections with its
+    weights, K = 16 filling a row and K = 20 taking the other path."""
+    from graspa_amd import synth
+    s = synth.config1(n_fw=400)
+    s.n_trial_pos = K
+    _, ref, got, summ = _widom_pair(400, 200, 31 + K, engine_mod.GRASPA_FP64, engine_mod, system=s)
+    assert ref["ok"].sum() > 150
+    assert np.array_equal(got["ok"], ref["ok"])
+    ok = ref["ok"] == 1
+    assert np.array_equal(got["first_bead"][ok], ref["first_bead"][ok])
+    assert np.array_equal(got["orientation"][ok], ref["orientation"][ok])
+    assert _close(got["rosen"], ref["rosen"], rtol=1e-9, atol=0.0)
+    assert summ.n == 200 and summ.n_ok == ref["ok"].sum()
+
+
 MIXED_DE_TOL = 1e-3    # internal units (10 J/mol): 1e-5 kJ/mol per insertion
 
 
