@@ -19,7 +19,9 @@ namespace graspa {
 
 int coulomb_table_octaves(double cut2)
 {
-  const int e_hi = (int)std::floor(std::log2(cut2)) + 1;
+  // 2^e_hi above cut2 (1 + 2^-20): r^2 < cut2 rounded to FP32 (relative
+  // error 2^-24) stays inside the last octave, so lookups need no upper clamp
+  const int e_hi = (int)std::floor(std::log2(cut2 * (1.0 + 0x1p-20))) + 1;
   return std::max(1, e_hi - kCoulTabELo);
 }
 

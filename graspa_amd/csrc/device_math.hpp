@@ -135,6 +135,16 @@ __device__ inline double lj_mixed(double rr, double a4e, double s2, double shift
   return fma(a4e, fma(s3, s3, -s3), -shift);      // s3 (s3 - 1) in one rounding
 }
 
+// The same 12-6 term from c12 = 4 eps sigma^12 and c6 = 4 eps sigma^6:
+// y^3 (c12 y^3 - c6) - shift with y = 1/r^2 (one multiply fewer than lj_mixed).
+__device__ inline double lj_mixed_c(double rr, double c12, double c6, double shift)
+{
+  double y = (double)__builtin_amdgcn_rcpf((float)rr);   // v_rcp_f32 seed
+  y = y * (2.0 - rr * y);
+  const double y3 = y * y * y;
+  return fma(y3, fma(c12, y3, -c6), -shift);
+}
+
 // exp(y) for -708 < y <= 0: Cody-Waite reduction y = n ln2 + r with a
 // two-part ln 2, degree-11 Taylor polynomial on |r| <= ln2/2 (truncation
 // 6e-15 relative), scaled by 2^n.

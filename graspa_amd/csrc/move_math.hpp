@@ -85,8 +85,13 @@ __device__ inline double coul_table(const double2* __restrict__ ct, const int ni
 {
   constexpr int kShift = 23 - kCoulTabMBits;
   const uint32_t bits = __float_as_uint((float)rr);
-  int idx = (int)(bits >> kShift) - ((127 + kCoulTabELo) << kCoulTabMBits);
-  idx = min(max(idx, 0), nint - 1);
+  // r^2 below the first octave (overlaps) reads row 0.  Every caller
+  // evaluates pairs inside the Coulomb cutoff only, and the table's last
+  // octave ends above cut^2 by more than FP32 rounding of r^2, so the index
+  // stays below nint without a clamp
+  (void)nint;
+  // (one saturating subtract, v_sub_u32 with clamp)
+  const unsigned idx = __builtin_elementwise_sub_sat(bits >> kShift, (unsigned)((127 + kCoulTabELo) << kCoulTabMBits));
   const double si = (double)__uint_as_float(bits & ~((1u << kShift) - 1u));
   const double d = rr - si;
   // the LDS copy's row stride; idx < 2^24, and the 24-bit multiply issues at

@@ -34,6 +34,7 @@
 // computed.
 // The semantics are Insertion_Body (mc_swap_utilities.h:3-133) with
 // Widom_Move_FirstBead_PARTIAL / Widom_Move_Chain_PARTIAL (mc_widom.h:305-614).
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -89,6 +90,7 @@ struct WidomArgs {
   int use_cells;
   int screen_chain;             // cell screen of the orientation sites (large frameworks only)
   int iso;                      // cubic cell with equal edges: the fast form's pair kernel works in cell fractions
+  int lj_shift;                 // some type pair has a nonzero LJ shift
   int cell_n[3];
   int cell_cap;
   const uint4* ctab;       // [cells * cell_cap] fixed-point x,y,z + type, ~0u = empty
@@ -552,7 +554,7 @@ __host__ __device__ constexpr int eval_fw_pad(int n) { return (n + 127) & ~127; 
 // the exact overlap flag; one wave per site, 64 framework atoms per step.
 // The framework (x, y, z, q and type) is staged in LDS when it fits.
 template <bool MIXED, bool CUBIC, bool FW_LDS, int W>
-__global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, const TrialPt* list,
+__global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 ? 8 : 1))) void widom_eval_kernel(const WidomArgs A, const TrialPt* list,
                                                                    const unsigned* counts, int groups, int chunk,
                                                                    int per_ins, double2* energy, uint32_t* mask,
                                                                    unsigned long long* stat_slot)
@@ -580,7 +582,13 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
   for (int i = threadIdx.x; i < A.M * T; i += blockDim.x) {
     const int a = i / T, ty = i % T;
     const double* f = A.ff64 + 4 * (ty * T + A.site_type[a]);
-    ffl[i] = make_double4(f[0], f[1], f[2], 0.0);
+    if (MIXED) {
+      // fast form: {c12, c6, shift} = {4 eps sigma^12, 4 eps sigma^6, shift}
+      const double s6 = f[1] * f[1] * f[1], c6 = f[0] * s6;
+      ffl[i] = make_double4(c6 * s6, c6, f[2], 0.0);
+    } else {
+      ffl[i] = make_double4(f[0], f[1], f[2], 0.0);
+    }
   }
   // the table sits at LDS offset 0 (lds_coul == 0): a row's address is one
   // multiply of its index
@@ -627,23 +635,13 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
     double v = 0.0, r = 0.0;
     bool o = false;
     if (MIXED) {
-      // straight-line pair terms with selects for the cutoffs, two atoms per
-      // lane per step (independent chains) under one wave-uniform branch that
-      // skips atom pairs beyond both cutoffs for all lanes.  r sums q_atom g(r^2)
-      // (one FMA on the cutoff-selected g); the prefactor and the site's charge
-      // multiply the wave's sum once
-      const double cmax = coul ? fmax(bc.cut_vdw2, bc.cut_coul2) : bc.cut_vdw2;
-      // the force-field row and the table row are read only by lanes inside
-      // the cutoffs (lanes outside issue no LDS access)
-      auto pair_fast = [&](double rr, double aq, int ty, double& v_, double& r_, bool& o_) {
-        if (rr < bc.cut_vdw2) {
-          const double4 ff = fr[ty];
-          const double e = lj_mixed(rr, ff.x, ff.y, ff.z);
-          v_ += e;
-          o_ = o_ || (e > bc.overlap) || (rr < 0.01);
-        }
-        if (coul && rr < bc.cut_coul2) r_ = fma(aq, coul_table(ct, nint, rr), r_);
-      };
+      // two atoms per lane per step (independent chains).  The LJ term and
+      // the real-space term sit behind lane-divergent cutoff branches (one
+      // branch when the two cutoffs are equal): the force-field row and the
+      // table row are read only by lanes inside them, and a step whose atoms
+      // are all beyond the cutoffs for every lane skips both terms.  r sums
+      // q_atom g(r^2) (one FMA); the prefactor and the site's charge multiply
+      // the wave's sum once
       double v1 = 0.0, r1 = 0.0;
       bool o1 = false;
       // squared distance to the site: in an equal-edged cube from cell
@@ -652,8 +650,29 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
       // coordinates through pbc_rint (15)
       const double Lc = bc.cell[0], L2 = Lc * Lc, iL = bc.inv[0];
       const double fx = mul_unfused(px, iL), fy = mul_unfused(py, iL), fz = mul_unfused(pz, iL);
-      auto run = [&](auto iso_c) {
-        constexpr bool ISO = decltype(iso_c)::value;
+      auto run = [&](auto iso_c, auto same_c, auto shift_c) {
+        constexpr bool ISO = decltype(iso_c)::value;      // distances from cell fractions
+        constexpr bool SAME = decltype(same_c)::value;    // equal LJ and Coulomb cutoffs
+        constexpr bool SHIFT = decltype(shift_c)::value;  // some LJ shift is nonzero (else the row's
+                                                          // third entry is not read)
+        auto lj = [&](double rr, int ty, double& v_, bool& o_) {
+          const double4* f = fr + ty;
+          const double2 cc = *reinterpret_cast<const double2*>(f);
+          const double e = lj_mixed_c(rr, cc.x, cc.y, SHIFT ? f->z : 0.0);
+          v_ += e;
+          o_ = o_ || (e > bc.overlap) || (rr < 0.01);
+        };
+        auto pair_fast = [&](double rr, double aq, int ty, double& v_, double& r_, bool& o_) {
+          if constexpr (SAME) {
+            if (rr < bc.cut_vdw2) {
+              lj(rr, ty, v_, o_);
+              if (coul) r_ = fma(aq, coul_table(ct, nint, rr), r_);
+            }
+          } else {
+            if (rr < bc.cut_vdw2) lj(rr, ty, v_, o_);
+            if (coul && rr < bc.cut_coul2) r_ = fma(aq, coul_table(ct, nint, rr), r_);
+          }
+        };
         // (the sums as explicit FMAs: both loops below round them alike, so
         // every launch shape gives the same sums)
         auto dist2 = [&](double ax, double ay, double az) {
@@ -668,15 +687,15 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
           }
         };
         if (FW_LDS) {
-          for (int j = lane; j < npad; j += 128) {
+          // (a wave-uniform step count: the loop test is scalar)
+          for (int it = 0, nit = npad >> 7; it < nit; ++it) {
+            const int j = (it << 7) + lane;
             const double4 a0 = fw4[j], a1 = fw4[j + 64];
             const int ty0 = fwt[j], ty1 = fwt[j + 64];
             const double rr0 = dist2(a0.x, a0.y, a0.z);
             const double rr1 = dist2(a1.x, a1.y, a1.z);
-            if (__any(rr0 < cmax || rr1 < cmax)) {
-              pair_fast(rr0, a0.w, ty0, v, r, o);
-              pair_fast(rr1, a1.w, ty1, v1, r1, o1);
-            }
+            pair_fast(rr0, a0.w, ty0, v, r, o);
+            pair_fast(rr1, a1.w, ty1, v1, r1, o1);
           }
         } else {
           const double sc = ISO ? iL : 1.0;
@@ -696,15 +715,21 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
             }
             const double rr0 = dist2(ax0, ay0, az0);
             const double rr1 = has1 ? dist2(ax1, ay1, az1) : 1e300;   // 1e300: contributes nothing
-            if (__any(rr0 < cmax || rr1 < cmax)) {
-              pair_fast(rr0, aq0, ty0, v, r, o);
-              pair_fast(rr1, aq1, ty1, v1, r1, o1);
-            }
+            pair_fast(rr0, aq0, ty0, v, r, o);
+            pair_fast(rr1, aq1, ty1, v1, r1, o1);
           }
         }
       };
-      if (CUBIC && A.iso) run(std::true_type{});
-      else run(std::false_type{});
+      auto run_same = [&](auto iso_c, auto shift_c) {
+        if (bc.cut_vdw2 == bc.cut_coul2) run(iso_c, std::true_type{}, shift_c);
+        else run(iso_c, std::false_type{}, shift_c);
+      };
+      auto run_shift = [&](auto iso_c) {
+        if (A.lj_shift) run_same(iso_c, std::true_type{});
+        else run_same(iso_c, std::false_type{});
+      };
+      if (CUBIC && A.iso) run_shift(std::true_type{});
+      else run_shift(std::false_type{});
       v += v1;
       r += r1;
       o = o || o1;
@@ -752,7 +777,7 @@ __global__ __launch_bounds__(W * 64) void widom_eval_kernel(const WidomArgs A, c
         const double rr = dx * dx + dy * dy + dz * dz;
         if (rr < bc.cut_vdw2) {
           const double4 ff = fr[ta];
-          const double e = MIXED ? lj_mixed(rr, ff.x, ff.y, ff.z) : lj_fp64(rr, ff.x, ff.y, ff.z);
+          const double e = MIXED ? lj_mixed_c(rr, ff.x, ff.y, ff.z) : lj_fp64(rr, ff.x, ff.y, ff.z);
           o = o || (e > bc.overlap) || (rr < 0.01);
           v += e;
         }
@@ -1523,6 +1548,7 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   for (int a = 0; a < c.molsize; ++a) qabs += std::fabs(c.q[a]);
   A.charged = (b->bc.use_charges && qabs > 1e-6) ? 1 : 0;
   A.iso = (b->bc.cubic && b->bc.cell[0] == b->bc.cell[4] && b->bc.cell[0] == b->bc.cell[8]) ? 1 : 0;
+  A.lj_shift = std::any_of(b->shift.begin(), b->shift.end(), [](double x) { return x != 0.0; }) ? 1 : 0;
   A.K = K; A.O = O; A.beta = beta; A.bias = bias; A.excl = excl; A.tail_de = tail_de;
   A.seed = seed; A.stream = stream; A.first_index = how.first_dev ? 0 : first_index; A.n_insert = n_insert;
   A.first_dev = how.first_dev;

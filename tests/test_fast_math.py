@@ -97,7 +97,9 @@ def _coul_table_eval(tab, rr):
     shift = 23 - MBITS
     bits = rr.astype(np.float32).view(np.uint32)
     idx = (bits >> shift).astype(np.int64) - ((127 - 7) << MBITS)
-    idx = np.clip(idx, 0, tab.size // (DEG + 1) - 1)
+    # the kernels clamp below only: inside the cutoff the index never passes the last row
+    idx = np.maximum(idx, 0)
+    assert idx.max() < tab.size // (DEG + 1)
     si = (bits & np.uint32((0xFFFFFFFF << shift) & 0xFFFFFFFF)).view(np.float32).astype(np.float64)
     d = rr - si
     c = tab.reshape(-1, DEG + 1)[idx]
@@ -124,6 +126,20 @@ def test_coulomb_table_accuracy(alpha, cut):
     err = np.abs(got - ref)
     assert err.max() < 1e-13, err.max()
     assert np.max(err / ref) < 1e-9
+
+
+@pytest.mark.parametrize("cut2", [144.0, 256.0 * (1.0 - 1e-9), 64.0 * (1.0 - 2.0 ** -26), 100.0, 256.0])
+def test_coulomb_table_covers_fp32_rounded_cutoff(cut2):
+    """The lookups clamp the row index below only: r^2 just under cut^2, whose
+    FP32 rounding can reach the next power of two, still falls in the last
+    octave of the table."""
+    tab = _coulomb_table(0.265058, cut2)
+    nrow = tab.size // (DEG + 1)
+    rr = np.array([np.nextafter(cut2, 0.0), cut2 * (1.0 - 2.0 ** -25), cut2 * (1.0 - 1e-12)])
+    bits = rr.astype(np.float32).view(np.uint32)
+    idx = (bits >> (23 - MBITS)).astype(np.int64) - ((127 - 7) << MBITS)
+    assert idx.max() < nrow, (idx, nrow)
+    _coul_table_eval(tab, rr)
 
 
 def test_coulomb_table_pins_a_wrong_interval():
