@@ -548,6 +548,9 @@ __device__ inline double mul_unfused(double a, double b)
 
 // Atoms of the pair kernel's LDS framework copy: whole steps of 128 (two per lane).
 __host__ __device__ constexpr int eval_fw_pad(int n) { return (n + 127) & ~127; }
+// Slot of atom i's type in that copy: the types of atoms 128 s + l and
+// 128 s + 64 + l sit side by side (one 8-byte read per lane and step).
+__device__ inline int fw_type_slot(int i) { return ((i >> 7) << 7) + ((i & 63) << 1) + ((i >> 6) & 1); }
 
 // B / D: full energies of the trial sites of a work list (VDW_Coulomb.cu:
 // 1183-1352 for one trial site): wave sums of LJ and real-space Coulomb and
@@ -612,7 +615,7 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
       fw4[i] = i < n ? make_double4(mul_unfused(A.fw[i], sc), mul_unfused(A.fw[n + i], sc), mul_unfused(A.fw[2 * n + i], sc),
                                     A.fw[3 * n + i])
                      : make_double4(qnan, qnan, qnan, 0.0);
-      fwt[i] = i < n ? A.fw_type[i] : 0;
+      fwt[fw_type_slot(i)] = i < n ? A.fw_type[i] : 0;
     }
   }
   __syncthreads();
@@ -691,7 +694,11 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
           for (int it = 0, nit = npad >> 7; it < nit; ++it) {
             const int j = (it << 7) + lane;
             const double4 a0 = fw4[j], a1 = fw4[j + 64];
-            const int ty0 = fwt[j], ty1 = fwt[j + 64];
+            // types of atoms j and j + 64 in one 8-byte read (kept whole rather
+            // than split into two reads, one of them under the cutoff branch)
+            const unsigned long long tt =
+                __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(fwt) + (j - 64 * it));
+            const int ty0 = (int)(unsigned)tt, ty1 = (int)(unsigned)(tt >> 32);
             const double rr0 = dist2(a0.x, a0.y, a0.z);
             const double rr1 = dist2(a1.x, a1.y, a1.z);
             pair_fast(rr0, a0.w, ty0, v, r, o);
@@ -740,7 +747,7 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
         int ty;
         if (FW_LDS) {
           const double4 a0 = fw4[j];
-          ax = a0.x; ay = a0.y; az = a0.z; aq = a0.w; ty = fwt[j];
+          ax = a0.x; ay = a0.y; az = a0.z; aq = a0.w; ty = fwt[fw_type_slot(j)];
         } else {
           ax = A.fw[j]; ay = A.fw[n + j]; az = A.fw[2 * n + j]; aq = A.fw[3 * n + j]; ty = A.fw_type[j];
         }
