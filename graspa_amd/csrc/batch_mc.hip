@@ -262,7 +262,7 @@ __device__ inline unsigned long long wave_or64(unsigned long long v)
 // r^2 < 0.01), erfc real space inside the Coulomb cutoff.
 template <bool MIXED, bool CUBIC>
 __device__ inline void pair_site(const BoxConst& bc, const double* __restrict__ ffl, const double2* __restrict__ ct,
-                                 const int nint, const int T, const BSite& s, const double ax, const double ay,
+                                 const int T, const BSite& s, const double ax, const double ay,
                                  const double az, const double aq, const int at, double& ev, double& er, uint32_t& ov,
                                  const uint32_t bit)
 {
@@ -278,7 +278,7 @@ __device__ inline void pair_site(const BoxConst& bc, const double* __restrict__ 
   }
   if (bc.use_charges && rr < bc.cut_coul2) {
     if (MIXED) {
-      er += (bc.prefactor * aq * s.q) * coul_table(ct, nint, rr);
+      er += (bc.prefactor * aq * s.q) * coul_table(ct, rr);
     } else {
       const double r = sqrt(rr);
       const double term = aq * s.q * erfc(bc.alpha * r);
@@ -337,7 +337,7 @@ __device__ void block_sums(const BBox& B, const BState& S, const int bk, const i
 
 template <bool MIXED, bool CUBIC>
 __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double* __restrict__ ffl,
-                          const double2* __restrict__ ct, const int nint, const double* __restrict__ fwl,
+                          const double2* __restrict__ ct, const double* __restrict__ fwl,
                           const double* __restrict__ gml, int* __restrict__ loc, const int n_sites, const int ex_lo,
                           const int ex_hi)
 {
@@ -484,7 +484,7 @@ __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double
 #pragma unroll
         for (int j = 0; j < kBGroup; ++j)
           if (j < cnt)
-            pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j], ar[j],
+            pair_site<MIXED, CUBIC>(bc, ffl, ct, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j], ar[j],
                                     ovb, 1u << j);
       }
     } else if (part == 0 && fwl) {
@@ -496,7 +496,7 @@ __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double
         const int at = ftl[i];
 #pragma unroll
         for (int j = 0; j < kBGroup; ++j)
-          if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j], ar[j],
+          if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j], ar[j],
                                           ovb, 1u << j);
       }
     } else if (part == 0) {
@@ -506,7 +506,7 @@ __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double
         const int at = B.fw_type[i];
 #pragma unroll
         for (int j = 0; j < kBGroup; ++j)
-          if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j], ar[j],
+          if (j < cnt) pair_site<MIXED, CUBIC>(bc, ffl, ct, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j], ar[j],
                                           ovb, 1u << j);
       }
     } else {
@@ -532,7 +532,7 @@ __device__ void pair_pass(const BBox& B, const BState& S, BMove& L, const double
 #pragma unroll
           for (int j = 0; j < kBGroup; ++j)
             if (j < cnt)
-              pair_site<MIXED, CUBIC>(bc, ffl, ct, nint, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j], ar[j],
+              pair_site<MIXED, CUBIC>(bc, ffl, ct, T, L.site[L.idx[g0 + j]], ax, ay, az, aq, at, av[j], ar[j],
                                           ovb, 1u << j);
         }
       }
@@ -802,7 +802,7 @@ enum { kHGV = 1, kGGV = 2, kHGR = 4, kGGR = 5, kHGE = 7, kGGE = 8, kTail = 9 };
 // ---------------------------------------------------------------------------
 template <bool MIXED, bool CUBIC>
 __device__ void single_body(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr, const double2* ct,
-                            int nint, const double* fwl, double* gml, int* loc, double2* eik)
+                            const double* fwl, double* gml, int* loc, double2* eik)
 {
   const int tid = threadIdx.x;
   const int M = L.M, start = L.start, cap = B.guest_cap, comp = L.comp;
@@ -844,7 +844,7 @@ __device__ void single_body(const BBox& B, BState& S, BMove& L, const double* ff
   if (tid < 2 * M) L.idx[tid] = (unsigned char)tid;
   if (tid == 0) L.ovl = 0ull;
   __syncthreads();
-  pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, gml, loc, 2 * M, start, start + M);
+  pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, fwl, gml, loc, 2 * M, start, start + M);
   if (tid == 0) {
     const unsigned long long newmask = ((1ull << M) - 1ull) << M;
     L.overlap = (L.ovl & newmask) ? 1 : 0;
@@ -905,7 +905,7 @@ __device__ inline void list_survivors(BMove& L, const int n, const int g)
 
 template <bool MIXED, bool CUBIC>
 __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr, const double2* ct,
-                     int nint, const double* fwl, double* gml, int* loc, double2* eik)
+                     const double* fwl, double* gml, int* loc, double2* eik)
 {
   const int tid = threadIdx.x;
   const int M = L.M, comp = L.comp, cap = B.guest_cap;
@@ -933,7 +933,7 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
   else if (tid == 0) L.smask = 0ull;
   if (tid == 0) list_survivors(L, K, 1);
   __syncthreads();
-  if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, gml, loc, L.n_eval, ex_lo, ex_hi);
+  if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, fwl, gml, loc, L.n_eval, ex_lo, ex_hi);
   if (tid < K) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) L.t4[tid][t] = L.e[tid][t];
@@ -983,7 +983,7 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
       else if (tid == 0) L.smask = 0ull;
       if (tid == 0) list_survivors(L, O, 1);
       __syncthreads();
-      if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, gml, loc, L.n_eval, ex_lo, ex_hi);
+      if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, fwl, gml, loc, L.n_eval, ex_lo, ex_hi);
       if (tid < O) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) L.t4[tid][t] = L.e[tid][t];
@@ -1036,7 +1036,7 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
     else if (tid == 0) L.smask = 0ull;
     if (tid == 0) list_survivors(L, O * chain, chain);
     __syncthreads();
-    if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, gml, loc, L.n_eval, ex_lo, ex_hi);
+    if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, fwl, gml, loc, L.n_eval, ex_lo, ex_hi);
     if (tid < O) {
       double e4[4] = {0.0, 0.0, 0.0, 0.0};
       for (int a = 0; a < chain; ++a)
@@ -1183,7 +1183,7 @@ __device__ void cbmc(const BBox& B, BState& S, BMove& L, const double* ffl, cons
 // excluded.
 template <bool MIXED, bool CUBIC>
 __device__ uint32_t trial_energies(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr,
-                                   const double2* ct, int nint, const double* fwl, double* gml, int* loc,
+                                   const double2* ct, const double* fwl, double* gml, int* loc,
                                    const int n_trials, const int chain, const int ex_lo, const int ex_hi)
 {
   const int tid = threadIdx.x;
@@ -1192,7 +1192,7 @@ __device__ uint32_t trial_energies(const BBox& B, BState& S, BMove& L, const dou
   else if (tid == 0) L.smask = 0ull;
   if (tid == 0) list_survivors(L, ns, chain);
   __syncthreads();
-  if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, nint, fwl, gml, loc, L.n_eval, ex_lo, ex_hi);
+  if (L.n_eval > 0) pair_pass<MIXED, CUBIC>(B, S, L, ffl, ct, fwl, gml, loc, L.n_eval, ex_lo, ex_hi);
   if (tid < n_trials) {
     double e4[4] = {0.0, 0.0, 0.0, 0.0};
     for (int a = 0; a < chain; ++a)
@@ -1242,7 +1242,7 @@ __device__ double retrace_weight(const BBox& B, BMove& L, const int n, const uin
 // kBDrawTrial + 3K + 3o, retrace orientation o at kBDrawTrial + 3K + 3O + 3o.
 template <bool MIXED, bool CUBIC>
 __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr,
-                            const double2* ct, int nint, const double* fwl, double* gml, int* loc, double2* eik)
+                            const double2* ct, const double* fwl, double* gml, int* loc, double2* eik)
 {
   const int tid = threadIdx.x;
   const int M = L.M, comp = L.comp, cap = B.guest_cap;
@@ -1261,7 +1261,7 @@ __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ff
     L.site[tid] = s;
   }
   __syncthreads();
-  uint32_t ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, K, 1, ex_lo, ex_hi);
+  uint32_t ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, fwl, gml, loc, K, 1, ex_lo, ex_hi);
   if (tid < 64) {
     double W = 1.0, others = 0.0;
     const int sel = select_wave0(B, L, K, ov, L.ud[5], false, false, &W, &others);
@@ -1296,7 +1296,7 @@ __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ff
       L.site[t] = s;
     }
     __syncthreads();
-    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, O, chain, ex_lo, ex_hi);
+    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, fwl, gml, loc, O, chain, ex_lo, ex_hi);
     if (tid < 64) {
       double W = L.W;
       const int sel = select_wave0(B, L, O, ov, L.ud[6], false, true, &W);
@@ -1328,7 +1328,7 @@ __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ff
     for (int t = 0; t < 4; ++t) L.eo[t] = 0.0;
   }
   __syncthreads();
-  ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, 1, 1, ex_lo, ex_hi);
+  ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, fwl, gml, loc, 1, 1, ex_lo, ex_hi);
   if (tid == 0) L.Wo = retrace_weight(B, L, 1, ov, L.stored, K);   // (own weight + StoredR) / K
   __syncthreads();
   // retrace: orientations about the old first bead, trial 0 its own
@@ -1353,7 +1353,7 @@ __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ff
       L.site[t] = s;
     }
     __syncthreads();
-    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, O, chain, ex_lo, ex_hi);
+    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, fwl, gml, loc, O, chain, ex_lo, ex_hi);
     if (tid == 0) L.Wo *= retrace_weight(B, L, O, ov, 0.0, O);
     __syncthreads();
   }
@@ -1417,7 +1417,7 @@ __device__ void reinsertion(const BBox& B, BState& S, BMove& L, const double* ff
 // at kBDrawTrial + 3K + 3o, retrace orientation o at kBDrawTrial + 3K + 3O + 3o.
 template <bool MIXED, bool CUBIC>
 __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* ffl, const float* thr,
-                              const double2* ct, int nint, const double* fwl, double* gml, int* loc, double2* eik)
+                              const double2* ct, const double* fwl, double* gml, int* loc, double2* eik)
 {
   const int tid = threadIdx.x;
   const int oc = L.comp, nc = L.nc, cap = B.guest_cap;
@@ -1436,7 +1436,7 @@ __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* 
     for (int t = 0; t < 4; ++t) L.eo[t] = 0.0;
   }
   __syncthreads();
-  uint32_t ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, 1, 1, ex_lo, ex_hi);
+  uint32_t ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, fwl, gml, loc, 1, 1, ex_lo, ex_hi);
   if (tid < 64) {
     double W = 1.0;
     const int sel = select_wave0(B, L, 1, ov, 0.5, false, false, &W);
@@ -1471,7 +1471,7 @@ __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* 
       L.site[t] = s;
     }
     __syncthreads();
-    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, O, chain, ex_lo, ex_hi);
+    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, fwl, gml, loc, O, chain, ex_lo, ex_hi);
     if (tid < 64) {
       double W = L.W;
       const int sel = select_wave0(B, L, O, ov, L.ud[6], false, true, &W);
@@ -1502,7 +1502,7 @@ __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* 
     L.site[0] = s;
   }
   __syncthreads();
-  ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, 1, 1, ex_lo, ex_hi);
+  ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, fwl, gml, loc, 1, 1, ex_lo, ex_hi);
   if (tid == 0) L.Wo = retrace_weight(B, L, 1, ov, 0.0, 1);
   __syncthreads();
   if (Mo > 1) {
@@ -1527,7 +1527,7 @@ __device__ void identity_swap(const BBox& B, BState& S, BMove& L, const double* 
       L.site[t] = s;
     }
     __syncthreads();
-    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, O, chain, ex_lo, ex_hi);
+    ov = trial_energies<MIXED, CUBIC>(B, S, L, ffl, thr, ct, fwl, gml, loc, O, chain, ex_lo, ex_hi);
     if (tid == 0) L.Wo *= retrace_weight(B, L, O, ov, 0.0, O);
     __syncthreads();
   }
@@ -1667,7 +1667,6 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
   double2* ct = reinterpret_cast<double2*>(dyn + nff4 + nthr);
   const int nct = MIXED ? ((coul_lds_doubles(B.coul_n) + 3) & ~3) : 0;
   double2* eik = reinterpret_cast<double2*>(dyn + nff4 + nthr + nct);
-  const int nint = B.coul_n / kCoulTabStride;
   // the framework goes to LDS for the launch when it fits after the tables
   const int fw_off = nff4 + nthr + nct + 2 * (2 * kBMaxMol * 3 * B.kstride);
   const int nf = B.n_fw;
@@ -1822,10 +1821,10 @@ __global__ __launch_bounds__(kBT) void batch_mc_kernel(const BBox* __restrict__ 
     }
     __syncthreads();
     const int type = L.type;
-    if (type == kMvTrans || type == kMvRot) single_body<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, eik);
-    else if (type == kMvReins) reinsertion<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, eik);
-    else if (type == kMvIdent) identity_swap<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, eik);
-    else if (type != kMvNone) cbmc<MIXED, CUBIC>(B, S, L, ffl, thr, ct, nint, fwl, gml, loc, eik);
+    if (type == kMvTrans || type == kMvRot) single_body<MIXED, CUBIC>(B, S, L, ffl, thr, ct, fwl, gml, loc, eik);
+    else if (type == kMvReins) reinsertion<MIXED, CUBIC>(B, S, L, ffl, thr, ct, fwl, gml, loc, eik);
+    else if (type == kMvIdent) identity_swap<MIXED, CUBIC>(B, S, L, ffl, thr, ct, fwl, gml, loc, eik);
+    else if (type != kMvNone) cbmc<MIXED, CUBIC>(B, S, L, ffl, thr, ct, fwl, gml, loc, eik);
     __syncthreads();
     if (tid == 0) {
       const int comp = L.comp;

@@ -81,15 +81,14 @@ __device__ inline void axis_table(const BoxConst& bc, double px, double py, doub
 // is read from the exponent and top mantissa bits of r^2 as FP32, the value is
 // a degree-kCoulTabDeg polynomial in r^2 - s_i (Horner).  Clamped indices only
 // occur for r^2 below 2^-7, i.e. pairs the overlap rule already rejects.
-__device__ inline double coul_table(const double2* __restrict__ ct, const int nint, const double rr)
+__device__ inline double coul_table(const double2* __restrict__ ct, const double rr)
 {
   constexpr int kShift = 23 - kCoulTabMBits;
   const uint32_t bits = __float_as_uint((float)rr);
   // r^2 below the first octave (overlaps) reads row 0.  Every caller
   // evaluates pairs inside the Coulomb cutoff only, and the table's last
   // octave ends above cut^2 by more than FP32 rounding of r^2, so the index
-  // stays below nint without a clamp
-  (void)nint;
+  // stays below the row count without a clamp
   // (one saturating subtract, v_sub_u32 with clamp)
   const unsigned idx = __builtin_elementwise_sub_sat(bits >> kShift, (unsigned)((127 + kCoulTabELo) << kCoulTabMBits));
   const double si = (double)__uint_as_float(bits & ~((1u << kShift) - 1u));

@@ -620,7 +620,6 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int nint = A.coul_n / kCoulTabStride;
   const int npts = (int)pre[G];
   unsigned long long n_eval = 0;
   for (int pi = blockIdx.x * W + wv; pi < npts; pi += gridDim.x * W) {
@@ -669,11 +668,11 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
           if constexpr (SAME) {
             if (rr < bc.cut_vdw2) {
               lj(rr, ty, v_, o_);
-              if (coul) r_ = fma(aq, coul_table(ct, nint, rr), r_);
+              if (coul) r_ = fma(aq, coul_table(ct, rr), r_);
             }
           } else {
             if (rr < bc.cut_vdw2) lj(rr, ty, v_, o_);
-            if (coul && rr < bc.cut_coul2) r_ = fma(aq, coul_table(ct, nint, rr), r_);
+            if (coul && rr < bc.cut_coul2) r_ = fma(aq, coul_table(ct, rr), r_);
           }
         };
         // (the sums as explicit FMAs: both loops below round them alike, so
@@ -790,7 +789,7 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
         }
         if (coul && rr < bc.cut_coul2) {
           if (MIXED) {
-            r = fma(gq, coul_table(ct, nint, rr), r);    // scaled with the framework sum below
+            r = fma(gq, coul_table(ct, rr), r);    // scaled with the framework sum below
           } else {
             const double rd = sqrt(rr);
             r += bc.prefactor * 1.0 * (gq * sq * erfc(bc.alpha * rd)) / rd;

@@ -244,6 +244,43 @@ def test_widom_mixed_within_tolerance(engine_mod):
     assert np.median(err) < 1e-6
 
 
+@pytest.mark.parametrize("cut_vdw,cut_coul,shifted", [(11.0, 12.0, True), (12.0, 10.0, False)])
+@pytest.mark.parametrize("framework_lds", [1, 0])
+def test_widom_mixed_unequal_cutoffs_and_shift(engine_mod, cut_vdw, cut_coul, shifted, framework_lds):
+    """The fast pair kernel's variants beside the default one (equal cutoffs,
+    no LJ shift): separate LJ and Coulomb cutoff branches, and force-field
+    rows whose shift is read, with the framework in LDS and read from memory;
+    the same constructions as the FP64 oracle and energies within tolerance."""
+    from graspa_amd import synth
+    from graspa_amd.deck import _shift_value
+    from oracle.binding import widom_records_to_numpy
+    s = synth.config1(n_fw=400)
+    s.cut_vdw, s.cut_coul = cut_vdw, cut_coul
+    nt = len(s.pseudo_names)
+    # shifted LJ (Get_Shifted_Value, read_data.cpp:737-748) at the VDW cutoff, or truncated
+    sh = np.zeros((nt, nt))
+    if shifted:
+        for i in range(nt):
+            for j in range(nt):
+                sh[i, j] = _shift_value(s.eps[i, j], s.sig[i, j], cut_vdw * cut_vdw)
+        assert np.any(sh != 0.0)
+    s.shift = sh
+    print("widom unequal cutoffs vdw %.1f coul %.1f shifted %d framework_lds %d" %
+          (cut_vdw, cut_coul, shifted, framework_lds), flush=True)
+    o = _oracle(s)
+    _, skf, ska, _, _, _ = o.ewald_total()
+    intra, selfe = o.exclusion(1)
+    ref = widom_records_to_numpy(o.widom_batch(1, ska, skf, seed=29, n_insert=300))
+    box = engine_mod.Box(s, precision=engine_mod.GRASPA_MIXED)
+    box.init_ewald()
+    box.set_eval_shape(12, 0, framework_lds)
+    res, _ = box.widom_batch(1, 300, seed=29, excl=intra + selfe)
+    got = engine_mod.widom_results_to_numpy(res)
+    box.close()
+    assert ref["ok"].sum() > 200
+    _check_mixed(ref, got)
+
+
 def test_widom_result_independent_of_batching(engine_mod):
     """Insertion i gives the same result alone, inside a larger batch, or
     offset by first_index: the counter-based streams make each result a pure
