@@ -53,7 +53,8 @@ constexpr int kSums = 6;         // W, W^2, W dE, n, n first bead placed, n comp
 constexpr int kRedBlock = 256;
 constexpr int kRedPer = 8;       // records per thread in the first reduction pass
 constexpr int kInsWaves = 16;    // waves per workgroup of the chain kernel (one live insertion each)
-constexpr int kFinWaves = 8;     // waves per workgroup of the finish kernel (256 VGPRs per lane available)
+constexpr int kFinWaves = 4;     // waves per workgroup of the finish kernel
+constexpr int kFinMaxPer = 8;    // finish workgroups per placed list at most
 constexpr int kFbWaves = 16;     // waves per workgroup of the first-bead kernel
 
 // Work lists are appended per workgroup of the screening kernels: workgroup g
@@ -139,6 +140,9 @@ struct WidomArgs {
   int dnv[3];               // its voxels per cell axis
   int* live;                // insertions with a first-bead trial left after the screen, sub-list
                             // of workgroup g at g * chunk
+  int* placed;              // insertions whose first bead was placed, sub-list of workgroup g at
+                            // g * chunk (in list 1's memory, free once list 1 is evaluated)
+  unsigned* placed_counts;  // [groups] their counts (in counts1's memory, likewise)
   unsigned long long* stat_part;   // per-workgroup work counts (screened, evaluated pairs)
   int groups, chunk;        // screening grid: workgroups and insertions per workgroup
   // outputs
@@ -394,9 +398,6 @@ __device__ inline const uint4* stage_screen(const WidomArgs& A, char* dyn, float
   return A.ctab;
 }
 
-// Appends this wave's surviving trials (bits of `alive`, n_sites sites each)
-// to its workgroup's sub-list (LDS counter wg_cnt); the sites of trial t are
-// W.pos[a * n_trials + t].
 // Records of an insertion whose construction failed (W = 0).
 __device__ inline void failed_record(const WidomArgs& A, int ins, int fb_placed)
 {
@@ -822,7 +823,7 @@ template <bool CUBIC>
 __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const WidomArgs A)
 {
   extern __shared__ __align__(16) char dyn[];
-  __shared__ unsigned wg_cnt;
+  __shared__ unsigned wg_cnt, wg_pl;
   __shared__ unsigned long long wg_st;
   const int T = A.bc.ntypes;
   float* thr = reinterpret_cast<float*>(dyn);
@@ -832,6 +833,7 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
   }
   if (threadIdx.x == 0) {
     wg_cnt = 0u;
+    wg_pl = 0u;
     wg_st = 0ull;
   }
   __syncthreads();
@@ -842,6 +844,18 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
   TrialPt* sub = A.list2 + (size_t)i0 * O * chain;
   const unsigned long long lt = (1ull << lane) - 1ull;
   unsigned long long st = 0;
+  int* placed = A.placed + i0;
+  // the placed insertions of this wave step (ballot bits `pm`, insertion in
+  // the lane of each bit) join the workgroup's placed list for the finish
+  // kernel; a live insertion whose first bead was not placed gets its W = 0
+  // record here
+  auto to_finish = [&](const unsigned long long pm, const bool on_lane, const int ins) {
+    if (!pm) return;
+    unsigned b = 0;
+    if (lane == 0) b = atomicAdd(&wg_pl, (unsigned)__popcll(pm));
+    b = (unsigned)__builtin_amdgcn_readfirstlane((int)b);
+    if (on_lane) placed[b + __popcll(pm & lt)] = ins;
+  };
   // orientations of a live insertion whose first bead was placed (the whole
   // wave; lane o * chain + a is orientation o's chain atom a)
   auto orient = [&](const int ins, const uint64_t idx, const double fb0, const double fb1, const double fb2) {
@@ -922,9 +936,11 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
       if (has && rl == 0) {
         A.state[ins] = S;
         A.mask2[ins] = 0u;
+        if (!fb_ok) failed_record(A, ins, 0);
       }
+      const unsigned long long okm = __ballot(fb_ok && rl == 0);
+      to_finish(okm, fb_ok && rl == 0, ins);
       if (chain > 0) {
-        const unsigned long long okm = __ballot(fb_ok && rl == 0);
         for (int r = 0; r < 4; ++r) {
           if (!((okm >> (16 * r)) & 1ull)) continue;
           const int ins_r = __builtin_amdgcn_readlane(ins, 16 * r);
@@ -965,7 +981,9 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
       if (lane == 0) {
         A.state[ins] = S;
         A.mask2[ins] = 0u;
+        if (!fb_ok) failed_record(A, ins, 0);
       }
+      to_finish(fb_ok ? 1ull : 0ull, fb_ok && lane == 0, ins);
       if (fb_ok && chain > 0) orient(ins, idx, S.fb[0], S.fb[1], S.fb[2]);
     }
   }
@@ -973,13 +991,17 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
   __syncthreads();
   if (threadIdx.x == 0) {
     A.counts2[blockIdx.x] = wg_cnt;
+    A.placed_counts[blockIdx.x] = wg_pl;
     A.stat_part[kMaxScreenGroups + blockIdx.x] = wg_st;
   }
 }
 
 // E: orientation selection, Fourier-space energy of the selected molecule
 // (Fourier_Ewald_Diff, Ewald_Energy_Functions.h:280-397, with Oldsize = 0)
-// and the record: W = W_fb W_ch exp(-beta (dE_ewald + dE_tail)).
+// and the record: W = W_fb W_ch exp(-beta (dE_ewald + dE_tail)), for the
+// insertions whose first bead was placed (the others have their records from
+// the first-bead or the chain kernel).  gridDim.x / groups workgroups share
+// the placed list of screening workgroup g, one wave per insertion.
 __global__ __launch_bounds__(kFinWaves * 64) void widom_finish_kernel(const WidomArgs A)
 {
   extern __shared__ __align__(16) char dyn[];
@@ -989,144 +1011,116 @@ __global__ __launch_bounds__(kFinWaves * 64) void widom_finish_kernel(const Wido
   const size_t wave_bytes = (size_t)M * 3 * KS * sizeof(double2) + kMaxMol * 3 * sizeof(double);
   double2* eik = reinterpret_cast<double2*>(dyn + (size_t)wv * wave_bytes);
   double (*mol)[3] = reinterpret_cast<double (*)[3]>(dyn + (size_t)wv * wave_bytes + (size_t)M * 3 * KS * sizeof(double2));
-  const unsigned n_live = A.live_counts[blockIdx.x];
-  // 64 live insertions per wave at a time, one per lane: those whose first
-  // bead was not placed get their record (W = 0) from their lane; the others
-  // are finished by the whole wave, one after another
-  for (unsigned l0 = wv * 64; l0 < n_live; l0 += kFinWaves * 64) {
-    const unsigned lk = l0 + lane;
-    const bool has = lk < n_live;
-    const int ins_l = has ? A.live[blockIdx.x * A.chunk + lk] : 0;
-    const bool placed = has && A.state[ins_l].ok != 0;
-    if (has && !placed) {
-      A.rec_w[ins_l] = 0.0;
-      A.rec_e[ins_l] = 0.0;
-      A.rec_f[ins_l] = 0;
+  const int per = gridDim.x / A.groups, g = blockIdx.x / per, part = blockIdx.x - g * per;
+  const unsigned n_pl = A.placed_counts[g];
+  const int* placed = A.placed + (size_t)g * A.chunk;
+  for (unsigned k = (unsigned)(part * kFinWaves + wv); k < n_pl; k += (unsigned)(per * kFinWaves)) {
+    const int ins = placed[k];
+    const uint64_t idx = (uint64_t)(first_of(A) + ins);
+    const InsState S = A.state[ins];
+    bool ok = S.ok != 0;
+    double Wtot = S.wfb, e_v = S.ev, e_r = S.er;
+    int orient = 0;
+    // selects, not an index into the loaded struct (which would place it in scratch)
+    if (lane < 3) mol[0][lane] = lane == 0 ? S.fb[0] : (lane == 1 ? S.fb[1] : S.fb[2]);
+    if (ok && chain > 0) {
+      const uint32_t dead2 = A.mask2[ins];
+      double cv = 0.0, cr = 0.0;
+      if (lane < O && !((dead2 >> lane) & 1u)) {
+        for (int a = 0; a < chain; ++a) {
+          const double2 e = A.e2[((size_t)ins * O + lane) * chain + a];
+          cv += e.x;
+          cr += e.y;
+        }
+      }
+      int so;
+      double Wch;
+      wave_select(cv, cr, dead2, O, draw(A, idx, 3 * K + 3 * O + 1), A.beta, A.bias, so, Wch);
+      if (so < 0 || !(S.wfb * Wch > 1e-150)) {
+        ok = false;
+      } else {
+        Wtot = S.wfb * Wch;
+        orient = so;
+        e_v += readlane_d(cv, so);
+        e_r += readlane_d(cr, so);
+        if (lane < chain) {
+          double p[3];
+          orient_pos(A, idx, so, lane, S.fb, p);
+          mol[1 + lane][0] = p[0]; mol[1 + lane][1] = p[1]; mol[1 + lane][2] = p[2];
+        }
+      }
+    }
+    wave_sync();
+    double ew = 0.0;
+    if (ok && A.charged) {
+      for (int t = lane; t < M * 3; t += 64) {
+        const int site = t / 3, axis = t % 3;
+        double s = bc.inv[axis] * mol[site][0] + bc.inv[3 + axis] * mol[site][1] + bc.inv[6 + axis] * mol[site][2];
+        s *= 2 * 3.14159265358979323846;
+        double sn, cs;
+        sincos(s, &sn, &cs);
+        const double2 e1 = make_double2(cs, sn);
+        double2* e = eik + (size_t)t * KS;
+        e[0] = make_double2(1.0, 0.0);
+        if (bc.kmax[axis] >= 1) e[1] = e1;
+        double2 cur = e1;
+        for (int k = 2; k <= bc.kmax[axis]; ++k) {
+          cur = cmul2(cur, e1);
+          e[k] = cur;
+        }
+      }
+      wave_sync();
+      double es = 0.0, ec = 0.0;
+      for (int kk = lane; kk < A.n_kact; kk += 64) {
+        const int4 kv = A.kact[kk];
+        const double t = A.ktemp_act[kk];
+        const int aky = abs(kv.y), akz = abs(kv.z);
+        double2 cn = make_double2(0.0, 0.0);
+        for (int i = 0; i < M; ++i) {
+          const double2* e = eik + (size_t)(3 * i) * KS;
+          double2 ey = e[KS + aky];
+          if (kv.y < 0) ey.y = -ey.y;
+          const double2 exy = cmul2(e[kv.x], ey);
+          double2 ez = e[2 * KS + akz];
+          if (kv.z < 0) ez.y = -ez.y;
+          const double2 pr = cmul2(exy, ez);
+          cn.x += A.site_q[i] * pr.x;
+          cn.y += A.site_q[i] * pr.y;
+        }
+        const double2 ov = A.sk_ads[kv.w];
+        const double2 nv = make_double2(ov.x + cn.x, ov.y + cn.y);
+        es += t * (nv.x * nv.x + nv.y * nv.y) - t * (ov.x * ov.x + ov.y * ov.y);
+        const double2 cvk = A.sk_fw[kv.w];
+        ec += t * (cvk.x * cn.x + cvk.y * cn.y);
+      }
+      es = wave_sum(es);
+      ec = wave_sum(ec);
+      ew = (es - A.excl) + 2.0 * ec;
+    }
+    if (lane == 0) {
+      double Wf = 0.0, etot = 0.0;
+      if (ok) {
+        Wf = Wtot * exp(-A.beta * ew) * exp(-A.beta * A.tail_de);
+        etot = e_v + e_r + ew + A.tail_de;
+      }
+      A.rec_w[ins] = Wf;
+      A.rec_e[ins] = etot;
+      A.rec_f[ins] = (S.ok ? 1 : 0) | (ok ? 2 : 0);
       if (A.res) {
         graspa_widom_result r;
-        r.rosen = 0.0;
-        r.energy = 0.0;
-        r.vdw = 0.0;
-        r.real = 0.0;
-        r.ewald = 0.0;
-        r.first_bead = -1;
-        r.orientation = -1;
-        r.ok = 0;
+        r.rosen = Wf;
+        r.energy = etot;
+        r.vdw = ok ? e_v : 0.0;
+        r.real = ok ? e_r : 0.0;
+        r.ewald = ok ? ew : 0.0;
+        r.first_bead = ok ? S.sel : -1;
+        r.orientation = ok ? orient : -1;
+        r.ok = ok ? 1 : 0;
         r._pad = 0;
-        A.res[ins_l] = r;
+        A.res[ins] = r;
       }
     }
-    unsigned long long pend = __ballot(placed);
-    while (pend) {
-      const int pl = __ffsll((long long)pend) - 1;
-      pend &= pend - 1ull;
-      const int ins = __builtin_amdgcn_readlane(ins_l, pl);
-      const uint64_t idx = (uint64_t)(first_of(A) + ins);
-      const InsState S = A.state[ins];
-      bool ok = S.ok != 0;
-      double Wtot = S.wfb, e_v = S.ev, e_r = S.er;
-      int orient = 0;
-      // selects, not an index into the loaded struct (which would place it in scratch)
-      if (lane < 3) mol[0][lane] = lane == 0 ? S.fb[0] : (lane == 1 ? S.fb[1] : S.fb[2]);
-      if (ok && chain > 0) {
-        const uint32_t dead2 = A.mask2[ins];
-        double cv = 0.0, cr = 0.0;
-        if (lane < O && !((dead2 >> lane) & 1u)) {
-          for (int a = 0; a < chain; ++a) {
-            const double2 e = A.e2[((size_t)ins * O + lane) * chain + a];
-            cv += e.x;
-            cr += e.y;
-          }
-        }
-        int so;
-        double Wch;
-        wave_select(cv, cr, dead2, O, draw(A, idx, 3 * K + 3 * O + 1), A.beta, A.bias, so, Wch);
-        if (so < 0 || !(S.wfb * Wch > 1e-150)) {
-          ok = false;
-        } else {
-          Wtot = S.wfb * Wch;
-          orient = so;
-          e_v += readlane_d(cv, so);
-          e_r += readlane_d(cr, so);
-          if (lane < chain) {
-            double p[3];
-            orient_pos(A, idx, so, lane, S.fb, p);
-            mol[1 + lane][0] = p[0]; mol[1 + lane][1] = p[1]; mol[1 + lane][2] = p[2];
-          }
-        }
-      }
-      wave_sync();
-      double ew = 0.0;
-      if (ok && A.charged) {
-        for (int t = lane; t < M * 3; t += 64) {
-          const int site = t / 3, axis = t % 3;
-          double s = bc.inv[axis] * mol[site][0] + bc.inv[3 + axis] * mol[site][1] + bc.inv[6 + axis] * mol[site][2];
-          s *= 2 * 3.14159265358979323846;
-          double sn, cs;
-          sincos(s, &sn, &cs);
-          const double2 e1 = make_double2(cs, sn);
-          double2* e = eik + (size_t)t * KS;
-          e[0] = make_double2(1.0, 0.0);
-          if (bc.kmax[axis] >= 1) e[1] = e1;
-          double2 cur = e1;
-          for (int k = 2; k <= bc.kmax[axis]; ++k) {
-            cur = cmul2(cur, e1);
-            e[k] = cur;
-          }
-        }
-        wave_sync();
-        double es = 0.0, ec = 0.0;
-        for (int kk = lane; kk < A.n_kact; kk += 64) {
-          const int4 kv = A.kact[kk];
-          const double t = A.ktemp_act[kk];
-          const int aky = abs(kv.y), akz = abs(kv.z);
-          double2 cn = make_double2(0.0, 0.0);
-          for (int i = 0; i < M; ++i) {
-            const double2* e = eik + (size_t)(3 * i) * KS;
-            double2 ey = e[KS + aky];
-            if (kv.y < 0) ey.y = -ey.y;
-            const double2 exy = cmul2(e[kv.x], ey);
-            double2 ez = e[2 * KS + akz];
-            if (kv.z < 0) ez.y = -ez.y;
-            const double2 pr = cmul2(exy, ez);
-            cn.x += A.site_q[i] * pr.x;
-            cn.y += A.site_q[i] * pr.y;
-          }
-          const double2 ov = A.sk_ads[kv.w];
-          const double2 nv = make_double2(ov.x + cn.x, ov.y + cn.y);
-          es += t * (nv.x * nv.x + nv.y * nv.y) - t * (ov.x * ov.x + ov.y * ov.y);
-          const double2 cvk = A.sk_fw[kv.w];
-          ec += t * (cvk.x * cn.x + cvk.y * cn.y);
-        }
-        es = wave_sum(es);
-        ec = wave_sum(ec);
-        ew = (es - A.excl) + 2.0 * ec;
-      }
-      if (lane == 0) {
-        double Wf = 0.0, etot = 0.0;
-        if (ok) {
-          Wf = Wtot * exp(-A.beta * ew) * exp(-A.beta * A.tail_de);
-          etot = e_v + e_r + ew + A.tail_de;
-        }
-        A.rec_w[ins] = Wf;
-        A.rec_e[ins] = etot;
-        A.rec_f[ins] = (S.ok ? 1 : 0) | (ok ? 2 : 0);
-        if (A.res) {
-          graspa_widom_result r;
-          r.rosen = Wf;
-          r.energy = etot;
-          r.vdw = ok ? e_v : 0.0;
-          r.real = ok ? e_r : 0.0;
-          r.ewald = ok ? ew : 0.0;
-          r.first_bead = ok ? S.sel : -1;
-          r.orientation = ok ? orient : -1;
-          r.ok = ok ? 1 : 0;
-          r._pad = 0;
-          A.res[ins] = r;
-        }
-      }
-      wave_sync();
-    }
+    wave_sync();
   }
 }
 
@@ -1576,6 +1570,11 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   A.counts2 = A.counts1 + kMaxScreenGroups;
   A.live_counts = A.counts2 + kMaxScreenGroups;
   A.live = W.live;
+  // list 1 and its counts are read by the first pair-energy launch only; the
+  // chain kernel then writes the placed lists there for the finish kernel
+  static_assert(sizeof(TrialPt) >= sizeof(int), "the placed lists fit in list 1");
+  A.placed = reinterpret_cast<int*>(W.list1);
+  A.placed_counts = A.counts1;
   A.stat_part = W.stat_part;
   for (int a = 0; a < c.molsize; ++a) {
     if (no_alloc) {
@@ -1633,7 +1632,14 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
     if (rc) return rc;
   }
   const size_t fin_lds = (size_t)kFinWaves * ((size_t)M * 3 * A.kstride * sizeof(double2) + kMaxMol * 3 * sizeof(double));
-  rc = launch_ins(b, widom_finish_kernel, A, fin_lds, "widom_finish_kernel", kFinWaves, A.groups);
+  {
+    // several workgroups per placed list (as many as stay resident), so its
+    // insertions spread over more waves than one workgroup's
+    const int occ = occupancy(b, widom_finish_kernel, kFinWaves * 64, fin_lds);
+    const long long slots = (long long)std::max(1, occ) * std::max(1, b->n_cu);
+    const int per = (int)std::max(1LL, std::min<long long>(kFinMaxPer, slots / A.groups));
+    rc = launch_ins(b, widom_finish_kernel, A, fin_lds, "widom_finish_kernel", kFinWaves, A.groups * per);
+  }
   if (rc) return rc;
   const int nblk = (n_insert + kRedBlock * kRedPer - 1) / (kRedBlock * kRedPer);
   hipLaunchKernelGGL(widom_reduce_kernel, dim3(nblk), dim3(kRedBlock), 0, b->stream, A.rec_w, A.rec_e, A.rec_f,
