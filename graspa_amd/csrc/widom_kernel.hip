@@ -552,6 +552,12 @@ __host__ __device__ constexpr int eval_fw_pad(int n) { return (n + 127) & ~127; 
 // Slot of atom i's type in that copy: the types of atoms 128 s + l and
 // 128 s + 64 + l sit side by side (one 8-byte read per lane and step).
 __device__ inline int fw_type_slot(int i) { return ((i >> 7) << 7) + ((i & 63) << 1) + ((i >> 6) & 1); }
+// Slot of atom i's (x, y) in that copy (double2 units), its (z, q) 64 slots
+// further: a step of 128 atoms is four blocks of 64 double2, (x, y) and
+// (z, q) of atoms 128 s + l, then of atoms 128 s + 64 + l, so the 64 lanes'
+// 16-byte reads of a block are contiguous (no two lanes of a pass share a
+// bank, where 32-byte records put lanes l and l + 8 on the same banks).
+__device__ inline int fw_xy_slot(int i) { return ((i >> 7) << 8) + (((i >> 6) & 1) << 7) + (i & 63); }
 
 // B / D: full energies of the trial sites of a work list (VDW_Coulomb.cu:
 // 1183-1352 for one trial site): wave sums of LJ and real-space Coulomb and
@@ -604,7 +610,7 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
   // NaN positions: every comparison with their r^2 is false, so they add
   // nothing, flag nothing and the steps need no bound check
   const int npad = eval_fw_pad(n);
-  double4* fw4 = reinterpret_cast<double4*>(dyn + A.lds_fw);
+  double2* fw2 = reinterpret_cast<double2*>(dyn + A.lds_fw);
   int* fwt = reinterpret_cast<int*>(dyn + A.lds_fw + (size_t)npad * sizeof(double4));
   if (FW_LDS) {
     const double qnan = __builtin_nan("");
@@ -613,9 +619,9 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
     for (int i = threadIdx.x; i < npad; i += blockDim.x) {
       // (products rounded on their own, as the path that reads the framework
       // from memory rounds them)
-      fw4[i] = i < n ? make_double4(mul_unfused(A.fw[i], sc), mul_unfused(A.fw[n + i], sc), mul_unfused(A.fw[2 * n + i], sc),
-                                    A.fw[3 * n + i])
-                     : make_double4(qnan, qnan, qnan, 0.0);
+      const int xy = fw_xy_slot(i);
+      fw2[xy] = i < n ? make_double2(mul_unfused(A.fw[i], sc), mul_unfused(A.fw[n + i], sc)) : make_double2(qnan, qnan);
+      fw2[xy + 64] = i < n ? make_double2(mul_unfused(A.fw[2 * n + i], sc), A.fw[3 * n + i]) : make_double2(qnan, 0.0);
       fwt[fw_type_slot(i)] = i < n ? A.fw_type[i] : 0;
     }
   }
@@ -693,7 +699,9 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
           // (a wave-uniform step count: the loop test is scalar)
           for (int it = 0, nit = npad >> 7; it < nit; ++it) {
             const int j = (it << 7) + lane;
-            const double4 a0 = fw4[j], a1 = fw4[j + 64];
+            const double2* st = fw2 + (it << 8) + lane;
+            const double2 xy0 = st[0], zq0 = st[64], xy1 = st[128], zq1 = st[192];
+            const double4 a0 = make_double4(xy0.x, xy0.y, zq0.x, zq0.y), a1 = make_double4(xy1.x, xy1.y, zq1.x, zq1.y);
             // types of atoms j and j + 64 in one 8-byte read (kept whole rather
             // than split into two reads, one of them under the cutoff branch)
             const unsigned long long tt =
@@ -746,8 +754,8 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
         double ax, ay, az, aq;
         int ty;
         if (FW_LDS) {
-          const double4 a0 = fw4[j];
-          ax = a0.x; ay = a0.y; az = a0.z; aq = a0.w; ty = fwt[fw_type_slot(j)];
+          const double2 xy = fw2[fw_xy_slot(j)], zq = fw2[fw_xy_slot(j) + 64];
+          ax = xy.x; ay = xy.y; az = zq.x; aq = zq.y; ty = fwt[fw_type_slot(j)];
         } else {
           ax = A.fw[j]; ay = A.fw[n + j]; az = A.fw[2 * n + j]; aq = A.fw[3 * n + j]; ty = A.fw_type[j];
         }
