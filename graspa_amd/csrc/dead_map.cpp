@@ -11,6 +11,12 @@
 // trial positions in a dense framework; unmarked voxels are left to the cell
 // screen and the exact evaluation.  Positions map to voxels through their
 // 32-bit fixed-point cell fractions: voxel = (f * nv) >> 32 per axis.
+//
+// The second map of a site type marks the certainly free voxels: every point
+// is farther than the overlap distance from every framework atom (the voxel's
+// nearest point beyond r_ov, with r_ov^2 raised by 1e-4 relative), so the cell
+// screen, whose thresholds lie below r_ov^2, cannot flag a trial there and the
+// screen is skipped; such trials go to the exact evaluation as before.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -32,19 +38,25 @@ void dead_map_dims(const graspa_box* b, int nv[3])
   for (int a = 0; a < 3; ++a) nv[a] = std::max(4, std::min(1024, (int)std::ceil(w[a] / kDeadVoxel)));
 }
 
-void build_dead_map(const graspa_box* b, int site_type, std::vector<uint32_t>& bits, int nv[3])
+void build_dead_map(const graspa_box* b, int site_type, std::vector<uint32_t>& bits, int nv[3],
+                    std::vector<uint32_t>* free_bits)
 {
   dead_map_dims(b, nv);
   const size_t nvox = (size_t)nv[0] * nv[1] * nv[2];
   bits.assign((nvox + 31) / 32, 0u);
+  if (free_bits) {
+    free_bits->assign((nvox + 31) / 32, ~0u);
+    if (nvox & 31) free_bits->back() = (1u << (nvox & 31)) - 1u;
+  }
   const BoxConst& bc = b->bc;
   const int T = bc.ntypes;
   const double* cell = bc.cell;   // columns a = cell[0..2], b = cell[3..5], c = cell[6..8]
-  std::vector<double> thr(T);
+  std::vector<double> thr(T), thr_near(T);
   for (int t = 0; t < T; ++t) {
     const int ij = t * T + site_type;
-    thr[t] = overlap_r2(4.0 * b->eps[ij], b->sig[ij] * b->sig[ij], b->shift[ij], bc.overlap, bc.cut_vdw2) *
-             (1.0 - 1.0e-5);
+    const double r2 = overlap_r2(4.0 * b->eps[ij], b->sig[ij] * b->sig[ij], b->shift[ij], bc.overlap, bc.cut_vdw2);
+    thr[t] = r2 * (1.0 - 1.0e-5);
+    thr_near[t] = r2 * (1.0 + 1.0e-4);
   }
   double wdt[3];
   {
@@ -58,10 +70,20 @@ void build_dead_map(const graspa_box* b, int site_type, std::vector<uint32_t>& b
   }
   const bool ortho = cell[1] == 0.0 && cell[2] == 0.0 && cell[3] == 0.0 && cell[5] == 0.0 && cell[6] == 0.0 &&
                      cell[7] == 0.0;
+  // a voxel's circumradius (half its longest body diagonal), for the nearest
+  // point of a skewed voxel: no point is closer than |centre| - radius
+  double vrad = 0.0;
+  for (int d = 0; d < 4; ++d) {
+    const double g[3] = {1.0 / nv[0], (d & 1 ? -1.0 : 1.0) / nv[1], (d & 2 ? -1.0 : 1.0) / nv[2]};
+    const double dx = cell[0] * g[0] + cell[3] * g[1] + cell[6] * g[2];
+    const double dy = cell[1] * g[0] + cell[4] * g[1] + cell[7] * g[2];
+    const double dz = cell[2] * g[0] + cell[5] * g[1] + cell[8] * g[2];
+    vrad = std::max(vrad, 0.5 * std::sqrt(dx * dx + dy * dy + dz * dz));
+  }
   for (int i = 0; i < b->n_fw; ++i) {
-    const double t2 = thr[b->fw_type[i]];
+    const double t2 = thr[b->fw_type[i]], tn2 = thr_near[b->fw_type[i]];
     if (!(t2 > 0.0)) continue;
-    const double r = std::sqrt(t2);
+    const double r = std::sqrt(free_bits ? tn2 : t2);
     const double* p = &b->fw_pos[3 * i];
     double s[3];
     for (int a = 0; a < 3; ++a) s[a] = bc.inv[a] * p[0] + bc.inv[3 + a] * p[1] + bc.inv[6 + a] * p[2];
@@ -75,6 +97,31 @@ void build_dead_map(const graspa_box* b, int site_type, std::vector<uint32_t>& b
       for (int iy = lo[1]; iy <= hi[1]; ++iy)
         for (int ix = lo[0]; ix <= hi[0]; ++ix) {
           const int ii[3] = {ix, iy, iz};
+          const int wx = ((ix % nv[0]) + nv[0]) % nv[0];
+          const int wy = ((iy % nv[1]) + nv[1]) % nv[1];
+          const int wz = ((iz % nv[2]) + nv[2]) % nv[2];
+          const size_t lin = (size_t)wx + (size_t)nv[0] * ((size_t)wy + (size_t)nv[1] * wz);
+          if (free_bits) {
+            // nearest point of the voxel to the atom
+            double near2 = 0.0;
+            if (ortho) {
+              for (int a = 0; a < 3; ++a) {
+                const double lo_f = (double)ii[a] / nv[a], hi_f = (double)(ii[a] + 1) / nv[a];
+                const double d = s[a] < lo_f ? lo_f - s[a] : (s[a] > hi_f ? s[a] - hi_f : 0.0);
+                near2 += d * d * cell[4 * a] * cell[4 * a];
+              }
+            } else {
+              double df[3];
+              for (int a = 0; a < 3; ++a) df[a] = ((double)ii[a] + 0.5) / nv[a] - s[a];
+              const double dx = cell[0] * df[0] + cell[3] * df[1] + cell[6] * df[2];
+              const double dy = cell[1] * df[0] + cell[4] * df[1] + cell[7] * df[2];
+              const double dz = cell[2] * df[0] + cell[5] * df[1] + cell[8] * df[2];
+              const double dc = std::max(0.0, std::sqrt(dx * dx + dy * dy + dz * dz) - vrad);
+              near2 = dc * dc;
+            }
+            if (near2 >= tn2) continue;
+            (*free_bits)[lin >> 5] &= ~(1u << (lin & 31));
+          }
           // farthest corner of the voxel from the atom
           double far2 = 0.0;
           if (ortho) {
@@ -96,10 +143,6 @@ void build_dead_map(const graspa_box* b, int site_type, std::vector<uint32_t>& b
             }
           }
           if (far2 >= t2) continue;
-          const int wx = ((ix % nv[0]) + nv[0]) % nv[0];
-          const int wy = ((iy % nv[1]) + nv[1]) % nv[1];
-          const int wz = ((iz % nv[2]) + nv[2]) % nv[2];
-          const size_t lin = (size_t)wx + (size_t)nv[0] * ((size_t)wy + (size_t)nv[1] * wz);
           bits[lin >> 5] |= 1u << (lin & 31);
         }
   }
@@ -112,17 +155,26 @@ int ensure_dead_map(graspa_box* b, int site_type, const uint32_t** out)
   *out = nullptr;
   if (site_type < 0 || site_type >= b->bc.ntypes || b->n_fw == 0) return GRASPA_OK;
   if ((int)b->d_dmap.size() < b->bc.ntypes) b->d_dmap.resize(b->bc.ntypes, nullptr);
+  if ((int)b->d_fmap.size() < b->bc.ntypes) b->d_fmap.resize(b->bc.ntypes, nullptr);
   if (!b->d_dmap[site_type]) {
-    std::vector<uint32_t> bits;
+    std::vector<uint32_t> bits, fbits;
     int nv[3];
-    build_dead_map(b, site_type, bits, nv);
-    uint32_t* d = nullptr;
-    if (hip_check(box_malloc(b, &d, bits.size() * sizeof(uint32_t)), "hipMalloc dead map")) return GRASPA_ERR_HIP;
-    if (hip_check(hipMemcpy(d, bits.data(), bits.size() * sizeof(uint32_t), hipMemcpyHostToDevice), "dead map upload")) {
+    build_dead_map(b, site_type, bits, nv, &fbits);
+    uint32_t *d = nullptr, *f = nullptr;
+    const size_t bytes = bits.size() * sizeof(uint32_t);
+    if (hip_check(box_malloc(b, &d, bytes), "hipMalloc dead map")) return GRASPA_ERR_HIP;
+    if (hip_check(box_malloc(b, &f, bytes), "hipMalloc free map")) {
       box_free(b, d);
       return GRASPA_ERR_HIP;
     }
+    if (hip_check(hipMemcpy(d, bits.data(), bytes, hipMemcpyHostToDevice), "dead map upload") ||
+        hip_check(hipMemcpy(f, fbits.data(), bytes, hipMemcpyHostToDevice), "free map upload")) {
+      box_free(b, d);
+      box_free(b, f);
+      return GRASPA_ERR_HIP;
+    }
     b->d_dmap[site_type] = d;
+    b->d_fmap[site_type] = f;
     for (int a = 0; a < 3; ++a) b->dmap_nv[a] = nv[a];
   }
   *out = b->d_dmap[site_type];
@@ -135,14 +187,20 @@ void free_dead_maps(graspa_box* b)
     box_free(b, d);
     d = nullptr;
   }
+  for (uint32_t*& f : b->d_fmap) {
+    box_free(b, f);
+    f = nullptr;
+  }
 }
 
 }  // namespace graspa
 
 using namespace graspa;
 
-extern "C" int graspa_dead_map(const graspa_box_params* p, int n_fw, const double* xyz, const int* type,
-                               int site_type, uint32_t* bits, int cap, int* nv)
+// Both host entry points: the map of the framework (dead = 0: certain
+// overlap, 1: certainly free) from the box parameters.
+static int host_voxel_map(const graspa_box_params* p, int n_fw, const double* xyz, const int* type, int site_type,
+                          uint32_t* bits, int cap, int* nv, int which)
 {
   if (!p || n_fw < 0 || (n_fw > 0 && (!xyz || !type)) || !nv || cap < 0 || p->ntypes < 1 ||
       site_type < 0 || site_type >= p->ntypes)
@@ -175,9 +233,22 @@ extern "C" int graspa_dead_map(const graspa_box_params* p, int n_fw, const doubl
   b.n_fw = n_fw;
   b.fw_pos.assign(xyz, xyz + 3 * (size_t)n_fw);
   b.fw_type.assign(type, type + n_fw);
-  std::vector<uint32_t> w;
-  build_dead_map(&b, site_type, w, nv);
+  std::vector<uint32_t> w, f;
+  build_dead_map(&b, site_type, w, nv, which ? &f : nullptr);
+  if (which) w.swap(f);
   if ((size_t)cap < w.size()) return (int)w.size();
   if (bits) std::memcpy(bits, w.data(), w.size() * sizeof(uint32_t));
   return (int)w.size();
+}
+
+extern "C" int graspa_dead_map(const graspa_box_params* p, int n_fw, const double* xyz, const int* type,
+                               int site_type, uint32_t* bits, int cap, int* nv)
+{
+  return host_voxel_map(p, n_fw, xyz, type, site_type, bits, cap, nv, 0);
+}
+
+extern "C" int graspa_free_map(const graspa_box_params* p, int n_fw, const double* xyz, const int* type,
+                               int site_type, uint32_t* bits, int cap, int* nv)
+{
+  return host_voxel_map(p, n_fw, xyz, type, site_type, bits, cap, nv, 1);
 }

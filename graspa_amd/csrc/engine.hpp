@@ -114,6 +114,7 @@ struct graspa_box {
   // maps of the framework's certain-overlap volume per probe-site type
   // (dead_map.cpp), built on first use
   std::vector<uint32_t*> d_dmap;
+  std::vector<uint32_t*> d_fmap;         // certainly free voxels, same layout (built with d_dmap)
   int dmap_nv[3] = {0, 0, 0};
   // real-space Coulomb table of the fast form (coulomb_table.cpp)
   double* d_coul = nullptr;
@@ -269,7 +270,8 @@ int build_screen_cells(graspa_box* b);
 // overlaps a framework atom for a probe site of the map's type.
 constexpr double kDeadVoxel = 0.15;
 void dead_map_dims(const graspa_box* b, int nv[3]);
-void build_dead_map(const graspa_box* b, int site_type, std::vector<uint32_t>& bits, int nv[3]);
+void build_dead_map(const graspa_box* b, int site_type, std::vector<uint32_t>& bits, int nv[3],
+                    std::vector<uint32_t>* free_bits = nullptr);
 int ensure_dead_map(graspa_box* b, int site_type, const uint32_t** out);
 void free_dead_maps(graspa_box* b);
 

@@ -137,6 +137,7 @@ struct WidomArgs {
   unsigned* counts2;        // [groups] sub-list lengths of list 2
   unsigned* live_counts;    // [groups] lengths of the live sub-lists
   const uint32_t* dmap[kMaxMol];   // certain-overlap voxel map of each molecule site's type (or null)
+  const uint32_t* fmap0;           // certainly-free voxel map of the first bead's type (or null)
   int dnv[3];               // its voxels per cell axis
   int* live;                // insertions with a first-bead trial left after the screen, sub-list
                             // of workgroup g at g * chunk
@@ -258,8 +259,9 @@ __device__ inline bool screen_lane(const WidomArgs& A, const uint4* __restrict__
   return false;
 }
 
-// Certain overlap from the voxel map: the voxel of the fixed-point site P.
-__device__ inline bool dead_voxel(const uint32_t* __restrict__ dm, const int* nv, const uint4 P)
+// Bit of a voxel map (certain overlap, or certainly free) at the voxel of the
+// fixed-point site P.
+__device__ inline bool voxel_bit(const uint32_t* __restrict__ dm, const int* nv, const uint4 P)
 {
   const uint32_t x = (uint32_t)(((uint64_t)P.x * (uint64_t)nv[0]) >> 32);
   const uint32_t y = (uint32_t)(((uint64_t)P.y * (uint64_t)nv[1]) >> 32);
@@ -463,8 +465,10 @@ __global__ __launch_bounds__(kFbWaves * 64) void widom_first_bead_kernel(const W
       p[1] = A.bc.cell[4] * u[1];
       p[2] = A.bc.cell[8] * u[2];
       P = to_fixed4(A.bc, p);
-      if (dm) hit = dead_voxel(dm, A.dnv, P);
-      undecided = !hit && A.use_cells;
+      if (dm) hit = voxel_bit(dm, A.dnv, P);
+      // a trial in a certainly free voxel skips the cell screen (it could
+      // not flag it)
+      undecided = !hit && A.use_cells && !(A.fmap0 && voxel_bit(A.fmap0, A.dnv, P));
     }
     // queue the undecided trials of the workgroup
     unsigned* qn = &q_n[step & 1];
@@ -878,7 +882,7 @@ __global__ __launch_bounds__(kInsWaves * 64) void widom_chain_kernel(const Widom
       quat_rotate(vec, u[0], u[1], u[2]);
       p[0] = fb0 + vec[0]; p[1] = fb1 + vec[1]; p[2] = fb2 + vec[2];
       const uint4 P = to_fixed4(A.bc, p);
-      if (A.dmap[1 + a]) hit = dead_voxel(A.dmap[1 + a], A.dnv, P);
+      if (A.dmap[1 + a]) hit = voxel_bit(A.dmap[1 + a], A.dnv, P);
       if (!hit && A.screen_chain) hit = screen_lane<CUBIC>(A, A.ctab, thr + (1 + a) * T, P, st);
     }
     const unsigned long long hits = __ballot(act && hit);
@@ -1596,6 +1600,10 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
     }
   }
   for (int d = 0; d < 3; ++d) A.dnv[d] = b->dmap_nv[d];
+  {
+    const int t0 = c.type[0];
+    A.fmap0 = (A.dmap[0] && t0 >= 0 && t0 < (int)b->d_fmap.size()) ? b->d_fmap[t0] : nullptr;
+  }
   A.groups = groups;
   A.chunk = chunk;
   A.res = d_res;

@@ -185,6 +185,8 @@ _SIGS = {
     "graspa_box_sync": (C.c_int, [C.c_void_p]),
     "graspa_glibc_rand_sequence": (C.c_int, [C.c_uint, C.c_int, _i32p]),
     "graspa_coulomb_table": (C.c_int, [C.c_double, C.c_double, _f64p, C.c_int]),
+    "graspa_free_map": (C.c_int, [C.POINTER(BoxParams), C.c_int, _f64p, _i32p, C.c_int, C.POINTER(C.c_uint32),
+                        C.c_int, _i32p]),
     "graspa_dead_map": (C.c_int, [C.POINTER(BoxParams), C.c_int, _f64p, _i32p, C.c_int, C.POINTER(C.c_uint32),
                                   C.c_int, _i32p]),
     "graspa_batch_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
@@ -264,22 +266,33 @@ def box_params(system, precision=GRASPA_FP64, device=0, keep=None):
     return p
 
 
-def dead_map(system, site_type):
-    """The certain-overlap voxel map of the system's framework for probe sites
-    of site_type (host computation): (bits as a bool array [nz, ny, nx], nv)."""
+def _voxel_map(fn, system, site_type):
     keep = []
     p = box_params(system, keep=keep)
     xyz = np.ascontiguousarray(system.fw_pos, dtype=np.float64).reshape(-1)
     ty = np.ascontiguousarray(system.fw_type, dtype=np.int32)
     nv = np.zeros(3, dtype=np.int32)
-    n = lib().graspa_dead_map(C.byref(p), len(ty), _p(xyz), _p(ty, _i32p), site_type, None, 0, _p(nv, _i32p))
+    f = getattr(lib(), fn)
+    n = f(C.byref(p), len(ty), _p(xyz), _p(ty, _i32p), site_type, None, 0, _p(nv, _i32p))
     if n < 0:
-        raise EngineError("graspa_dead_map failed")
+        raise EngineError("%s failed" % fn)
     w = np.zeros(n, dtype=np.uint32)
-    lib().graspa_dead_map(C.byref(p), len(ty), _p(xyz), _p(ty, _i32p), site_type,
-                          w.ctypes.data_as(C.POINTER(C.c_uint32)), n, _p(nv, _i32p))
+    f(C.byref(p), len(ty), _p(xyz), _p(ty, _i32p), site_type, w.ctypes.data_as(C.POINTER(C.c_uint32)), n,
+      _p(nv, _i32p))
     bits = np.unpackbits(w.view(np.uint8), bitorder="little")[: int(nv[0]) * int(nv[1]) * int(nv[2])]
     return bits.reshape(int(nv[2]), int(nv[1]), int(nv[0])).astype(bool), tuple(int(x) for x in nv)
+
+
+def dead_map(system, site_type):
+    """The certain-overlap voxel map of the system's framework for probe sites
+    of site_type (host computation): (bits as a bool array [nz, ny, nx], nv)."""
+    return _voxel_map("graspa_dead_map", system, site_type)
+
+
+def free_map(system, site_type):
+    """The certainly-free voxel map (same voxels as dead_map): voxels whose
+    every point is beyond the overlap distance of every framework atom."""
+    return _voxel_map("graspa_free_map", system, site_type)
 
 
 class Box:

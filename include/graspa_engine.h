@@ -561,6 +561,15 @@ int  graspa_coulomb_table(double alpha, double cut2, double* out, int cap);
 int  graspa_dead_map(const graspa_box_params* params, int n_fw, const double* xyz, const int* type, int site_type,
                      uint32_t* bits, int cap, int* nv);
 
+/* Host helper with no counterpart in the reference: the companion map of
+ * certainly free voxels (same voxels and bit layout as graspa_dead_map; bit
+ * set when every point of the voxel is farther than the overlap distance from
+ * every framework atom, with the squared distance raised by 1e-4 relative).
+ * The first-bead screen skips trials there: the cell screen could not flag
+ * them.  Same arguments and return values as graspa_dead_map. */
+int  graspa_free_map(const graspa_box_params* params, int n_fw, const double* xyz, const int* type, int site_type,
+                     uint32_t* bits, int cap, int* nv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,3 +79,47 @@ def test_unmarked_space_exists_and_map_depends_on_type():
     m4, _ = engine.dead_map(s, 4)
     assert 0.05 < m3.mean() < 0.95
     assert not np.array_equal(m3, m4)
+
+
+@pytest.mark.parametrize("name", ["cubic", "triclinic"])
+def test_free_voxels_hold_no_close_pair(name):
+    """A certainly-free voxel (the first-bead screen skips trials there) has no
+    point within the overlap distance of any framework atom: checked at the
+    corners and random points of free voxels with an independent numpy
+    minimum-image distance against the closed-form overlap radius; the free
+    and certain-overlap maps never share a voxel.  A map that tested the voxel
+    centre (or dropped the margin) would put points near a voxel face inside
+    the overlap radius."""
+    from graspa_amd import engine
+    s = dict(_systems())[name]
+    cell = np.asarray(s.cell, dtype=float).reshape(3, 3).T
+    inv = np.linalg.inv(cell)
+    fw = np.asarray(s.fw_pos, dtype=float)
+    ty = np.asarray(s.fw_type)
+    T = s.ntypes
+    rng = np.random.default_rng(5)
+    for site_type in (3, 4):
+        f, nv = engine.free_map(s, site_type)
+        d, nv2 = engine.dead_map(s, site_type)
+        assert nv == nv2 and not np.any(f & d)
+        free = np.argwhere(f)
+        assert len(free) > 0.01 * f.size
+        # overlap radius per framework type: E_LJ(r) = OverlapCriteria or r = 0.1 A
+        eps = np.asarray(s.eps).reshape(T, T)[:, site_type]
+        sig = np.asarray(s.sig).reshape(T, T)[:, site_type]
+        sh = np.asarray(s.shift).reshape(T, T)[:, site_type]
+        c = (s.overlap + sh) / (4.0 * eps)
+        r2_ov = np.maximum(0.01, sig * sig / np.cbrt(0.5 * (1.0 + np.sqrt(1.0 + 4.0 * c))))
+        pick = free[rng.choice(len(free), 300, replace=False)][:, ::-1].astype(float)
+        nva = np.array(nv, dtype=float)
+        pts = [(pick + np.array([(k >> a) & 1 for a in range(3)], dtype=float)) / nva for k in range(8)]
+        pts.append((pick + rng.uniform(0, 1, pick.shape)) / nva)
+        cart = np.concatenate(pts) @ cell.T
+        for p in cart:
+            dd = fw - p
+            sf = dd @ inv.T
+            sf -= np.round(sf)
+            dd = sf @ cell.T
+            rr = (dd * dd).sum(1)
+            assert np.all(rr >= r2_ov[ty]), (p, rr.min())
+        assert not _overlapping(s, cart[:200], site_type).any()
