@@ -1136,31 +1136,6 @@ __global__ __launch_bounds__(kFinWaves * 64) void widom_finish_kernel(const Wido
   }
 }
 
-// Work counts: per-workgroup partials summed in a fixed order: screened pairs
-// of the first-bead and chain workgroups ([0, n_screen) and
-// [kMaxScreenGroups, kMaxScreenGroups + n_screen)), then evaluated pairs of
-// the two pair-energy launches (kMaxEvalGroups slots each).
-__global__ void widom_stat_kernel(const unsigned long long* part, int n_screen, int n_eval1, int n_eval2,
-                                  unsigned long long* stat, unsigned long long* stat_tot)
-{
-  __shared__ unsigned long long s[2][64];
-  unsigned long long a = 0, e = 0;
-  for (int i = threadIdx.x; i < n_screen; i += 64) a += part[i] + part[kMaxScreenGroups + i];
-  for (int i = threadIdx.x; i < n_eval1; i += 64) e += part[2 * kMaxScreenGroups + i];
-  for (int i = threadIdx.x; i < n_eval2; i += 64) e += part[2 * kMaxScreenGroups + kMaxEvalGroups + i];
-  s[0][threadIdx.x] = a;
-  s[1][threadIdx.x] = e;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long x = 0, y = 0;
-    for (int i = 0; i < 64; ++i) { x += s[0][i]; y += s[1][i]; }
-    stat[0] = x;
-    stat[1] = y;
-    stat_tot[0] += x;
-    stat_tot[1] += y;
-  }
-}
-
 // Sums of the per-insertion records over fixed index ranges, in a fixed order
 // (deterministic): block b covers records [b*2048, (b+1)*2048).
 __global__ __launch_bounds__(kRedBlock) void widom_reduce_kernel(const double* rw, const double* re, const int* rf,
@@ -1192,10 +1167,27 @@ __global__ __launch_bounds__(kRedBlock) void widom_reduce_kernel(const double* r
   if (threadIdx.x < kSums) part[kSums * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
 
+// The batch's sums of the reduction partials, added to the running totals,
+// and (threads 0-63) its work counts: per-workgroup partials summed in a
+// fixed order, screened pairs of the first-bead and chain workgroups ([0,
+// n_screen) and [kMaxScreenGroups, kMaxScreenGroups + n_screen)), then
+// evaluated pairs of the two pair-energy launches (kMaxEvalGroups slots each).
 __global__ __launch_bounds__(kRedBlock) void widom_final_kernel(const double* part, int nblk, double* out,
-                                                                double* tot)
+                                                                double* tot, const unsigned long long* wpart,
+                                                                int n_screen, int n_eval1, int n_eval2,
+                                                                unsigned long long* stat, unsigned long long* stat_tot)
 {
+  static_assert(kRedBlock >= 64, "the work counts take the first 64 threads");
   __shared__ double s[kSums][kRedBlock];
+  __shared__ unsigned long long ws[2][64];
+  if (threadIdx.x < 64) {
+    unsigned long long a = 0, e = 0;
+    for (int i = threadIdx.x; i < n_screen; i += 64) a += wpart[i] + wpart[kMaxScreenGroups + i];
+    for (int i = threadIdx.x; i < n_eval1; i += 64) e += wpart[2 * kMaxScreenGroups + i];
+    for (int i = threadIdx.x; i < n_eval2; i += 64) e += wpart[2 * kMaxScreenGroups + kMaxEvalGroups + i];
+    ws[0][threadIdx.x] = a;
+    ws[1][threadIdx.x] = e;
+  }
   double acc[kSums] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nblk; b += kRedBlock)
     for (int q = 0; q < kSums; ++q) acc[q] += part[kSums * b + q];
@@ -1210,6 +1202,14 @@ __global__ __launch_bounds__(kRedBlock) void widom_final_kernel(const double* pa
     const double v = s[threadIdx.x][0];
     out[threadIdx.x] = v;
     tot[threadIdx.x] += v;   // running totals: batches of a box run in order on its stream
+  }
+  if (threadIdx.x == 0) {
+    unsigned long long x = 0, y = 0;
+    for (int i = 0; i < 64; ++i) { x += ws[0][i]; y += ws[1][i]; }
+    stat[0] = x;
+    stat[1] = y;
+    stat_tot[0] += x;
+    stat_tot[1] += y;
   }
 }
 
@@ -1662,11 +1662,9 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
                      n_insert, W.part);
   if (hip_check(hipGetLastError(), "widom_reduce launch")) return GRASPA_ERR_HIP;
   hipLaunchKernelGGL(widom_final_kernel, dim3(1), dim3(kRedBlock), 0, b->stream, W.part, nblk, b->d_wacc,
-                     b->d_wacc + kWaccTotals);
+                     b->d_wacc + kWaccTotals, A.stat_part, A.groups, egrid[0], egrid[1], A.stat,
+                     reinterpret_cast<unsigned long long*>(b->d_wacc + kWaccTotals + 8));
   if (hip_check(hipGetLastError(), "widom_final launch")) return GRASPA_ERR_HIP;
-  hipLaunchKernelGGL(widom_stat_kernel, dim3(1), dim3(64), 0, b->stream, A.stat_part, A.groups, egrid[0], egrid[1],
-                     A.stat, reinterpret_cast<unsigned long long*>(b->d_wacc + kWaccTotals + 8));
-  if (hip_check(hipGetLastError(), "widom_stat launch")) return GRASPA_ERR_HIP;
   if (summary) {
     double acc[12];
     if (hip_check(hipMemcpyAsync(acc, b->d_wacc, 12 * sizeof(double), hipMemcpyDeviceToHost, b->stream),
