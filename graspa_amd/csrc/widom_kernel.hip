@@ -633,13 +633,22 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int npts = (int)pre[G];
   unsigned long long n_eval = 0;
-  for (int pi = blockIdx.x * W + wv; pi < npts; pi += gridDim.x * W) {
-    // sub-list g holding point pi: the last g with pre[g] <= pi
-    int lo = 0, hi = G - 1;
+  // each wave takes a contiguous range of the points (which wave evaluates
+  // which point does not change any result): one binary search for its
+  // first point's sub-list, then the sub-list index only moves forward
+  const int nwave = gridDim.x * W, per_wave = (npts + nwave - 1) / nwave;
+  const int p_begin = (blockIdx.x * W + wv) * per_wave, p_end = min(npts, p_begin + per_wave);
+  int lo = 0;
+  if (p_begin < p_end) {
+    // sub-list g holding point p_begin: the last g with pre[g] <= p_begin
+    int hi = G - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if ((int)pre[mid] <= pi) lo = mid; else hi = mid - 1;
+      if ((int)pre[mid] <= p_begin) lo = mid; else hi = mid - 1;
     }
+  }
+  for (int pi = p_begin; pi < p_end; ++pi) {
+    while (lo + 1 < G && (int)pre[lo + 1] <= pi) ++lo;
     const TrialPt P = list[(size_t)lo * chunk * per_ins + (pi - (int)pre[lo])];
     const double px = P.x, py = P.y, pz = P.z;
     const double sq = A.site_q[P.site];
