@@ -633,21 +633,28 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int npts = (int)pre[G];
   unsigned long long n_eval = 0;
-  // each wave takes a contiguous range of the points (which wave evaluates
-  // which point does not change any result): one binary search for its
-  // first point's sub-list, then the sub-list index only moves forward
-  const int nwave = gridDim.x * W, per_wave = (npts + nwave - 1) / nwave;
-  const int p_begin = (blockIdx.x * W + wv) * per_wave, p_end = min(npts, p_begin + per_wave);
+  // which wave evaluates which point does not change any result.  A long
+  // list is cut into a contiguous range per wave (one binary search for its
+  // first point's sub-list, then the sub-list index only moves forward); a
+  // short one (under four points per wave) is dealt out point by point, which
+  // keeps the orientation sites of one insertion, whose overlap flags share a
+  // mask word, on waves of different workgroups
+  const int nwave = gridDim.x * W, wid = blockIdx.x * W + wv;
+  const int per_wave = (npts + nwave - 1) / nwave;
+  const bool ranges = per_wave >= 4;
+  const int p_begin = ranges ? wid * per_wave : wid, p_end = ranges ? min(npts, p_begin + per_wave) : npts;
+  const int p_step = ranges ? 1 : nwave;
   int lo = 0;
-  if (p_begin < p_end) {
-    // sub-list g holding point p_begin: the last g with pre[g] <= p_begin
-    int hi = G - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if ((int)pre[mid] <= p_begin) lo = mid; else hi = mid - 1;
+  for (int pi = p_begin; pi < p_end; pi += p_step) {
+    if (!ranges || pi == p_begin) {
+      // sub-list g holding point pi: the last g with pre[g] <= pi
+      int hi = G - 1;
+      lo = 0;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)pre[mid] <= pi) lo = mid; else hi = mid - 1;
+      }
     }
-  }
-  for (int pi = p_begin; pi < p_end; ++pi) {
     while (lo + 1 < G && (int)pre[lo + 1] <= pi) ++lo;
     const TrialPt P = list[(size_t)lo * chunk * per_ins + (pi - (int)pre[lo])];
     const double px = P.x, py = P.y, pz = P.z;
@@ -1177,23 +1184,23 @@ __global__ __launch_bounds__(kRedBlock) void widom_reduce_kernel(const double* r
 }
 
 // The batch's sums of the reduction partials, added to the running totals,
-// and (threads 0-63) its work counts: per-workgroup partials summed in a
-// fixed order, screened pairs of the first-bead and chain workgroups ([0,
-// n_screen) and [kMaxScreenGroups, kMaxScreenGroups + n_screen)), then
-// evaluated pairs of the two pair-energy launches (kMaxEvalGroups slots each).
+// and its work counts, sums of per-workgroup partials (integers): screened
+// pairs of the first-bead and chain workgroups ([0, n_screen) and
+// [kMaxScreenGroups, kMaxScreenGroups + n_screen)), then evaluated pairs of
+// the two pair-energy launches (kMaxEvalGroups slots each).
 __global__ __launch_bounds__(kRedBlock) void widom_final_kernel(const double* part, int nblk, double* out,
                                                                 double* tot, const unsigned long long* wpart,
                                                                 int n_screen, int n_eval1, int n_eval2,
                                                                 unsigned long long* stat, unsigned long long* stat_tot)
 {
-  static_assert(kRedBlock >= 64, "the work counts take the first 64 threads");
   __shared__ double s[kSums][kRedBlock];
-  __shared__ unsigned long long ws[2][64];
-  if (threadIdx.x < 64) {
+  __shared__ unsigned long long ws[2][kRedBlock];
+  {
+    // (integer sums: exact in any order)
     unsigned long long a = 0, e = 0;
-    for (int i = threadIdx.x; i < n_screen; i += 64) a += wpart[i] + wpart[kMaxScreenGroups + i];
-    for (int i = threadIdx.x; i < n_eval1; i += 64) e += wpart[2 * kMaxScreenGroups + i];
-    for (int i = threadIdx.x; i < n_eval2; i += 64) e += wpart[2 * kMaxScreenGroups + kMaxEvalGroups + i];
+    for (int i = threadIdx.x; i < n_screen; i += kRedBlock) a += wpart[i] + wpart[kMaxScreenGroups + i];
+    for (int i = threadIdx.x; i < n_eval1; i += kRedBlock) e += wpart[2 * kMaxScreenGroups + i];
+    for (int i = threadIdx.x; i < n_eval2; i += kRedBlock) e += wpart[2 * kMaxScreenGroups + kMaxEvalGroups + i];
     ws[0][threadIdx.x] = a;
     ws[1][threadIdx.x] = e;
   }
@@ -1214,7 +1221,7 @@ __global__ __launch_bounds__(kRedBlock) void widom_final_kernel(const double* pa
   }
   if (threadIdx.x == 0) {
     unsigned long long x = 0, y = 0;
-    for (int i = 0; i < 64; ++i) { x += ws[0][i]; y += ws[1][i]; }
+    for (int i = 0; i < kRedBlock; ++i) { x += ws[0][i]; y += ws[1][i]; }
     stat[0] = x;
     stat[1] = y;
     stat_tot[0] += x;
