@@ -636,8 +636,8 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
   // which wave evaluates which point does not change any result.  A long
   // list is cut into a contiguous range per wave (one binary search for its
   // first point's sub-list, then the sub-list index only moves forward); a
-  // short one (under four points per wave) is dealt out point by point (with
-  // ranges the orientation list of a batch ran slower, 29 against 36 us)
+  // short one (under four points per wave) is dealt out point by point (cut
+  // into ranges, the orientation list of a batch took 36 us instead of 29)
   const int nwave = gridDim.x * W, wid = blockIdx.x * W + wv;
   const int per_wave = (npts + nwave - 1) / nwave;
   const bool ranges = per_wave >= 4;
