@@ -35,6 +35,7 @@
 // The semantics are Insertion_Body (mc_swap_utilities.h:3-133) with
 // Widom_Move_FirstBead_PARTIAL / Widom_Move_Chain_PARTIAL (mc_widom.h:305-614).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -91,7 +92,9 @@ struct WidomArgs {
   int use_cells;
   int screen_chain;             // cell screen of the orientation sites (large frameworks only)
   int iso;                      // cubic cell with equal edges: the fast form's pair kernel works in cell fractions
-  int lj_shift;                 // some type pair has a nonzero LJ shift
+  int lj_general;               // the fast pair loop's general form: some LJ shift is nonzero, or a
+                                // molecule site's LJ with some framework type at r = 0.1 A stays
+                                // below twice OverlapCriteria (the r^2 < 0.01 test is then needed)
   int cell_n[3];
   int cell_cap;
   const uint4* ctab;       // [cells * cell_cap] fixed-point x,y,z + type, ~0u = empty
@@ -678,17 +681,19 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
       // coordinates through pbc_rint (15)
       const double Lc = bc.cell[0], L2 = Lc * Lc, iL = bc.inv[0];
       const double fx = mul_unfused(px, iL), fy = mul_unfused(py, iL), fz = mul_unfused(pz, iL);
-      auto run = [&](auto iso_c, auto same_c, auto shift_c) {
+      auto run = [&](auto iso_c, auto same_c, auto general_c) {
         constexpr bool ISO = decltype(iso_c)::value;      // distances from cell fractions
         constexpr bool SAME = decltype(same_c)::value;    // equal LJ and Coulomb cutoffs
-        constexpr bool SHIFT = decltype(shift_c)::value;  // some LJ shift is nonzero (else the row's
-                                                          // third entry is not read)
+        // general form: the shift is read and r^2 < 0.01 tested; else every
+        // pair closer than 0.1 A has an LJ term above OverlapCriteria (or NaN
+        // at r = 0), so !(e <= overlap) alone flags it as the reference does
+        constexpr bool GENERAL = decltype(general_c)::value;
         auto lj = [&](double rr, int ty, double& v_, bool& o_) {
           const double4* f = fr + ty;
           const double2 cc = *reinterpret_cast<const double2*>(f);
-          const double e = lj_mixed_c(rr, cc.x, cc.y, SHIFT ? f->z : 0.0);
+          const double e = lj_mixed_c(rr, cc.x, cc.y, GENERAL ? f->z : 0.0);
           v_ += e;
-          o_ = o_ || (e > bc.overlap) || (rr < 0.01);
+          o_ = o_ || !(e <= bc.overlap) || (GENERAL && rr < 0.01);
         };
         auto pair_fast = [&](double rr, double aq, int ty, double& v_, double& r_, bool& o_) {
           if constexpr (SAME) {
@@ -754,16 +759,16 @@ __global__ __launch_bounds__(W * 64) __attribute__((amdgpu_waves_per_eu(W == 16 
           }
         }
       };
-      auto run_same = [&](auto iso_c, auto shift_c) {
-        if (bc.cut_vdw2 == bc.cut_coul2) run(iso_c, std::true_type{}, shift_c);
-        else run(iso_c, std::false_type{}, shift_c);
+      auto run_same = [&](auto iso_c, auto general_c) {
+        if (bc.cut_vdw2 == bc.cut_coul2) run(iso_c, std::true_type{}, general_c);
+        else run(iso_c, std::false_type{}, general_c);
       };
-      auto run_shift = [&](auto iso_c) {
-        if (A.lj_shift) run_same(iso_c, std::true_type{});
+      auto run_general = [&](auto iso_c) {
+        if (A.lj_general) run_same(iso_c, std::true_type{});
         else run_same(iso_c, std::false_type{});
       };
-      if (CUBIC && A.iso) run_shift(std::true_type{});
-      else run_shift(std::false_type{});
+      if (CUBIC && A.iso) run_general(std::true_type{});
+      else run_general(std::false_type{});
       v += v1;
       r += r1;
       o = o || o1;
@@ -1575,7 +1580,19 @@ int launch_widom(graspa_box* b, int comp, int K, int O, double beta, int bias, d
   for (int a = 0; a < c.molsize; ++a) qabs += std::fabs(c.q[a]);
   A.charged = (b->bc.use_charges && qabs > 1e-6) ? 1 : 0;
   A.iso = (b->bc.cubic && b->bc.cell[0] == b->bc.cell[4] && b->bc.cell[0] == b->bc.cell[8]) ? 1 : 0;
-  A.lj_shift = std::any_of(b->shift.begin(), b->shift.end(), [](double x) { return x != 0.0; }) ? 1 : 0;
+  {
+    // the fast pair loop's general form (see lj_general)
+    bool general = std::any_of(b->shift.begin(), b->shift.end(), [](double x) { return x != 0.0; });
+    const int T = b->bc.ntypes;
+    for (int a = 0; a < c.molsize && !general; ++a)
+      for (int t = 0; t < T && !general; ++t) {
+        const int ij = t * T + c.type[a];
+        const double s6 = std::pow(b->sig[ij] * b->sig[ij] / 0.01, 3.0);
+        const double e01 = 4.0 * b->eps[ij] * (s6 * s6 - s6) - b->shift[ij];
+        if (!(e01 > 2.0 * b->bc.overlap)) general = true;
+      }
+    A.lj_general = general ? 1 : 0;
+  }
   A.K = K; A.O = O; A.beta = beta; A.bias = bias; A.excl = excl; A.tail_de = tail_de;
   A.seed = seed; A.stream = stream; A.first_index = how.first_dev ? 0 : first_index; A.n_insert = n_insert;
   A.first_dev = how.first_dev;

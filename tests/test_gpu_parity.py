@@ -281,6 +281,18 @@ def test_widom_mixed_unequal_cutoffs_and_shift(engine_mod, cut_vdw, cut_coul, sh
     _check_mixed(ref, got)
 
 
+def test_widom_mixed_n2_massless_site(engine_mod):
+    """N2 has a massless centre with no LJ (epsilon 0): its pairs are overlaps
+    only by r^2 < 0.01, so the fast pair loop must take its general form (the
+    explicit r^2 test); same constructions as the FP64 oracle."""
+    from graspa_amd import synth
+    s = synth.build(n_fw=400, L=24.0, species=("N2",), seed=11, T=300.0, P=1e5, probs=dict(widom=1.0))
+    s.use_max_step, s.max_step_per_cycle = True, 1
+    _, ref, got, _ = _widom_pair(400, 300, 37, engine_mod.GRASPA_MIXED, engine_mod, system=s)
+    assert ref["ok"].sum() > 200
+    _check_mixed(ref, got)
+
+
 def test_widom_result_independent_of_batching(engine_mod):
     """Insertion i gives the same result alone, inside a larger batch, or
     offset by first_index: the counter-based streams make each result a pure
