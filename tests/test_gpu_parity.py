@@ -293,6 +293,28 @@ def test_widom_mixed_n2_massless_site(engine_mod):
     _check_mixed(ref, got)
 
 
+@pytest.mark.parametrize("form", ["fp64", "mixed"])
+def test_widom_single_site_species(engine_mod, form):
+    """A one-site molecule (united-atom methane, uncharged): no orientations,
+    so the chain kernel's placed list goes straight to the finish kernel
+    (no Fourier term); the same insertions, weights and energies as the
+    FP64 oracle."""
+    from graspa_amd import synth
+    s = synth.build(n_fw=400, L=24.0, species=("methane",), seed=13, T=300.0, P=1e5, probs=dict(widom=1.0))
+    prec = engine_mod.GRASPA_FP64 if form == "fp64" else engine_mod.GRASPA_MIXED
+    _, ref, got, summ = _widom_pair(400, 300, 41, prec, engine_mod, system=s)
+    assert ref["ok"].sum() > 200
+    if form == "fp64":
+        assert np.array_equal(got["ok"], ref["ok"])
+        ok = ref["ok"] == 1
+        assert np.array_equal(got["first_bead"][ok], ref["first_bead"][ok])
+        assert _close(got["energy"], ref["energy"], atol=1e-8)
+        assert _close(got["rosen"], ref["rosen"], rtol=1e-9, atol=0.0)
+    else:
+        _check_mixed(ref, got)
+    assert summ.n == 300 and summ.n_ok == ref["ok"].sum()
+
+
 def test_widom_result_independent_of_batching(engine_mod):
     """Insertion i gives the same result alone, inside a larger batch, or
     offset by first_index: the counter-based streams make each result a pure
